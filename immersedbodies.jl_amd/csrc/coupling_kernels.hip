@@ -1,0 +1,394 @@
+// Fluid-body coupling kernels: regularized-delta weights, E (interpolation onto panels),
+// rot(E^T f) (regularisation onto the fluid, scattered without atomics by a gather over
+// panels sorted by column), the dense B^-1 product with the force output, and the
+// conjugate-gradient reductions kept in FP64.
+#include <hip/hip_runtime.h>
+#include <limits.h>
+
+#include "ibpm_device.h"
+#include "ibpm_internal.h"
+
+namespace ibpm {
+
+// delta_h of coupling.jl:91-148: the Yang3 smoothed 3-point kernel times h. The
+// polynomial coefficients are the reference's literal constants (they define the
+// kernel); asin and sqrt are the device library's.
+__device__ double delta_h(double rf, double rb, double dr) {
+  const double r = fabs(rf - rb);
+  const double r1 = r / dr;
+  const double r2 = r1 * r1;
+  const double r3 = r2 * r1;
+  const double r4 = r3 * r1;
+  if (r1 <= 1.0) {
+    const double a5 = asin((1.0 / 2.0) * sqrt(3.0) * (2.0 * r1 - 1.0));
+    const double a8 = sqrt(1.0 - 12.0 * r2 + 12.0 * r1);
+    return 4.166666667e-2 * r4 + (-0.1388888889 + 3.472222222e-2 * a8) * r3 +
+           (-7.121664902e-2 - 5.208333333e-2 * a8 + 0.2405626122 * a5) * r2 +
+           (-0.2405626122 * a5 - 0.3792313933 + 0.1012731481 * a8) * r1 +
+           8.0187537413e-2 * a5 - 4.195601852e-2 * a8 + 0.6485698427;
+  } else if (r1 <= 2.0) {
+    const double a6 = asin((1.0 / 2.0) * sqrt(3.0) * (-3.0 + 2.0 * r1));
+    const double a9 = sqrt(-23.0 + 36.0 * r1 - 12.0 * r2);
+    return -6.250000000e-2 * r4 + (0.4861111111 - 1.736111111e-2 * a9) * r3 +
+           (-1.143175026 + 7.812500000e-2 * a9 - 0.1202813061 * a6) * r2 +
+           (0.8751991178 + 0.3608439183 * a6 - 0.1548032407 * a9) * r1 - 0.2806563809 * a6 +
+           8.22848104e-3 + 0.1150173611 * a9;
+  } else if (r1 <= 3.0) {
+    const double a1 = asin((1.0 / 2.0 * (2.0 * r1 - 5.0)) * sqrt(3.0));
+    const double a7 = sqrt(-71.0 - 12.0 * r2 + 60.0 * r1);
+    return 2.083333333e-2 * r4 + (3.472222222e-3 * a7 - 0.2638888889) * r3 +
+           (1.214391675 - 2.604166667e-2 * a7 + 2.405626122e-2 * a1) * r2 +
+           (-0.1202813061 * a1 - 2.449273192 + 7.262731481e-2 * a7) * r1 + 0.1523563211 * a1 +
+           1.843201677 - 7.306134259e-2 * a7;
+  }
+  return 0.0;
+}
+
+// ---------------------------------------------------------------------------------
+// Panel kinematics and weights. For moving bodies (bodies.jl:12-31):
+//   xb = r0 + R(theta) rb,  ub = v0 + Omega R'(theta) rb,  R' = [-s -c; c -s]
+// then _update!(reg, ...) (coupling.jl:27-54): nearest flux cell
+//   (ix, iy) = round((xb - x0)/h + 0.75), 3 < ix <= nx-3, 3 < iy <= ny-3
+// and the 7 x 7 weights at the u and v flux points around it.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void body_update_kernel(const BodyDev b, int only_moving) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= b.npanel) return;
+  const int n = b.npanel;
+  const int body = b.body_of[p];
+  const bool mv = b.moving[body] != 0;
+  if (only_moving && !mv) return;
+  double px, py;
+  if (mv) {
+    const double* m = b.motion + 6 * body;
+    const double r0x = m[0], r0y = m[1], v0x = m[2], v0y = m[3], ang = m[4], om = m[5];
+    const double c = cos(ang), s = sin(ang);
+    const double rx = b.xb_ref[p], ry = b.xb_ref[n + p];
+    px = r0x + (c * rx + -s * ry);
+    py = r0y + (s * rx + c * ry);
+    const double Rv11 = om * -s, Rv12 = om * -c, Rv21 = om * c, Rv22 = om * -s;
+    b.xb[p] = px;
+    b.xb[n + p] = py;
+    b.ub[p] = v0x + (Rv11 * rx + Rv12 * ry);
+    b.ub[n + p] = v0y + (Rv21 * rx + Rv22 * ry);
+  } else {
+    px = b.xb[p];
+    py = b.xb[n + p];
+  }
+  const double h = b.h;
+  const int ix = (int)rint((px - b.x0) / h + 1.0 - 0.25);
+  const int iy = (int)rint((py - b.y0) / h + 1.0 - 0.25);
+  const bool ok = (3 < ix && ix <= b.nx - 3) && (3 < iy && iy <= b.ny - 3);
+  if (!ok) {
+    atomicMax(b.err, n - p);  // records the smallest offending panel index
+    b.coords[2 * p] = 4;      // keep later gathers inside the plane
+    b.coords[2 * p + 1] = 4;
+    return;
+  }
+  b.coords[2 * p] = ix;
+  b.coords[2 * p + 1] = iy;
+  double dxu[7], dyu[7], dxv[7], dyv[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    const double X = b.x0 + h * (double)(ix + k - 4);
+    const double Y = b.y0 + h * (double)(iy + k - 4);
+    dxu[k] = delta_h(X, px, h);
+    dyu[k] = delta_h(Y + h / 2, py, h);
+    dxv[k] = delta_h(X + h / 2, px, h);
+    dyv[k] = delta_h(Y, py, h);
+  }
+  double* w = b.w + (size_t)p * 98;
+#pragma unroll
+  for (int bb = 0; bb < 7; ++bb)
+#pragma unroll
+    for (int a = 0; a < 7; ++a) {
+      w[bb * 7 + a] = dxu[a] * dyu[bb];
+      w[49 + bb * 7 + a] = dxv[a] * dyv[bb];
+    }
+}
+
+hipError_t launch_body_update(const BodyDev& b, int only_moving, hipStream_t st) {
+  if (b.npanel == 0) return hipSuccess;
+  hipLaunchKernelGGL(body_update_kernel, dim3((b.npanel + 255) / 256), dim3(256), 0, st, b,
+                     only_moving);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Sort panels by flux column ix (bitonic sort of (ix, p) keys in LDS, one work-group)
+// and build xstart[c] = number of panels with ix < c, c = 0 .. nx+2. rot(E^T f) then
+// gathers, for each vertex, only the panels of the 8 columns that can reach it.
+// ---------------------------------------------------------------------------------
+constexpr int SORT_MAX = 8192;
+
+__global__ __launch_bounds__(1024) void body_sort_kernel(const BodyDev b) {
+  __shared__ unsigned keys[SORT_MAX];
+  const int n = b.npanel;
+  int np2 = 1;
+  while (np2 < n) np2 <<= 1;
+  for (int i = threadIdx.x; i < np2; i += 1024)
+    keys[i] = i < n ? ((unsigned)b.coords[2 * i] << 13) | (unsigned)i : 0xffffffffu;
+  __syncthreads();
+  for (int k = 2; k <= np2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < np2; i += 1024) {
+        const int l = i ^ j;
+        if (l > i) {
+          const unsigned a = keys[i], c = keys[l];
+          const bool up = (i & k) == 0;
+          if ((a > c) == up) {
+            keys[i] = c;
+            keys[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < n; i += 1024) b.perm[i] = (int)(keys[i] & 8191u);
+  for (int c = threadIdx.x; c <= b.nx + 2; c += 1024) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if ((int)(keys[mid] >> 13) < c) lo = mid + 1;
+      else hi = mid;
+    }
+    b.xstart[c] = lo;
+  }
+}
+
+hipError_t launch_body_sort(const BodyDev& b, hipStream_t st) {
+  if (b.npanel == 0) return hipSuccess;
+  hipLaunchKernelGGL(body_sort_kernel, dim3(1), dim3(1024), 0, st, b);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// E (regT!, coupling.jl:57-69) of the level-1 flux curl(psi1) + q0, minus ub*h
+// (couple_surface_rigid, coupling.jl:239-244). One thread per (panel, component, rhs);
+// the 7 x 7 dot product runs in the reference's column-major order.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void gather_kernel(const GatherArgs a) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int n = a.npanel;
+  if (t >= 2 * n) return;
+  const int rhs = blockIdx.y;
+  const int comp = t >= n;
+  const int p = comp ? t - n : t;
+  const int nx = a.g.nx, ny = a.g.ny;
+  const double* p1 = a.psi1[rhs];
+  const double* p2 = a.psi2[rhs];
+  const int ix = a.coords[2 * p], iy = a.coords[2 * p + 1];
+  const double* w = a.w + (size_t)p * 98 + comp * 49;
+  double s = 0.0;
+  for (int bb = 0; bb < 7; ++bb) {
+    const int j = iy + bb - 3;
+    for (int aa = 0; aa < 7; ++aa) {
+      const int i = ix + aa - 3;
+      double qv;
+      if (!comp)  // u(i,j) = psi_e(i-1, j) - psi_e(i-1, j-1)
+        qv = (ext_value(p1, p2, nx, ny, i - 1, j, 1.0) - ext_value(p1, p2, nx, ny, i - 1, j - 1, 1.0)) + a.q0u;
+      else  // v(i,j) = psi_e(i-1, j-1) - psi_e(i, j-1)
+        qv = (ext_value(p1, p2, nx, ny, i - 1, j - 1, 1.0) - ext_value(p1, p2, nx, ny, i, j - 1, 1.0)) + a.q0v;
+      s += qv * w[bb * 7 + aa];
+    }
+  }
+  if (a.ub) s -= a.ub[t] * a.h;
+  a.out[rhs][t] = s;
+}
+
+hipError_t launch_gather(const GatherArgs& a, hipStream_t st) {
+  if (a.npanel == 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_kernel, dim3((2 * a.npanel + 255) / 256, a.nrhs), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// rot(E^T f) on the level-1 plane (reg! coupling.jl:72-85, then rot! fluid.jl:101-111):
+//   Gamma(I,J) = (u(I+1,J) - u(I+1,J+1)) + (v(I+1,J+1) - v(I,J+1)),
+//   u(i,j) = sum over panels p with |i-ix_p| <= 3, |j-iy_p| <= 3 of w_u[p] f_x[p].
+// Every vertex writes its own value (zero away from the bodies): no atomics.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void scatter_rot_kernel(const ScatterArgs a) {
+  const GridDims g = a.g;
+  const int rhs = blockIdx.z;
+  const int I = blockIdx.x * 64 + (threadIdx.x & 63) + 1;
+  const int J = blockIdx.y * 4 + (threadIdx.x >> 6) + 1;
+  if (I > g.Nx || J > g.Ny) return;
+  const int n = a.npanel;
+  const int c0 = max(I - 3, 0);
+  const int c1 = min(I + 5, g.nx + 2);  // columns ix in [I-3, I+4]
+  const int s0 = a.xstart[c0], s1 = a.xstart[c1];
+  double ua = 0.0, ub = 0.0, va = 0.0, vb = 0.0;
+  const double* f = a.f[rhs];
+  for (int s = s0; s < s1; ++s) {
+    const int p = a.perm[s];
+    const int ix = a.coords[2 * p], iy = a.coords[2 * p + 1];
+    const int db = J - iy + 3;  // b index of row J; row J+1 has db+1
+    if (db < -1 || db > 6) continue;
+    const double* w = a.w + (size_t)p * 98;
+    const double fx = f[p], fy = f[n + p];
+    const int au = I + 1 - ix + 3;  // a index of column I+1
+    const int av = I - ix + 3;      // a index of column I
+    if (au >= 0 && au <= 6) {
+      if (db >= 0) ua += w[db * 7 + au] * fx;
+      if (db + 1 <= 6) {
+        ub += w[(db + 1) * 7 + au] * fx;
+        va += w[49 + (db + 1) * 7 + au] * fy;
+      }
+    }
+    if (av >= 0 && av <= 6 && db + 1 <= 6) vb += w[49 + (db + 1) * 7 + av] * fy;
+  }
+  const double v = (ua - ub) + (va - vb);
+  const size_t idx = (size_t)(I - 1) + (size_t)(J - 1) * g.Nx;
+  if (a.out_f32)
+    ((float*)a.out[rhs])[idx] = (float)v;
+  else
+    ((double*)a.out[rhs])[idx] = v;
+}
+
+hipError_t launch_scatter_rot(const ScatterArgs& a, hipStream_t st) {
+  dim3 grid((a.g.Nx + 63) / 64, (a.g.Ny + 3) / 4, a.nrhs);
+  hipLaunchKernelGGL(scatter_rot_kernel, grid, dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Dense B^-1 product (coupling.jl:191) and update_vars! (problem.jl:676-685):
+//   Ft = Binv * b;  fb = Ft * h / dt;  total = column sums of fb (FP64).
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void gemv_kernel(int m, const double* __restrict__ M,
+                                                   const double* __restrict__ b,
+                                                   double* __restrict__ x) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  double acc = 0.0;
+  for (int j = 0; j < m; ++j) acc += M[i + (size_t)j * m] * b[j];
+  x[i] = acc;
+}
+
+__global__ __launch_bounds__(1024) void forces_kernel(int n, const double* __restrict__ Ft,
+                                                      double* __restrict__ fb,
+                                                      double* __restrict__ total, double h,
+                                                      double dt) {
+  __shared__ double sx[1024], sy[1024];
+  double ax = 0.0, ay = 0.0;
+  for (int p = threadIdx.x; p < n; p += 1024) {
+    const double fx = Ft[p] * h / dt, fy = Ft[n + p] * h / dt;
+    fb[p] = fx;
+    fb[n + p] = fy;
+    ax += fx;
+    ay += fy;
+  }
+  sx[threadIdx.x] = ax;
+  sy[threadIdx.x] = ay;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sx[threadIdx.x] += sx[threadIdx.x + s];
+      sy[threadIdx.x] += sy[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    total[0] = sx[0];
+    total[1] = sy[0];
+  }
+}
+
+hipError_t launch_gemv(int m, const double* M, const double* b, double* x, hipStream_t st) {
+  if (m == 0) return hipSuccess;
+  hipLaunchKernelGGL(gemv_kernel, dim3((m + 255) / 256), dim3(256), 0, st, m, M, b, x);
+  return hipGetLastError();
+}
+
+hipError_t launch_forces(int n, const double* Ft, double* fb, double* total, double h, double dt,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(forces_kernel, dim3(1), dim3(1024), 0, st, n, Ft, fb, total, h, dt);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Conjugate gradient of IterativeSolvers.jl's cg! (unpreconditioned CGIterable), as
+// called by _Binv_iterative (coupling.jl:194-202), with FP64 reductions in one
+// work-group. scal: [0] residual, [1] prev residual, [2] tolerance, [3] iterations,
+// [4] converged flag, [5] ||b||.
+// ---------------------------------------------------------------------------------
+__device__ double block_sum_1024(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(1024) void cg_init_kernel(const CgDev c) {
+  __shared__ double sh[1024];
+  double rr = 0.0, bb = 0.0;
+  for (int i = threadIdx.x; i < c.n; i += 1024) {
+    const double r = c.b[i] - c.c[i];
+    c.r[i] = r;
+    rr += r * r;
+    bb += c.b[i] * c.b[i];
+  }
+  const double res = sqrt(block_sum_1024(rr, sh));
+  const double bn = sqrt(block_sum_1024(bb, sh));
+  double tol = c.reltol * (c.tol_mode == 1 ? bn : res);
+  if (c.abstol > tol) tol = c.abstol;
+  const bool conv = (0 >= c.maxiter) || res <= tol;
+  // first search direction: u = r + beta*0 with beta = res^2 / 1^2
+  for (int i = threadIdx.x; i < c.n; i += 1024) c.u[i] = conv ? 0.0 : c.r[i];
+  if (threadIdx.x == 0) {
+    c.scal[0] = res;
+    c.scal[1] = 1.0;
+    c.scal[2] = tol;
+    c.scal[3] = 0.0;
+    c.scal[4] = conv ? 1.0 : 0.0;
+    c.scal[5] = bn;
+  }
+}
+
+__global__ __launch_bounds__(1024) void cg_step_kernel(const CgDev c) {
+  __shared__ double sh[1024];
+  if (c.scal[4] != 0.0) return;
+  const double res = c.scal[0];
+  double uc = 0.0;
+  for (int i = threadIdx.x; i < c.n; i += 1024) uc += c.u[i] * c.c[i];
+  const double alpha = res * res / block_sum_1024(uc, sh);
+  double rr = 0.0;
+  for (int i = threadIdx.x; i < c.n; i += 1024) {
+    c.x[i] += alpha * c.u[i];
+    const double r = c.r[i] - alpha * c.c[i];
+    c.r[i] = r;
+    rr += r * r;
+  }
+  const double nres = sqrt(block_sum_1024(rr, sh));
+  const double iters = c.scal[3] + 1.0;
+  const bool conv = iters >= (double)c.maxiter || nres <= c.scal[2];
+  if (!conv) {
+    const double beta = nres * nres / (res * res);
+    for (int i = threadIdx.x; i < c.n; i += 1024) c.u[i] = c.r[i] + beta * c.u[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    c.scal[1] = res;
+    c.scal[0] = nres;
+    c.scal[3] = iters;
+    c.scal[4] = conv ? 1.0 : 0.0;
+  }
+}
+
+hipError_t launch_cg_init(const CgDev& c, hipStream_t st) {
+  hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(1024), 0, st, c);
+  return hipGetLastError();
+}
+hipError_t launch_cg_step(const CgDev& c, hipStream_t st) {
+  hipLaunchKernelGGL(cg_step_kernel, dim3(1), dim3(1024), 0, st, c);
+  return hipGetLastError();
+}
+
+}  // namespace ibpm

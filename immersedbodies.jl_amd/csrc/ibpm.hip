@@ -1,0 +1,1031 @@
+// libibpm: the IBPM time step (advance! of ImmersedBodies.jl's CnabSolver) on one MI355X.
+//
+// Host orchestration of the kernels in dst_kernels.hip, stencil_kernels.hip and
+// coupling_kernels.hip behind the C ABI of include/ibpm.h. Everything the step touches
+// lives in HBM in the reference's layout; the host only launches kernels, evaluates
+// nothing per grid point, and synchronises once per CG iteration (moving bodies) or
+// once per ibpm_advance call (static bodies).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/ibpm.h"
+#include "ibpm_internal.h"
+
+using namespace ibpm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(IBPM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  return hipMalloc((void**)p, n * sizeof(T));
+}
+
+// One plane of a DST-I solve  x = S_x S_y (diag . (S_x b S_y)), optionally
+// x = base -/+ solve, where the system operator is a I + c K (for refinement).
+struct SolveJob {
+  const double* b;
+  const double* ring;  // apply_bc! from this coarse plane (vort2flux), or null
+  const double* diag;
+  double* x;
+  int epi;             // EPI_STORE, EPI_SUB (x = base - solve)
+  const double* base;
+  double a, c;
+};
+
+}  // namespace
+
+struct ibpm_solver {
+  GridDims g{};
+  int L = 0;
+  double x0 = 0, y0 = 0, h = 0, Re = 0, dt = 0, t0 = 0;
+  int index = 0;
+  int n = 0, nbody = 0, moving = 0;
+  int precision = IBPM_FP64, refine = 1;
+  int cg_maxiter = 5000, cg_tol_mode = 0;
+  double cg_reltol = 1e-12, cg_abstol = 0.0;
+  int device = 0;
+  hipStream_t st = nullptr;
+  double fs[2] = {0.0, 0.0};
+  std::vector<double> hc;  // grid step per level
+
+  // fields
+  double *q = nullptr, *G = nullptr, *psi = nullptr, *nl1 = nullptr, *nl2 = nullptr;
+  double *Gs = nullptr, *rhs = nullptr;
+  double *T = nullptr, *U = nullptr, *R = nullptr;  // kMaxPlanes scratch planes each
+  double *dpois = nullptr, *dAinv = nullptr;
+  void *SoX = nullptr, *SeX = nullptr, *SoY = nullptr, *SeY = nullptr;  // FP64 tables
+  void *BfX = nullptr, *BfY = nullptr;                                  // bf16 tables
+  // bodies
+  double *xb = nullptr, *ub = nullptr, *xb_ref = nullptr, *Ft = nullptr, *fb = nullptr;
+  double *Ftmp = nullptr, *total = nullptr, *motion = nullptr;
+  int *body_of = nullptr, *moving_d = nullptr, *coords = nullptr, *perm = nullptr;
+  int *xstart = nullptr, *err = nullptr;
+  double* w = nullptr;
+  double* Binv = nullptr;
+  // B operator scratch (ncol columns)
+  int bcols = 0;
+  double *Gtmp = nullptr, *psitmp = nullptr, *Sc = nullptr;
+  // CG
+  double *cg_r = nullptr, *cg_u = nullptr, *cg_c = nullptr, *cg_scal = nullptr;
+  double* h_scal = nullptr;  // pinned
+  // host copies
+  std::vector<double> ds;
+  std::vector<double> q0_shadow;
+  bool q0_shadow_valid = false;
+  std::vector<int> body_start, body_count, body_moving;
+
+  size_t plane() const { return (size_t)g.nG; }
+};
+
+namespace {
+
+void free_all(ibpm_solver* s) {
+  void* ptrs[] = {s->q, s->G, s->psi, s->nl1, s->nl2, s->Gs, s->rhs, s->T, s->U, s->R,
+                  s->dpois, s->dAinv, s->SoX, s->SeX, s->SoY, s->SeY, s->BfX, s->BfY,
+                  s->xb, s->ub, s->xb_ref, s->Ft, s->fb, s->Ftmp, s->total, s->motion,
+                  s->body_of, s->moving_d, s->coords, s->perm, s->xstart, s->err, s->w,
+                  s->Binv, s->Gtmp, s->psitmp, s->Sc, s->cg_r, s->cg_u, s->cg_c, s->cg_scal};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (s->h_scal) (void)hipHostFree(s->h_scal);
+  if (s->st) (void)hipStreamDestroy(s->st);
+}
+
+const void* tab_odd(const ibpm_solver* s, bool xdir) {
+  if (s->precision == IBPM_MIXED) return xdir ? s->BfX : s->BfY;
+  return xdir ? s->SoX : s->SoY;
+}
+const void* tab_even(const ibpm_solver* s, bool xdir) {
+  if (s->precision == IBPM_MIXED) {
+    const int N = xdir ? s->g.Nx : s->g.Ny;
+    const uint16_t* base = (const uint16_t*)(xdir ? s->BfX : s->BfY);
+    return base + sine_table_bf16_odd_count(N);
+  }
+  return xdir ? s->SeX : s->SeY;
+}
+
+// One pass over up to kMaxPlanes planes.
+int run_pass(ibpm_solver* s, PassBatch& pb, int np, bool xdir, int epi, bool ring, bool in_f32,
+             bool out_f32) {
+  const int Nk = xdir ? s->g.Nx : s->g.Ny;
+  const int Nb = xdir ? s->g.Ny : s->g.Nx;
+  for (int i = 0; i < np; ++i) {
+    pb.p[i].Nk = Nk;
+    pb.p[i].Nb = Nb;
+    pb.p[i].nx = s->g.nx;
+    pb.p[i].ny = s->g.ny;
+    pb.p[i].So = tab_odd(s, xdir);
+    pb.p[i].Se = tab_even(s, xdir);
+  }
+  const int H = (Nk + 1) / 2;
+  if (s->precision == IBPM_MIXED)
+    HIPCHK(launch_dst_pass_mixed(pb, np, Nb, H, epi, ring, in_f32, out_f32, s->st));
+  else
+    HIPCHK(launch_dst_pass_f64(pb, np, Nb, H, epi, ring, s->st));
+  return IBPM_OK;
+}
+
+// DST-I solves of up to kMaxPlanes jobs; scratch planes T[i], U[i], R[i].
+int solve_chunk(ibpm_solver* s, const SolveJob* jobs, int nj) {
+  const size_t P = s->plane();
+  const bool mixed = s->precision == IBPM_MIXED;
+  PassBatch pb;
+  auto Tp = [&](int i) { return (void*)(s->T + i * P); };
+  auto Up = [&](int i) { return (void*)(s->U + i * P); };
+  auto four_passes = [&](bool from_residual, bool correction) -> int {
+    bool any_ring = false;
+    // pass 1: x-direction, b (+ring) -> T
+    memset(&pb, 0, sizeof pb);
+    for (int i = 0; i < nj; ++i) {
+      pb.p[i].in = from_residual ? (const void*)(s->R + i * P) : (const void*)jobs[i].b;
+      pb.p[i].out = Tp(i);
+      pb.p[i].ring = from_residual ? nullptr : jobs[i].ring;
+      if (pb.p[i].ring) any_ring = true;
+    }
+    if (any_ring) {
+      // planes without a ring are not allowed in a ring launch: split
+      PassBatch a, b;
+      memset(&a, 0, sizeof a);
+      memset(&b, 0, sizeof b);
+      int na = 0, nb = 0;
+      int ia[kMaxPlanes], ib[kMaxPlanes];
+      for (int i = 0; i < nj; ++i) {
+        if (pb.p[i].ring) { ia[na] = i; a.p[na++] = pb.p[i]; }
+        else { ib[nb] = i; b.p[nb++] = pb.p[i]; }
+      }
+      (void)ia; (void)ib;
+      if (na) { int rc = run_pass(s, a, na, true, EPI_STORE, true, false, mixed); if (rc) return rc; }
+      if (nb) { int rc = run_pass(s, b, nb, true, EPI_STORE, false, false, mixed); if (rc) return rc; }
+    } else {
+      int rc = run_pass(s, pb, nj, true, EPI_STORE, false, from_residual && mixed, mixed);
+      if (rc) return rc;
+    }
+    // pass 2: y-direction, T -> U, times diag
+    memset(&pb, 0, sizeof pb);
+    for (int i = 0; i < nj; ++i) {
+      pb.p[i].in = Tp(i);
+      pb.p[i].out = Up(i);
+      pb.p[i].diag = jobs[i].diag;
+    }
+    int rc = run_pass(s, pb, nj, false, EPI_DIAG, false, mixed, mixed);
+    if (rc) return rc;
+    // pass 3: x-direction, U -> T
+    memset(&pb, 0, sizeof pb);
+    for (int i = 0; i < nj; ++i) {
+      pb.p[i].in = Up(i);
+      pb.p[i].out = Tp(i);
+    }
+    rc = run_pass(s, pb, nj, true, EPI_STORE, false, mixed, mixed);
+    if (rc) return rc;
+    // pass 4: y-direction, T -> x with the job's epilogue
+    memset(&pb, 0, sizeof pb);
+    int epi = EPI_STORE;
+    for (int i = 0; i < nj; ++i) {
+      pb.p[i].in = Tp(i);
+      pb.p[i].out = jobs[i].x;
+      if (correction) {
+        epi = jobs[i].epi == EPI_SUB ? EPI_SUB : EPI_ADD;
+        pb.p[i].base = jobs[i].x;
+      } else {
+        epi = jobs[i].epi;
+        pb.p[i].base = jobs[i].base;
+      }
+    }
+    return run_pass(s, pb, nj, false, epi, false, mixed, false);
+  };
+  int rc = four_passes(false, false);
+  if (rc) return rc;
+  if (mixed) {
+    for (int sweep = 0; sweep < s->refine; ++sweep) {
+      ResidualArgs ra;
+      memset(&ra, 0, sizeof ra);
+      ra.g = s->g;
+      ra.nplanes = nj;
+      ra.r_f32 = 1;
+      for (int i = 0; i < nj; ++i) {
+        ra.b[i] = jobs[i].b;
+        ra.x[i] = jobs[i].x;
+        ra.r[i] = s->R + i * P;
+        ra.a[i] = jobs[i].a;
+        ra.c[i] = jobs[i].c;
+        ra.ring[i] = jobs[i].ring;
+        ra.xbase[i] = jobs[i].epi == EPI_SUB ? jobs[i].base : nullptr;
+      }
+      HIPCHK(launch_residual(ra, s->st));
+      rc = four_passes(true, true);
+      if (rc) return rc;
+    }
+  }
+  return IBPM_OK;
+}
+
+int solve(ibpm_solver* s, const SolveJob* jobs, int nj) {
+  // all jobs of a chunk must agree on the epilogue kind
+  int i = 0;
+  while (i < nj) {
+    int k = i + 1;
+    while (k < nj && k - i < kMaxPlanes && jobs[k].epi == jobs[i].epi) ++k;
+    int rc = solve_chunk(s, jobs + i, k - i);
+    if (rc) return rc;
+    i = k;
+  }
+  return IBPM_OK;
+}
+
+// vort2flux (fluid.jl:175-206) for ncol independent columns of L planes each.
+// With `outer`, the coarse levels outside the centre come from outer (final pass of the
+// step); otherwise coarsify! writes the centre region in place. q: curl all levels.
+int vort2flux(ibpm_solver* s, int ncol, double* const* Gc, double* const* psic,
+              double* const* outer, double* const* qc) {
+  const size_t P = s->plane();
+  const int L = s->L;
+  for (int c = 0; c < ncol; ++c)
+    for (int l = 1; l < L; ++l)
+      HIPCHK(launch_coarsify(s->g, Gc[c] + l * P, Gc[c] + (l - 1) * P,
+                             outer && outer[c] ? outer[c] + l * P : nullptr, s->st));
+  std::vector<SolveJob> jobs(ncol);
+  for (int l = L - 1; l >= 0; --l) {
+    for (int c = 0; c < ncol; ++c) {
+      SolveJob& j = jobs[c];
+      j.b = Gc[c] + l * P;
+      j.ring = l + 1 < L ? psic[c] + (l + 1) * P : nullptr;
+      j.diag = s->dpois;
+      j.x = psic[c] + l * P;
+      j.epi = EPI_STORE;
+      j.base = nullptr;
+      j.a = 0.0;
+      j.c = 1.0;
+    }
+    int rc = solve(s, jobs.data(), ncol);
+    if (rc) return rc;
+  }
+  if (qc)
+    for (int c = 0; c < ncol; ++c) HIPCHK(launch_curl_all(s->g, L, qc[c], psic[c], s->st));
+  return IBPM_OK;
+}
+
+double cA(const ibpm_solver* s, int l) { return s->dt / (2.0 * s->Re * (s->hc[l] * s->hc[l])); }
+
+// B operator (coupling.jl:150-177) on ncol columns: x_c = E vort2flux(Ainv1 rot(E^T z_c))[1]
+int apply_B(ibpm_solver* s, int ncol, const double* const* z, double* const* x) {
+  const size_t P = s->plane();
+  const int L = s->L;
+  for (int c0 = 0; c0 < ncol; c0 += 64) {
+    const int nc = std::min(64, ncol - c0);
+    ScatterArgs sa;
+    memset(&sa, 0, sizeof sa);
+    sa.g = s->g;
+    sa.npanel = s->n;
+    sa.nrhs = nc;
+    sa.coords = s->coords;
+    sa.w = s->w;
+    sa.perm = s->perm;
+    sa.xstart = s->xstart;
+    for (int c = 0; c < nc; ++c) {
+      sa.f[c] = z[c0 + c];
+      sa.out[c] = s->Sc + (size_t)(c0 + c) * P;
+    }
+    HIPCHK(launch_scatter_rot(sa, s->st));
+  }
+  std::vector<SolveJob> jobs(ncol);
+  for (int c = 0; c < ncol; ++c) {
+    SolveJob& j = jobs[c];
+    j.b = s->Sc + (size_t)c * P;
+    j.ring = nullptr;
+    j.diag = s->dAinv;
+    j.x = s->Gtmp + (size_t)c * L * P;
+    j.epi = EPI_STORE;
+    j.base = nullptr;
+    j.a = 1.0;
+    j.c = cA(s, 0);
+  }
+  int rc = solve(s, jobs.data(), ncol);
+  if (rc) return rc;
+  std::vector<double*> Gc(ncol), pc(ncol);
+  for (int c = 0; c < ncol; ++c) {
+    Gc[c] = s->Gtmp + (size_t)c * L * P;
+    pc[c] = s->psitmp + (size_t)c * L * P;
+  }
+  rc = vort2flux(s, ncol, Gc.data(), pc.data(), nullptr, nullptr);
+  if (rc) return rc;
+  for (int c0 = 0; c0 < ncol; c0 += 64) {
+    const int nc = std::min(64, ncol - c0);
+    GatherArgs ga;
+    memset(&ga, 0, sizeof ga);
+    ga.g = s->g;
+    ga.npanel = s->n;
+    ga.nrhs = nc;
+    ga.coords = s->coords;
+    ga.w = s->w;
+    for (int c = 0; c < nc; ++c) {
+      ga.psi1[c] = pc[c0 + c];
+      ga.psi2[c] = L > 1 ? pc[c0 + c] + P : nullptr;
+      ga.out[c] = x[c0 + c];
+    }
+    ga.q0u = ga.q0v = 0.0;
+    ga.ub = nullptr;
+    ga.h = s->h;
+    HIPCHK(launch_gather(ga, s->st));
+  }
+  return IBPM_OK;
+}
+
+int alloc_bscratch(ibpm_solver* s, int ncol) {
+  const size_t P = s->plane();
+  if (s->Gtmp) (void)hipFree(s->Gtmp);
+  if (s->psitmp) (void)hipFree(s->psitmp);
+  if (s->Sc) (void)hipFree(s->Sc);
+  s->Gtmp = s->psitmp = s->Sc = nullptr;
+  s->bcols = ncol;
+  if (ncol == 0) return IBPM_OK;
+  HIPCHK(dalloc(&s->Gtmp, (size_t)ncol * s->L * P));
+  HIPCHK(dalloc(&s->psitmp, (size_t)ncol * s->L * P));
+  HIPCHK(dalloc(&s->Sc, (size_t)ncol * P));
+  // coarse levels of Gtmp stay zero outside the centre region (Gamma_tmp .= 0)
+  HIPCHK(hipMemsetAsync(s->Gtmp, 0, (size_t)ncol * s->L * P * sizeof(double), s->st));
+  return IBPM_OK;
+}
+
+// LU with partial pivoting and inverse (LAPACK getrf/getri in inv(), coupling.jl:190).
+bool host_inverse(int n, std::vector<double>& A, std::vector<double>& Ainv) {
+  std::vector<int> piv(n);
+  auto a = [&](int i, int j) -> double& { return A[i + (size_t)j * n]; };
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double mx = fabs(a(k, k));
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(a(i, k)) > mx) { mx = fabs(a(i, k)); p = i; }
+    if (mx == 0.0) return false;
+    piv[k] = p;
+    if (p != k)
+      for (int j = 0; j < n; ++j) std::swap(a(k, j), a(p, j));
+    const double inv = 1.0 / a(k, k);
+    for (int i = k + 1; i < n; ++i) a(i, k) *= inv;
+    for (int j = k + 1; j < n; ++j) {
+      const double akj = a(k, j);
+      if (akj != 0.0) {
+        double* cj = &A[(size_t)j * n];
+        const double* ck = &A[(size_t)k * n];
+        for (int i = k + 1; i < n; ++i) cj[i] -= ck[i] * akj;
+      }
+    }
+  }
+  Ainv.assign((size_t)n * n, 0.0);
+  for (int col = 0; col < n; ++col) {
+    double* x = &Ainv[(size_t)col * n];
+    x[col] = 1.0;
+    for (int k = 0; k < n; ++k)
+      if (piv[k] != k) std::swap(x[k], x[piv[k]]);
+    for (int j = 0; j < n; ++j) {  // forward, unit lower
+      const double xj = x[j];
+      if (xj != 0.0)
+        for (int i = j + 1; i < n; ++i) x[i] -= a(i, j) * xj;
+    }
+    for (int j = n - 1; j >= 0; --j) {  // backward
+      x[j] /= a(j, j);
+      const double xj = x[j];
+      if (xj != 0.0)
+        for (int i = 0; i < j; ++i) x[i] -= a(i, j) * xj;
+    }
+  }
+  return true;
+}
+
+int check_body_error(ibpm_solver* s) {
+  int e = 0;
+  HIPCHK(hipMemcpyAsync(&e, s->err, sizeof(int), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  if (e) {
+    const int p = s->n - e;  // 0-based panel
+    double px = 0, py = 0;
+    (void)hipMemcpy(&px, s->xb + p, sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipMemcpy(&py, s->xb + s->n + p, sizeof(double), hipMemcpyDeviceToHost);
+    int zero = 0;
+    (void)hipMemcpy(s->err, &zero, sizeof(int), hipMemcpyHostToDevice);
+    char msg[256];
+    snprintf(msg, sizeof msg, "Panel index=%d at (%.17g, %.17g) outside allowed region", p + 1, px, py);
+    return fail(IBPM_E_OUTSIDE_REGION, msg);
+  }
+  return IBPM_OK;
+}
+
+BodyDev body_dev(ibpm_solver* s) {
+  BodyDev b;
+  b.npanel = s->n;
+  b.nbody = s->nbody;
+  b.x0 = s->x0;
+  b.y0 = s->y0;
+  b.h = s->h;
+  b.nx = s->g.nx;
+  b.ny = s->g.ny;
+  b.xb = s->xb;
+  b.ub = s->ub;
+  b.xb_ref = s->xb_ref;
+  b.body_of = s->body_of;
+  b.moving = s->moving_d;
+  b.motion = s->motion;
+  b.coords = s->coords;
+  b.w = s->w;
+  b.perm = s->perm;
+  b.xstart = s->xstart;
+  b.err = s->err;
+  return b;
+}
+
+// couple_surface_rigid (coupling.jl:233-251) incl. the CG of moving bodies
+int couple_surface(ibpm_solver* s, ibpm_step_stats* stats) {
+  const size_t P = s->plane();
+  GatherArgs ga;
+  memset(&ga, 0, sizeof ga);
+  ga.g = s->g;
+  ga.npanel = s->n;
+  ga.nrhs = 1;
+  ga.coords = s->coords;
+  ga.w = s->w;
+  ga.psi1[0] = s->psi;
+  ga.psi2[0] = s->L > 1 ? s->psi + P : nullptr;
+  ga.out[0] = s->Ftmp;
+  ga.q0u = s->fs[0] * s->hc[0];
+  ga.q0v = s->fs[1] * s->hc[0];
+  ga.ub = s->ub;
+  ga.h = s->h;
+  HIPCHK(launch_gather(ga, s->st));
+  if (!s->moving) {
+    HIPCHK(launch_gemv(2 * s->n, s->Binv, s->Ftmp, s->Ft, s->st));
+    if (stats) {
+      stats->cg_iters = 0;
+      stats->cg_converged = 1;
+      stats->cg_residual = 0.0;
+    }
+    return IBPM_OK;
+  }
+  // cg! with x = Ft as the initial guess (coupling.jl:194-202, solver.jl:137)
+  CgDev c;
+  c.n = 2 * s->n;
+  c.x = s->Ft;
+  c.r = s->cg_r;
+  c.u = s->cg_u;
+  c.c = s->cg_c;
+  c.b = s->Ftmp;
+  c.scal = s->cg_scal;
+  c.maxiter = s->cg_maxiter;
+  c.reltol = s->cg_reltol;
+  c.abstol = s->cg_abstol;
+  c.tol_mode = s->cg_tol_mode;
+  const double* zc[1] = {s->Ft};
+  double* xc[1] = {s->cg_c};
+  int rc = apply_B(s, 1, zc, xc);
+  if (rc) return rc;
+  HIPCHK(launch_cg_init(c, s->st));
+  HIPCHK(hipMemcpyAsync(s->h_scal, s->cg_scal, 6 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  rc = check_body_error(s);
+  if (rc) return rc;
+  while (s->h_scal[4] == 0.0) {
+    const double* zu[1] = {s->cg_u};
+    rc = apply_B(s, 1, zu, xc);
+    if (rc) return rc;
+    HIPCHK(launch_cg_step(c, s->st));
+    HIPCHK(hipMemcpyAsync(s->h_scal, s->cg_scal, 6 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+  }
+  if (stats) {
+    stats->cg_iters = (int)s->h_scal[3];
+    stats->cg_residual = s->h_scal[0];
+    stats->cg_converged = s->h_scal[0] <= s->h_scal[2] ? 1 : 0;
+  }
+  return IBPM_OK;
+}
+
+int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stats* stats) {
+  const size_t P = s->plane();
+  const int L = s->L;
+  // inc_time_index!, freestream (solver.jl:164-167)
+  s->index += 1;
+  s->fs[0] = fsv ? fsv[0] : 0.0;
+  s->fs[1] = fsv ? fsv[1] : 0.0;
+  s->q0_shadow_valid = false;
+  // _update_body_motion! (bodies.jl:43-51)
+  if (s->moving) {
+    HIPCHK(hipMemcpyAsync(s->motion, motion, 6 * s->nbody * sizeof(double), hipMemcpyHostToDevice, s->st));
+    BodyDev b = body_dev(s);
+    HIPCHK(launch_body_update(b, 1, s->st));
+    HIPCHK(launch_body_sort(b, s->st));
+  }
+  // get_trial_state (fluid.jl:467-506): nonlinear term and rhs of every level, then the
+  // implicit solves of every level in one batch (levels only read the old state)
+  NonlinArgs na;
+  memset(&na, 0, sizeof na);
+  na.g = s->g;
+  na.nlevel = L;
+  na.q = s->q;
+  na.G = s->G;
+  na.nl_new = s->nl1;
+  na.nl_old = s->nl2;
+  na.rhs = s->rhs;
+  for (int l = 0; l < L; ++l) {
+    const double hc = s->hc[l];
+    na.q0u[l] = s->fs[0] * hc;
+    na.q0v[l] = s->fs[1] * hc;
+    na.inv_hc2[l] = 1.0 / (hc * hc);
+    na.cA[l] = cA(s, l);
+    na.fac_bc[l] = 0.25 * s->dt / (s->Re * (hc * hc));
+  }
+  na.dt = s->dt;
+  na.beta1 = 1.5;
+  na.beta2 = -0.5;
+  HIPCHK(launch_nonlin_rhs(na, s->st));
+  std::vector<SolveJob> jobs(L);
+  for (int l = 0; l < L; ++l) {
+    SolveJob& j = jobs[l];
+    j.b = s->rhs + l * P;
+    j.ring = nullptr;
+    j.diag = s->dAinv + l * P;
+    j.x = s->Gs + l * P;
+    j.epi = EPI_STORE;
+    j.base = nullptr;
+    j.a = 1.0;
+    j.c = cA(s, l);
+  }
+  int rc = solve(s, jobs.data(), L);
+  if (rc) return rc;
+  std::swap(s->nl1, s->nl2);  // _cycle!
+  {
+    double* Gc[1] = {s->Gs};
+    double* pc[1] = {s->psi};
+    rc = vort2flux(s, 1, Gc, pc, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  // couple_surface!, then project_circ! (coupling.jl:217-231): Gamma = Gamma_s,
+  // Gamma[:,1] -= Ainv1 rot(E^T Ft)
+  if (s->n > 0) {
+    rc = couple_surface(s, stats);
+    if (rc) return rc;
+    HIPCHK(launch_forces(s->n, s->Ft, s->fb, s->total, s->h, s->dt, s->st));
+    ScatterArgs sa;
+    memset(&sa, 0, sizeof sa);
+    sa.g = s->g;
+    sa.npanel = s->n;
+    sa.nrhs = 1;
+    sa.coords = s->coords;
+    sa.w = s->w;
+    sa.perm = s->perm;
+    sa.xstart = s->xstart;
+    sa.f[0] = s->Ft;
+    sa.out[0] = s->rhs;
+    HIPCHK(launch_scatter_rot(sa, s->st));
+    SolveJob j;
+    j.b = s->rhs;
+    j.ring = nullptr;
+    j.diag = s->dAinv;
+    j.x = s->G;
+    j.epi = EPI_SUB;
+    j.base = s->Gs;
+    j.a = 1.0;
+    j.c = cA(s, 0);
+    rc = solve(s, &j, 1);
+    if (rc) return rc;
+    double* Gc[1] = {s->G};
+    double* pc[1] = {s->psi};
+    double* oc[1] = {s->Gs};
+    double* qc[1] = {s->q};
+    rc = vort2flux(s, 1, Gc, pc, L > 1 ? oc : nullptr, qc);
+    if (rc) return rc;
+  } else {
+    HIPCHK(launch_copy(s->G, s->Gs, (size_t)L * P, s->st));
+    double* Gc[1] = {s->G};
+    double* pc[1] = {s->psi};
+    double* qc[1] = {s->q};
+    rc = vort2flux(s, 1, Gc, pc, nullptr, qc);
+    if (rc) return rc;
+    if (stats) {
+      stats->cg_iters = 0;
+      stats->cg_converged = 1;
+      stats->cg_residual = 0.0;
+    }
+  }
+  return IBPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ibpm_default_options(ibpm_options* o) {
+  if (!o) return;
+  memset(o, 0, sizeof *o);
+  o->precision = IBPM_FP64;
+  o->refine_sweeps = 1;
+  o->cg_maxiter = 5000;
+  o->cg_reltol = 1e-12;
+  o->cg_abstol = 0.0;
+  o->cg_tol_mode = 0;
+  o->device = 0;
+}
+
+const char* ibpm_last_error(void) { return g_err.c_str(); }
+
+int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out) {
+  if (!p || !out) return fail(IBPM_E_ARG, "null argument");
+  *out = nullptr;
+  ibpm_options opt;
+  ibpm_default_options(&opt);
+  if (o) opt = *o;
+  if (p->nx % 4 || p->ny % 4 || p->nx < 8 || p->ny < 8)
+    return fail(IBPM_E_ARG, "nx and ny must be multiples of 4 and at least 8");
+  if (p->nlevel < 1 || p->nlevel > 16) return fail(IBPM_E_ARG, "nlevel must be in 1..16");
+  if (p->npanel < 0 || p->npanel > 8191) return fail(IBPM_E_ARG, "npanel must be in 0..8191");
+  if (p->npanel > 0 && (!p->xb || p->nbody < 1 || !p->body_start || !p->body_count || !p->body_moving))
+    return fail(IBPM_E_ARG, "panels need xb and a body partition");
+  if (opt.precision != IBPM_FP64 && opt.precision != IBPM_MIXED)
+    return fail(IBPM_E_ARG, "unknown precision");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= opt.device)
+    return fail(IBPM_E_NO_DEVICE, "no HIP device");
+  HIPCHK(hipSetDevice(opt.device));
+
+  ibpm_solver* s = new ibpm_solver();
+  s->g = make_dims(p->nx, p->ny);
+  s->L = p->nlevel;
+  s->x0 = p->x0;
+  s->y0 = p->y0;
+  s->h = p->h;
+  s->Re = p->Re;
+  s->dt = p->dt;
+  s->t0 = p->t0;
+  s->n = p->npanel;
+  s->nbody = p->npanel > 0 ? p->nbody : 0;
+  s->precision = opt.precision;
+  s->refine = std::max(0, opt.refine_sweeps);
+  s->cg_maxiter = opt.cg_maxiter > 0 ? opt.cg_maxiter : 5000;
+  s->cg_reltol = opt.cg_reltol > 0 ? opt.cg_reltol : 1e-12;
+  s->cg_abstol = opt.cg_abstol;
+  s->cg_tol_mode = opt.cg_tol_mode;
+  s->device = opt.device;
+  s->hc.resize(s->L);
+  for (int l = 0; l < s->L; ++l) s->hc[l] = pow(2.0, l) * s->h;
+
+  auto bail = [&](int rc) {
+    std::string m = g_err;
+    free_all(s);
+    delete s;
+    g_err = m;
+    return rc;
+  };
+#define CK(expr)                                                                        \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return bail(fail(IBPM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_))); \
+  } while (0)
+
+  CK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+  const size_t P = s->plane(), Q = (size_t)s->g.nq, L = s->L;
+  CK(dalloc(&s->q, Q * L));
+  CK(dalloc(&s->G, P * L));
+  CK(dalloc(&s->psi, P * L));
+  CK(dalloc(&s->nl1, P * L));
+  CK(dalloc(&s->nl2, P * L));
+  CK(dalloc(&s->Gs, P * L));
+  CK(dalloc(&s->rhs, P * L));
+  CK(dalloc(&s->T, P * kMaxPlanes));
+  CK(dalloc(&s->U, P * kMaxPlanes));
+  if (s->precision == IBPM_MIXED) CK(dalloc(&s->R, P * kMaxPlanes));
+  CK(hipMemsetAsync(s->q, 0, Q * L * 8, s->st));
+  CK(hipMemsetAsync(s->G, 0, P * L * 8, s->st));
+  CK(hipMemsetAsync(s->psi, 0, P * L * 8, s->st));
+  CK(hipMemsetAsync(s->nl1, 0, P * L * 8, s->st));
+  CK(hipMemsetAsync(s->nl2, 0, P * L * 8, s->st));
+  CK(hipMemsetAsync(s->Gs, 0, P * L * 8, s->st));
+
+  // spectral diagonals: scale / Lambda (fluid.jl:117-134, :155, :168-173)
+  {
+    const int nx = s->g.nx, ny = s->g.ny, Nx = s->g.Nx, Ny = s->g.Ny;
+    const double scale = 1.0 / (4.0 * nx * ny);
+    std::vector<double> lam(P), d(P);
+    for (int j = 1; j <= Ny; ++j)
+      for (int i = 1; i <= Nx; ++i)
+        lam[(i - 1) + (size_t)(j - 1) * Nx] = -2.0 * (cos(M_PI * i / nx) + cos(M_PI * j / ny) - 2.0);
+    CK(dalloc(&s->dpois, P));
+    for (size_t k = 0; k < P; ++k) d[k] = scale / lam[k];
+    CK(hipMemcpy(s->dpois, d.data(), P * 8, hipMemcpyHostToDevice));
+    CK(dalloc(&s->dAinv, P * L));
+    for (int l = 0; l < s->L; ++l) {
+      const double hc = s->hc[l];
+      for (size_t k = 0; k < P; ++k) d[k] = scale / (1.0 + lam[k] * s->dt / (2.0 * s->Re * (hc * hc)));
+      CK(hipMemcpy(s->dAinv + l * P, d.data(), P * 8, hipMemcpyHostToDevice));
+    }
+  }
+  // sine tables
+  {
+    auto build = [&](int N, void** So, void** Se, void** Bf) -> hipError_t {
+      std::vector<double> o(sine_table_odd_count(N)), e(std::max<size_t>(1, sine_table_even_count(N)));
+      build_sine_tables_f64(N, o.data(), e.data());
+      hipError_t r = dalloc((double**)So, o.size());
+      if (r) return r;
+      r = dalloc((double**)Se, e.size());
+      if (r) return r;
+      r = hipMemcpy(*So, o.data(), o.size() * 8, hipMemcpyHostToDevice);
+      if (r) return r;
+      r = hipMemcpy(*Se, e.data(), e.size() * 8, hipMemcpyHostToDevice);
+      if (r) return r;
+      if (s->precision == IBPM_MIXED) {
+        const size_t no = sine_table_bf16_odd_count(N), ne = sine_table_bf16_even_count(N);
+        std::vector<uint16_t> bf(no + ne);
+        const size_t half_o = no / 2, half_e = ne / 2;
+        build_sine_tables_bf16(N, bf.data(), bf.data() + half_o, bf.data() + no, bf.data() + no + half_e);
+        r = dalloc((uint16_t**)Bf, bf.size());
+        if (r) return r;
+        r = hipMemcpy(*Bf, bf.data(), bf.size() * 2, hipMemcpyHostToDevice);
+      }
+      return r;
+    };
+    CK(build(s->g.Nx, &s->SoX, &s->SeX, &s->BfX));
+    CK(build(s->g.Ny, &s->SoY, &s->SeY, &s->BfY));
+  }
+  // bodies
+  const int n = s->n;
+  CK(dalloc(&s->xb, 2 * (size_t)n));
+  CK(dalloc(&s->ub, 2 * (size_t)n));
+  CK(dalloc(&s->xb_ref, 2 * (size_t)n));
+  CK(dalloc(&s->Ft, 2 * (size_t)n));
+  CK(dalloc(&s->fb, 2 * (size_t)n));
+  CK(dalloc(&s->Ftmp, 2 * (size_t)n));
+  CK(dalloc(&s->total, 2));
+  CK(dalloc(&s->motion, 6 * (size_t)std::max(1, s->nbody)));
+  CK(dalloc(&s->body_of, (size_t)n));
+  CK(dalloc(&s->moving_d, (size_t)std::max(1, s->nbody)));
+  CK(dalloc(&s->coords, 2 * (size_t)n));
+  CK(dalloc(&s->perm, (size_t)n));
+  CK(dalloc(&s->xstart, (size_t)s->g.nx + 3));
+  CK(dalloc(&s->err, 1));
+  CK(dalloc(&s->w, 98 * (size_t)n));
+  CK(dalloc(&s->cg_scal, 8));
+  CK(hipHostMalloc((void**)&s->h_scal, 8 * sizeof(double), hipHostMallocDefault));
+  // allocations hold max(count, 1) elements (dalloc)
+  auto zero_bytes = [](size_t count, size_t elem) { return std::max<size_t>(count, 1) * elem; };
+  CK(hipMemsetAsync(s->Ft, 0, zero_bytes(2 * (size_t)n, 8), s->st));
+  CK(hipMemsetAsync(s->fb, 0, zero_bytes(2 * (size_t)n, 8), s->st));
+  CK(hipMemsetAsync(s->total, 0, 16, s->st));
+  CK(hipMemsetAsync(s->ub, 0, zero_bytes(2 * (size_t)n, 8), s->st));
+  CK(hipMemsetAsync(s->xb, 0, zero_bytes(2 * (size_t)n, 8), s->st));
+  CK(hipMemsetAsync(s->err, 0, sizeof(int), s->st));
+  CK(hipMemsetAsync(s->coords, 0, zero_bytes(2 * (size_t)n, sizeof(int)), s->st));
+  CK(hipMemsetAsync(s->w, 0, zero_bytes(98 * (size_t)n, 8), s->st));
+  s->ds.assign(n, 0.0);
+  if (n > 0) {
+    std::vector<int> bof(n, 0), mv(s->nbody, 0);
+    s->body_start.assign(p->body_start, p->body_start + s->nbody);
+    s->body_count.assign(p->body_count, p->body_count + s->nbody);
+    s->body_moving.assign(p->body_moving, p->body_moving + s->nbody);
+    for (int b = 0; b < s->nbody; ++b) {
+      const int st0 = p->body_start[b], cnt = p->body_count[b];
+      if (st0 < 0 || cnt < 0 || st0 + cnt > n) return bail(fail(IBPM_E_ARG, "body partition out of range"));
+      for (int k = st0; k < st0 + cnt; ++k) bof[k] = b;
+      mv[b] = p->body_moving[b] ? 1 : 0;
+      if (mv[b]) s->moving = 1;
+    }
+    CK(hipMemcpy(s->body_of, bof.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->moving_d, mv.data(), s->nbody * sizeof(int), hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->xb_ref, p->xb, 2 * n * 8, hipMemcpyHostToDevice));
+    // _init!(StaticBody): xb, ds taken now, ub = 0; moving bodies keep zeros until the
+    // first step (bodies.jl:1-12, :33-41)
+    std::vector<double> xb0(2 * n, 0.0);
+    for (int b = 0; b < s->nbody; ++b) {
+      if (mv[b]) continue;
+      const int st0 = p->body_start[b], cnt = p->body_count[b];
+      for (int k = st0; k < st0 + cnt; ++k) {
+        xb0[k] = p->xb[k];
+        xb0[n + k] = p->xb[n + k];
+        s->ds[k] = p->ds ? p->ds[k] : 0.0;
+      }
+    }
+    CK(hipMemcpy(s->xb, xb0.data(), 2 * n * 8, hipMemcpyHostToDevice));
+    BodyDev bd = body_dev(s);
+    CK(launch_body_update(bd, 0, s->st));  // static panels: weights now
+    // moving panels are skipped by only_moving=0? no: static branch reads xb (zeros for
+    // moving bodies) -- recompute moving ones at the first step; clear their error.
+    CK(hipStreamSynchronize(s->st));
+    if (!s->moving) {
+      int rc = check_body_error(s);
+      if (rc) return bail(rc);
+    } else {
+      int e = 0;
+      CK(hipMemcpy(&e, s->err, sizeof(int), hipMemcpyDeviceToHost));
+      if (e) {
+        // an error from a static panel is real; one from a moving panel at zero is not
+        int rc = check_body_error(s);
+        const int pbad = n - e;
+        if (!mv[bof[pbad]]) return bail(rc);
+        g_err.clear();
+      }
+    }
+    CK(launch_body_sort(bd, s->st));
+  }
+  // B^-1 (static) or CG buffers (moving): coupling.jl:179-202
+  if (n > 0) {
+    if (!s->moving) {
+      const int m = 2 * n;
+      const int ncol = std::min(m, 32);
+      int rc = alloc_bscratch(s, ncol);
+      if (rc) return bail(rc);
+      double *Bm = nullptr, *E = nullptr;
+      CK(dalloc(&Bm, (size_t)m * m));
+      CK(dalloc(&E, (size_t)m * ncol));
+      std::vector<double> eh((size_t)m * ncol);
+      for (int c0 = 0; c0 < m; c0 += ncol) {
+        const int nc = std::min(ncol, m - c0);
+        std::fill(eh.begin(), eh.end(), 0.0);
+        for (int c = 0; c < nc; ++c) eh[(size_t)c * m + c0 + c] = 1.0;
+        CK(hipMemcpyAsync(E, eh.data(), eh.size() * 8, hipMemcpyHostToDevice, s->st));
+        std::vector<const double*> z(nc);
+        std::vector<double*> x(nc);
+        for (int c = 0; c < nc; ++c) {
+          z[c] = E + (size_t)c * m;
+          x[c] = Bm + (size_t)(c0 + c) * m;
+        }
+        rc = apply_B(s, nc, z.data(), x.data());
+        if (rc) { (void)hipFree(Bm); (void)hipFree(E); return bail(rc); }
+        CK(hipStreamSynchronize(s->st));
+      }
+      std::vector<double> Bh((size_t)m * m), Bi;
+      CK(hipMemcpy(Bh.data(), Bm, Bh.size() * 8, hipMemcpyDeviceToHost));
+      (void)hipFree(Bm);
+      (void)hipFree(E);
+      if (!host_inverse(m, Bh, Bi)) return bail(fail(IBPM_E_SINGULAR, "B is singular"));
+      CK(dalloc(&s->Binv, (size_t)m * m));
+      CK(hipMemcpy(s->Binv, Bi.data(), Bi.size() * 8, hipMemcpyHostToDevice));
+      rc = alloc_bscratch(s, 0);
+      if (rc) return bail(rc);
+    } else {
+      int rc = alloc_bscratch(s, 1);
+      if (rc) return bail(rc);
+      CK(dalloc(&s->cg_r, 2 * (size_t)n));
+      CK(dalloc(&s->cg_u, 2 * (size_t)n));
+      CK(dalloc(&s->cg_c, 2 * (size_t)n));
+    }
+  }
+  CK(hipStreamSynchronize(s->st));
+#undef CK
+  *out = s;
+  return IBPM_OK;
+}
+
+int ibpm_advance(ibpm_solver* s, int nsteps, const double* freestream, const double* motion,
+                 ibpm_step_stats* stats) {
+  if (!s || nsteps < 0) return fail(IBPM_E_ARG, "bad arguments");
+  if (s->moving && !motion) return fail(IBPM_E_ARG, "moving bodies need motion");
+  HIPCHK(hipSetDevice(s->device));
+  for (int k = 0; k < nsteps; ++k) {
+    ibpm_step_stats* sk = stats ? stats + k : nullptr;
+    int rc = step(s, freestream ? freestream + 2 * k : nullptr,
+                  motion ? motion + (size_t)6 * s->nbody * k : nullptr, sk);
+    if (rc) return rc;
+    if (sk && s->n > 0) HIPCHK(hipMemcpyAsync(sk->force, s->total, 16, hipMemcpyDeviceToHost, s->st));
+    if (sk && s->n == 0) sk->force[0] = sk->force[1] = 0.0;
+  }
+  HIPCHK(hipStreamSynchronize(s->st));
+  return IBPM_OK;
+}
+
+size_t ibpm_field_size(const ibpm_solver* s, int field) {
+  if (!s) return 0;
+  const size_t P = s->plane(), Q = s->g.nq, L = s->L, n = s->n;
+  switch (field) {
+    case IBPM_Q: case IBPM_Q0: return Q * L;
+    case IBPM_GAMMA: case IBPM_PSI: case IBPM_NONLIN1: case IBPM_NONLIN2: return P * L;
+    case IBPM_XB: case IBPM_UB: case IBPM_FTILDE: case IBPM_FB: return 2 * n;
+    case IBPM_TOTAL_FORCE: case IBPM_FREESTREAM: return 2;
+    case IBPM_DS: return n;
+  }
+  return 0;
+}
+
+static double* field_ptr(ibpm_solver* s, int field) {
+  switch (field) {
+    case IBPM_Q: return s->q;
+    case IBPM_GAMMA: return s->G;
+    case IBPM_PSI: return s->psi;
+    case IBPM_NONLIN1: return s->nl1;
+    case IBPM_NONLIN2: return s->nl2;
+    case IBPM_XB: return s->xb;
+    case IBPM_UB: return s->ub;
+    case IBPM_FTILDE: return s->Ft;
+    case IBPM_FB: return s->fb;
+    case IBPM_TOTAL_FORCE: return s->total;
+  }
+  return nullptr;
+}
+
+int ibpm_read(ibpm_solver* s, int field, double* dst, size_t n) {
+  if (!s || !dst) return fail(IBPM_E_ARG, "null argument");
+  const size_t need = ibpm_field_size(s, field);
+  if (need == 0 && field != IBPM_DS && field != IBPM_XB) return fail(IBPM_E_ARG, "unknown field");
+  if (n != need) return fail(IBPM_E_ARG, "size mismatch: expected " + std::to_string(need));
+  if (need == 0) return IBPM_OK;
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipStreamSynchronize(s->st));
+  if (field == IBPM_Q0) {
+    if (s->q0_shadow_valid) {
+      memcpy(dst, s->q0_shadow.data(), need * 8);
+      return IBPM_OK;
+    }
+    // base_flux! of a static grid (fluid.jl:22-32)
+    const size_t Q = s->g.nq, nu = s->g.nu;
+    for (int l = 0; l < s->L; ++l) {
+      for (size_t k = 0; k < nu; ++k) dst[l * Q + k] = s->index > 0 ? s->fs[0] * s->hc[l] : 0.0;
+      for (size_t k = nu; k < Q; ++k) dst[l * Q + k] = s->index > 0 ? s->fs[1] * s->hc[l] : 0.0;
+    }
+    return IBPM_OK;
+  }
+  if (field == IBPM_DS) {
+    memcpy(dst, s->ds.data(), need * 8);
+    return IBPM_OK;
+  }
+  if (field == IBPM_FREESTREAM) {
+    dst[0] = s->fs[0];
+    dst[1] = s->fs[1];
+    return IBPM_OK;
+  }
+  HIPCHK(hipMemcpy(dst, field_ptr(s, field), need * 8, hipMemcpyDeviceToHost));
+  return IBPM_OK;
+}
+
+int ibpm_write(ibpm_solver* s, int field, const double* src, size_t n) {
+  if (!s || !src) return fail(IBPM_E_ARG, "null argument");
+  const size_t need = ibpm_field_size(s, field);
+  if (need == 0 && field != IBPM_DS && field != IBPM_XB) return fail(IBPM_E_ARG, "unknown field");
+  if (n != need) return fail(IBPM_E_ARG, "size mismatch: expected " + std::to_string(need));
+  if (need == 0) return IBPM_OK;
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipStreamSynchronize(s->st));
+  switch (field) {
+    case IBPM_Q0:
+      s->q0_shadow.assign(src, src + need);
+      s->q0_shadow_valid = true;
+      return IBPM_OK;
+    case IBPM_DS:
+      s->ds.assign(src, src + need);
+      return IBPM_OK;
+    case IBPM_FREESTREAM:
+      s->fs[0] = src[0];
+      s->fs[1] = src[1];
+      return IBPM_OK;
+    case IBPM_TOTAL_FORCE:
+      return fail(IBPM_E_ARG, "total force is derived from fb");
+  }
+  HIPCHK(hipMemcpy(field_ptr(s, field), src, need * 8, hipMemcpyHostToDevice));
+  return IBPM_OK;
+}
+
+int ibpm_get_index(const ibpm_solver* s) { return s ? s->index : 0; }
+
+int ibpm_set_index(ibpm_solver* s, int index) {
+  if (!s) return fail(IBPM_E_ARG, "null solver");
+  s->index = index;
+  return IBPM_OK;
+}
+
+double ibpm_get_time(const ibpm_solver* s) {
+  // set_time_index!: t = t0 + dt (index - 1) (problem.jl:703-708)
+  return s ? s->t0 + s->dt * (double)(s->index - 1) : 0.0;
+}
+
+int ibpm_synchronize(ibpm_solver* s) {
+  if (!s) return fail(IBPM_E_ARG, "null solver");
+  HIPCHK(hipStreamSynchronize(s->st));
+  return IBPM_OK;
+}
+
+void ibpm_destroy(ibpm_solver* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->device);
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  free_all(s);
+  delete s;
+}
+
+}  // extern "C"

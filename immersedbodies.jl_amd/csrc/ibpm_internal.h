@@ -1,0 +1,206 @@
+// Internal declarations shared by the HIP translation units of libibpm.
+//
+// Layout of every field in HBM is the reference's (ImmersedBodies.jl, problem.jl:363-389):
+// column-major float64, one plane per multi-domain level.
+//   Gamma / psi plane : Nx x Ny, Nx = nx-1, Ny = ny-1, x fastest
+//   q plane           : u ((nx+1) x ny) followed by v (nx x (ny+1))
+//   bc vector         : left (ny+1), right (ny+1), bottom (nx+1), top (nx+1)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace ibpm {
+
+// Geometry of one multi-domain grid (same for all its levels).
+struct GridDims {
+  int nx, ny;   // cells per level
+  int Nx, Ny;   // interior vertices per direction
+  int nu;       // (nx+1)*ny
+  int nq;       // nu + nx*(ny+1)
+  int nG;       // Nx*Ny
+};
+
+inline GridDims make_dims(int nx, int ny) {
+  GridDims d;
+  d.nx = nx; d.ny = ny; d.Nx = nx - 1; d.Ny = ny - 1;
+  d.nu = (nx + 1) * ny;
+  d.nq = d.nu + nx * (ny + 1);
+  d.nG = d.Nx * d.Ny;
+  return d;
+}
+
+// One plane of one sine-contraction pass.
+//   in  : Nk x Nb, k contiguous (in[k + b*Nk])
+//   out : Nb x Nk, b contiguous (out[b + j*Nb])
+// Optional operands:
+//   diag : multiply the result elementwise (same layout as out)
+//   base : epilogue operand of EPI_SUB / EPI_ADD
+//   ring : coarse-level plane whose ring supplies boundary values added to the input
+//          (apply_bc with fac = 1, first pass of a vort2flux solve only)
+struct PassPlane {
+  const void* in;
+  void* out;
+  const double* diag;
+  const double* base;
+  const double* ring;
+  const void* So;  // odd-row sine table of length-Nk transform
+  const void* Se;  // even-row sine table
+  int Nk, Nb;
+  int nx, ny;      // grid of the ring (for apply_bc index maps)
+  int pad0, pad1;
+};
+
+enum Epilogue : int {
+  EPI_STORE = 0,   // out = val
+  EPI_DIAG = 1,    // out = val * diag
+  EPI_SUB = 2,     // out = base - val
+  EPI_ADD = 3,     // out = base + val
+};
+
+enum Precision : int { PREC_F64 = 0, PREC_MIXED = 1 };
+
+// Up to kMaxPlanes planes of one pass travel by value in the kernel arguments.
+constexpr int kMaxPlanes = 16;
+struct PassBatch {
+  PassPlane p[kMaxPlanes];
+};
+
+// Host launchers (dst_kernels.hip).
+hipError_t launch_dst_pass_f64(const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
+                               bool ring, hipStream_t st);
+hipError_t launch_dst_pass_mixed(const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
+                                 bool ring, bool in_f32, bool out_f32, hipStream_t st);
+
+// Sine tables for a DST-I of length N (N+1 even). Host builds, returns element counts.
+void build_sine_tables_f64(int N, double* So, double* Se);
+// Split bf16 hi/lo tables, laid out for the MFMA B operand (k contiguous per column).
+void build_sine_tables_bf16(int N, uint16_t* So_hi, uint16_t* So_lo, uint16_t* Se_hi,
+                            uint16_t* Se_lo);
+size_t sine_table_odd_count(int N);
+size_t sine_table_even_count(int N);
+size_t sine_table_bf16_odd_count(int N);
+size_t sine_table_bf16_even_count(int N);
+
+// ---- stencil kernels (stencil_kernels.hip) ----
+// Nonlinear term and CNAB right-hand side for every level in one launch
+// (fluid.jl:441-455 and :467-497).
+struct NonlinArgs {
+  GridDims g;
+  int nlevel;
+  const double* q;       // nq x L
+  const double* G;       // nG x L (state circulation)
+  double* nl_new;        // nG x L, written
+  const double* nl_old;  // nG x L
+  double* rhs;           // nG x L, written
+  double q0u[16], q0v[16];  // uniform base flux per level (static grid)
+  double inv_hc2[16];    // 1/hc^2 per level
+  double cA[16];         // dt / (2 Re hc^2): A = I - cA*K
+  double fac_bc[16];     // 0.25*dt/(Re hc^2)
+  double dt, beta1, beta2;
+};
+hipError_t launch_nonlin_rhs(const NonlinArgs& a, hipStream_t st);
+
+// coarsify! (fluid.jl:208-228) from level `fine` into `coarse`; when `outer` is not null
+// the points outside the centre region are copied from it (fused state copy).
+hipError_t launch_coarsify(GridDims g, double* coarse, const double* fine, const double* outer,
+                           hipStream_t st);
+
+// curl! with boundary streamfunction (fluid.jl:230-276) for every level in one launch.
+hipError_t launch_curl_all(GridDims g, int nlevel, double* q, const double* psi, hipStream_t st);
+
+// r = b - (a*I + c*K) x  (5-point K = rot(curl), zero Dirichlet) for nplanes planes.
+struct ResidualArgs {
+  GridDims g;
+  int nplanes;
+  const double* b[16];
+  const double* x[16];
+  void* r[16];
+  double a[16], c[16];
+  const double* ring[16];  // vort2flux boundary values (added to b), may be null
+  const double* xbase[16]; // when set, x stands for xbase - x (solution stored as base - x)
+  int r_f32;
+};
+hipError_t launch_residual(const ResidualArgs& a, hipStream_t st);
+
+// ---- coupling kernels (coupling_kernels.hip) ----
+struct BodyDev {
+  int npanel;
+  int nbody;
+  double x0, y0, h;
+  int nx, ny;
+  double* xb;         // 2n
+  double* ub;         // 2n
+  const double* xb_ref;  // 2n reference positions
+  const int* body_of;    // n
+  const int* moving;     // nbody
+  const double* motion;  // 6 x nbody
+  int* coords;        // 2n (ix, iy), 1-based
+  double* w;          // n x 98
+  int* perm;          // n, panels sorted by ix
+  int* xstart;        // nx + 3
+  int* err;           // first offending panel + 1, 0 if none
+};
+hipError_t launch_body_update(const BodyDev& b, int only_moving, hipStream_t st);
+hipError_t launch_body_sort(const BodyDev& b, hipStream_t st);
+
+// F = E * (curl(psi1) + q0) - ub*h (couple_surface_rigid's right-hand side), and E of an
+// explicit flux plane when q1 is given instead of psi.
+struct GatherArgs {
+  GridDims g;
+  int npanel;
+  int nrhs;
+  const int* coords;
+  const double* w;
+  const double* psi1[64];   // level-1 streamfunction per rhs (curl on the fly)
+  const double* psi2[64];   // level-2 streamfunction for boundary values (null: zero)
+  double* out[64];          // 2n per rhs
+  double q0u, q0v;          // uniform base flux of level 1
+  const double* ub;         // subtract ub*h if not null
+  double h;
+};
+hipError_t launch_gather(const GatherArgs& a, hipStream_t st);
+
+// Gamma1 = rot(E^T f) on the full level-1 plane (coupling.jl:72-85 then fluid.jl:101-111).
+struct ScatterArgs {
+  GridDims g;
+  int npanel;
+  int nrhs;
+  const int* coords;
+  const double* w;
+  const int* perm;
+  const int* xstart;
+  const double* f[64];
+  void* out[64];
+  int out_f32;
+};
+hipError_t launch_scatter_rot(const ScatterArgs& a, hipStream_t st);
+
+// x = M b for a dense column-major m x m matrix.
+hipError_t launch_gemv(int m, const double* M, const double* b, double* x, hipStream_t st);
+// fb = Ft*h/dt and the total force (column sums), FP64.
+hipError_t launch_forces(int n, const double* Ft, double* fb, double* total, double h, double dt,
+                         hipStream_t st);
+
+// Conjugate gradient helpers (single work-group kernels, FP64 reductions).
+struct CgDev {
+  int n;
+  double* x;
+  double* r;
+  double* u;
+  double* c;
+  const double* b;
+  double* scal;  // [0] residual, [1] prev_residual, [2] tol, [3] iters, [4] converged
+  int maxiter;
+  double reltol, abstol;
+  int tol_mode;
+};
+hipError_t launch_cg_init(const CgDev& c, hipStream_t st);
+hipError_t launch_cg_step(const CgDev& c, hipStream_t st);
+
+hipError_t launch_fill(double* p, size_t n, double v, hipStream_t st);
+hipError_t launch_copy(double* dst, const double* src, size_t n, hipStream_t st);
+hipError_t launch_axpy_f32(double* x, const float* y, size_t n, hipStream_t st);
+
+}  // namespace ibpm

@@ -1,0 +1,235 @@
+// Grid-stencil kernels of the IBPM step: the fused nonlinear term + CNAB right-hand side,
+// coarsify!, curl! and the residual used by the mixed-precision refinement.
+// All are HBM-bound sweeps; one thread owns one output value and reads its neighbours
+// through L2 (neighbouring threads share every load).
+#include <hip/hip_runtime.h>
+
+#include "ibpm_device.h"
+#include "ibpm_internal.h"
+
+namespace ibpm {
+
+// ---------------------------------------------------------------------------------
+// Nonlinear term and CNAB right-hand side (fluid.jl:352-455 and :467-497), every level
+// in one launch (blockIdx.z = level). Per interior vertex (I, J):
+//   nonlin = (fqx(I+1,J) - fqx(I+1,J+1)) + (fqy(I+1,J+1) - fqy(I,J+1))) / hc^2
+//   fqx(i,j) = (Qy(i,j+1) Ge(i-1,j) + Qy(i,j) Ge(i-1,j-1)) / 2
+//   fqy(i,j) = -(Qx(i+1,j) Ge(i,j-1) + Qx(i,j) Ge(i-1,j-1)) / 2
+//   Qx, Qy   = face averages of q + q0 (avg_flux!)
+// where Ge is the circulation on the level's vertices with the boundary ring taken from
+// the next coarser level times 0.25 (get_bc!, then bc .*= 0.25 at fluid.jl:480).
+//   rhs = A Gamma + dt (beta1 nonlin_new + beta2 nonlin_old) + rhsbc
+// with A = I - dt/(2 Re hc^2) K applied as the 5-point stencil (K = rot curl with zero
+// Dirichlet values, the operator the DST diagonalises) and rhsbc = apply_bc! of the
+// unscaled ring with fac = 0.25 dt / (Re hc^2).
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
+  const GridDims g = a.g;
+  const int lev = blockIdx.z;
+  const int I = blockIdx.x * 64 + (threadIdx.x & 63) + 1;
+  const int J = blockIdx.y * 4 + (threadIdx.x >> 6) + 1;
+  if (I > g.Nx || J > g.Ny) return;
+  const int nx = g.nx, ny = g.ny;
+  const double* q = a.q + (size_t)lev * g.nq;
+  const double* qx = q;
+  const double* qy = q + g.nu;
+  const double* G = a.G + (size_t)lev * g.nG;
+  const double* Gc = lev + 1 < a.nlevel ? a.G + (size_t)(lev + 1) * g.nG : nullptr;
+  const double q0u = a.q0u[lev], q0v = a.q0v[lev];
+
+#define QX(i, j) qx[((i)-1) + (size_t)((j)-1) * (nx + 1)]
+#define QY(i, j) qy[((i)-1) + (size_t)((j)-1) * nx]
+#define GE(i, j) ext_value(G, Gc, nx, ny, (i), (j), 0.25)
+  auto Qx = [&](int i, int j) { return (((QX(i, j) + QX(i, j - 1)) + q0u) + q0u) / 2; };
+  auto Qy = [&](int i, int j) { return (((QY(i, j) + QY(i - 1, j)) + q0v) + q0v) / 2; };
+
+  const double g00 = GE(I, J);
+  const double gm0 = GE(I - 1, J);
+  const double gp0 = GE(I + 1, J);
+  const double g0m = GE(I, J - 1);
+  const double g0p = GE(I, J + 1);
+  const double qy1 = Qy(I + 1, J), qy2 = Qy(I + 1, J + 1), qy3 = Qy(I + 1, J + 2);
+  const double qxa = Qx(I, J + 1), qxb = Qx(I + 1, J + 1), qxc = Qx(I + 2, J + 1);
+
+  const double fqx_a = (qy2 * g00 + qy1 * g0m) / 2;   // fqx(I+1, J)
+  const double fqx_b = (qy3 * g0p + qy2 * g00) / 2;   // fqx(I+1, J+1)
+  const double fqy_a = -(qxc * gp0 + qxb * g00) / 2;  // fqy(I+1, J+1)
+  const double fqy_b = -(qxb * g00 + qxa * gm0) / 2;  // fqy(I, J+1)
+  const double nl = ((fqx_a - fqx_b) + (fqy_a - fqy_b)) * a.inv_hc2[lev];
+#undef QX
+#undef QY
+#undef GE
+
+  const size_t idx = (size_t)(I - 1) + (size_t)(J - 1) * g.Nx + (size_t)lev * g.nG;
+  a.nl_new[idx] = nl;
+
+  // A Gamma with zero Dirichlet neighbours
+  const double* Gl = G;
+  auto Gz = [&](int i, int j) -> double {
+    return (i >= 1 && i <= g.Nx && j >= 1 && j <= g.Ny) ? Gl[(i - 1) + (size_t)(j - 1) * g.Nx] : 0.0;
+  };
+  const double gc = Gz(I, J);
+  const double Kg = 4.0 * gc - Gz(I - 1, J) - Gz(I + 1, J) - Gz(I, J - 1) - Gz(I, J + 1);
+  double v = gc - a.cA[lev] * Kg;
+  v = v + (a.dt * a.beta1) * nl;
+  v = v + (a.dt * a.beta2) * a.nl_old[idx];
+  if (Gc) {
+    const double fac = a.fac_bc[lev];
+    double rb = 0.0;
+    if (J == 1) rb += fac * ring_value(Gc, nx, ny, 2, I + 1);
+    if (J == g.Ny) rb += fac * ring_value(Gc, nx, ny, 3, I + 1);
+    if (I == 1) rb += fac * ring_value(Gc, nx, ny, 0, J + 1);
+    if (I == g.Nx) rb += fac * ring_value(Gc, nx, ny, 1, J + 1);
+    v = v + rb;
+  }
+  a.rhs[idx] = v;
+}
+
+hipError_t launch_nonlin_rhs(const NonlinArgs& a, hipStream_t st) {
+  dim3 grid((a.g.Nx + 63) / 64, (a.g.Ny + 3) / 4, a.nlevel);
+  hipLaunchKernelGGL(nonlin_rhs_kernel, grid, dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// coarsify! (fluid.jl:208-228): coarse vertex (Ic, Jc) of the centre region
+// [nx/4+1, 3nx/4-1] x [ny/4+1, 3ny/4-1] takes the 1-1/2-1/4 full weighting of fine
+// vertex (2(Ic-nx/4), 2(Jc-ny/4)). With `outer`, the rest of the coarse plane is
+// copied from it (project_circ's state.Gamma .= Gamma_s fused into the final pass).
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void coarsify_kernel(GridDims g, double* __restrict__ coarse,
+                                                       const double* __restrict__ fine,
+                                                       const double* __restrict__ outer) {
+  const int nx = g.nx, ny = g.ny;
+  int Ic, Jc;
+  if (outer) {
+    Ic = blockIdx.x * 64 + (threadIdx.x & 63) + 1;
+    Jc = blockIdx.y * 4 + (threadIdx.x >> 6) + 1;
+  } else {
+    Ic = nx / 4 + 1 + blockIdx.x * 64 + (threadIdx.x & 63);
+    Jc = ny / 4 + 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
+  }
+  if (Ic > g.Nx || Jc > g.Ny) return;
+  const size_t idx = (size_t)(Ic - 1) + (size_t)(Jc - 1) * g.Nx;
+  const bool centre = Ic >= nx / 4 + 1 && Ic <= 3 * nx / 4 - 1 && Jc >= ny / 4 + 1 && Jc <= 3 * ny / 4 - 1;
+  if (!centre) {
+    if (outer) coarse[idx] = outer[idx];
+    return;
+  }
+  const int i = 2 * (Ic - nx / 4), j = 2 * (Jc - ny / 4);
+#define GF(ii, jj) fine[((ii)-1) + (size_t)((jj)-1) * g.Nx]
+  coarse[idx] = GF(i, j) + 0.5 * (GF(i + 1, j) + GF(i, j + 1) + GF(i - 1, j) + GF(i, j - 1)) +
+                0.25 * (GF(i + 1, j + 1) + GF(i + 1, j - 1) + GF(i - 1, j - 1) + GF(i - 1, j + 1));
+#undef GF
+}
+
+hipError_t launch_coarsify(GridDims g, double* coarse, const double* fine, const double* outer,
+                           hipStream_t st) {
+  dim3 grid;
+  if (outer) {
+    grid = dim3((g.Nx + 63) / 64, (g.Ny + 3) / 4, 1);
+  } else {
+    const int ni = g.nx / 2 - 1, nj = g.ny / 2 - 1;
+    grid = dim3((ni + 63) / 64, (nj + 3) / 4, 1);
+  }
+  hipLaunchKernelGGL(coarsify_kernel, grid, dim3(256), 0, st, g, coarse, fine, outer);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// curl! with boundary streamfunction (fluid.jl:230-276) for all levels at once:
+//   u(i,j) = psi_e(i-1, j) - psi_e(i-1, j-1),  v(i,j) = psi_e(i-1, j-1) - psi_e(i, j-1)
+// psi_e = psi on interior vertices, get_bc! ring of the next coarser level on the
+// boundary (zero on the coarsest level, fluid.jl:187-190).
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void curl_all_kernel(GridDims g, int nlevel, double* __restrict__ q,
+                                                       const double* __restrict__ psi) {
+  const int lev = blockIdx.y;
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (size_t)g.nq) return;
+  const int nx = g.nx, ny = g.ny;
+  const double* p = psi + (size_t)lev * g.nG;
+  const double* pc = lev + 1 < nlevel ? psi + (size_t)(lev + 1) * g.nG : nullptr;
+  double v;
+  if (e < (size_t)g.nu) {
+    const int i = (int)(e % (nx + 1)) + 1, j = (int)(e / (nx + 1)) + 1;
+    v = ext_value(p, pc, nx, ny, i - 1, j, 1.0) - ext_value(p, pc, nx, ny, i - 1, j - 1, 1.0);
+  } else {
+    const size_t f = e - g.nu;
+    const int i = (int)(f % nx) + 1, j = (int)(f / nx) + 1;
+    v = ext_value(p, pc, nx, ny, i - 1, j - 1, 1.0) - ext_value(p, pc, nx, ny, i, j - 1, 1.0);
+  }
+  q[(size_t)lev * g.nq + e] = v;
+}
+
+hipError_t launch_curl_all(GridDims g, int nlevel, double* q, const double* psi, hipStream_t st) {
+  dim3 grid((g.nq + 255) / 256, nlevel, 1);
+  hipLaunchKernelGGL(curl_all_kernel, grid, dim3(256), 0, st, g, nlevel, q, psi);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Residual of the Poisson / Helmholtz system in FP64: r = b_e - (a x + c K x), K the
+// 5-point rot curl with zero Dirichlet values, b_e = b plus the apply_bc! ring terms
+// when a ring plane is given. Feeds the mixed-precision refinement sweeps.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void residual_kernel(const ResidualArgs a) {
+  const GridDims g = a.g;
+  const int pl = blockIdx.z;
+  const int I = blockIdx.x * 64 + (threadIdx.x & 63) + 1;
+  const int J = blockIdx.y * 4 + (threadIdx.x >> 6) + 1;
+  if (I > g.Nx || J > g.Ny) return;
+  const double* x = a.x[pl];
+  const double* xb = a.xbase[pl];
+  auto X = [&](int i, int j) -> double {
+    if (!(i >= 1 && i <= g.Nx && j >= 1 && j <= g.Ny)) return 0.0;
+    const size_t k = (i - 1) + (size_t)(j - 1) * g.Nx;
+    return xb ? xb[k] - x[k] : x[k];
+  };
+  const size_t idx = (size_t)(I - 1) + (size_t)(J - 1) * g.Nx;
+  double b = a.b[pl][idx];
+  const double* rc = a.ring[pl];
+  if (rc) {
+    if (J == 1) b += ring_value(rc, g.nx, g.ny, 2, I + 1);
+    if (J == g.Ny) b += ring_value(rc, g.nx, g.ny, 3, I + 1);
+    if (I == 1) b += ring_value(rc, g.nx, g.ny, 0, J + 1);
+    if (I == g.Nx) b += ring_value(rc, g.nx, g.ny, 1, J + 1);
+  }
+  const double xc = X(I, J);
+  const double Kx = 4.0 * xc - X(I - 1, J) - X(I + 1, J) - X(I, J - 1) - X(I, J + 1);
+  const double r = b - (a.a[pl] * xc + a.c[pl] * Kx);
+  if (a.r_f32)
+    ((float*)a.r[pl])[idx] = (float)r;
+  else
+    ((double*)a.r[pl])[idx] = r;
+}
+
+hipError_t launch_residual(const ResidualArgs& a, hipStream_t st) {
+  dim3 grid((a.g.Nx + 63) / 64, (a.g.Ny + 3) / 4, a.nplanes);
+  hipLaunchKernelGGL(residual_kernel, grid, dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// small utilities
+// ---------------------------------------------------------------------------------
+__global__ void fill_kernel(double* p, size_t n, double v) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void copy_kernel(double* __restrict__ d, const double* __restrict__ s, size_t n) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) d[i] = s[i];
+}
+hipError_t launch_fill(double* p, size_t n, double v, hipStream_t st) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_copy(double* d, const double* s, size_t n, hipStream_t st) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, s, n);
+  return hipGetLastError();
+}
+
+}  // namespace ibpm

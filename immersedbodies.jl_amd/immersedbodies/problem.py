@@ -1,0 +1,355 @@
+"""Problem description: the host-side mirror of ImmersedBodies.jl's ``problem.jl``.
+
+The names, argument meanings and errors follow the reference so a user of the Julia
+package finds the same objects (paths relative to the reference repository):
+
+* ``CartesianGrid``, ``MultiDomainGrid``, ``fluid_grid``, ``gridstep``, ``subdomain``,
+  ``coords``, ``GridIndices``                               problem.jl:269-472
+* ``Fluid``, ``default_gridstep``                            problem.jl:534-556
+* ``CNAB``, ``default_scheme``, ``timestep``                 problem.jl:566-596
+* ``Panels``, ``StaticBody``, ``MovingRigidBody``,
+  ``RigidBodyTransform``, ``Bodies``                         problem.jl:1-261
+* ``Problem``, ``State``                                     problem.jl:598-708
+* ``boundary_total_force``                                   problem.jl:853-856
+
+Only the static grid is supported (the reference's ``MovingGrid`` is outside the
+accelerated path).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional, Tuple
+
+import numpy as np
+
+
+def _floor_sig1(x: float) -> float:
+    """floor(x; sigdigits=1) of Julia."""
+    if x == 0:
+        return 0.0
+    e = math.floor(math.log10(abs(x)))
+    p = 10.0 ** e
+    return math.floor(x / p) * p
+
+
+@dataclass(frozen=True)
+class CartesianGrid:
+    """Cartesian grid with lower-left corner ``x0``, cell counts ``n`` and step ``h``
+    (problem.jl:269-281)."""
+
+    x0: Tuple[float, float]
+    n: Tuple[int, int]
+    h: float
+
+    @staticmethod
+    def from_limits(h, xlims, ylims) -> "CartesianGrid":
+        x0 = (float(xlims[0]), float(ylims[0]))
+        n = (int(round((xlims[1] - xlims[0]) / h)), int(round((ylims[1] - ylims[0]) / h)))
+        return CartesianGrid(x0, n, float(h))
+
+
+@dataclass(frozen=True)
+class GridIndices:
+    """Sizes and boundary offsets of the flattened fields (problem.jl:363-389)."""
+
+    nx: int
+    ny: int
+
+    @property
+    def nu(self):
+        return (self.nx + 1) * self.ny
+
+    @property
+    def nv(self):
+        return self.nx * (self.ny + 1)
+
+    @property
+    def nq(self):
+        return self.nu + self.nv
+
+    @property
+    def nG(self):
+        return (self.nx - 1) * (self.ny - 1)
+
+    @property
+    def L(self):
+        return 0
+
+    @property
+    def R(self):
+        return self.ny + 1
+
+    @property
+    def B(self):
+        return 2 * (self.ny + 1)
+
+    @property
+    def T(self):
+        return 2 * (self.ny + 1) + self.nx + 1
+
+
+@dataclass(frozen=True)
+class MultiDomainGrid:
+    """``nlevel`` grids with a common centre, level ``l`` having step ``2^(l-1) h``
+    (problem.jl:398-429)."""
+
+    base: CartesianGrid
+    nlevel: int
+
+    @property
+    def inds(self) -> GridIndices:
+        return GridIndices(*self.base.n)
+
+    @property
+    def h(self):
+        return self.base.h
+
+
+def fluid_grid(*, h=None, xlims, ylims, nlevel):
+    """``fluid_grid(; h, xlims, ylims, nlevel)`` (problem.jl:459-464). Without ``h`` the
+    step is chosen by ``Fluid`` from the Reynolds number."""
+    if h is None:
+        return ("extents", tuple(xlims), tuple(ylims), int(nlevel))
+    return MultiDomainGrid(CartesianGrid.from_limits(h, xlims, ylims), int(nlevel))
+
+
+def gridstep(grid, level: int = 1) -> float:
+    """Grid step on level ``level`` (problem.jl:416-417)."""
+    if isinstance(grid, CartesianGrid):
+        return grid.h
+    if isinstance(grid, Problem):
+        return gridstep(grid.fluid.grid, level)
+    return 2.0 ** (level - 1) * grid.base.h
+
+
+def subdomain(grid: MultiDomainGrid, level: int) -> CartesianGrid:
+    """Level ``level`` as a CartesianGrid (problem.jl:424-429)."""
+    b = grid.base
+    h = 2.0 ** (level - 1) * b.h
+    x0 = tuple(b.x0[i] + b.n[i] * (b.h - h) / 2 for i in range(2))
+    return CartesianGrid(x0, b.n, h)
+
+
+def coords(grid, pts: str = "vertices"):
+    """Coordinates of grid points: ``vertices``, ``u``, ``v`` or ``gamma`` (problem.jl:311-338)."""
+    if isinstance(grid, MultiDomainGrid):
+        return [coords(subdomain(grid, l), pts) for l in range(1, grid.nlevel + 1)]
+    xs = grid.x0[0] + grid.h * np.arange(grid.n[0] + 1)
+    ys = grid.x0[1] + grid.h * np.arange(grid.n[1] + 1)
+    mid = lambda r: r[:-1] + grid.h / 2
+    if pts == "vertices":
+        return xs, ys
+    if pts == "u":
+        return xs, mid(ys)
+    if pts == "v":
+        return mid(xs), ys
+    if pts == "gamma":
+        return xs[1:-1], ys[1:-1]
+    raise ValueError(pts)
+
+
+def default_gridstep(Re: float) -> float:
+    """floor(2/Re; sigdigits=1) (problem.jl:556)."""
+    return _floor_sig1(2.0 / Re)
+
+
+@dataclass
+class StaticGrid:
+    """A grid that is static in the global frame (problem.jl:508-513)."""
+
+
+@dataclass
+class Fluid:
+    """``Fluid(; grid, Re, freestream_vel, grid_motion)`` (problem.jl:534-552)."""
+
+    grid: object
+    Re: float
+    freestream_vel: Callable[[float], Tuple[float, float]] = lambda t: (0.0, 0.0)
+    grid_motion: object = field(default_factory=StaticGrid)
+
+    def __post_init__(self):
+        if isinstance(self.grid, tuple) and self.grid and self.grid[0] == "extents":
+            _, xl, yl, nl = self.grid
+            self.grid = MultiDomainGrid(
+                CartesianGrid.from_limits(default_gridstep(self.Re), xl, yl), nl)
+        if not isinstance(self.grid_motion, StaticGrid):
+            raise NotImplementedError("only StaticGrid is supported by the GPU solver")
+
+
+@dataclass(frozen=True)
+class CNAB:
+    """2-step Crank-Nicolson / Adams-Bashforth scheme (problem.jl:566-577)."""
+
+    dt: float
+    n: int = 2
+
+    def __post_init__(self):
+        if self.n != 2:
+            raise ValueError("only 2-step CNAB is currently supported")
+
+    @property
+    def beta(self):
+        return (1.5, -0.5)
+
+
+def default_scheme(grid, *, Umax, cfl=0.1) -> CNAB:
+    """dt = floor(cfl h / Umax; sigdigits=1) (problem.jl:590-596)."""
+    return CNAB(dt=_floor_sig1(cfl * gridstep(grid) / Umax))
+
+
+def timestep(x) -> float:
+    if isinstance(x, CNAB):
+        return x.dt
+    return x.scheme.dt
+
+
+@dataclass
+class Panels:
+    """Panel positions ``xb`` (n x 2) and lengths ``ds`` (n) (problem.jl:9-12)."""
+
+    xb: np.ndarray
+    ds: np.ndarray
+
+    def __post_init__(self):
+        self.xb = np.asarray(self.xb, dtype=np.float64).reshape(-1, 2)
+        self.ds = np.asarray(self.ds, dtype=np.float64).reshape(-1)
+
+    @property
+    def n(self):
+        return self.xb.shape[0]
+
+
+@dataclass(frozen=True)
+class RigidBodyTransform:
+    """Position, velocity, angle and angular velocity of a rigid body (problem.jl:233-239)."""
+
+    pos: Tuple[float, float]
+    vel: Tuple[float, float]
+    angle: float
+    angular_vel: float
+
+    def as_array(self):
+        return np.array([self.pos[0], self.pos[1], self.vel[0], self.vel[1], self.angle,
+                         self.angular_vel], dtype=np.float64)
+
+
+@dataclass
+class StaticBody:
+    """A body fixed at ``panels`` (problem.jl:220-226)."""
+
+    panels: Panels
+
+    @property
+    def moving(self):
+        return False
+
+
+@dataclass
+class MovingRigidBody:
+    """A rigid body at ``motion(t)`` applied to ``panels`` (problem.jl:241-261)."""
+
+    motion: Callable[[float], RigidBodyTransform]
+    panels: Panels
+
+    @property
+    def moving(self):
+        return True
+
+
+class Bodies:
+    """``Bodies(; name=body, ...)`` (problem.jl:136-178): panels of all bodies are
+    concatenated in the order given."""
+
+    def __init__(self, **bodies):
+        self.byname: Dict[str, object] = dict(bodies)
+        self.sections: List[Tuple[str, object, int, int]] = []
+        off = 0
+        for name, body in bodies.items():
+            n = body.panels.n
+            self.sections.append((name, body, off, n))
+            off += n
+        self.npanel = off
+        self.static = all(not b.moving for b in bodies.values())
+
+    def panel_range(self, name):
+        for nm, _, off, n in self.sections:
+            if nm == name:
+                return range(off, off + n)
+        raise KeyError(name)
+
+    def __getitem__(self, name):
+        return self.byname[name]
+
+    def xb(self):
+        if not self.sections:
+            return np.zeros((0, 2))
+        return np.concatenate([b.panels.xb for _, b, _, _ in self.sections], axis=0)
+
+    def ds(self):
+        if not self.sections:
+            return np.zeros(0)
+        return np.concatenate([b.panels.ds for _, b, _, _ in self.sections])
+
+    def motion_array(self, t: float) -> np.ndarray:
+        """6 values per body at time t (zeros for static bodies)."""
+        out = np.zeros(6 * max(len(self.sections), 1))
+        for k, (_, b, _, _) in enumerate(self.sections):
+            if b.moving:
+                out[6 * k:6 * k + 6] = b.motion(t).as_array()
+        return out
+
+
+def n_panels(x) -> int:
+    if isinstance(x, Bodies):
+        return x.npanel
+    return x.panels.n
+
+
+@dataclass
+class Problem:
+    """``Problem(; fluid, bodies, scheme)`` (problem.jl:598-603)."""
+
+    fluid: Fluid
+    bodies: Bodies
+    scheme: CNAB
+
+
+class State:
+    """Host copy of the solver state (problem.jl:626-665). Arrays use the reference's
+    column-major layout, stored here as (nlevel, n) row blocks: ``q[l]`` is the flux
+    column of level l+1."""
+
+    def __init__(self, prob: Problem, t0: float = 0.0):
+        g = prob.fluid.grid
+        ind = g.inds
+        L = g.nlevel
+        n = prob.bodies.npanel
+        self.prob = prob
+        self.t0 = float(t0)
+        self.q = np.zeros((L, ind.nq))
+        self.q0 = np.zeros((L, ind.nq))
+        self.Gamma = np.zeros((L, ind.nG))
+        self.psi = np.zeros((L, ind.nG))
+        self.nonlin = [np.zeros((L, ind.nG)), np.zeros((L, ind.nG))]
+        self.xb = np.zeros((n, 2))
+        self.ds = np.zeros(n)
+        self.ub = np.zeros((n, 2))
+        self.Ftilde = np.zeros(2 * n)
+        self.fb = np.zeros((n, 2))
+        self.freestream_vel = np.zeros(2)
+        self.set_time_index(0)
+
+    def set_time_index(self, index: int):
+        """t = t0 + dt (index - 1) (problem.jl:703-708)."""
+        self.index = int(index)
+        self.t = self.t0 + timestep(self.prob) * (self.index - 1)
+
+    def inc_time_index(self):
+        self.set_time_index(self.index + 1)
+
+
+def boundary_total_force(state: State, inds=None):
+    """Total [x, y] force on the panels (problem.jl:853-856)."""
+    fb = state.fb if inds is None else state.fb[inds]
+    return fb.sum(axis=0)

@@ -1,0 +1,229 @@
+"""Solver protocol of ImmersedBodies.jl (solver.jl) over the HIP library.
+
+``create_solver``, ``advance``, ``getstate``, ``solve`` and ``solve_`` (``solve!``) keep
+the reference's names, argument meanings and time-index convention (solver.jl:1-76,
+problem.jl:703-708). ``HipCnabSolver`` stands where ``CnabSolver`` (solver.jl:78-190)
+stands: its state lives in HBM and is copied to the host ``State`` only when
+``getstate`` (or ``sync_state``) is called; forces are returned every step.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import _lib
+from .problem import Problem, State, timestep
+
+
+class AbstractSolver:
+    """Solves a Problem (solver.jl:6): implement ``advance`` and ``getstate``."""
+
+    def advance(self):
+        raise NotImplementedError
+
+    def getstate(self) -> State:
+        raise NotImplementedError
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class HipCnabSolver(AbstractSolver):
+    """CNAB IBPM solver running on the GPU through libibpm.
+
+    Keyword arguments:
+      precision: "fp64" (sine contractions on FP64 MFMA) or "mixed" (3xBF16 split products
+                 with FP32 accumulation and FP64 residual refinement)
+      refine_sweeps: refinement sweeps per solve in mixed precision
+      cg: dict of maxiter / reltol / abstol for moving bodies (solver.jl:137 defaults
+          5000 / 1e-12 / 0); cg_tol_mode 0 is IterativeSolvers' reltol*||r0||,
+          1 is reltol*||b||
+      device: HIP device ordinal
+    """
+
+    def __init__(self, state: State, *, precision="fp64", refine_sweeps=1, cg=None,
+                 cg_tol_mode=0, device=0):
+        L = _lib.load()
+        self.state = state
+        prob = state.prob
+        g = prob.fluid.grid
+        bodies = prob.bodies
+        n = bodies.npanel
+        self._n = n
+        self._nbody = len(bodies.sections)
+        self._moving = not bodies.static
+        xb = np.ascontiguousarray(bodies.xb().T.reshape(-1))  # column-major n x 2
+        ds = np.ascontiguousarray(bodies.ds())
+        bs = np.array([s[2] for s in bodies.sections] or [0], dtype=np.int32)
+        bc = np.array([s[3] for s in bodies.sections] or [0], dtype=np.int32)
+        bm = np.array([int(s[1].moving) for s in bodies.sections] or [0], dtype=np.int32)
+        self._keep = (xb, ds, bs, bc, bm)
+        p = _lib.Problem(g.base.n[0], g.base.n[1], g.nlevel, g.base.x0[0], g.base.x0[1],
+                         g.base.h, prob.fluid.Re, timestep(prob), state.t0, n,
+                         _dp(xb) if n else None, _dp(ds) if n else None, self._nbody,
+                         bs.ctypes.data_as(C.POINTER(C.c_int)),
+                         bc.ctypes.data_as(C.POINTER(C.c_int)),
+                         bm.ctypes.data_as(C.POINTER(C.c_int)))
+        o = _lib.Options()
+        L.ibpm_default_options(C.byref(o))
+        o.precision = {"fp64": _lib.IBPM_FP64, "mixed": _lib.IBPM_MIXED}[precision]
+        o.refine_sweeps = int(refine_sweeps)
+        cg = dict(cg or {})
+        o.cg_maxiter = int(cg.get("maxiter", 5000))
+        o.cg_reltol = float(cg.get("reltol", 1e-12))
+        o.cg_abstol = float(cg.get("abstol", 0.0))
+        o.cg_tol_mode = int(cg_tol_mode)
+        o.device = int(device)
+        h = C.c_void_p()
+        _lib.check(L.ibpm_create(C.byref(p), C.byref(o), C.byref(h)))
+        self._h = h
+        self.precision = precision
+        self.last_stats = None
+        # a State created before the solver may carry a non-zero index (resume)
+        if state.index != 0:
+            _lib.check(L.ibpm_set_index(self._h, state.index))
+            self.write_state(state)
+
+    # -- lifetime --
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.load().ibpm_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- stepping --
+    def advance(self, nsteps: int = 1):
+        """advance! (solver.jl:155-190), ``nsteps`` times. Freestream and body motion are
+        evaluated on the host at each new time and passed to the library."""
+        st = self.state
+        prob = st.prob
+        dt = timestep(prob)
+        fs = np.zeros(2 * nsteps)
+        mot = np.zeros(6 * max(self._nbody, 1) * nsteps) if self._moving else None
+        for k in range(nsteps):
+            t = st.t0 + dt * (st.index + k)  # time after inc_time_index!
+            fs[2 * k:2 * k + 2] = prob.fluid.freestream_vel(t)
+            if self._moving:
+                mot[6 * self._nbody * k:6 * self._nbody * (k + 1)] = \
+                    prob.bodies.motion_array(t)[:6 * self._nbody]
+        stats = (_lib.StepStats * nsteps)()
+        rc = _lib.load().ibpm_advance(self._h, nsteps, _dp(fs),
+                                      _dp(mot) if mot is not None else None, stats)
+        _lib.check(rc)
+        st.set_time_index(st.index + nsteps)
+        st.freestream_vel = fs[-2:].copy() if nsteps else st.freestream_vel
+        self.last_stats = stats
+        return stats
+
+    def forces(self, stats=None):
+        stats = self.last_stats if stats is None else stats
+        return np.array([[s.force[0], s.force[1]] for s in stats])
+
+    def cg_iters(self, stats=None):
+        stats = self.last_stats if stats is None else stats
+        return np.array([s.cg_iters for s in stats])
+
+    # -- state exchange --
+    def read(self, name: str) -> np.ndarray:
+        L = _lib.load()
+        f = _lib.FIELDS[name]
+        n = L.ibpm_field_size(self._h, f)
+        out = np.zeros(n)
+        _lib.check(L.ibpm_read(self._h, f, _dp(out), n))
+        return out
+
+    def write(self, name: str, values):
+        L = _lib.load()
+        f = _lib.FIELDS[name]
+        a = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(-1))
+        _lib.check(L.ibpm_write(self._h, f, _dp(a), a.size))
+
+    def sync_state(self) -> State:
+        st = self.state
+        g = st.prob.fluid.grid
+        Lv = g.nlevel
+        n = self._n
+        st.q = self.read("q").reshape(Lv, -1)
+        st.q0 = self.read("q0").reshape(Lv, -1)
+        st.Gamma = self.read("Gamma").reshape(Lv, -1)
+        st.psi = self.read("psi").reshape(Lv, -1)
+        st.nonlin = [self.read("nonlin1").reshape(Lv, -1), self.read("nonlin2").reshape(Lv, -1)]
+        st.xb = self.read("xb").reshape(2, n).T.copy() if n else np.zeros((0, 2))
+        st.ub = self.read("ub").reshape(2, n).T.copy() if n else np.zeros((0, 2))
+        st.ds = self.read("ds") if n else np.zeros(0)
+        st.Ftilde = self.read("Ftilde")
+        st.fb = self.read("fb").reshape(2, n).T.copy() if n else np.zeros((0, 2))
+        st.freestream_vel = self.read("freestream")
+        return st
+
+    def write_state(self, st: State):
+        """Push a host State (e.g. from load_state) to the device."""
+        n = self._n
+        self.write("q", st.q)
+        self.write("q0", st.q0)
+        self.write("Gamma", st.Gamma)
+        self.write("psi", st.psi)
+        self.write("nonlin1", st.nonlin[0])
+        self.write("nonlin2", st.nonlin[1])
+        if n:
+            self.write("xb", st.xb.T)
+            self.write("ub", st.ub.T)
+            self.write("ds", st.ds)
+            self.write("Ftilde", st.Ftilde)
+            self.write("fb", st.fb.T)
+        self.write("freestream", st.freestream_vel)
+        _lib.check(_lib.load().ibpm_set_index(self._h, st.index))
+
+    def getstate(self) -> State:
+        return self.sync_state()
+
+
+def create_solver(x, **kw) -> HipCnabSolver:
+    """create_solver(prob_or_state; kw...) (solver.jl:28,152)."""
+    state = State(x) if isinstance(x, Problem) else x
+    return HipCnabSolver(state, **kw)
+
+
+def advance(solver: AbstractSolver, nsteps: int = 1):
+    """advance!(solver)."""
+    return solver.advance(nsteps)
+
+
+def getstate(solver: AbstractSolver) -> State:
+    return solver.getstate()
+
+
+def solve_(f: Optional[Callable], solver: HipCnabSolver, *, t: float, log: bool = False,
+           chunk: int = 1):
+    """solve!(f, solver; t) (solver.jl:37-62): n = round((t - state.t)/dt) steps, calling
+    f(state, i, n) after each step. With chunk > 1 the steps run in groups of ``chunk``
+    between callbacks (forces of every step are still returned)."""
+    st = solver.state
+    dt = timestep(st.prob)
+    n = int(round((t - st.t) / dt))
+    forces = []
+    i = 0
+    while i < n:
+        k = min(chunk, n - i)
+        stats = solver.advance(k)
+        forces.append(solver.forces(stats))
+        i += k
+        if f is not None:
+            f(st, i, n)
+        if log and (i - 1) % 10 == 0:
+            print("i = ", st.index, "  t = ", st.t)
+    return np.concatenate(forces) if forces else np.zeros((0, 2))
+
+
+def solve(f: Optional[Callable], prob: Problem, *, t: float, solver_kw=None, **kw):
+    """solve(f, prob; t, solver_kw) (solver.jl:71-76)."""
+    solver = create_solver(State(prob), **(solver_kw or {}))
+    return solve_(f, solver, t=t, **kw)

@@ -1,0 +1,127 @@
+/*
+ * libibpm: MI355X (gfx950) implementation of the IBPM time step of ImmersedBodies.jl.
+ *
+ * C ABI: plain pointers and sizes, column-major float64 arrays in the reference's layout
+ * (problem.jl:363-389, :626-665). The library owns device memory; callers own every host
+ * buffer they pass. A solver is not thread-safe (the reference's CnabSolver is not
+ * either: its closures share temporaries, solver.jl:343-358).
+ *
+ * Every entry point returns IBPM_OK (0) or an error code; ibpm_last_error() gives the
+ * message of the last failure on the calling thread.
+ *
+ * Reference interface replaced (paths relative to the reference repository):
+ *   ibpm_create       <- create_solver(state; fftw, cg) / CnabSolver(state), solver.jl:84-152
+ *                        (includes _init_body_motion!, bodies.jl:33-41, and Binv_operator,
+ *                        coupling.jl:179-202)
+ *   ibpm_advance      <- advance!(solver::CnabSolver), solver.jl:155-190, nsteps times;
+ *                        freestream_vel(t) and body motion(t) are evaluated by the caller
+ *                        (problem.jl:546, :259) and passed in per step
+ *   ibpm_read/_write  <- the State fields read by getstate/savers and written by
+ *                        load_state! (problem.jl:626-665, io.jl:42-71)
+ *   ibpm_get_index/_set_index <- state.index / set_time_index! (problem.jl:703-708)
+ *   ibpm_last_error   <- the exceptions of coupling.jl:40 ("outside allowed region")
+ *   ibpm_destroy      <- garbage collection of the solver
+ */
+#ifndef IBPM_H
+#define IBPM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ibpm_solver ibpm_solver;
+
+enum ibpm_status {
+  IBPM_OK = 0,
+  IBPM_E_ARG = 1,            /* invalid argument or size */
+  IBPM_E_OUTSIDE_REGION = 2, /* a panel too close to the level-1 edge (coupling.jl:39-41) */
+  IBPM_E_HIP = 3,            /* HIP runtime failure */
+  IBPM_E_SINGULAR = 4,       /* dense B is singular */
+  IBPM_E_NO_DEVICE = 5       /* no gfx950 device */
+};
+
+enum ibpm_precision {
+  IBPM_FP64 = 0,  /* sine contractions on FP64 MFMA */
+  IBPM_MIXED = 1  /* 3xBF16 split products, FP32 accumulation, FP64 refinement */
+};
+
+enum ibpm_field {
+  IBPM_Q = 0,           /* nq x nlevel   velocity flux            state.q  */
+  IBPM_Q0 = 1,          /* nq x nlevel   base flux                state.q0 */
+  IBPM_GAMMA = 2,       /* nG x nlevel   circulation              state.Γ  */
+  IBPM_PSI = 3,         /* nG x nlevel   streamfunction           state.ψ  */
+  IBPM_NONLIN1 = 4,     /* nG x nlevel   state.nonlin[1]                   */
+  IBPM_NONLIN2 = 5,     /* nG x nlevel   state.nonlin[2]                   */
+  IBPM_XB = 6,          /* npanel x 2    panel positions          state.panels.xb */
+  IBPM_UB = 7,          /* npanel x 2    panel velocities         state.panels.ub */
+  IBPM_FTILDE = 8,      /* 2 npanel      state.F̃b                          */
+  IBPM_FB = 9,          /* npanel x 2    state.fb                          */
+  IBPM_TOTAL_FORCE = 10,/* 2             boundary_total_force(state)       */
+  IBPM_DS = 11,         /* npanel        state.panels.ds                   */
+  IBPM_FREESTREAM = 12  /* 2             state.freestream_vel              */
+};
+
+/* Problem description (Problem + State(prob; t0) of problem.jl). */
+typedef struct ibpm_problem {
+  int nx, ny, nlevel;       /* cells per level (multiples of 4), levels      */
+  double x0, y0, h;         /* lower-left corner and step of level 1          */
+  double Re, dt;            /* Reynolds number, CNAB time step                */
+  double t0;                /* time at index 1                                */
+  int npanel;
+  const double* xb;         /* npanel x 2 column-major reference positions    */
+  const double* ds;         /* npanel panel lengths (stored, not used)        */
+  int nbody;
+  const int* body_start;    /* first panel of each body                       */
+  const int* body_count;
+  const int* body_moving;   /* 0: StaticBody, 1: MovingRigidBody              */
+} ibpm_problem;
+
+typedef struct ibpm_options {
+  int precision;      /* enum ibpm_precision                                     */
+  int refine_sweeps;  /* IBPM_MIXED: FP64 residual refinement sweeps per solve   */
+  int cg_maxiter;     /* moving bodies: default 5000 (solver.jl:137)             */
+  double cg_reltol;   /* default 1e-12 (solver.jl:137)                           */
+  double cg_abstol;   /* default 0                                               */
+  int cg_tol_mode;    /* 0: reltol*||r0|| (IterativeSolvers), 1: reltol*||b||    */
+  int device;         /* HIP device ordinal                                      */
+} ibpm_options;
+
+typedef struct ibpm_step_stats {
+  int cg_iters;        /* CG iterations of the step (0 for static bodies)        */
+  int cg_converged;    /* 1 when the CG stopping test was met                    */
+  double cg_residual;  /* final CG residual norm                                 */
+  double force[2];     /* boundary_total_force after the step                    */
+} ibpm_step_stats;
+
+/* Defaults of the reference: FP64, cg maxiter 5000, reltol 1e-12. */
+void ibpm_default_options(ibpm_options* o);
+
+int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out);
+
+/* Advance nsteps steps. freestream: 2 x nsteps (ux, uy at each new time). motion:
+ * 6 x nbody x nsteps (pos x, pos y, vel x, vel y, angle, angular_vel of every body at
+ * each new time; entries of static bodies are ignored), or NULL when no body moves.
+ * stats: nsteps entries or NULL. */
+int ibpm_advance(ibpm_solver* s, int nsteps, const double* freestream, const double* motion,
+                 ibpm_step_stats* stats);
+
+int ibpm_read(ibpm_solver* s, int field, double* dst, size_t n);
+int ibpm_write(ibpm_solver* s, int field, const double* src, size_t n);
+size_t ibpm_field_size(const ibpm_solver* s, int field);
+
+int ibpm_get_index(const ibpm_solver* s);
+int ibpm_set_index(ibpm_solver* s, int index);
+double ibpm_get_time(const ibpm_solver* s);
+
+/* Wait for queued work of the solver's stream. */
+int ibpm_synchronize(ibpm_solver* s);
+
+const char* ibpm_last_error(void);
+void ibpm_destroy(ibpm_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IBPM_H */

@@ -1,0 +1,112 @@
+"""Workload builders shared by the tests, the smoke check and the benchmark.
+
+Each builder returns an ``immersedbodies.Problem``; ``oracle_for`` builds the FP64 CPU
+oracle of the same problem so both run on identical inputs.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+import immersedbodies as ib
+from immersedbodies import curves
+
+
+def cylinder(nx=60, ny=60, nlevel=3, h=0.05, x0=(-1.0, -1.5), Re=100.0, dt=0.004, r=0.5,
+             npts=None, center=(0.0, 0.0), U=(1.0, 0.0)):
+    """Stationary circular cylinder in a uniform stream (the reference's solver tests
+    and the 200^2 / 1024^2 cylinder workloads)."""
+    grid = ib.MultiDomainGrid(ib.CartesianGrid(tuple(x0), (nx, ny), h), nlevel)
+    fluid = ib.Fluid(grid=grid, Re=Re, freestream_vel=lambda t: U)
+    c = curves.Circle(r=r, center=center)
+    pan = curves.partition(c, int(npts)) if npts else curves.panels(grid, c)
+    bodies = ib.Bodies(cyl=ib.StaticBody(pan))
+    return ib.Problem(fluid=fluid, bodies=bodies, scheme=ib.CNAB(dt=dt))
+
+
+def cylinder_200():
+    """200 x 200 cells on [-1,3] x [-2,2], h = 0.02, 3 levels, Re 100, dt 0.004, 157 points."""
+    return cylinder(nx=200, ny=200, nlevel=3, h=0.02, x0=(-1.0, -2.0), Re=100.0, dt=0.004,
+                    npts=157)
+
+
+def cylinder_1024(nlevel=5, dt=0.001):
+    """1024 x 1024 cells on [-2,6] x [-4,4], h = 1/128, 5 levels, Re 200, 402 points."""
+    return cylinder(nx=1024, ny=1024, nlevel=nlevel, h=1.0 / 128, x0=(-2.0, -4.0), Re=200.0,
+                    dt=dt, npts=402)
+
+
+def two_lines(nx=60, ny=60, nlevel=3, h=0.05, x0=(-1.0, -1.5), Re=100.0, dt=0.004):
+    """Two static line segments (test/solver.jl:21-29)."""
+    grid = ib.MultiDomainGrid(ib.CartesianGrid(tuple(x0), (nx, ny), h), nlevel)
+    fluid = ib.Fluid(grid=grid, Re=Re, freestream_vel=lambda t: (1.0, 0.0))
+    l1 = curves.panels(grid, curves.LineSegment((-0.2, 0.4), (0.2, 0.1)))
+    l2 = curves.panels(grid, curves.LineSegment((-0.2, -0.1), (0.2, -0.4)))
+    bodies = ib.Bodies(line1=ib.StaticBody(l1), line2=ib.StaticBody(l2))
+    return ib.Problem(fluid=fluid, bodies=bodies, scheme=ib.CNAB(dt=dt))
+
+
+def moving_cylinder(nx=60, ny=60, nlevel=3, h=0.05, x0=(-1.0, -1.5), Re=100.0, dt=0.004):
+    """Moving cylinder of test/solver.jl:30-41."""
+    grid = ib.MultiDomainGrid(ib.CartesianGrid(tuple(x0), (nx, ny), h), nlevel)
+    fluid = ib.Fluid(grid=grid, Re=Re, freestream_vel=lambda t: (1.0, 0.0))
+    pan = curves.panels(grid, curves.Circle(r=0.5, center=(0.0, 0.0)))
+    motion = lambda t: ib.RigidBodyTransform(pos=(0.3 * math.cos(t), 0.1 * math.sin(t)),
+                                             vel=(-0.3 * math.sin(t), 0.1 * math.cos(t)),
+                                             angle=0.1 * t, angular_vel=0.1)
+    bodies = ib.Bodies(cyl=ib.MovingRigidBody(motion, pan))
+    return ib.Problem(fluid=fluid, bodies=bodies, scheme=ib.CNAB(dt=dt))
+
+
+def pitching_plate(nx=64, ny=64, nlevel=3, h=1.0 / 32, x0=(-1.0, -1.0), Re=200.0, dt=0.002,
+                   npts=40, k=0.5):
+    """Flat plate of length 1 pitching about midchord, alpha = 20 + 10 sin(2 pi f t) deg,
+    k = pi f (the learned-warm-start workload, reduced size)."""
+    grid = ib.MultiDomainGrid(ib.CartesianGrid(tuple(x0), (nx, ny), h), nlevel)
+    fluid = ib.Fluid(grid=grid, Re=Re, freestream_vel=lambda t: (1.0, 0.0))
+    pan = curves.partition(curves.LineSegment((-0.5, 0.0), (0.5, 0.0)), npts)
+    f = k / math.pi
+    deg = math.pi / 180
+
+    def motion(t):
+        a = 20 * deg + 10 * deg * math.sin(2 * math.pi * f * t)
+        adot = 10 * deg * 2 * math.pi * f * math.cos(2 * math.pi * f * t)
+        return ib.RigidBodyTransform(pos=(0.0, 0.0), vel=(0.0, 0.0), angle=-a, angular_vel=-adot)
+
+    bodies = ib.Bodies(plate=ib.MovingRigidBody(motion, pan))
+    return ib.Problem(fluid=fluid, bodies=bodies, scheme=ib.CNAB(dt=dt))
+
+
+def no_body(nx=32, ny=32, nlevel=2, h=0.1, x0=(-1.6, -1.6), Re=50.0, dt=0.01):
+    grid = ib.MultiDomainGrid(ib.CartesianGrid(tuple(x0), (nx, ny), h), nlevel)
+    fluid = ib.Fluid(grid=grid, Re=Re, freestream_vel=lambda t: (1.0, 0.3))
+    return ib.Problem(fluid=fluid, bodies=ib.Bodies(), scheme=ib.CNAB(dt=dt))
+
+
+def oracle_for(prob, **kw):
+    """FP64 CPU oracle of ``prob`` (test infrastructure)."""
+    import oracle as O
+
+    g = prob.fluid.grid
+    grid = O.Grid(g.base.n[0], g.base.n[1], g.nlevel, g.base.x0[0], g.base.x0[1], g.base.h)
+    b = prob.bodies
+    bodies = [(off, n, body.moving) for (_, body, off, n) in b.sections]
+    return O.OracleSolver(grid, prob.fluid.Re, prob.scheme.dt, b.xb(), b.ds(), bodies, **kw)
+
+
+def oracle_advance(orc, prob, nsteps=1):
+    """Advance the oracle with the freestream and motion the host evaluates."""
+    forces = []
+    for _ in range(nsteps):
+        t = orc.next_time()
+        orc.advance(prob.fluid.freestream_vel(t), prob.bodies.motion_array(t))
+        forces.append(orc.field("total_force"))
+    return np.array(forces)
+
+
+def relerr(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    d = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (d if d > 0 else 1.0))

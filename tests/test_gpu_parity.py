@@ -1,0 +1,88 @@
+"""GPU parity: libibpm (through its C ABI) against the FP64 CPU oracle on identical inputs.
+
+Tolerances: the FP64 path must match fields within 1e-10 and forces within 1e-9
+(relative, 2-norm over the array); the mixed path (3xBF16 products, FP32 accumulation,
+FP64 refinement) must match forces within 1e-6.
+"""
+import numpy as np
+import pytest
+
+import cases
+import immersedbodies as ib
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 1e-10
+FORCE_TOL = 1e-9
+
+
+def _compare_state(solver, orc, field_tol=FIELD_TOL, force_tol=FORCE_TOL, fields=None):
+    fields = fields or ["Gamma", "psi", "q", "nonlin1", "nonlin2", "Ftilde", "fb", "xb", "ub"]
+    errs = {}
+    for name in fields:
+        a, b = solver.read(name), orc.field(name)
+        errs[name] = cases.relerr(a, b)
+    for name, e in errs.items():
+        tol = force_tol if name in ("Ftilde", "fb") else field_tol
+        assert e <= tol, f"{name}: rel err {e:.3e} > {tol:.1e} ({errs})"
+    return errs
+
+
+@pytest.mark.parametrize("builder", [cases.cylinder, cases.two_lines])
+def test_static_fp64_matches_oracle(builder):
+    prob = builder()
+    print("case", builder.__name__, flush=True)
+    solver = ib.create_solver(prob)
+    orc = cases.oracle_for(prob)
+    nsteps = 6
+    stats = solver.advance(nsteps)
+    f_ref = cases.oracle_advance(orc, prob, nsteps)
+    f_gpu = solver.forces(stats)
+    assert cases.relerr(f_gpu, f_ref) <= FORCE_TOL, (f_gpu, f_ref)
+    _compare_state(solver, orc)
+    assert solver.state.index == nsteps
+    assert abs(solver.state.t - 5 * prob.scheme.dt) < 1e-15
+
+
+def test_no_body_uniform_flow_stays_irrotational():
+    prob = cases.no_body()
+    solver = ib.create_solver(prob)
+    orc = cases.oracle_for(prob)
+    solver.advance(5)
+    cases.oracle_advance(orc, prob, 5)
+    G = solver.read("Gamma")
+    assert np.abs(G).max() < 1e-12
+    _compare_state(solver, orc, fields=["q", "Gamma", "psi"])
+
+
+def test_moving_cylinder_cg_matches_oracle():
+    prob = cases.moving_cylinder()
+    print("case moving_cylinder", flush=True)
+    solver = ib.create_solver(prob)
+    orc = cases.oracle_for(prob)
+    nsteps = 4
+    stats = solver.advance(nsteps)
+    f_ref = cases.oracle_advance(orc, prob, nsteps)
+    assert cases.relerr(solver.forces(stats), f_ref) <= 1e-8
+    iters = solver.cg_iters(stats)
+    assert (iters > 0).all()
+    _compare_state(solver, orc, field_tol=1e-9, force_tol=1e-8)
+
+
+def test_mixed_precision_forces():
+    prob = cases.cylinder()
+    solver = ib.create_solver(prob, precision="mixed", refine_sweeps=2)
+    orc = cases.oracle_for(prob)
+    nsteps = 6
+    stats = solver.advance(nsteps)
+    f_ref = cases.oracle_advance(orc, prob, nsteps)
+    e = cases.relerr(solver.forces(stats), f_ref)
+    print("mixed force rel err", e)
+    assert e <= 1e-6
+
+
+def test_outside_region_error():
+    # a cylinder touching the grid edge (test/solver.jl:66-79)
+    prob = cases.cylinder(nx=16, ny=20, nlevel=3, h=0.1, x0=(-1.0, -1.0), Re=20.0, dt=0.01)
+    with pytest.raises(Exception, match="outside allowed region"):
+        ib.create_solver(prob)
