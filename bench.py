@@ -1,0 +1,159 @@
+#!/usr/bin/env python3
+"""Benchmark of the IBPM time step on one MI355X.
+
+Workload: stationary circular cylinder, 1024 x 1024 cells on [-2,6] x [-4,4] (h = 1/128),
+5 multi-domain levels, Re 200, 402 panels, dt 0.001 (default_scheme with CFL 0.2),
+impulsive start, synthetic (no data files). One "step" is one advance! of the solver.
+
+Prints one JSON line: steps/s of the mixed-precision path (headline), the FP64 path's
+steps/s, the force error of the mixed path against the FP64 path, CG iterations per step,
+the roofline of the dominant kernel (the sine-contraction pass) measured with HIP events
+on the solver's stream, and the FP64 CPU oracle timed on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "immersedbodies.jl_amd"), os.path.join(ROOT, "tests")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+PEAK = {"mixed": 2500.0, "fp64": 78.6}  # dense MFMA TFLOP/s: bf16, fp64
+HBM_PEAK = 8000.0  # GB/s
+
+
+def time_solver(prob, precision, refine, steps, warmup):
+    import immersedbodies as ib
+
+    t0 = time.perf_counter()
+    solver = ib.create_solver(prob, precision=precision, refine_sweeps=refine)
+    setup_s = time.perf_counter() - t0
+    wstats = solver.advance(warmup) if warmup else []
+    fw = solver.forces(wstats) if warmup else np.zeros((0, 2))
+    from immersedbodies import _lib
+
+    _lib.check(_lib.load().ibpm_synchronize(solver._h))
+    t0 = time.perf_counter()
+    stats = solver.advance(steps)  # returns after the stream is synchronised
+    dt = time.perf_counter() - t0
+    forces = np.concatenate([fw, solver.forces(stats)])
+    iters = solver.cg_iters(stats)
+    return solver, dt, setup_s, forces, iters
+
+
+def kernel_roofline(solver, kernel, reps, peak_tflops):
+    import ctypes as C
+
+    from immersedbodies import _lib
+
+    ms, fl, by = C.c_double(), C.c_double(), C.c_double()
+    _lib.check(_lib.load().ibpm_time_kernel(solver._h, kernel, reps, C.byref(ms), C.byref(fl),
+                                            C.byref(by)))
+    sec = ms.value * 1e-3
+    if kernel == 0:
+        ach = fl.value / sec / 1e12
+        return {"bound": "mfma", "achieved": round(ach, 3), "peak": peak_tflops,
+                "unit": "TFLOP/s", "frac": round(ach / peak_tflops, 4), "traffic": None,
+                "avg_launch_ms": round(ms.value, 5), "flops_per_launch": fl.value,
+                "min_bytes_per_launch": by.value,
+                "hbm_GBps_at_min_bytes": round(by.value / sec / 1e9, 1)}
+    ach = by.value / sec / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK, 4), "traffic": None,
+            "avg_launch_ms": round(ms.value, 5), "bytes_per_launch": by.value}
+
+
+def cpu_baseline(prob, steps):
+    """FP64 CPU oracle on the host cores for a bounded number of steps of the same
+    workload (B^-1 replaced by the identity: the dense assembly is setup, the per-step
+    cost is the same GEMV)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import cases
+    import oracle as O
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    O.set_threads(threads)
+    orc = cases.oracle_for(prob, timing_only=True)
+    cases.oracle_advance(orc, prob, 1)
+    t0 = time.perf_counter()
+    cases.oracle_advance(orc, prob, steps)
+    dt = time.perf_counter() - t0
+    orc.close()
+    return {"value": round(steps / dt, 4), "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the same 1024x1024 5-level cylinder step after 1 "
+                      "warm-up step, FP64 C/OpenMP restatement (FFT-based DST-I), B^-1 "
+                      "replaced by the identity (same per-step GEMV cost)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--refine", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--levels", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(every problem fits one MI355X); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import cases
+
+    prob = cases.cylinder_1024(nlevel=args.levels)
+    steps, warm = args.steps, args.warmup
+    head, dt_h, setup_h, f_h, it_h = time_solver(prob, args.precision, args.refine, steps, warm)
+    roof = kernel_roofline(head, 0, 50, PEAK[args.precision])
+    roof_nl = kernel_roofline(head, 1, 50, None)
+    head.close()
+    out_extra = {}
+    if args.precision == "mixed":
+        fsteps = args.fp64_steps or steps
+        ref, dt_r, setup_r, f_r, _ = time_solver(prob, "fp64", 0, fsteps, warm)
+        ref.close()
+        n = min(len(f_h), len(f_r))
+        rel = np.linalg.norm(f_h[:n] - f_r[:n], axis=1) / np.linalg.norm(f_r[:n], axis=1)
+        out_extra = {"fp64_steps_per_s": round(fsteps / dt_r, 3),
+                     "speedup_vs_fp64": round((steps / dt_h) / (fsteps / dt_r), 3),
+                     "force_rel_err_vs_fp64_max": float(rel.max()),
+                     "force_rel_err_vs_fp64_final": float(rel[-1]),
+                     "fp64_setup_s": round(setup_r, 3)}
+    res = {
+        "metric": "IBPM time steps/s at 1024^2 (mixed/batched vs FP64); CG iters/step; force rel err",
+        "value": round(steps / dt_h, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": round(1e3 * dt_h / steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3 split / fp32 accumulate + fp64" if args.precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"stationary cylinder, 1024x1024 cells, {args.levels} levels, Re 200, "
+                               "402 panels, dt 0.001, impulsive start",
+                   "precision": args.precision, "refine_sweeps": args.refine},
+        "cg_iters_per_step": float(np.mean(it_h)) if len(it_h) else 0.0,
+        "setup_s": round(setup_h, 3),
+        "final_force": [float(f_h[-1, 0]), float(f_h[-1, 1])],
+        "roofline": roof,
+        "roofline_nonlinear_kernel": roof_nl,
+    }
+    res.update(out_extra)
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(prob, args.cpu_steps)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

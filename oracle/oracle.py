@@ -40,7 +40,7 @@ class _Config(C.Structure):
                 ("npanel", C.c_int), ("xb", _dp), ("ds", _dp), ("nbody", C.c_int),
                 ("body_start", _ip), ("body_count", _ip), ("body_moving", _ip),
                 ("cg_maxiter", C.c_int), ("cg_reltol", C.c_double), ("cg_abstol", C.c_double),
-                ("cg_tol_mode", C.c_int)]
+                ("cg_tol_mode", C.c_int), ("timing_only", C.c_int)]
 
 
 def lib():
@@ -273,7 +273,8 @@ class OracleSolver:
     """
 
     def __init__(self, grid: Grid, Re, dt, xb, ds=None, bodies=None, t0=0.0,
-                 cg_maxiter=5000, cg_reltol=1e-12, cg_abstol=0.0, cg_tol_mode=0):
+                 cg_maxiter=5000, cg_reltol=1e-12, cg_abstol=0.0, cg_tol_mode=0,
+                 timing_only=False):
         xb = np.asarray(xb, dtype=np.float64).reshape(-1, 2)
         n = xb.shape[0]
         self.grid, self.n = grid, n
@@ -287,7 +288,7 @@ class OracleSolver:
         self.nbody = len(bodies)
         cfg = _Config(grid.c, Re, dt, t0, n, _p(self._xb), _p(self._ds), self.nbody,
                       _pi(self._bs), _pi(self._bc), _pi(self._bm), cg_maxiter, cg_reltol,
-                      cg_abstol, cg_tol_mode)
+                      cg_abstol, cg_tol_mode, int(timing_only))
         err = C.create_string_buffer(512)
         self._h = lib().ibo_create(C.byref(cfg), err, 512)
         if not self._h:
