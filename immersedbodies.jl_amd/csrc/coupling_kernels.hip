@@ -8,6 +8,12 @@
 #include "ibpm_device.h"
 #include "ibpm_internal.h"
 
+// No fused multiply-add in this file: the flux-point coordinates x0 + h*k feed the branch
+// tests of delta_h (r/h <= 1, 2, 3). Panels that sit exactly on grid lines put r/h on a
+// branch point, and the reference rounds the product and the sum separately; fusing them
+// can move r/h across the branch and change a weight in its 10th digit.
+#pragma clang fp contract(off)
+
 namespace ibpm {
 
 // delta_h of coupling.jl:91-148: the Yang3 smoothed 3-point kernel times h. The

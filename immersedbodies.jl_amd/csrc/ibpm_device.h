@@ -9,7 +9,7 @@ namespace ibpm {
 // get_bc! entry (fluid.jl:278-306): value on `side` (0 left, 1 right, 2 bottom, 3 top)
 // at 1-based position k of the boundary vector, interpolated from the coarse plane r.
 // ---------------------------------------------------------------------------------
-__device__ __forceinline__ double ring_value(const double* __restrict__ r, int nx, int ny,
+__host__ __device__ __forceinline__ double ring_value(const double* __restrict__ r, int nx, int ny,
                                              int side, int k) {
   const int Nx = nx - 1;
   if (side < 2) {
@@ -29,7 +29,7 @@ __device__ __forceinline__ double ring_value(const double* __restrict__ r, int n
 // coarser plane rc (zero when rc is null), times scale. A corner vertex belongs to two
 // sides of the boundary vector; both entries interpolate the same coarse vertex, so
 // the side chosen here does not change the value.
-__device__ __forceinline__ double ext_value(const double* __restrict__ p,
+__host__ __device__ __forceinline__ double ext_value(const double* __restrict__ p,
                                             const double* __restrict__ rc, int nx, int ny,
                                             int i, int j, double scale) {
   if (i > 0 && i < nx && j > 0 && j < ny) return p[(i - 1) + (size_t)(j - 1) * (nx - 1)];
