@@ -101,6 +101,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--levels", type=int, default=5)
+    ap.add_argument("--no-ref", action="store_true", help="skip the FP64 comparison run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -116,7 +117,7 @@ def main():
     roof_nl = kernel_roofline(head, 1, 50, None)
     head.close()
     out_extra = {}
-    if args.precision == "mixed":
+    if args.precision == "mixed" and not args.no_ref:
         fsteps = args.fp64_steps or steps
         ref, dt_r, setup_r, f_r, _ = time_solver(prob, "fp64", 0, fsteps, warm)
         ref.close()

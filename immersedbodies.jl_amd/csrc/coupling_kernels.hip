@@ -263,14 +263,23 @@ hipError_t launch_scatter_rot(const ScatterArgs& a, hipStream_t st) {
 // Dense B^-1 product (coupling.jl:191) and update_vars! (problem.jl:676-685):
 //   Ft = Binv * b;  fb = Ft * h / dt;  total = column sums of fb (FP64).
 // ---------------------------------------------------------------------------------
+// 16 rows per work-group; the 16 column groups of a row reduce through LDS.
 __global__ __launch_bounds__(256) void gemv_kernel(int m, const double* __restrict__ M,
                                                    const double* __restrict__ b,
                                                    double* __restrict__ x) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= m) return;
+  __shared__ double part[16][17];
+  const int r = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const int i = blockIdx.x * 16 + r;
   double acc = 0.0;
-  for (int j = 0; j < m; ++j) acc += M[i + (size_t)j * m] * b[j];
-  x[i] = acc;
+  if (i < m)
+    for (int j = g; j < m; j += 16) acc += M[i + (size_t)j * m] * b[j];
+  part[g][r] = acc;
+  __syncthreads();
+  if (g == 0 && i < m) {
+    double s = 0.0;
+    for (int k = 0; k < 16; ++k) s += part[k][r];
+    x[i] = s;
+  }
 }
 
 __global__ __launch_bounds__(1024) void forces_kernel(int n, const double* __restrict__ Ft,
@@ -304,7 +313,7 @@ __global__ __launch_bounds__(1024) void forces_kernel(int n, const double* __res
 
 hipError_t launch_gemv(int m, const double* M, const double* b, double* x, hipStream_t st) {
   if (m == 0) return hipSuccess;
-  hipLaunchKernelGGL(gemv_kernel, dim3((m + 255) / 256), dim3(256), 0, st, m, M, b, x);
+  hipLaunchKernelGGL(gemv_kernel, dim3((m + 15) / 16), dim3(256), 0, st, m, M, b, x);
   return hipGetLastError();
 }
 

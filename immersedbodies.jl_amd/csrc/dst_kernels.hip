@@ -21,7 +21,6 @@
 #include <math.h>
 #include <string.h>
 
-#include "ibpm_device.h"
 #include "ibpm_internal.h"
 
 namespace ibpm {
@@ -35,48 +34,65 @@ __device__ __forceinline__ double ld(const T* p, size_t i) {
   return (double)p[i];
 }
 
-// Input element (k0, b) of the first pass, with the apply_bc! edge terms (fac = 1,
-// fluid.jl:308-338) when a ring plane is given.
-template <typename InT, bool RING>
+// Input element (k0, b) of a pass (k0 along the contracted index).
+template <typename InT>
 __device__ __forceinline__ double load_in(const PassPlane& d, int k0, int b) {
-  const InT* in = (const InT*)d.in;
-  double v = ld(in, (size_t)k0 + (size_t)b * d.Nk);
-  if (RING) {
-    const int I = k0 + 1, J = b + 1;
-    const int Nx = d.Nk, Ny = d.Nb;
-    if (J == 1) v += ring_value(d.ring, d.nx, d.ny, 2, I + 1);
-    if (J == Ny) v += ring_value(d.ring, d.nx, d.ny, 3, I + 1);
-    if (I == 1) v += ring_value(d.ring, d.nx, d.ny, 0, J + 1);
-    if (I == Nx) v += ring_value(d.ring, d.nx, d.ny, 1, J + 1);
-  }
-  return v;
+  return ld((const InT*)d.in, (size_t)k0 + (size_t)b * d.Nk);
 }
 
+// Output element (b, jc) with the pass epilogue. EPI_DIAG applies the spectral diagonal
+// D = dscale / (da + lambda dnum / dden), lambda = -2((cos_b + cos_j) - 2): the
+// reference's Delta_inv (da = 0, dnum = dden = 1) and Ainv (da = 1, dnum = dt,
+// dden = 2 Re hc^2) eigenvalues (fluid.jl:117-134, :168-173), in its evaluation order.
 template <typename OutT, int EPI>
-__device__ __forceinline__ void store_out(const PassPlane& d, size_t idx, double val) {
+__device__ __forceinline__ void store_out(const PassPlane& d, int b, int jc, double val) {
+  const size_t idx = (size_t)b + (size_t)jc * d.Nb;
   OutT* out = (OutT*)d.out;
-  if (EPI == EPI_DIAG) val *= d.diag[idx];
+  if (EPI == EPI_DIAG) {
+    const double lam = -2.0 * ((d.cosb[b] + d.cosj[jc]) - 2.0);
+    val *= d.dscale / (d.da + lam * d.dnum / d.dden);
+  }
   if (EPI == EPI_SUB) val = d.base[idx] - val;
   if (EPI == EPI_ADD) val = d.base[idx] + val;
   out[idx] = (OutT)val;
 }
 
+// XCD-aware tile order. Work-groups are dealt round-robin to the 8 XCDs (each with its own
+// L2); logical tiles are numbered with the m-tile fastest, so the tiles that share a
+// plane's rows of input (same plane, same b-tile) are consecutive, and work-group w takes
+// logical tile (w % 8) * ceil(T/8) + w / 8: each XCD walks a contiguous range and reads
+// its rows of input from its own L2.
+__device__ __forceinline__ bool xcd_tile(const PassBatch& b, int& z, int& bt, int& mt) {
+  const int T = b.nplanes * b.nbt * b.nmt;
+  const int per = (T + 7) / 8;
+  const int w = blockIdx.x;
+  const int t = (w & 7) * per + (w >> 3);
+  if (t >= T) return false;
+  mt = t % b.nmt;
+  bt = (t / b.nmt) % b.nbt;
+  z = t / (b.nmt * b.nbt);
+  return true;
+}
+
 // ---------------------------------------------------------------------------------
-// FP64 pass. Work-group tile: 64 values of b x 32 odd rows + 32 even rows; K chunk 16.
-// 4 waves: wave w computes parity (w>>1) on b-half (w&1), a 32 x 32 block as 2 x 2
-// MFMA 16x16x4 tiles.
+// FP64 pass. Work-group tile: 64 values of b x (64 odd + 64 even) output rows, K chunk
+// 16. 4 waves: wave w computes parity (w>>1) on b-half (w&1), a 32 x 64 block as 2 x 4
+// MFMA 16x16x4 tiles. Input chunk and sine operands are staged through LDS; the global
+// loads of the next chunk are issued before the matrix-core work of the current one.
 // ---------------------------------------------------------------------------------
 constexpr int F64_TB = 64;
-constexpr int F64_TM = 32;
+constexpr int F64_TM = 64;
 constexpr int F64_KC = 16;
 
-template <typename InT, typename OutT, int EPI, bool RING>
+template <typename InT, typename OutT, int EPI>
 __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch) {
-  const PassPlane& d = batch.p[blockIdx.z];
+  int z, bt, mt;
+  if (!xcd_tile(batch, z, bt, mt)) return;
+  const PassPlane& d = batch.p[z];
   const int Nk = d.Nk, Nb = d.Nb;
   const int H = (Nk + 1) / 2;
-  const int b0 = blockIdx.x * F64_TB;
-  const int m0 = blockIdx.y * F64_TM;
+  const int b0 = bt * F64_TB;
+  const int m0 = mt * F64_TM;
   if (b0 >= Nb || m0 >= H) return;
 
   __shared__ double Ps[F64_KC][F64_TB + 1];
@@ -92,96 +108,100 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
   const double* SoT = (const double*)d.So;
   const double* SeT = (const double*)d.Se;
 
-  d4 acc[2][2];
+  d4 acc[2][4];
+#pragma unroll
   for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
 
+  double ra[4], rm[4], rso[4], rse[4];
+  auto fetch = [&](int kc) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = tid + 256 * r;
+      const int k1 = kc + (e & 15) + 1;  // 1-based k; 16 lanes read 16 consecutive k
+      const int b = b0 + (e >> 4);
+      double va = 0.0, vm = 0.0;
+      if (k1 <= H && b < Nb) {
+        va = load_in<InT>(d, k1 - 1, b);
+        if (k1 < H) vm = load_in<InT>(d, Nk - k1, b);
+      }
+      ra[r] = va;
+      rm[r] = vm;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = tid + 256 * r;
+      const int k1 = kc + (e >> 6) + 1;
+      const int m = m0 + (e & 63);
+      rso[r] = (k1 <= H && m < H) ? SoT[(size_t)(k1 - 1) * H + m] : 0.0;
+      rse[r] = (k1 <= H - 1 && m < H - 1) ? SeT[(size_t)(k1 - 1) * (H - 1) + m] : 0.0;
+    }
+  };
+  fetch(0);
+  const int col = lane & 15;
+  const int kq = lane >> 4;
   for (int kc = 0; kc < H; kc += F64_KC) {
-    // fold-and-transpose of the input chunk into LDS
+    // fold into LDS: P = x_k + x_{M-k}, Q = x_k - x_{M-k}; k = H has no partner (Q = 0)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int e = tid + 256 * r;
       const int kk = e & 15;
-      const int bb = e >> 4;
-      const int k1 = kc + kk + 1;  // 1-based k
-      const int b = b0 + bb;
-      double p = 0.0, qv = 0.0;
-      if (k1 <= H && b < Nb) {
-        const double va = load_in<InT, RING>(d, k1 - 1, b);
-        if (k1 == H) {
-          p = va;
-        } else {
-          const double vm = load_in<InT, RING>(d, Nk - k1, b);
-          p = va + vm;
-          qv = va - vm;
-        }
-      }
-      Ps[kk][bb] = p;
-      Qs[kk][bb] = qv;
+      const bool mid = kc + kk + 1 == H;
+      Ps[kk][e >> 4] = ra[r] + rm[r];
+      Qs[kk][e >> 4] = mid ? 0.0 : ra[r] - rm[r];
     }
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
+    for (int r = 0; r < 4; ++r) {
       const int e = tid + 256 * r;
-      const int mm = e & 31;
-      const int kk = e >> 5;
-      const int k1 = kc + kk + 1;
-      const int m = m0 + mm;
-      So_s[kk][mm] = (k1 <= H && m < H) ? SoT[(size_t)(k1 - 1) * H + m] : 0.0;
-      Se_s[kk][mm] = (k1 <= H - 1 && m < H - 1) ? SeT[(size_t)(k1 - 1) * (H - 1) + m] : 0.0;
+      So_s[e >> 6][e & 63] = rso[r];
+      Se_s[e >> 6][e & 63] = rse[r];
     }
     __syncthreads();
-    const int col = lane & 15;
-    const int kq = lane >> 4;
+    if (kc + F64_KC < H) fetch(kc + F64_KC);
+    const double(*A)[F64_TB + 1] = parity == 0 ? Ps : Qs;
+    const double(*Bs)[F64_TM + 1] = parity == 0 ? So_s : Se_s;
 #pragma unroll
     for (int ks = 0; ks < F64_KC / 4; ++ks) {
       const int krow = ks * 4 + kq;
-      double a0, a1, bv0, bv1;
-      if (parity == 0) {
-        a0 = Ps[krow][bh * 32 + col];
-        a1 = Ps[krow][bh * 32 + 16 + col];
-        bv0 = So_s[krow][col];
-        bv1 = So_s[krow][16 + col];
-      } else {
-        a0 = Qs[krow][bh * 32 + col];
-        a1 = Qs[krow][bh * 32 + 16 + col];
-        bv0 = Se_s[krow][col];
-        bv1 = Se_s[krow][16 + col];
+      const double a0 = A[krow][bh * 32 + col];
+      const double a1 = A[krow][bh * 32 + 16 + col];
+#pragma unroll
+      for (int ms = 0; ms < 4; ++ms) {
+        const double bv = Bs[krow][ms * 16 + col];
+        acc[0][ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc[0][ms], 0, 0, 0);
+        acc[1][ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc[1][ms], 0, 0, 0);
       }
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv1, acc[1][1], 0, 0, 0);
     }
     __syncthreads();
   }
 
   // epilogue: the f64 16x16 accumulator holds D[row = lane/16 + 4r][col = lane%16]
-  const int col = lane & 15;
   const int rq = lane >> 4;
   const int mlim = parity == 0 ? H : H - 1;
 #pragma unroll
   for (int bs = 0; bs < 2; ++bs)
 #pragma unroll
-    for (int ms = 0; ms < 2; ++ms) {
+    for (int ms = 0; ms < 4; ++ms) {
       const int m = m0 + ms * 16 + col;
       if (m >= mlim) continue;
       const int jc = parity == 0 ? 2 * m : 2 * m + 1;  // 0-based output row index j-1
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int b = b0 + bh * 32 + bs * 16 + rq + 4 * r;
-        if (b < Nb) store_out<OutT, EPI>(d, (size_t)b + (size_t)jc * Nb, acc[bs][ms][r]);
+        if (b < Nb) store_out<OutT, EPI>(d, b, jc, acc[bs][ms][r]);
       }
     }
 }
 
 // ---------------------------------------------------------------------------------
 // Mixed pass: fp32 operands split into bf16 hi + lo, three MFMA products, fp32 sums.
-// Work-group tile: 64 b x (32 odd + 32 even) rows, K chunk 32 (one MFMA k-step).
+// Work-group tile: 64 b x (64 odd + 64 even) rows, K chunk 32 (one MFMA k-step).
 // The sine tables are stored per output row with k contiguous and zero padding to a
-// multiple of 32 in both k and m, so the B operand is one 16-byte load per lane.
+// multiple of 32 in both k and m; a chunk's 64 rows of both tables are staged in LDS.
 // ---------------------------------------------------------------------------------
 constexpr int MX_TB = 64;
-constexpr int MX_TM = 32;
+constexpr int MX_TM = 64;
 constexpr int MX_KC = 32;
 constexpr int MX_LDK = MX_KC + 8;  // padded LDS row (bf16 elements)
 
@@ -190,20 +210,23 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
   lo = (__bf16)(x - (float)hi);
 }
 
-template <typename InT, typename OutT, int EPI, bool RING>
+template <typename InT, typename OutT, int EPI>
 __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch batch) {
-  const PassPlane& d = batch.p[blockIdx.z];
+  int z, bt, mt;
+  if (!xcd_tile(batch, z, bt, mt)) return;
+  const PassPlane& d = batch.p[z];
   const int Nk = d.Nk, Nb = d.Nb;
   const int H = (Nk + 1) / 2;
   const int Hp = (H + 31) & ~31;
-  const int b0 = blockIdx.x * MX_TB;
-  const int m0 = blockIdx.y * MX_TM;
+  const int b0 = bt * MX_TB;
+  const int m0 = mt * MX_TM;
   if (b0 >= Nb || m0 >= H) return;
 
-  __shared__ __attribute__((aligned(16))) __bf16 Ph[MX_TB][MX_LDK];
-  __shared__ __attribute__((aligned(16))) __bf16 Pl[MX_TB][MX_LDK];
-  __shared__ __attribute__((aligned(16))) __bf16 Qh[MX_TB][MX_LDK];
-  __shared__ __attribute__((aligned(16))) __bf16 Ql[MX_TB][MX_LDK];
+  // A operand (folded input, split) and B operand (sine rows, split) of one chunk
+  __shared__ __attribute__((aligned(16))) __bf16 Ah[2][MX_TB][MX_LDK];  // [P|Q] hi
+  __shared__ __attribute__((aligned(16))) __bf16 Al[2][MX_TB][MX_LDK];  // [P|Q] lo
+  __shared__ __attribute__((aligned(16))) __bf16 Bh[2][MX_TM][MX_LDK];  // [odd|even] hi
+  __shared__ __attribute__((aligned(16))) __bf16 Bl[2][MX_TM][MX_LDK];  // [odd|even] lo
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -213,74 +236,95 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
   const int col = lane & 15;
   const int kq = lane >> 4;
 
-  const __bf16* Sh = (const __bf16*)(parity == 0 ? d.So : d.Se);
   // hi and lo tables are stored back to back: [hi (Mp x Hp)][lo (Mp x Hp)]
-  const int Mp = (H + MX_TM - 1) / MX_TM * MX_TM;
+  const int Mp = (H + 31) / 32 * 32;
   const size_t tab = (size_t)Mp * Hp;
-  const __bf16* Sl = Sh + tab;
+  const __bf16* So = (const __bf16*)d.So;
+  const __bf16* Se = (const __bf16*)d.Se;
 
-  f4 acc[2][2];
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
-
-  // loader mapping: thread -> row bb = tid/4, k sub-block ks8 = (tid%4)*8 .. +7
-  const int lb = tid >> 2;
-  const int lk = (tid & 3) * 8;
-
-  for (int kc = 0; kc < H; kc += MX_KC) {
-    {
-      const int b = b0 + lb;
-      __bf16 ph[8], pl[8], qh[8], ql[8];
+  f4 acc[2][4];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int k1 = kc + lk + i + 1;
-        double p = 0.0, qv = 0.0;
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+
+  // loader mapping for the input: element pair (2kp, 2kp+1) of row bb, 16 lanes per row
+  double ra[4][2], rm[4][2];
+  // loader mapping for the sine rows: 16-byte piece (row sm, k8) of table st (4 tables)
+  const int sm = tid >> 2, sk = (tid & 3) * 8;
+  bf8 rs[4];
+  auto fetch = [&](int kc) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = tid + 256 * r;
+      const int kp = e & 15;
+      const int b = b0 + (e >> 4);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int k1 = kc + 2 * kp + j + 1;
+        double va = 0.0, vm = 0.0;
         if (k1 <= H && b < Nb) {
-          const double va = load_in<InT, RING>(d, k1 - 1, b);
-          if (k1 == H) {
-            p = va;
-          } else {
-            const double vm = load_in<InT, RING>(d, Nk - k1, b);
-            p = va + vm;
-            qv = va - vm;
-          }
+          va = load_in<InT>(d, k1 - 1, b);
+          if (k1 < H) vm = load_in<InT>(d, Nk - k1, b);
         }
-        split_bf16((float)p, ph[i], pl[i]);
-        split_bf16((float)qv, qh[i], ql[i]);
+        ra[r][j] = va;
+        rm[r][j] = vm;
       }
-      *(bf8*)&Ph[lb][lk] = *(bf8*)ph;
-      *(bf8*)&Pl[lb][lk] = *(bf8*)pl;
-      *(bf8*)&Qh[lb][lk] = *(bf8*)qh;
-      *(bf8*)&Ql[lb][lk] = *(bf8*)ql;
     }
+    const size_t off = (size_t)(m0 + sm) * Hp + kc + sk;  // m0 + sm < Mp by padding
+    if (m0 + sm < Mp) {
+      rs[0] = *(const bf8*)(So + off);
+      rs[1] = *(const bf8*)(So + tab + off);
+      rs[2] = *(const bf8*)(Se + off);
+      rs[3] = *(const bf8*)(Se + tab + off);
+    } else {
+      for (int t = 0; t < 4; ++t)
+        for (int i = 0; i < 8; ++i) rs[t][i] = (__bf16)0.f;
+    }
+  };
+  fetch(0);
+  for (int kc = 0; kc < H; kc += MX_KC) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = tid + 256 * r;
+      const int kp = e & 15, bb = e >> 4;
+      __bf16 ph[2], pl[2], qh[2], ql[2];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const bool mid = kc + 2 * kp + j + 1 == H;  // k = H has no mirror partner
+        split_bf16((float)(ra[r][j] + rm[r][j]), ph[j], pl[j]);
+        split_bf16(mid ? 0.f : (float)(ra[r][j] - rm[r][j]), qh[j], ql[j]);
+      }
+      typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+      *(bf2*)&Ah[0][bb][2 * kp] = (bf2){ph[0], ph[1]};
+      *(bf2*)&Al[0][bb][2 * kp] = (bf2){pl[0], pl[1]};
+      *(bf2*)&Ah[1][bb][2 * kp] = (bf2){qh[0], qh[1]};
+      *(bf2*)&Al[1][bb][2 * kp] = (bf2){ql[0], ql[1]};
+    }
+    *(bf8*)&Bh[0][sm][sk] = rs[0];
+    *(bf8*)&Bl[0][sm][sk] = rs[1];
+    *(bf8*)&Bh[1][sm][sk] = rs[2];
+    *(bf8*)&Bl[1][sm][sk] = rs[3];
     __syncthreads();
-    bf8 ah[2], al[2], bhv[2], blv[2];
+    if (kc + MX_KC < H) fetch(kc + MX_KC);
+    bf8 ah[2], al[2];
 #pragma unroll
     for (int bs = 0; bs < 2; ++bs) {
       const int row = bh * 32 + bs * 16 + col;
-      if (parity == 0) {
-        ah[bs] = *(const bf8*)&Ph[row][kq * 8];
-        al[bs] = *(const bf8*)&Pl[row][kq * 8];
-      } else {
-        ah[bs] = *(const bf8*)&Qh[row][kq * 8];
-        al[bs] = *(const bf8*)&Ql[row][kq * 8];
-      }
+      ah[bs] = *(const bf8*)&Ah[parity][row][kq * 8];
+      al[bs] = *(const bf8*)&Al[parity][row][kq * 8];
     }
 #pragma unroll
-    for (int ms = 0; ms < 2; ++ms) {
-      const int m = m0 + ms * 16 + col;
-      const size_t off = (size_t)m * Hp + kc + kq * 8;
-      bhv[ms] = *(const bf8*)(Sh + off);
-      blv[ms] = *(const bf8*)(Sl + off);
-    }
+    for (int ms = 0; ms < 4; ++ms) {
+      const bf8 bhv = *(const bf8*)&Bh[parity][ms * 16 + col][kq * 8];
+      const bf8 blv = *(const bf8*)&Bl[parity][ms * 16 + col][kq * 8];
 #pragma unroll
-    for (int bs = 0; bs < 2; ++bs)
-#pragma unroll
-      for (int ms = 0; ms < 2; ++ms) {
-        acc[bs][ms] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bs], bhv[ms], acc[bs][ms], 0, 0, 0);
-        acc[bs][ms] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bs], blv[ms], acc[bs][ms], 0, 0, 0);
-        acc[bs][ms] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bs], bhv[ms], acc[bs][ms], 0, 0, 0);
+      for (int bs = 0; bs < 2; ++bs) {
+        acc[bs][ms] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bs], bhv, acc[bs][ms], 0, 0, 0);
+        acc[bs][ms] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bs], blv, acc[bs][ms], 0, 0, 0);
+        acc[bs][ms] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bs], bhv, acc[bs][ms], 0, 0, 0);
       }
+    }
     __syncthreads();
   }
 
@@ -290,14 +334,14 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
 #pragma unroll
   for (int bs = 0; bs < 2; ++bs)
 #pragma unroll
-    for (int ms = 0; ms < 2; ++ms) {
+    for (int ms = 0; ms < 4; ++ms) {
       const int m = m0 + ms * 16 + col;
       if (m >= mlim) continue;
       const int jc = parity == 0 ? 2 * m : 2 * m + 1;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int b = b0 + bh * 32 + bs * 16 + 4 * rq + r;
-        if (b < Nb) store_out<OutT, EPI>(d, (size_t)b + (size_t)jc * Nb, (double)acc[bs][ms][r]);
+        if (b < Nb) store_out<OutT, EPI>(d, b, jc, (double)acc[bs][ms][r]);
       }
     }
 }
@@ -305,46 +349,44 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
 // ---------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------
-#define IBPM_DISPATCH_EPI(KERNEL, InT, OutT, RING)                                         \
+#define IBPM_DISPATCH_EPI(KERNEL, InT, OutT)                                               \
   switch (epi) {                                                                           \
-    case EPI_STORE: hipLaunchKernelGGL((KERNEL<InT, OutT, EPI_STORE, RING>), grid, block, 0, st, batch); break; \
-    case EPI_DIAG: hipLaunchKernelGGL((KERNEL<InT, OutT, EPI_DIAG, RING>), grid, block, 0, st, batch); break;   \
-    case EPI_SUB: hipLaunchKernelGGL((KERNEL<InT, OutT, EPI_SUB, RING>), grid, block, 0, st, batch); break;     \
-    default: hipLaunchKernelGGL((KERNEL<InT, OutT, EPI_ADD, RING>), grid, block, 0, st, batch); break;          \
+    case EPI_STORE: hipLaunchKernelGGL((KERNEL<InT, OutT, EPI_STORE>), grid, block, 0, st, batch); break; \
+    case EPI_DIAG: hipLaunchKernelGGL((KERNEL<InT, OutT, EPI_DIAG>), grid, block, 0, st, batch); break;   \
+    case EPI_SUB: hipLaunchKernelGGL((KERNEL<InT, OutT, EPI_SUB>), grid, block, 0, st, batch); break;     \
+    default: hipLaunchKernelGGL((KERNEL<InT, OutT, EPI_ADD>), grid, block, 0, st, batch); break;          \
   }
 
-hipError_t launch_dst_pass_f64(const PassBatch& batch, int nplanes, int maxNb, int maxH, int epi,
-                               bool ring, hipStream_t st) {
-  dim3 grid((maxNb + F64_TB - 1) / F64_TB, (maxH + F64_TM - 1) / F64_TM, nplanes);
+static dim3 tile_grid(PassBatch& b, int nplanes, int maxNb, int maxH, int TB, int TM) {
+  b.nbt = (maxNb + TB - 1) / TB;
+  b.nmt = (maxH + TM - 1) / TM;
+  b.nplanes = nplanes;
+  const int T = nplanes * b.nbt * b.nmt;
+  return dim3(8 * ((T + 7) / 8));
+}
+
+hipError_t launch_dst_pass_f64(const PassBatch& batch_in, int nplanes, int maxNb, int maxH, int epi,
+                               hipStream_t st) {
+  PassBatch batch = batch_in;
+  dim3 grid = tile_grid(batch, nplanes, maxNb, maxH, F64_TB, F64_TM);
   dim3 block(256);
-  if (ring) {
-    IBPM_DISPATCH_EPI(dst_pass_f64_kernel, double, double, true)
-  } else {
-    IBPM_DISPATCH_EPI(dst_pass_f64_kernel, double, double, false)
-  }
+  IBPM_DISPATCH_EPI(dst_pass_f64_kernel, double, double)
   return hipGetLastError();
 }
 
-hipError_t launch_dst_pass_mixed(const PassBatch& batch, int nplanes, int maxNb, int maxH, int epi,
-                                 bool ring, bool in_f32, bool out_f32, hipStream_t st) {
-  dim3 grid((maxNb + MX_TB - 1) / MX_TB, (maxH + MX_TM - 1) / MX_TM, nplanes);
+hipError_t launch_dst_pass_mixed(const PassBatch& batch_in, int nplanes, int maxNb, int maxH, int epi,
+                                 bool in_f32, bool out_f32, hipStream_t st) {
+  PassBatch batch = batch_in;
+  dim3 grid = tile_grid(batch, nplanes, maxNb, maxH, MX_TB, MX_TM);
   dim3 block(256);
   if (!in_f32 && out_f32) {
-    if (ring) {
-      IBPM_DISPATCH_EPI(dst_pass_mixed_kernel, double, float, true)
-    } else {
-      IBPM_DISPATCH_EPI(dst_pass_mixed_kernel, double, float, false)
-    }
+    IBPM_DISPATCH_EPI(dst_pass_mixed_kernel, double, float)
   } else if (in_f32 && out_f32) {
-    IBPM_DISPATCH_EPI(dst_pass_mixed_kernel, float, float, false)
+    IBPM_DISPATCH_EPI(dst_pass_mixed_kernel, float, float)
   } else if (in_f32 && !out_f32) {
-    IBPM_DISPATCH_EPI(dst_pass_mixed_kernel, float, double, false)
+    IBPM_DISPATCH_EPI(dst_pass_mixed_kernel, float, double)
   } else {
-    if (ring) {
-      IBPM_DISPATCH_EPI(dst_pass_mixed_kernel, double, double, true)
-    } else {
-      IBPM_DISPATCH_EPI(dst_pass_mixed_kernel, double, double, false)
-    }
+    IBPM_DISPATCH_EPI(dst_pass_mixed_kernel, double, double)
   }
   return hipGetLastError();
 }
@@ -390,7 +432,7 @@ static inline float bf16_to_float(uint16_t b) {
 static size_t bf16_tab(int N) {
   const size_t H = (size_t)(N + 1) / 2;
   const size_t Hp = (H + 31) & ~(size_t)31;
-  const size_t Mp = (H + MX_TM - 1) / MX_TM * MX_TM;
+  const size_t Mp = (H + 31) / 32 * 32;
   return Mp * Hp;
 }
 size_t sine_table_bf16_odd_count(int N) { return 2 * bf16_tab(N); }

@@ -43,12 +43,12 @@ hipError_t dalloc(T** p, size_t n) {
   return hipMalloc((void**)p, n * sizeof(T));
 }
 
-// One plane of a DST-I solve  x = S_x S_y (diag . (S_x b S_y)), optionally
-// x = base -/+ solve, where the system operator is a I + c K (for refinement).
+// One plane of a DST-I solve  x = S_x S_y (D . (S_x b S_y)) with the spectral diagonal
+// D = scale / (da + lambda dnum / dden), optionally x = base - solve; the system operator
+// is a I + c K (for the refinement residual).
 struct SolveJob {
   const double* b;
-  const double* ring;  // apply_bc! from this coarse plane (vort2flux), or null
-  const double* diag;
+  double da, dnum, dden;
   double* x;
   int epi;             // EPI_STORE, EPI_SUB (x = base - solve)
   const double* base;
@@ -70,14 +70,19 @@ struct ibpm_solver {
   hipStream_t st = nullptr;
   double fs[2] = {0.0, 0.0};
   std::vector<double> hc;  // grid step per level
+  double scale = 0.0;      // 1 / (4 nx ny): the two unnormalised DST-I scale factors
+  int lvstride = 0;        // ring-line length: max(nx, ny)/2 + 1
 
   // fields
   double *q = nullptr, *G = nullptr, *psi = nullptr, *nl1 = nullptr, *nl2 = nullptr;
   double *Gs = nullptr, *rhs = nullptr;
   double *T = nullptr, *U = nullptr, *R = nullptr;  // kMaxPlanes scratch planes each
-  double *dpois = nullptr, *dAinv = nullptr;
+  double *cosx = nullptr, *cosy = nullptr;  // cos(pi k / n): spectral eigenvalues
   void *SoX = nullptr, *SeX = nullptr, *SoY = nullptr, *SeY = nullptr;  // FP64 tables
   void *BfX = nullptr, *BfY = nullptr;                                  // bf16 tables
+  // multi-level ring chain: sine rows/columns, per-plane factors, per-chain scratch
+  double *edge = nullptr, *pw = nullptr;
+  double *fac = nullptr, *rpart = nullptr, *rlv = nullptr;
   // bodies
   double *xb = nullptr, *ub = nullptr, *xb_ref = nullptr, *Ft = nullptr, *fb = nullptr;
   double *Ftmp = nullptr, *total = nullptr, *motion = nullptr;
@@ -104,10 +109,11 @@ namespace {
 
 void free_all(ibpm_solver* s) {
   void* ptrs[] = {s->q, s->G, s->psi, s->nl1, s->nl2, s->Gs, s->rhs, s->T, s->U, s->R,
-                  s->dpois, s->dAinv, s->SoX, s->SeX, s->SoY, s->SeY, s->BfX, s->BfY,
+                  s->cosx, s->cosy, s->SoX, s->SeX, s->SoY, s->SeY, s->BfX, s->BfY,
                   s->xb, s->ub, s->xb_ref, s->Ft, s->fb, s->Ftmp, s->total, s->motion,
                   s->body_of, s->moving_d, s->coords, s->perm, s->xstart, s->err, s->w,
-                  s->Binv, s->Gtmp, s->psitmp, s->Sc, s->cg_r, s->cg_u, s->cg_c, s->cg_scal};
+                  s->Binv, s->Gtmp, s->psitmp, s->Sc, s->cg_r, s->cg_u, s->cg_c, s->cg_scal,
+                  s->edge, s->pw, s->fac, s->rpart, s->rlv};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->h_scal) (void)hipHostFree(s->h_scal);
@@ -128,83 +134,63 @@ const void* tab_even(const ibpm_solver* s, bool xdir) {
 }
 
 // One pass over up to kMaxPlanes planes.
-int run_pass(ibpm_solver* s, PassBatch& pb, int np, bool xdir, int epi, bool ring, bool in_f32,
+int run_pass(ibpm_solver* s, PassBatch& pb, int np, bool xdir, int epi, bool in_f32,
              bool out_f32) {
   const int Nk = xdir ? s->g.Nx : s->g.Ny;
   const int Nb = xdir ? s->g.Ny : s->g.Nx;
   for (int i = 0; i < np; ++i) {
     pb.p[i].Nk = Nk;
     pb.p[i].Nb = Nb;
-    pb.p[i].nx = s->g.nx;
-    pb.p[i].ny = s->g.ny;
     pb.p[i].So = tab_odd(s, xdir);
     pb.p[i].Se = tab_even(s, xdir);
+    pb.p[i].cosb = xdir ? s->cosy : s->cosx;
+    pb.p[i].cosj = xdir ? s->cosx : s->cosy;
+    pb.p[i].dscale = s->scale;
   }
   const int H = (Nk + 1) / 2;
   if (s->precision == IBPM_MIXED)
-    HIPCHK(launch_dst_pass_mixed(pb, np, Nb, H, epi, ring, in_f32, out_f32, s->st));
+    HIPCHK(launch_dst_pass_mixed(pb, np, Nb, H, epi, in_f32, out_f32, s->st));
   else
-    HIPCHK(launch_dst_pass_f64(pb, np, Nb, H, epi, ring, s->st));
+    HIPCHK(launch_dst_pass_f64(pb, np, Nb, H, epi, s->st));
   return IBPM_OK;
 }
 
-// DST-I solves of up to kMaxPlanes jobs; scratch planes T[i], U[i], R[i].
+// DST-I solves x = S (D . (S b S)) S of up to kMaxPlanes independent planes (the
+// implicit Helmholtz solves); scratch planes T[i], U[i], R[i]. In mixed precision every
+// solve is followed by `refine` sweeps x += solve(b - (a I + c K) x) with an FP64 residual.
 int solve_chunk(ibpm_solver* s, const SolveJob* jobs, int nj) {
   const size_t P = s->plane();
   const bool mixed = s->precision == IBPM_MIXED;
   PassBatch pb;
-  auto Tp = [&](int i) { return (void*)(s->T + i * P); };
-  auto Up = [&](int i) { return (void*)(s->U + i * P); };
   auto four_passes = [&](bool from_residual, bool correction) -> int {
-    bool any_ring = false;
-    // pass 1: x-direction, b (+ring) -> T
     memset(&pb, 0, sizeof pb);
     for (int i = 0; i < nj; ++i) {
       pb.p[i].in = from_residual ? (const void*)(s->R + i * P) : (const void*)jobs[i].b;
-      pb.p[i].out = Tp(i);
-      pb.p[i].ring = from_residual ? nullptr : jobs[i].ring;
-      if (pb.p[i].ring) any_ring = true;
+      pb.p[i].out = s->T + i * P;
     }
-    if (any_ring) {
-      // planes without a ring are not allowed in a ring launch: split
-      PassBatch a, b;
-      memset(&a, 0, sizeof a);
-      memset(&b, 0, sizeof b);
-      int na = 0, nb = 0;
-      int ia[kMaxPlanes], ib[kMaxPlanes];
-      for (int i = 0; i < nj; ++i) {
-        if (pb.p[i].ring) { ia[na] = i; a.p[na++] = pb.p[i]; }
-        else { ib[nb] = i; b.p[nb++] = pb.p[i]; }
-      }
-      (void)ia; (void)ib;
-      if (na) { int rc = run_pass(s, a, na, true, EPI_STORE, true, false, mixed); if (rc) return rc; }
-      if (nb) { int rc = run_pass(s, b, nb, true, EPI_STORE, false, false, mixed); if (rc) return rc; }
-    } else {
-      int rc = run_pass(s, pb, nj, true, EPI_STORE, false, from_residual && mixed, mixed);
-      if (rc) return rc;
-    }
-    // pass 2: y-direction, T -> U, times diag
+    int rc = run_pass(s, pb, nj, true, EPI_STORE, from_residual && mixed, mixed);
+    if (rc) return rc;
     memset(&pb, 0, sizeof pb);
     for (int i = 0; i < nj; ++i) {
-      pb.p[i].in = Tp(i);
-      pb.p[i].out = Up(i);
-      pb.p[i].diag = jobs[i].diag;
+      pb.p[i].in = s->T + i * P;
+      pb.p[i].out = s->U + i * P;
+      pb.p[i].da = jobs[i].da;
+      pb.p[i].dnum = jobs[i].dnum;
+      pb.p[i].dden = jobs[i].dden;
     }
-    int rc = run_pass(s, pb, nj, false, EPI_DIAG, false, mixed, mixed);
+    rc = run_pass(s, pb, nj, false, EPI_DIAG, mixed, mixed);
     if (rc) return rc;
-    // pass 3: x-direction, U -> T
     memset(&pb, 0, sizeof pb);
     for (int i = 0; i < nj; ++i) {
-      pb.p[i].in = Up(i);
-      pb.p[i].out = Tp(i);
+      pb.p[i].in = s->U + i * P;
+      pb.p[i].out = s->T + i * P;
     }
-    rc = run_pass(s, pb, nj, true, EPI_STORE, false, mixed, mixed);
+    rc = run_pass(s, pb, nj, true, EPI_STORE, mixed, mixed);
     if (rc) return rc;
-    // pass 4: y-direction, T -> x with the job's epilogue
     memset(&pb, 0, sizeof pb);
     int epi = EPI_STORE;
     for (int i = 0; i < nj; ++i) {
-      pb.p[i].in = Tp(i);
+      pb.p[i].in = s->T + i * P;
       pb.p[i].out = jobs[i].x;
       if (correction) {
         epi = jobs[i].epi == EPI_SUB ? EPI_SUB : EPI_ADD;
@@ -214,7 +200,7 @@ int solve_chunk(ibpm_solver* s, const SolveJob* jobs, int nj) {
         pb.p[i].base = jobs[i].base;
       }
     }
-    return run_pass(s, pb, nj, false, epi, false, mixed, false);
+    return run_pass(s, pb, nj, false, epi, mixed, false);
   };
   int rc = four_passes(false, false);
   if (rc) return rc;
@@ -231,7 +217,7 @@ int solve_chunk(ibpm_solver* s, const SolveJob* jobs, int nj) {
         ra.r[i] = s->R + i * P;
         ra.a[i] = jobs[i].a;
         ra.c[i] = jobs[i].c;
-        ra.ring[i] = jobs[i].ring;
+        ra.ring[i] = nullptr;
         ra.xbase[i] = jobs[i].epi == EPI_SUB ? jobs[i].base : nullptr;
       }
       HIPCHK(launch_residual(ra, s->st));
@@ -255,6 +241,81 @@ int solve(ibpm_solver* s, const SolveJob* jobs, int nj) {
   return IBPM_OK;
 }
 
+// Multi-level Poisson solve of vort2flux for ng columns of L planes (ng*L <= kMaxPlanes):
+// one batched forward pass pair, the FP64 ring chain level L..2, one batched inverse pass
+// pair whose first pass adds the ring terms. in[p], out[p]: plane p = column*L + level.
+int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double* const* out,
+              int epi) {
+  const size_t P = s->plane();
+  const int L = s->L, np = ng * L;
+  const bool mixed = s->precision == IBPM_MIXED;
+  const size_t nf = 2 * (size_t)s->g.Nx + 2 * (size_t)s->g.Ny;
+  PassBatch pb;
+  memset(&pb, 0, sizeof pb);
+  for (int p = 0; p < np; ++p) {
+    pb.p[p].in = in[p];
+    pb.p[p].out = s->T + p * P;
+  }
+  int rc = run_pass(s, pb, np, true, EPI_STORE, in_f32, mixed);
+  if (rc) return rc;
+  memset(&pb, 0, sizeof pb);
+  for (int p = 0; p < np; ++p) {
+    pb.p[p].in = s->T + p * P;
+    pb.p[p].out = s->U + p * P;
+  }
+  rc = run_pass(s, pb, np, false, EPI_STORE, mixed, mixed);
+  if (rc) return rc;
+  // ring chain, coarse to fine: E_l = D . (F_l + R_l) in place and, for l >= 1, the ring
+  // factors of level l-1 from E_l
+  RingChainArgs ra;
+  memset(&ra, 0, sizeof ra);
+  ra.nx = s->g.nx;
+  ra.ny = s->g.ny;
+  ra.Nx = s->g.Nx;
+  ra.Ny = s->g.Ny;
+  ra.F_f32 = mixed ? 1 : 0;
+  ra.cosx = s->cosx;
+  ra.cosy = s->cosy;
+  ra.scale = s->scale;
+  ra.edge = s->edge;
+  ra.pw = s->pw;
+  ra.SoX = (const double*)s->SoX;
+  ra.SeX = (const double*)s->SeX;
+  ra.SoY = (const double*)s->SoY;
+  ra.SeY = (const double*)s->SeY;
+  ra.lvstride = s->lvstride;
+  const size_t npart = ring_partial_count(s->g.Nx, s->g.Ny);
+  const size_t npa = ring_partial_a_count(s->g.Nx, s->g.Ny);
+  for (int l = L - 1; l >= 0; --l) {
+    for (int c = 0; c < ng; ++c) {
+      ra.F[c] = s->U + (c * L + l) * P;
+      ra.fac_in[c] = l == L - 1 ? nullptr : s->fac + (c * L + l) * nf;
+      ra.fac_out[c] = l > 0 ? s->fac + (c * L + l - 1) * nf : nullptr;
+      ra.part_a[c] = s->rpart + c * npart;
+      ra.part_c[c] = s->rpart + c * npart + npa;
+      ra.lv[c] = s->rlv + (size_t)c * 4 * s->lvstride;
+    }
+    if (l > 0)
+      HIPCHK(launch_ring_step(ra, ng, s->st));
+    else
+      HIPCHK(launch_ring_project(ra, ng, false, s->st));
+  }
+  memset(&pb, 0, sizeof pb);
+  for (int p = 0; p < np; ++p) {
+    pb.p[p].in = s->U + p * P;
+    pb.p[p].out = s->T + p * P;
+  }
+  rc = run_pass(s, pb, np, true, EPI_STORE, mixed, mixed);
+  if (rc) return rc;
+  memset(&pb, 0, sizeof pb);
+  for (int p = 0; p < np; ++p) {
+    pb.p[p].in = s->T + p * P;
+    pb.p[p].out = out[p];
+    pb.p[p].base = out[p];
+  }
+  return run_pass(s, pb, np, false, epi, mixed, false);
+}
+
 // vort2flux (fluid.jl:175-206) for ncol independent columns of L planes each.
 // With `outer`, the coarse levels outside the centre come from outer (final pass of the
 // step); otherwise coarsify! writes the centre region in place. q: curl all levels.
@@ -262,25 +323,50 @@ int vort2flux(ibpm_solver* s, int ncol, double* const* Gc, double* const* psic,
               double* const* outer, double* const* qc) {
   const size_t P = s->plane();
   const int L = s->L;
+  const bool mixed = s->precision == IBPM_MIXED;
   for (int c = 0; c < ncol; ++c)
     for (int l = 1; l < L; ++l)
       HIPCHK(launch_coarsify(s->g, Gc[c] + l * P, Gc[c] + (l - 1) * P,
                              outer && outer[c] ? outer[c] + l * P : nullptr, s->st));
-  std::vector<SolveJob> jobs(ncol);
-  for (int l = L - 1; l >= 0; --l) {
-    for (int c = 0; c < ncol; ++c) {
-      SolveJob& j = jobs[c];
-      j.b = Gc[c] + l * P;
-      j.ring = l + 1 < L ? psic[c] + (l + 1) * P : nullptr;
-      j.diag = s->dpois;
-      j.x = psic[c] + l * P;
-      j.epi = EPI_STORE;
-      j.base = nullptr;
-      j.a = 0.0;
-      j.c = 1.0;
-    }
-    int rc = solve(s, jobs.data(), ncol);
+  const int gmax = std::max(1, kMaxPlanes / L);
+  for (int c0 = 0; c0 < ncol; c0 += gmax) {
+    const int ng = std::min(gmax, ncol - c0);
+    const void* in[kMaxPlanes];
+    double* out[kMaxPlanes];
+    for (int c = 0; c < ng; ++c)
+      for (int l = 0; l < L; ++l) {
+        in[c * L + l] = Gc[c0 + c] + l * P;
+        out[c * L + l] = psic[c0 + c] + l * P;
+      }
+    int rc = v2f_group(s, ng, in, false, out, EPI_STORE);
     if (rc) return rc;
+    if (mixed) {
+      // refinement of the coupled multi-level system: r_l = Gamma_l + apply_bc(get_bc(
+      // psi_{l+1})) - K psi_l, corrected through the same solve
+      for (int sweep = 0; sweep < s->refine; ++sweep) {
+        ResidualArgs ra;
+        memset(&ra, 0, sizeof ra);
+        ra.g = s->g;
+        ra.nplanes = ng * L;
+        ra.r_f32 = 1;
+        const void* rin[kMaxPlanes];
+        for (int c = 0; c < ng; ++c)
+          for (int l = 0; l < L; ++l) {
+            const int p = c * L + l;
+            ra.b[p] = Gc[c0 + c] + l * P;
+            ra.x[p] = psic[c0 + c] + l * P;
+            ra.r[p] = s->R + p * P;
+            ra.a[p] = 0.0;
+            ra.c[p] = 1.0;
+            ra.ring[p] = l + 1 < L ? psic[c0 + c] + (l + 1) * P : nullptr;
+            ra.xbase[p] = nullptr;
+            rin[p] = s->R + p * P;
+          }
+        HIPCHK(launch_residual(ra, s->st));
+        rc = v2f_group(s, ng, rin, true, out, EPI_ADD);
+        if (rc) return rc;
+      }
+    }
   }
   if (qc)
     for (int c = 0; c < ncol; ++c) HIPCHK(launch_curl_all(s->g, L, qc[c], psic[c], s->st));
@@ -314,8 +400,9 @@ int apply_B(ibpm_solver* s, int ncol, const double* const* z, double* const* x) 
   for (int c = 0; c < ncol; ++c) {
     SolveJob& j = jobs[c];
     j.b = s->Sc + (size_t)c * P;
-    j.ring = nullptr;
-    j.diag = s->dAinv;
+    j.da = 1.0;
+    j.dnum = s->dt;
+    j.dden = 2.0 * s->Re * (s->hc[0] * s->hc[0]);
     j.x = s->Gtmp + (size_t)c * L * P;
     j.epi = EPI_STORE;
     j.base = nullptr;
@@ -562,8 +649,9 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
   for (int l = 0; l < L; ++l) {
     SolveJob& j = jobs[l];
     j.b = s->rhs + l * P;
-    j.ring = nullptr;
-    j.diag = s->dAinv + l * P;
+    j.da = 1.0;
+    j.dnum = s->dt;
+    j.dden = 2.0 * s->Re * (s->hc[l] * s->hc[l]);
     j.x = s->Gs + l * P;
     j.epi = EPI_STORE;
     j.base = nullptr;
@@ -599,8 +687,9 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
     HIPCHK(launch_scatter_rot(sa, s->st));
     SolveJob j;
     j.b = s->rhs;
-    j.ring = nullptr;
-    j.diag = s->dAinv;
+    j.da = 1.0;
+    j.dnum = s->dt;
+    j.dden = 2.0 * s->Re * (s->hc[0] * s->hc[0]);
     j.x = s->G;
     j.epi = EPI_SUB;
     j.base = s->Gs;
@@ -721,23 +810,19 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   CK(hipMemsetAsync(s->nl2, 0, P * L * 8, s->st));
   CK(hipMemsetAsync(s->Gs, 0, P * L * 8, s->st));
 
-  // spectral diagonals: scale / Lambda (fluid.jl:117-134, :155, :168-173)
+  // spectral eigenvalues lambda = -2((cos(pi i/nx) + cos(pi j/ny)) - 2) and the DST-I
+  // scale 1/(4 nx ny) (fluid.jl:117-134, :155, :168-173): the diagonals are evaluated in
+  // the kernels from the two cosine vectors
   {
     const int nx = s->g.nx, ny = s->g.ny, Nx = s->g.Nx, Ny = s->g.Ny;
-    const double scale = 1.0 / (4.0 * nx * ny);
-    std::vector<double> lam(P), d(P);
-    for (int j = 1; j <= Ny; ++j)
-      for (int i = 1; i <= Nx; ++i)
-        lam[(i - 1) + (size_t)(j - 1) * Nx] = -2.0 * (cos(M_PI * i / nx) + cos(M_PI * j / ny) - 2.0);
-    CK(dalloc(&s->dpois, P));
-    for (size_t k = 0; k < P; ++k) d[k] = scale / lam[k];
-    CK(hipMemcpy(s->dpois, d.data(), P * 8, hipMemcpyHostToDevice));
-    CK(dalloc(&s->dAinv, P * L));
-    for (int l = 0; l < s->L; ++l) {
-      const double hc = s->hc[l];
-      for (size_t k = 0; k < P; ++k) d[k] = scale / (1.0 + lam[k] * s->dt / (2.0 * s->Re * (hc * hc)));
-      CK(hipMemcpy(s->dAinv + l * P, d.data(), P * 8, hipMemcpyHostToDevice));
-    }
+    s->scale = 1.0 / (4.0 * nx * ny);
+    std::vector<double> cx(Nx), cy(Ny);
+    for (int i = 1; i <= Nx; ++i) cx[i - 1] = cos(M_PI * i / nx);
+    for (int j = 1; j <= Ny; ++j) cy[j - 1] = cos(M_PI * j / ny);
+    CK(dalloc(&s->cosx, (size_t)Nx));
+    CK(dalloc(&s->cosy, (size_t)Ny));
+    CK(hipMemcpy(s->cosx, cx.data(), Nx * 8, hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->cosy, cy.data(), Ny * 8, hipMemcpyHostToDevice));
   }
   // sine tables
   {
@@ -765,6 +850,35 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
     };
     CK(build(s->g.Nx, &s->SoX, &s->SeX, &s->BfX));
     CK(build(s->g.Ny, &s->SoY, &s->SeY, &s->BfY));
+    // ring chain tables: the first and last rows of S (apply_bc! edges) and the ring-line
+    // projection weights, with exact argument reduction
+    const int nx = s->g.nx, ny = s->g.ny, Nx = s->g.Nx, Ny = s->g.Ny;
+    auto s2 = [](long long num, long long n) {
+      return 2.0 * sin(M_PI * (double)(num % (2 * n)) / (double)n);
+    };
+    std::vector<double> ed(2 * Nx + 2 * Ny), pw(2 * Ny + 2 * Nx);
+    for (int k = 1; k <= Nx; ++k) {
+      ed[k - 1] = s2(k, nx);                       // S_x[k, 1]
+      ed[Nx + k - 1] = s2((long long)k * Nx, nx);  // S_x[k, Nx]
+      pw[2 * Ny + k - 1] = s2((long long)k * (nx / 4), nx);           // S_x[nx/4, k]
+      pw[2 * Ny + Nx + k - 1] = s2((long long)k * (3 * nx / 4), nx);  // S_x[3nx/4, k]
+    }
+    for (int k = 1; k <= Ny; ++k) {
+      ed[2 * Nx + k - 1] = s2(k, ny);
+      ed[2 * Nx + Ny + k - 1] = s2((long long)k * Ny, ny);
+      pw[k - 1] = s2((long long)k * (ny / 4), ny);             // S_y[k, ny/4]
+      pw[Ny + k - 1] = s2((long long)k * (3 * ny / 4), ny);    // S_y[k, 3ny/4]
+    }
+    CK(dalloc(&s->edge, ed.size()));
+    CK(dalloc(&s->pw, pw.size()));
+    CK(hipMemcpy(s->edge, ed.data(), ed.size() * 8, hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->pw, pw.data(), pw.size() * 8, hipMemcpyHostToDevice));
+    const size_t nf = ed.size(), nbc = 2 * (size_t)(nx + 1) + 2 * (size_t)(ny + 1);
+    CK(dalloc(&s->fac, kMaxPlanes * nf));
+    CK(dalloc(&s->rpart, kMaxChains * ring_partial_count(Nx, Ny)));
+    s->lvstride = std::max(nx, ny) / 2 + 1;
+    CK(dalloc(&s->rlv, kMaxChains * 4 * (size_t)s->lvstride));
+    (void)nbc;
   }
   // bodies
   const int n = s->n;
@@ -1020,6 +1134,56 @@ int ibpm_synchronize(ibpm_solver* s) {
   return IBPM_OK;
 }
 
+int ibpm_apply(ibpm_solver* s, int op, int level, const double* in, double* out, size_t n) {
+  if (!s || !in || !out) return fail(IBPM_E_ARG, "null argument");
+  const size_t P = s->plane();
+  const int L = s->L;
+  const size_t need = op == IBPM_OP_VORT2FLUX ? P * L : P;
+  if (n != need) return fail(IBPM_E_ARG, "size mismatch: expected " + std::to_string(need));
+  if (op == IBPM_OP_HELMHOLTZ && (level < 1 || level > L)) return fail(IBPM_E_ARG, "bad level");
+  if (op < IBPM_OP_POISSON || op > IBPM_OP_VORT2FLUX) return fail(IBPM_E_ARG, "unknown op");
+  HIPCHK(hipSetDevice(s->device));
+  // staging: rhs planes hold the input, Gs planes (vort2flux) the output; both are step
+  // scratch, overwritten by the next step. The solver's stream does not synchronise with
+  // the null stream, so every host copy waits for it explicitly.
+  HIPCHK(hipStreamSynchronize(s->st));
+  HIPCHK(hipMemcpy(s->rhs, in, need * 8, hipMemcpyHostToDevice));
+  if (op == IBPM_OP_VORT2FLUX) {
+    double* Gc[1] = {s->rhs};
+    double* pc[1] = {s->Gs};
+    int rc = vort2flux(s, 1, Gc, pc, nullptr, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s->st));
+    HIPCHK(hipMemcpy(out, s->Gs, need * 8, hipMemcpyDeviceToHost));
+    return IBPM_OK;
+  }
+  SolveJob j;
+  j.b = s->rhs;
+  j.x = s->rhs + P;
+  j.epi = EPI_STORE;
+  j.base = nullptr;
+  if (op == IBPM_OP_POISSON) {
+    j.da = 0.0;
+    j.dnum = 1.0;
+    j.dden = 1.0;
+    j.a = 0.0;
+    j.c = 1.0;
+  } else {
+    const double hc = s->hc[level - 1];
+    j.da = 1.0;
+    j.dnum = s->dt;
+    j.dden = 2.0 * s->Re * (hc * hc);
+    j.a = 1.0;
+    j.c = cA(s, level - 1);
+  }
+  if (L < 2) return fail(IBPM_E_ARG, "operator access needs nlevel >= 2 (scratch planes)");
+  int rc = solve(s, &j, 1);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(s->st));
+  HIPCHK(hipMemcpy(out, s->rhs + P, need * 8, hipMemcpyDeviceToHost));
+  return IBPM_OK;
+}
+
 int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, double* flops,
                      double* bytes) {
   if (!s || reps < 1 || !avg_ms || !flops || !bytes) return fail(IBPM_E_ARG, "bad arguments");
@@ -1036,7 +1200,7 @@ int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, doubl
         pb.p[i].in = s->rhs + i * P;
         pb.p[i].out = s->T + i * P;
       }
-      return run_pass(s, pb, L, true, EPI_STORE, false, false, mixed);
+      return run_pass(s, pb, L, true, EPI_STORE, false, mixed);
     }
     HIPCHK(launch_nonlin_rhs(na, s->st));
     return IBPM_OK;

@@ -34,22 +34,19 @@ inline GridDims make_dims(int nx, int ny) {
 // One plane of one sine-contraction pass.
 //   in  : Nk x Nb, k contiguous (in[k + b*Nk])
 //   out : Nb x Nk, b contiguous (out[b + j*Nb])
-// Optional operands:
-//   diag : multiply the result elementwise (same layout as out)
-//   base : epilogue operand of EPI_SUB / EPI_ADD
-//   ring : coarse-level plane whose ring supplies boundary values added to the input
-//          (apply_bc with fac = 1, first pass of a vort2flux solve only)
+// EPI_DIAG multiplies by the spectral diagonal D = dscale / (da + lambda*dnum/dden) with
+// lambda = -2((cos_b[b] + cos_j[j]) - 2) (the reference's _lap_eigs, fluid.jl:168-173);
+// EPI_SUB / EPI_ADD combine with `base`.
 struct PassPlane {
   const void* in;
   void* out;
-  const double* diag;
   const double* base;
-  const double* ring;
-  const void* So;  // odd-row sine table of length-Nk transform
-  const void* Se;  // even-row sine table
+  const double* cosb;  // cos(pi k / n) along the output's b direction
+  const double* cosj;  // along the transformed direction
+  const void* So;      // odd-row sine table of the length-Nk transform
+  const void* Se;      // even-row sine table
+  double dscale, da, dnum, dden;
   int Nk, Nb;
-  int nx, ny;      // grid of the ring (for apply_bc index maps)
-  int pad0, pad1;
 };
 
 enum Epilogue : int {
@@ -65,13 +62,43 @@ enum Precision : int { PREC_F64 = 0, PREC_MIXED = 1 };
 constexpr int kMaxPlanes = 16;
 struct PassBatch {
   PassPlane p[kMaxPlanes];
+  int nbt, nmt, nplanes;  // tile grid (filled by the launcher)
 };
 
 // Host launchers (dst_kernels.hip).
 hipError_t launch_dst_pass_f64(const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
-                               bool ring, hipStream_t st);
+                               hipStream_t st);
 hipError_t launch_dst_pass_mixed(const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
-                                 bool ring, bool in_f32, bool out_f32, hipStream_t st);
+                                 bool in_f32, bool out_f32, hipStream_t st);
+
+// ---- multi-level ring chain (ring_kernels.hip) ----
+constexpr int kMaxChains = 16;
+struct RingChainArgs {
+  int nx, ny, Nx, Ny;
+  void* F[kMaxChains];              // spectral plane of the level, per chain (E in place)
+  int F_f32;
+  const double* cosx;               // cos(pi kx / nx), kx = 1..Nx
+  const double* cosy;
+  double scale;                     // 1 / (4 nx ny)
+  double* lv[kMaxChains];           // streamfunction on the 4 ring lines (4 x lvstride)
+  int lvstride;
+  const double* fac_in[kMaxChains]; // ring factors of this level (null: none)
+  double* fac_out[kMaxChains];      // ring factors of the next finer level
+  double* part_a[kMaxChains];
+  double* part_c[kMaxChains];
+  const double* edge;               // sx1, sxN, sy1, syN
+  const double* pw;                 // S_y[:, ny/4], S_y[:, 3ny/4], S_x[nx/4, :], S_x[3nx/4, :]
+  const double* SoX;                // folded FP64 sine tables (see dst_kernels.hip)
+  const double* SeX;
+  const double* SoY;
+  const double* SeY;
+};
+// E = D . (F + R) in place, plus (do_proj) the ring projections of the level.
+hipError_t launch_ring_project(const RingChainArgs& a, int nchain, bool do_proj, hipStream_t st);
+// project + ring line values + factors of the next finer level.
+hipError_t launch_ring_step(const RingChainArgs& a, int nchain, hipStream_t st);
+size_t ring_partial_count(int Nx, int Ny);
+size_t ring_partial_a_count(int Nx, int Ny);
 
 // Sine tables for a DST-I of length N (N+1 even). Host builds, returns element counts.
 void build_sine_tables_f64(int N, double* So, double* Se);

@@ -28,7 +28,11 @@ FIELDS = {"q": 0, "q0": 1, "Gamma": 2, "psi": 3, "nonlin1": 4, "nonlin2": 5, "xb
 # every symbol include/ibpm.h declares
 EXPORTS = ("ibpm_default_options", "ibpm_create", "ibpm_advance", "ibpm_read", "ibpm_write",
            "ibpm_field_size", "ibpm_get_index", "ibpm_set_index", "ibpm_get_time",
-           "ibpm_synchronize", "ibpm_last_error", "ibpm_destroy", "ibpm_time_kernel")
+           "ibpm_synchronize", "ibpm_last_error", "ibpm_destroy", "ibpm_time_kernel", "ibpm_apply")
+
+IBPM_OP_POISSON = 0
+IBPM_OP_HELMHOLTZ = 1
+IBPM_OP_VORT2FLUX = 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -97,6 +101,8 @@ def load():
     L.ibpm_destroy.restype = None
     L.ibpm_time_kernel.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
     L.ibpm_time_kernel.restype = C.c_int
+    L.ibpm_apply.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_size_t]
+    L.ibpm_apply.restype = C.c_int
     _lib = L
     return L
 

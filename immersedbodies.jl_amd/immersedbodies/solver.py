@@ -146,6 +146,16 @@ class HipCnabSolver(AbstractSolver):
         a = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(-1))
         _lib.check(L.ibpm_write(self._h, f, _dp(a), a.size))
 
+    def apply(self, op: str, x, level: int = 1) -> np.ndarray:
+        """Apply one building block of the step on the GPU: "poisson" (Delta_inv),
+        "helmholtz" (Ainv of a 1-based level) or "vort2flux" (psi of all levels)."""
+        code = {"poisson": _lib.IBPM_OP_POISSON, "helmholtz": _lib.IBPM_OP_HELMHOLTZ,
+                "vort2flux": _lib.IBPM_OP_VORT2FLUX}[op]
+        a = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1))
+        out = np.zeros_like(a)
+        _lib.check(_lib.load().ibpm_apply(self._h, code, int(level), _dp(a), _dp(out), a.size))
+        return out
+
     def sync_state(self) -> State:
         st = self.state
         g = st.prob.fluid.grid

@@ -118,6 +118,16 @@ double ibpm_get_time(const ibpm_solver* s);
 /* Wait for queued work of the solver's stream. */
 int ibpm_synchronize(ibpm_solver* s);
 
+/* Operator access for checking the step's building blocks one at a time (used by the
+ * parity tests), in the solver's precision:
+ *   IBPM_OP_POISSON     Delta_inv_operator, one plane (fluid.jl:117-119, :145-166)
+ *   IBPM_OP_HELMHOLTZ   Ainv[level] of A_operators (fluid.jl:121-143), level is 1-based
+ *   IBPM_OP_VORT2FLUX   streamfunction of vort2flux (fluid.jl:175-206): in = Gamma
+ *                       (nG x nlevel), out = psi (nG x nlevel); in is not modified
+ * n is the number of doubles of in and out. */
+enum ibpm_op { IBPM_OP_POISSON = 0, IBPM_OP_HELMHOLTZ = 1, IBPM_OP_VORT2FLUX = 2 };
+int ibpm_apply(ibpm_solver* s, int op, int level, const double* in, double* out, size_t n);
+
 /* Instrumentation, no counterpart in the reference: time `reps` launches of one kernel of
  * the step with HIP events on the solver's stream, on the solver's current fields.
  *   kernel 0: sine-contraction pass of the batched implicit solve (all levels at once)
