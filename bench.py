@@ -96,7 +96,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
-    ap.add_argument("--refine", type=int, default=1)
+    ap.add_argument("--refine", type=int, default=2)
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")

@@ -727,7 +727,7 @@ void ibpm_default_options(ibpm_options* o) {
   if (!o) return;
   memset(o, 0, sizeof *o);
   o->precision = IBPM_FP64;
-  o->refine_sweeps = 1;
+  o->refine_sweeps = 2;
   o->cg_maxiter = 5000;
   o->cg_reltol = 1e-12;
   o->cg_abstol = 0.0;

@@ -44,7 +44,7 @@ class HipCnabSolver(AbstractSolver):
       device: HIP device ordinal
     """
 
-    def __init__(self, state: State, *, precision="fp64", refine_sweeps=1, cg=None,
+    def __init__(self, state: State, *, precision="fp64", refine_sweeps=2, cg=None,
                  cg_tol_mode=0, device=0):
         L = _lib.load()
         self.state = state
