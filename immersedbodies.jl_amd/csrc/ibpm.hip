@@ -29,6 +29,17 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+}  // namespace
+
+namespace ibpm {
+int set_error(int code, const char* msg) {
+  g_err = msg;
+  return code;
+}
+}  // namespace ibpm
+
+namespace {
+
 #define HIPCHK(expr)                                                                  \
   do {                                                                                \
     hipError_t e_ = (expr);                                                           \
@@ -95,7 +106,13 @@ struct ibpm_solver {
   double *Gtmp = nullptr, *psitmp = nullptr, *Sc = nullptr;
   // CG
   double *cg_r = nullptr, *cg_u = nullptr, *cg_c = nullptr, *cg_scal = nullptr;
-  double* h_scal = nullptr;  // pinned
+  double* h_scal = nullptr;  // pinned: [0..5] CG scalars, [8..10] warm-start scalars
+  // CG warm start: extrapolation history F~b of the two steps before the previous one,
+  // the extrapolated guess x_e, B x_e, and (rms(r_e), ||r_e||, ||b||)
+  int ws_mode = IBPM_WS_PREVIOUS, ws_record = 0, nhist = 0;
+  ibpm_net* net = nullptr;
+  double *ws_h0 = nullptr, *ws_h1 = nullptr, *ws_xe = nullptr, *ws_Bxe = nullptr;
+  double* ws_scal = nullptr;
   // host copies
   std::vector<double> ds;
   std::vector<double> q0_shadow;
@@ -113,7 +130,8 @@ void free_all(ibpm_solver* s) {
                   s->xb, s->ub, s->xb_ref, s->Ft, s->fb, s->Ftmp, s->total, s->motion,
                   s->body_of, s->moving_d, s->coords, s->perm, s->xstart, s->err, s->w,
                   s->Binv, s->Gtmp, s->psitmp, s->Sc, s->cg_r, s->cg_u, s->cg_c, s->cg_scal,
-                  s->edge, s->pw, s->fac, s->rpart, s->rlv};
+                  s->edge, s->pw, s->fac, s->rpart, s->rlv, s->ws_h0, s->ws_h1, s->ws_xe,
+                  s->ws_Bxe, s->ws_scal};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->h_scal) (void)hipHostFree(s->h_scal);
@@ -569,9 +587,11 @@ int couple_surface(ibpm_solver* s, ibpm_step_stats* stats) {
     }
     return IBPM_OK;
   }
-  // cg! with x = Ft as the initial guess (coupling.jl:194-202, solver.jl:137)
+  // cg! (coupling.jl:194-202, solver.jl:137). The reference starts from the previous F~b;
+  // the warm-start modes only change that initial guess (ibpm_set_warm_start).
+  const int n2 = 2 * s->n;
   CgDev c;
-  c.n = 2 * s->n;
+  c.n = n2;
   c.x = s->Ft;
   c.r = s->cg_r;
   c.u = s->cg_u;
@@ -582,15 +602,74 @@ int couple_surface(ibpm_solver* s, ibpm_step_stats* stats) {
   c.reltol = s->cg_reltol;
   c.abstol = s->cg_abstol;
   c.tol_mode = s->cg_tol_mode;
-  const double* zc[1] = {s->Ft};
   double* xc[1] = {s->cg_c};
-  int rc = apply_B(s, 1, zc, xc);
+  int rc = IBPM_OK;
+  int bapp = 0;
+  const bool use_ws = s->ws_mode != IBPM_WS_PREVIOUS || s->ws_record;
+  const bool need_res = s->ws_mode == IBPM_WS_LEARNED || s->ws_record;
+  WsArgs wa;
+  memset(&wa, 0, sizeof wa);
+  if (use_ws) {
+    // x_e from F~b of the last three steps; then shift the history
+    HIPCHK(launch_ws_extrap(n2, s->Ft, s->ws_h0, s->ws_h1, s->nhist, s->ws_xe, s->st));
+    std::swap(s->ws_h0, s->ws_h1);
+    HIPCHK(launch_copy(s->ws_h0, s->Ft, n2, s->st));
+    s->nhist = std::min(s->nhist + 1, 3);
+    if (need_res) {
+      const double* ze[1] = {s->ws_xe};
+      double* xe[1] = {s->ws_Bxe};
+      rc = apply_B(s, 1, ze, xe);
+      if (rc) return rc;
+      ++bapp;
+      wa.n = s->n;
+      wa.b = s->Ftmp;
+      wa.Bxe = s->ws_Bxe;
+      wa.xb = s->xb;
+      wa.ub = s->ub;
+      wa.body_of = s->body_of;
+      wa.x0 = s->x0;
+      wa.y0 = s->y0;
+      wa.h = s->h;
+      wa.xe = s->ws_xe;
+      wa.F = s->Ft;
+      wa.scal = s->ws_scal;
+      rc = net_features(s->net, wa, s->st);
+      if (rc) return rc;
+    }
+    if (s->ws_mode == IBPM_WS_LEARNED) {
+      rc = net_predict(s->net, wa, s->Ft, s->st);
+      if (rc) return rc;
+    } else if (s->ws_mode == IBPM_WS_EXTRAPOLATE) {
+      HIPCHK(launch_copy(s->Ft, s->ws_xe, n2, s->st));
+    }
+  }
+  const double* zc[1] = {s->Ft};
+  rc = apply_B(s, 1, zc, xc);
   if (rc) return rc;
+  ++bapp;
   HIPCHK(launch_cg_init(c, s->st));
   HIPCHK(hipMemcpyAsync(s->h_scal, s->cg_scal, 6 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  if (need_res)
+    HIPCHK(hipMemcpyAsync(s->h_scal + 8, s->ws_scal, 3 * sizeof(double), hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
   rc = check_body_error(s);
   if (rc) return rc;
+  int ws_used = 0;
+  double base_res = need_res ? s->h_scal[9] : 0.0;
+  if (s->ws_mode == IBPM_WS_LEARNED) {
+    // keep the network's guess only when it beats x_e; otherwise restart from x_e, whose
+    // B x_e is already known
+    ws_used = s->h_scal[0] <= base_res ? 1 : 0;
+    if (!ws_used) {
+      HIPCHK(launch_copy(s->Ft, s->ws_xe, n2, s->st));
+      HIPCHK(launch_copy(s->cg_c, s->ws_Bxe, n2, s->st));
+      HIPCHK(launch_cg_init(c, s->st));
+      HIPCHK(hipMemcpyAsync(s->h_scal, s->cg_scal, 6 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+      HIPCHK(hipStreamSynchronize(s->st));
+    }
+  }
+  const double bnorm = s->h_scal[5];
+  const double res0 = s->h_scal[0];
   while (s->h_scal[4] == 0.0) {
     const double* zu[1] = {s->cg_u};
     rc = apply_B(s, 1, zu, xc);
@@ -599,10 +678,18 @@ int couple_surface(ibpm_solver* s, ibpm_step_stats* stats) {
     HIPCHK(hipMemcpyAsync(s->h_scal, s->cg_scal, 6 * sizeof(double), hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
   }
+  if (s->ws_record) {
+    rc = net_record(s->net, wa, s->st);
+    if (rc) return rc;
+  }
   if (stats) {
     stats->cg_iters = (int)s->h_scal[3];
     stats->cg_residual = s->h_scal[0];
     stats->cg_converged = s->h_scal[0] <= s->h_scal[2] ? 1 : 0;
+    stats->b_applies = bapp + stats->cg_iters;
+    stats->ws_used = ws_used;
+    stats->ws_base_residual = bnorm > 0.0 ? base_res / bnorm : 0.0;
+    stats->ws_residual = bnorm > 0.0 ? res0 / bnorm : 0.0;
   }
   return IBPM_OK;
 }
@@ -898,7 +985,7 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   CK(dalloc(&s->err, 1));
   CK(dalloc(&s->w, 98 * (size_t)n));
   CK(dalloc(&s->cg_scal, 8));
-  CK(hipHostMalloc((void**)&s->h_scal, 8 * sizeof(double), hipHostMallocDefault));
+  CK(hipHostMalloc((void**)&s->h_scal, 16 * sizeof(double), hipHostMallocDefault));
   // allocations hold max(count, 1) elements (dalloc)
   auto zero_bytes = [](size_t count, size_t elem) { return std::max<size_t>(count, 1) * elem; };
   CK(hipMemsetAsync(s->Ft, 0, zero_bytes(2 * (size_t)n, 8), s->st));
@@ -1000,6 +1087,11 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
       CK(dalloc(&s->cg_r, 2 * (size_t)n));
       CK(dalloc(&s->cg_u, 2 * (size_t)n));
       CK(dalloc(&s->cg_c, 2 * (size_t)n));
+      CK(dalloc(&s->ws_h0, 2 * (size_t)n));
+      CK(dalloc(&s->ws_h1, 2 * (size_t)n));
+      CK(dalloc(&s->ws_xe, 2 * (size_t)n));
+      CK(dalloc(&s->ws_Bxe, 2 * (size_t)n));
+      CK(dalloc(&s->ws_scal, 4));
     }
   }
   CK(hipStreamSynchronize(s->st));
@@ -1015,6 +1107,7 @@ int ibpm_advance(ibpm_solver* s, int nsteps, const double* freestream, const dou
   HIPCHK(hipSetDevice(s->device));
   for (int k = 0; k < nsteps; ++k) {
     ibpm_step_stats* sk = stats ? stats + k : nullptr;
+    if (sk) memset(sk, 0, sizeof *sk);
     int rc = step(s, freestream ? freestream + 2 * k : nullptr,
                   motion ? motion + (size_t)6 * s->nbody * k : nullptr, sk);
     if (rc) return rc;
@@ -1112,6 +1205,7 @@ int ibpm_write(ibpm_solver* s, int field, const double* src, size_t n) {
       return fail(IBPM_E_ARG, "total force is derived from fb");
   }
   HIPCHK(hipMemcpy(field_ptr(s, field), src, need * 8, hipMemcpyHostToDevice));
+  if (field == IBPM_FTILDE) s->nhist = 0;  // the extrapolation history no longer applies
   return IBPM_OK;
 }
 
@@ -1120,6 +1214,23 @@ int ibpm_get_index(const ibpm_solver* s) { return s ? s->index : 0; }
 int ibpm_set_index(ibpm_solver* s, int index) {
   if (!s) return fail(IBPM_E_ARG, "null solver");
   s->index = index;
+  s->nhist = 0;
+  return IBPM_OK;
+}
+
+int ibpm_set_warm_start(ibpm_solver* s, int mode, ibpm_net* net, int record) {
+  if (!s) return fail(IBPM_E_ARG, "null solver");
+  if (mode < IBPM_WS_PREVIOUS || mode > IBPM_WS_LEARNED) return fail(IBPM_E_ARG, "unknown warm-start mode");
+  if ((mode == IBPM_WS_LEARNED || record) && !net)
+    return fail(IBPM_E_ARG, "IBPM_WS_LEARNED and recording need a network");
+  if (net && net_device(net) != s->device)
+    return fail(IBPM_E_ARG, "the network lives on another device than the solver");
+  if (net && s->n > net_max_points(net))
+    return fail(IBPM_E_ARG, "the solver has more body points than the network's max_points");
+  s->ws_mode = mode;
+  s->ws_record = record ? 1 : 0;
+  s->net = net;
+  s->nhist = 0;
   return IBPM_OK;
 }
 

@@ -11,7 +11,12 @@
 #include <stddef.h>
 #include <stdint.h>
 
+struct ibpm_net;  // warm-start network (include/ibpm.h)
+
 namespace ibpm {
+
+// Sets the thread-local message returned by ibpm_last_error() and returns `code`.
+int set_error(int code, const char* msg);
 
 // Geometry of one multi-domain grid (same for all its levels).
 struct GridDims {
@@ -229,5 +234,102 @@ hipError_t launch_cg_step(const CgDev& c, hipStream_t st);
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t st);
 hipError_t launch_copy(double* dst, const double* src, size_t n, hipStream_t st);
 hipError_t launch_axpy_f32(double* x, const float* y, size_t n, hipStream_t st);
+
+// ---- warm-start network (net_kernels.hip, net.hip) ----
+constexpr int kNetFeatures = 10;  // per-point input channels
+constexpr int kNetInPad = 32;     // input channels as stored (MFMA K granularity)
+constexpr int kNetOutPad = 32;    // output channels as computed (2 used)
+
+// Y (Rpad x Cout) = conv(X (Rpad x Cin), W (Cout x K x Cin)) over rows of equal seg.
+struct ConvArgs {
+  const uint16_t* X;  // bf16
+  int Cin;
+  const uint16_t* W;  // bf16
+  int Cout;
+  int K;
+  const float* bias;      // Cout (CONV_RELU, CONV_LINEAR_F32)
+  const int* seg;         // R sequence ids
+  int R, Rpad;            // rows, rows rounded up to 64 (pad rows are written as 0)
+  void* Y;                // bf16, or float for CONV_LINEAR_F32
+  const uint16_t* mask;   // CONV_MASK: Rpad x Cout bf16, keep where > 0
+};
+enum ConvEpi : int { CONV_RELU = 0, CONV_LINEAR_F32 = 1, CONV_MASK = 2 };
+hipError_t launch_conv(const ConvArgs& a, int epi, hipStream_t st);
+
+// dW (Cout x K x Cin) and db (Cout), stored contiguously in `out`, from dY and X.
+struct WgradArgs {
+  const uint16_t* dY;
+  int Cout;
+  const uint16_t* X;
+  int Cin;
+  const int* seg;
+  int R, K;
+  int rows_per_chunk;  // multiple of 32
+  float* part;         // chunks x (Cout*K*Cin + Cout)
+};
+int wgrad_chunks(int R, int rows_per_chunk);
+hipError_t launch_wgrad(const WgradArgs& a, float* out, hipStream_t st);
+
+int mse_blocks(int Rpad, int Cp);
+hipError_t launch_mse(const float* y, int Cp, const float* tgt, int R, int Rpad, uint16_t* dY,
+                      double* partial, double* loss_out, hipStream_t st);
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, size_t n, float lr, float b1,
+                       float b2, float eps, int t, hipStream_t st);
+hipError_t launch_cast_weights(const float* w, int Cout, int K, int Cin, uint16_t* wb,
+                               uint16_t* wt, hipStream_t st);
+
+struct GatherBatchArgs {
+  const float* feat;   // training set: capacity x stride x kNetFeatures
+  const float* tgt;    // capacity x stride x 2
+  const int* body;     // capacity x stride
+  const int* counts;   // rows per sample
+  int stride;          // rows reserved per sample (max_points)
+  const int* idx;      // S sample indices
+  const int* off;      // S+1 row offsets
+  int S, R, Rpad;
+  float inv_scale;     // 1 / output scale
+  uint16_t* A0;        // Rpad x kNetInPad
+  float* tgt_out;      // R x 2
+  int* seg;            // Rpad
+};
+hipError_t launch_gather_batch(const GatherBatchArgs& a, int max_points, hipStream_t st);
+hipError_t launch_pack_rows(const float* x, const int* seg_in, int R, int Rpad, uint16_t* A0,
+                            int* seg, hipStream_t st);
+hipError_t launch_sumsq(const float* x, size_t n, double* partial, int nblocks, hipStream_t st);
+
+// Solver side of the warm start.
+struct WsArgs {
+  int n;                  // body points
+  const double* b;        // CG right-hand side (2n)
+  const double* Bxe;      // B x_e (2n)
+  const double* xb;       // positions (2n)
+  const double* ub;       // velocities (2n)
+  const int* body_of;     // n
+  double x0, y0, h;       // level-1 origin and step
+  const double* xe;       // extrapolated guess
+  const double* F;        // converged F~b (record)
+  double* scal;           // [0] rms(r_e), [1] ||r_e||, [2] ||b||
+  float* feat;            // n x kNetFeatures (written)
+  uint16_t* A0;           // Rpad x kNetInPad (written)
+  int* seg;               // Rpad (written)
+  int Rpad;
+};
+hipError_t launch_ws_extrap(int n2, const double* F1, const double* F2, const double* F3, int nhist,
+                            double* xe, hipStream_t st);
+hipError_t launch_ws_features(const WsArgs& a, hipStream_t st);
+hipError_t launch_ws_apply(int n, const double* xe, const float* y, int Cp, const double* scal,
+                           float out_scale, double* x0, hipStream_t st);
+hipError_t launch_ws_record(const WsArgs& a, int stride, float* ds_feat, float* ds_tgt, int* ds_body,
+                            hipStream_t st);
+
+// Network entry points used by a solver step (net.hip); all run on the solver's stream.
+int net_device(const ::ibpm_net* net);
+int net_max_points(const ::ibpm_net* net);
+// Features of the step into the network's input rows (fills a.feat, a.A0, a.seg, a.Rpad).
+int net_features(::ibpm_net* net, WsArgs& a, hipStream_t st);
+// x0 = x_e + rms(r_e) * scale * net(features) after net_features.
+int net_predict(::ibpm_net* net, const WsArgs& a, double* x0, hipStream_t st);
+// Append (features, (F - x_e) / rms(r_e)) to the training set after net_features.
+int net_record(::ibpm_net* net, const WsArgs& a, hipStream_t st);
 
 }  // namespace ibpm

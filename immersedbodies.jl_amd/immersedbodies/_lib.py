@@ -28,7 +28,17 @@ FIELDS = {"q": 0, "q0": 1, "Gamma": 2, "psi": 3, "nonlin1": 4, "nonlin2": 5, "xb
 # every symbol include/ibpm.h declares
 EXPORTS = ("ibpm_default_options", "ibpm_create", "ibpm_advance", "ibpm_read", "ibpm_write",
            "ibpm_field_size", "ibpm_get_index", "ibpm_set_index", "ibpm_get_time",
-           "ibpm_synchronize", "ibpm_last_error", "ibpm_destroy", "ibpm_time_kernel", "ibpm_apply")
+           "ibpm_synchronize", "ibpm_last_error", "ibpm_destroy", "ibpm_time_kernel", "ibpm_apply",
+           "ibpm_set_warm_start", "ibpm_net_default_config", "ibpm_net_create",
+           "ibpm_net_destroy", "ibpm_net_dataset_size", "ibpm_net_clear_dataset",
+           "ibpm_net_train", "ibpm_net_param_count", "ibpm_net_get_params",
+           "ibpm_net_set_params", "ibpm_net_forward", "ibpm_net_gradients", "ibpm_net_step")
+
+IBPM_WS_PREVIOUS = 0
+IBPM_WS_EXTRAPOLATE = 1
+IBPM_WS_LEARNED = 2
+NET_FEATURES = 10
+NET_OUTPUTS = 2
 
 IBPM_OP_POISSON = 0
 IBPM_OP_HELMHOLTZ = 1
@@ -54,7 +64,14 @@ class Options(C.Structure):
 
 class StepStats(C.Structure):
     _fields_ = [("cg_iters", C.c_int), ("cg_converged", C.c_int), ("cg_residual", C.c_double),
-                ("force", C.c_double * 2)]
+                ("force", C.c_double * 2), ("b_applies", C.c_int), ("ws_used", C.c_int),
+                ("ws_base_residual", C.c_double), ("ws_residual", C.c_double)]
+
+
+class NetConfig(C.Structure):
+    _fields_ = [("channels", C.c_int), ("layers", C.c_int), ("kernel", C.c_int),
+                ("max_points", C.c_int), ("capacity", C.c_int), ("max_batch", C.c_int),
+                ("seed", C.c_ulonglong), ("device", C.c_int)]
 
 
 class IbpmError(RuntimeError):
@@ -103,6 +120,33 @@ def load():
     L.ibpm_time_kernel.restype = C.c_int
     L.ibpm_apply.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_size_t]
     L.ibpm_apply.restype = C.c_int
+    _fp = C.POINTER(C.c_float)
+    L.ibpm_set_warm_start.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+    L.ibpm_set_warm_start.restype = C.c_int
+    L.ibpm_net_default_config.argtypes = [C.POINTER(NetConfig)]
+    L.ibpm_net_default_config.restype = None
+    L.ibpm_net_create.argtypes = [C.POINTER(NetConfig), C.POINTER(C.c_void_p)]
+    L.ibpm_net_create.restype = C.c_int
+    L.ibpm_net_destroy.argtypes = [C.c_void_p]
+    L.ibpm_net_destroy.restype = None
+    L.ibpm_net_dataset_size.argtypes = [C.c_void_p]
+    L.ibpm_net_dataset_size.restype = C.c_int
+    L.ibpm_net_clear_dataset.argtypes = [C.c_void_p]
+    L.ibpm_net_clear_dataset.restype = C.c_int
+    L.ibpm_net_train.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp]
+    L.ibpm_net_train.restype = C.c_int
+    L.ibpm_net_param_count.argtypes = [C.c_void_p]
+    L.ibpm_net_param_count.restype = C.c_size_t
+    L.ibpm_net_get_params.argtypes = [C.c_void_p, _fp, C.c_size_t]
+    L.ibpm_net_get_params.restype = C.c_int
+    L.ibpm_net_set_params.argtypes = [C.c_void_p, _fp, C.c_size_t]
+    L.ibpm_net_set_params.restype = C.c_int
+    L.ibpm_net_forward.argtypes = [C.c_void_p, _fp, _ip, C.c_int, _fp]
+    L.ibpm_net_forward.restype = C.c_int
+    L.ibpm_net_gradients.argtypes = [C.c_void_p, _fp, _ip, _fp, C.c_int, _fp, C.c_size_t, _dp]
+    L.ibpm_net_gradients.restype = C.c_int
+    L.ibpm_net_step.argtypes = [C.c_void_p, _fp, _ip, _fp, C.c_int, C.c_double, _dp]
+    L.ibpm_net_step.restype = C.c_int
     _lib = L
     return L
 

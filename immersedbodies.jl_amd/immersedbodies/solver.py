@@ -131,6 +131,22 @@ class HipCnabSolver(AbstractSolver):
         stats = self.last_stats if stats is None else stats
         return np.array([s.cg_iters for s in stats])
 
+    def b_applies(self, stats=None):
+        stats = self.last_stats if stats is None else stats
+        return np.array([s.b_applies for s in stats])
+
+    def set_warm_start(self, mode: str = "previous", net=None, record: bool = False):
+        """Initial guess of the moving-body CG: "previous" (the reference's previous F̃b),
+        "extrapolate" (quadratic extrapolation of the last three F̃b) or "learned"
+        (extrapolation corrected by ``net``, a ``WarmStartNet``). ``record`` appends one
+        training sample per step to ``net``."""
+        from .warmstart import MODES
+
+        self._net = net  # keep the network alive while the solver uses it
+        _lib.check(_lib.load().ibpm_set_warm_start(self._h, MODES[mode],
+                                                   net.handle if net is not None else None,
+                                                   int(bool(record))))
+
     # -- state exchange --
     def read(self, name: str) -> np.ndarray:
         L = _lib.load()

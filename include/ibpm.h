@@ -93,6 +93,12 @@ typedef struct ibpm_step_stats {
   int cg_converged;    /* 1 when the CG stopping test was met                    */
   double cg_residual;  /* final CG residual norm                                 */
   double force[2];     /* boundary_total_force after the step                    */
+  int b_applies;       /* applications of the B operator in the step (CG initial
+                          residual + iterations + warm-start residual)            */
+  int ws_used;         /* IBPM_WS_LEARNED: 1 when the network's guess was kept    */
+  double ws_base_residual;  /* ||b - B x_e|| / ||b|| of the extrapolated guess x_e
+                               (0 when not computed)                              */
+  double ws_residual;  /* ||b - B x0|| / ||b|| of the guess CG started from       */
 } ibpm_step_stats;
 
 /* Defaults of the reference: FP64, cg maxiter 5000, reltol 1e-12. */
@@ -139,6 +145,89 @@ int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, doubl
 
 const char* ibpm_last_error(void);
 void ibpm_destroy(ibpm_solver* s);
+
+/* ---- Warm start of the CG surface-force solve of moving bodies ----
+ *
+ * No counterpart in the reference, which starts cg! from the previous step's F̃b
+ * (coupling.jl:194-202, :246-247). The modes below replace only the initial guess: CG
+ * still runs to the same stopping test, so forces agree with IBPM_WS_PREVIOUS to the CG
+ * tolerance whatever the guess.
+ *
+ *   IBPM_WS_PREVIOUS     x0 = F̃b of the previous step (the reference)
+ *   IBPM_WS_EXTRAPOLATE  x0 = x_e, the quadratic extrapolation 3F1 - 3F2 + F3 of the last
+ *                        three steps' F̃b (linear, then previous, while fewer exist)
+ *   IBPM_WS_LEARNED      x0 = x_e + s * net(features), where s = rms(r_e), r_e = b - B x_e
+ *                        (one extra application of B), and the network (ibpm_net) maps
+ *                        per-point features to the correction (F̃b - x_e) / s. When
+ *                        ||b - B x0|| > ||r_e|| (or is not finite) CG starts from x_e
+ *                        instead, without another application of B.
+ *
+ * Per-point features, in this channel order (each body is one sequence, in panel order):
+ *   0,1  r_e (x, y) / rms(r_e)
+ *   2,3  sin, cos of 2 pi (x - x0) / h       4,5  sin, cos of 2 pi (y - y0) / h
+ *   6,7  unit tangent of the body at the point (central differences, one-sided at ends)
+ *   8,9  ub (x, y) / rms(ub)  (0 when the body is at rest)
+ */
+enum ibpm_warm_start { IBPM_WS_PREVIOUS = 0, IBPM_WS_EXTRAPOLATE = 1, IBPM_WS_LEARNED = 2 };
+#define IBPM_NET_FEATURES 10
+#define IBPM_NET_OUTPUTS 2
+
+typedef struct ibpm_net ibpm_net;
+
+/* 1-D convolutional network over body points: `layers` convolutions of odd width
+ * `kernel` with zero padding, IBPM_NET_FEATURES -> channels -> ... -> IBPM_NET_OUTPUTS,
+ * ReLU between layers. BF16 weights and activations, FP32 accumulation on MFMA, FP32
+ * master weights trained with Adam. */
+typedef struct ibpm_net_config {
+  int channels;        /* hidden channels, a multiple of 64 (default 128)       */
+  int layers;          /* >= 2 (default 4)                                      */
+  int kernel;          /* 3, 5 or 7 (default 5)                                 */
+  int max_points;      /* largest body-point count of one sample (default 1024) */
+  int capacity;        /* samples the training set holds (default 20000); once
+                          full, new samples overwrite the oldest                */
+  int max_batch;       /* largest training minibatch in samples (default 64)    */
+  unsigned long long seed;  /* weight initialisation and minibatch sampling      */
+  int device;
+} ibpm_net_config;
+
+void ibpm_net_default_config(ibpm_net_config* c);
+int ibpm_net_create(const ibpm_net_config* c, ibpm_net** out);
+void ibpm_net_destroy(ibpm_net* net);
+
+/* Select the warm start of a solver. net: required for IBPM_WS_LEARNED or record != 0,
+ * and must outlive its use by the solver. record != 0 appends one training sample
+ * (features, (F̃b - x_e) / rms(r_e)) per step with moving bodies to the net's training
+ * set; it computes r_e (one extra application of B) in every mode. */
+int ibpm_set_warm_start(ibpm_solver* s, int mode, ibpm_net* net, int record);
+
+int ibpm_net_dataset_size(const ibpm_net* net);  /* samples held */
+int ibpm_net_clear_dataset(ibpm_net* net);
+
+/* `iters` Adam steps (betas 0.9, 0.999, eps 1e-8) on minibatches of `batch` samples drawn
+ * uniformly from the training set, learning rate decaying from lr to lr/10 along a
+ * cosine; loss of the mean squared error of the normalised output. The output scale
+ * (rms of the training targets) is fixed at the first call. loss: iters entries or NULL. */
+int ibpm_net_train(ibpm_net* net, int iters, int batch, double lr, double* loss);
+
+/* Parameters as float32, per layer the weight [cout][cin][kernel] then the bias [cout],
+ * followed by the output scale (one value). */
+size_t ibpm_net_param_count(const ibpm_net* net);
+int ibpm_net_get_params(const ibpm_net* net, float* p, size_t n);
+/* Also resets the Adam state (moments and step count). */
+int ibpm_net_set_params(ibpm_net* net, const float* p, size_t n);
+
+/* Network on host rows (at most max_batch * max_points): x is rows x IBPM_NET_FEATURES,
+ * seg gives each row's sequence id (rows of one sequence are consecutive; ids >= 0),
+ * y is rows x IBPM_NET_OUTPUTS in normalised
+ * units (multiply by the output scale for (F̃b - x_e) / rms(r_e)). */
+int ibpm_net_forward(ibpm_net* net, const float* x, const int* seg, int rows, float* y);
+/* Loss mean((y - target)^2) over rows x IBPM_NET_OUTPUTS and its gradient with respect to
+ * the parameters (layout of ibpm_net_get_params; the output-scale entry is 0). */
+int ibpm_net_gradients(ibpm_net* net, const float* x, const int* seg, const float* target,
+                       int rows, float* grad, size_t n, double* loss);
+/* One Adam step (count of steps taken so far + 1) at learning rate lr on the given rows. */
+int ibpm_net_step(ibpm_net* net, const float* x, const int* seg, const float* target, int rows,
+                  double lr, double* loss);
 
 #ifdef __cplusplus
 }
