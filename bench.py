@@ -8,7 +8,8 @@ impulsive start, synthetic (no data files). One "step" is one advance! of the so
 Prints one JSON line: steps/s of the mixed-precision path (headline), the FP64 path's
 steps/s, the force error of the mixed path against the FP64 path, CG iterations per step,
 the roofline of the dominant kernel (the sine-contraction pass) measured with HIP events
-on the solver's stream, and the FP64 CPU oracle timed on the host cores.
+on the solver's stream, the FP64 CPU oracle timed on the host cores, and CG iterations per
+step of a pitching plate with the learned warm start against the reference's guess.
 """
 import argparse
 import json
@@ -24,6 +25,10 @@ for p in (os.path.join(ROOT, "immersedbodies.jl_amd"), os.path.join(ROOT, "tests
 import numpy as np  # noqa: E402
 
 PEAK = {"mixed": 2500.0, "fp64": 78.6}  # dense MFMA TFLOP/s: bf16, fp64
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 HBM_PEAK = 8000.0  # GB/s
 
 
@@ -90,6 +95,77 @@ def cpu_baseline(prob, steps):
                       "replaced by the identity (same per-step GEMV cost)"}
 
 
+def warm_start_bench(args):
+    """Learned CG warm start on a pitching plate (512^2, 4 levels, 33 points, CG to a
+    1e-8 ||b||-relative residual): record samples on four reduced frequencies, train on the
+    GPU, and compare CG iterations per step on a fifth, held-out frequency against CG from
+    the previous step's F~b (the reference) and from the extrapolated guess alone."""
+    import immersedbodies as ib
+    from immersedbodies import _lib
+    from immersedbodies.warmstart import WarmStartNet
+    import cases
+
+    ks_train = (0.2, 0.45, 0.7, 0.95)
+    k_test = 0.6
+    per = max(1, args.ws_samples // len(ks_train))
+    cg = dict(maxiter=5000, reltol=1e-8)
+
+    def make(k):
+        return ib.create_solver(cases.pitching_plate_512(k), precision=args.precision,
+                                refine_sweeps=args.refine, cg=cg, cg_tol_mode=1)
+
+    net = WarmStartNet(channels=128, layers=4, kernel=5, max_points=64,
+                       capacity=per * len(ks_train), max_batch=64, seed=1)
+    t0 = time.perf_counter()
+    for k in ks_train:
+        s = make(k)
+        s.set_warm_start("extrapolate", net=net, record=True)
+        s.advance(per)
+        s.close()
+        log(f"warm start: recorded {per} steps at k={k}")
+    t_rec = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss = net.train(args.ws_iters, batch=64, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    log(f"warm start: trained {args.ws_iters} iterations in {t_train:.1f} s")
+    modes = {}
+    forces = {}
+    for mode in ("previous", "extrapolate", "learned"):
+        s = make(k_test)
+        s.set_warm_start("extrapolate")
+        s.advance(args.ws_warm)
+        s.set_warm_start(mode, net=net if mode == "learned" else None)
+        _lib.check(_lib.load().ibpm_synchronize(s._h))
+        t0 = time.perf_counter()
+        st = s.advance(args.ws_eval_steps)
+        dt = time.perf_counter() - t0
+        modes[mode] = {"cg_iters_per_step": round(float(s.cg_iters(st).mean()), 3),
+                       "b_applies_per_step": round(float(s.b_applies(st).mean()), 3),
+                       "steps_per_s": round(args.ws_eval_steps / dt, 2)}
+        if mode == "learned":
+            modes[mode]["learned_guess_kept"] = float(np.mean([x.ws_used for x in st]))
+        forces[mode] = s.forces(st)
+        s.close()
+        log(f"warm start: {mode} {modes[mode]}")
+    ref = forces["previous"]
+    err = {m: float(np.max(np.linalg.norm(forces[m] - ref, axis=1) / np.linalg.norm(ref, axis=1)))
+           for m in ("extrapolate", "learned")}
+    return {
+        "workload": "pitching plate, 512x512 cells, 4 levels, Re 200, dt 0.002, 33 points "
+                    "(spacing 2h), alpha = 20+10 sin(2 pi f t) deg; train k = "
+                    f"{list(ks_train)}, test k = {k_test}; CG reltol 1e-8 of ||b||",
+        "train_samples": per * len(ks_train), "record_s": round(t_rec, 2),
+        "train_iters": args.ws_iters, "train_s": round(t_train, 2),
+        "train_loss_first_last": [float(loss[:50].mean()), float(loss[-50:].mean())],
+        "eval_steps": args.ws_eval_steps, "modes": modes,
+        "cg_iter_reduction_vs_previous": round(modes["previous"]["cg_iters_per_step"] /
+                                               max(modes["learned"]["cg_iters_per_step"], 1e-9), 3),
+        "b_apply_reduction_vs_previous": round(modes["previous"]["b_applies_per_step"] /
+                                               modes["learned"]["b_applies_per_step"], 3),
+        "force_rel_err_vs_previous_max": err,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +178,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--levels", type=int, default=5)
     ap.add_argument("--no-ref", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-ws", action="store_true", help="skip the learned warm-start section")
+    ap.add_argument("--ws-samples", type=int, default=4000)
+    ap.add_argument("--ws-iters", type=int, default=3000)
+    ap.add_argument("--ws-warm", type=int, default=200)
+    ap.add_argument("--ws-eval-steps", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -113,6 +194,7 @@ def main():
     prob = cases.cylinder_1024(nlevel=args.levels)
     steps, warm = args.steps, args.warmup
     head, dt_h, setup_h, f_h, it_h = time_solver(prob, args.precision, args.refine, steps, warm)
+    log(f"{args.precision}: {steps / dt_h:.2f} steps/s")
     roof = kernel_roofline(head, 0, 50, PEAK[args.precision])
     roof_nl = kernel_roofline(head, 1, 50, None)
     head.close()
@@ -121,6 +203,7 @@ def main():
         fsteps = args.fp64_steps or steps
         ref, dt_r, setup_r, f_r, _ = time_solver(prob, "fp64", 0, fsteps, warm)
         ref.close()
+        log(f"fp64: {fsteps / dt_r:.2f} steps/s")
         n = min(len(f_h), len(f_r))
         rel = np.linalg.norm(f_h[:n] - f_r[:n], axis=1) / np.linalg.norm(f_r[:n], axis=1)
         out_extra = {"fp64_steps_per_s": round(fsteps / dt_r, 3),
@@ -151,6 +234,8 @@ def main():
         "roofline_nonlinear_kernel": roof_nl,
     }
     res.update(out_extra)
+    if not args.no_ws:
+        res["warm_start"] = warm_start_bench(args)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(prob, args.cpu_steps)
     print(json.dumps(res), flush=True)

@@ -78,6 +78,16 @@ def pitching_plate(nx=64, ny=64, nlevel=3, h=1.0 / 32, x0=(-1.0, -1.0), Re=200.0
     return ib.Problem(fluid=fluid, bodies=bodies, scheme=ib.CNAB(dt=dt))
 
 
+def pitching_plate_512(k=0.5, npts=33):
+    """512 x 512 cells on [-2,6] x [-4,4] (h = 1/64), 4 levels, Re 200, dt 0.002: plate of
+    length 1 pitching about midchord, alpha = 20 + 10 sin(2 pi f t) deg, k = pi f. The
+    points sit at the reference's panel spacing 2h (Panels(grid, curve), curves.jl:53-55):
+    33 points. At 200 points (spacing 0.32 h) B is numerically singular and CG cannot reach
+    a 1e-8 residual."""
+    return pitching_plate(nx=512, ny=512, nlevel=4, h=1.0 / 64, x0=(-2.0, -4.0), Re=200.0,
+                          dt=0.002, npts=npts, k=k)
+
+
 def no_body(nx=32, ny=32, nlevel=2, h=0.1, x0=(-1.6, -1.6), Re=50.0, dt=0.01):
     grid = ib.MultiDomainGrid(ib.CartesianGrid(tuple(x0), (nx, ny), h), nlevel)
     fluid = ib.Fluid(grid=grid, Re=Re, freestream_vel=lambda t: (1.0, 0.3))
