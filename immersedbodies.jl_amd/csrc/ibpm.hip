@@ -17,6 +17,7 @@
 
 #include "../../include/ibpm.h"
 #include "ibpm_internal.h"
+#include "solver.h"
 
 using namespace ibpm;
 
@@ -68,59 +69,6 @@ struct SolveJob {
 
 }  // namespace
 
-struct ibpm_solver {
-  GridDims g{};
-  int L = 0;
-  double x0 = 0, y0 = 0, h = 0, Re = 0, dt = 0, t0 = 0;
-  int index = 0;
-  int n = 0, nbody = 0, moving = 0;
-  int precision = IBPM_FP64, refine = 1;
-  int cg_maxiter = 5000, cg_tol_mode = 0;
-  double cg_reltol = 1e-12, cg_abstol = 0.0;
-  int device = 0;
-  hipStream_t st = nullptr;
-  double fs[2] = {0.0, 0.0};
-  std::vector<double> hc;  // grid step per level
-  double scale = 0.0;      // 1 / (4 nx ny): the two unnormalised DST-I scale factors
-  int lvstride = 0;        // ring-line length: max(nx, ny)/2 + 1
-
-  // fields
-  double *q = nullptr, *G = nullptr, *psi = nullptr, *nl1 = nullptr, *nl2 = nullptr;
-  double *Gs = nullptr, *rhs = nullptr;
-  double *T = nullptr, *U = nullptr, *R = nullptr;  // kMaxPlanes scratch planes each
-  double *cosx = nullptr, *cosy = nullptr;  // cos(pi k / n): spectral eigenvalues
-  void *SoX = nullptr, *SeX = nullptr, *SoY = nullptr, *SeY = nullptr;  // FP64 tables
-  void *BfX = nullptr, *BfY = nullptr;                                  // bf16 tables
-  // multi-level ring chain: sine rows/columns, per-plane factors, per-chain scratch
-  double *edge = nullptr, *pw = nullptr;
-  double *fac = nullptr, *rpart = nullptr, *rlv = nullptr;
-  // bodies
-  double *xb = nullptr, *ub = nullptr, *xb_ref = nullptr, *Ft = nullptr, *fb = nullptr;
-  double *Ftmp = nullptr, *total = nullptr, *motion = nullptr;
-  int *body_of = nullptr, *moving_d = nullptr, *coords = nullptr, *perm = nullptr;
-  int *xstart = nullptr, *err = nullptr;
-  double* w = nullptr;
-  double* Binv = nullptr;
-  // B operator scratch (ncol columns)
-  int bcols = 0;
-  double *Gtmp = nullptr, *psitmp = nullptr, *Sc = nullptr;
-  // CG
-  double *cg_r = nullptr, *cg_u = nullptr, *cg_c = nullptr, *cg_scal = nullptr;
-  double* h_scal = nullptr;  // pinned: [0..5] CG scalars, [8..10] warm-start scalars
-  // CG warm start: extrapolation history F~b of the two steps before the previous one,
-  // the extrapolated guess x_e, B x_e, and (rms(r_e), ||r_e||, ||b||)
-  int ws_mode = IBPM_WS_PREVIOUS, ws_record = 0, nhist = 0;
-  ibpm_net* net = nullptr;
-  double *ws_h0 = nullptr, *ws_h1 = nullptr, *ws_xe = nullptr, *ws_Bxe = nullptr;
-  double* ws_scal = nullptr;
-  // host copies
-  std::vector<double> ds;
-  std::vector<double> q0_shadow;
-  bool q0_shadow_valid = false;
-  std::vector<int> body_start, body_count, body_moving;
-
-  size_t plane() const { return (size_t)g.nG; }
-};
 
 namespace {
 
@@ -808,6 +756,12 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
 
 }  // namespace
 
+namespace ibpm {
+int solver_step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stats* stats) {
+  return step(s, fsv, motion, stats);
+}
+}  // namespace ibpm
+
 extern "C" {
 
 void ibpm_default_options(ibpm_options* o) {
@@ -1150,7 +1104,7 @@ static double* field_ptr(ibpm_solver* s, int field) {
 int ibpm_read(ibpm_solver* s, int field, double* dst, size_t n) {
   if (!s || !dst) return fail(IBPM_E_ARG, "null argument");
   const size_t need = ibpm_field_size(s, field);
-  if (need == 0 && field != IBPM_DS && field != IBPM_XB) return fail(IBPM_E_ARG, "unknown field");
+  if (field < IBPM_Q || field > IBPM_FREESTREAM) return fail(IBPM_E_ARG, "unknown field");
   if (n != need) return fail(IBPM_E_ARG, "size mismatch: expected " + std::to_string(need));
   if (need == 0) return IBPM_OK;
   HIPCHK(hipSetDevice(s->device));
@@ -1184,7 +1138,7 @@ int ibpm_read(ibpm_solver* s, int field, double* dst, size_t n) {
 int ibpm_write(ibpm_solver* s, int field, const double* src, size_t n) {
   if (!s || !src) return fail(IBPM_E_ARG, "null argument");
   const size_t need = ibpm_field_size(s, field);
-  if (need == 0 && field != IBPM_DS && field != IBPM_XB) return fail(IBPM_E_ARG, "unknown field");
+  if (field < IBPM_Q || field > IBPM_FREESTREAM) return fail(IBPM_E_ARG, "unknown field");
   if (n != need) return fail(IBPM_E_ARG, "size mismatch: expected " + std::to_string(need));
   if (need == 0) return IBPM_OK;
   HIPCHK(hipSetDevice(s->device));

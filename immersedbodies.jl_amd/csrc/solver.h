@@ -1,0 +1,70 @@
+// State of one solver (ibpm_solver of include/ibpm.h), shared by the translation units
+// that drive it (ibpm.hip: the step; batch.hip: many solvers captured in one graph).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/ibpm.h"
+#include "ibpm_internal.h"
+
+struct ibpm_solver {
+  ibpm::GridDims g{};
+  int L = 0;
+  double x0 = 0, y0 = 0, h = 0, Re = 0, dt = 0, t0 = 0;
+  int index = 0;
+  int n = 0, nbody = 0, moving = 0;
+  int precision = IBPM_FP64, refine = 1;
+  int cg_maxiter = 5000, cg_tol_mode = 0;
+  double cg_reltol = 1e-12, cg_abstol = 0.0;
+  int device = 0;
+  hipStream_t st = nullptr;
+  double fs[2] = {0.0, 0.0};
+  std::vector<double> hc;  // grid step per level
+  double scale = 0.0;      // 1 / (4 nx ny): the two unnormalised DST-I scale factors
+  int lvstride = 0;        // ring-line length: max(nx, ny)/2 + 1
+
+  // fields
+  double *q = nullptr, *G = nullptr, *psi = nullptr, *nl1 = nullptr, *nl2 = nullptr;
+  double *Gs = nullptr, *rhs = nullptr;
+  double *T = nullptr, *U = nullptr, *R = nullptr;  // kMaxPlanes scratch planes each
+  double *cosx = nullptr, *cosy = nullptr;  // cos(pi k / n): spectral eigenvalues
+  void *SoX = nullptr, *SeX = nullptr, *SoY = nullptr, *SeY = nullptr;  // FP64 tables
+  void *BfX = nullptr, *BfY = nullptr;                                  // bf16 tables
+  // multi-level ring chain: sine rows/columns, per-plane factors, per-chain scratch
+  double *edge = nullptr, *pw = nullptr;
+  double *fac = nullptr, *rpart = nullptr, *rlv = nullptr;
+  // bodies
+  double *xb = nullptr, *ub = nullptr, *xb_ref = nullptr, *Ft = nullptr, *fb = nullptr;
+  double *Ftmp = nullptr, *total = nullptr, *motion = nullptr;
+  int *body_of = nullptr, *moving_d = nullptr, *coords = nullptr, *perm = nullptr;
+  int *xstart = nullptr, *err = nullptr;
+  double* w = nullptr;
+  double* Binv = nullptr;
+  // B operator scratch (ncol columns)
+  int bcols = 0;
+  double *Gtmp = nullptr, *psitmp = nullptr, *Sc = nullptr;
+  // CG
+  double *cg_r = nullptr, *cg_u = nullptr, *cg_c = nullptr, *cg_scal = nullptr;
+  double* h_scal = nullptr;  // pinned: [0..5] CG scalars, [8..10] warm-start scalars
+  // CG warm start: extrapolation history F~b of the two steps before the previous one,
+  // the extrapolated guess x_e, B x_e, and (rms(r_e), ||r_e||, ||b||)
+  int ws_mode = IBPM_WS_PREVIOUS, ws_record = 0, nhist = 0;
+  ibpm_net* net = nullptr;
+  double *ws_h0 = nullptr, *ws_h1 = nullptr, *ws_xe = nullptr, *ws_Bxe = nullptr;
+  double* ws_scal = nullptr;
+  // host copies
+  std::vector<double> ds;
+  std::vector<double> q0_shadow;
+  bool q0_shadow_valid = false;
+  std::vector<int> body_start, body_count, body_moving;
+
+  size_t plane() const { return (size_t)g.nG; }
+};
+
+namespace ibpm {
+// One advance! of a solver (solver.jl:155-190) queued on s->st: no host synchronisation
+// for static bodies. fsv: freestream at the new time; motion: 6 x nbody or null.
+int solver_step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stats* stats);
+}  // namespace ibpm

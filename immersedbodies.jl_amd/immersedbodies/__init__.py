@@ -7,6 +7,7 @@ from .problem import (CNAB, Bodies, CartesianGrid, Fluid, GridIndices, MovingRig
 from .solver import (AbstractSolver, HipCnabSolver, advance, create_solver, getstate, solve,
                      solve_)
 from . import curves
+from .batch import Batch
 
 __all__ = [
     "CNAB", "Bodies", "CartesianGrid", "Fluid", "GridIndices", "MovingRigidBody",

@@ -32,7 +32,8 @@ EXPORTS = ("ibpm_default_options", "ibpm_create", "ibpm_advance", "ibpm_read", "
            "ibpm_set_warm_start", "ibpm_net_default_config", "ibpm_net_create",
            "ibpm_net_destroy", "ibpm_net_dataset_size", "ibpm_net_clear_dataset",
            "ibpm_net_train", "ibpm_net_param_count", "ibpm_net_get_params",
-           "ibpm_net_set_params", "ibpm_net_forward", "ibpm_net_gradients", "ibpm_net_step")
+           "ibpm_net_set_params", "ibpm_net_forward", "ibpm_net_gradients", "ibpm_net_step",
+           "ibpm_batch_create", "ibpm_batch_advance", "ibpm_batch_destroy")
 
 IBPM_WS_PREVIOUS = 0
 IBPM_WS_EXTRAPOLATE = 1
@@ -147,6 +148,12 @@ def load():
     L.ibpm_net_gradients.restype = C.c_int
     L.ibpm_net_step.argtypes = [C.c_void_p, _fp, _ip, _fp, C.c_int, C.c_double, _dp]
     L.ibpm_net_step.restype = C.c_int
+    L.ibpm_batch_create.argtypes = [C.c_int, C.POINTER(C.c_void_p), _dp, C.POINTER(C.c_void_p)]
+    L.ibpm_batch_create.restype = C.c_int
+    L.ibpm_batch_advance.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.ibpm_batch_advance.restype = C.c_int
+    L.ibpm_batch_destroy.argtypes = [C.c_void_p]
+    L.ibpm_batch_destroy.restype = None
     _lib = L
     return L
 

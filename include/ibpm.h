@@ -146,6 +146,27 @@ int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, doubl
 const char* ibpm_last_error(void);
 void ibpm_destroy(ibpm_solver* s);
 
+/* ---- Batched ensembles ----
+ *
+ * No counterpart in the reference, where each CnabSolver advances alone
+ * (solver.jl:155-190). A batch advances many independent solvers together: any grid
+ * sizes, level counts, body-point counts and Reynolds numbers, static bodies or none
+ * (a moving body's CG synchronises with the host every iteration). One step of every
+ * solver is captured into a hipGraph (two, one per parity of the nonlinear-term swap)
+ * spread over concurrent streams; each batch step is one graph replay. Every solver
+ * ends in the same state as when advanced alone.
+ *
+ * freestream: 2 x nprob, the velocity of each problem, fixed for the batch (NULL: 0).
+ * The batch must be destroyed before its solvers. Between batch steps the solvers may be
+ * read, written or advanced alone, as long as all of them advance by the same parity of
+ * steps (otherwise ibpm_batch_advance fails). */
+typedef struct ibpm_batch ibpm_batch;
+int ibpm_batch_create(int nprob, ibpm_solver* const* solvers, const double* freestream,
+                      ibpm_batch** out);
+/* forces: nsteps x nprob x 2 (total force of every problem after every step) or NULL. */
+int ibpm_batch_advance(ibpm_batch* b, int nsteps, double* forces);
+void ibpm_batch_destroy(ibpm_batch* b);
+
 /* ---- Warm start of the CG surface-force solve of moving bodies ----
  *
  * No counterpart in the reference, which starts cg! from the previous step's F̃b

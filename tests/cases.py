@@ -88,6 +88,37 @@ def pitching_plate_512(k=0.5, npts=33):
                           dt=0.002, npts=npts, k=k)
 
 
+def ellipse_body(nx=96, ny=64, nlevel=2, h=0.05, x0=(-1.2, -1.6), Re=250.0, dt=None, a=0.5,
+                 b=0.2, aoa_deg=10.0, spacing=None, U=(1.0, 0.0)):
+    """Stationary ellipse (semi-axes a, b) at angle of attack ``aoa_deg`` in a uniform
+    stream, points spaced ``spacing`` (default h). dt defaults to CFL 0.2 (h/5)."""
+    grid = ib.MultiDomainGrid(ib.CartesianGrid(tuple(x0), (nx, ny), h), nlevel)
+    fluid = ib.Fluid(grid=grid, Re=Re, freestream_vel=lambda t: U)
+    curve = curves.ellipse(a, b, angle=-math.radians(aoa_deg))
+    pan = curves.partition(curve, float(spacing or h))
+    bodies = ib.Bodies(ellipse=ib.StaticBody(pan))
+    return ib.Problem(fluid=fluid, bodies=bodies, scheme=ib.CNAB(dt=dt or 0.2 * h))
+
+
+def ensemble(nprob=64, seed=2024, sizes=(256, 512, 768, 1024), levels=(3, 4, 5)):
+    """Batched-ensemble workload: finest grids N x N drawn from ``sizes`` on [-2,6] x [-4,4]
+    (h = 8/N), levels from ``levels``, Re uniform in [50, 300], one ellipse per problem
+    (a = 0.5, b uniform in [0.1, 0.5], angle of attack uniform in [0, 30] deg, points
+    spaced h), dt = h/5 (CFL 0.2), numpy default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nprob):
+        N = int(rng.choice(sizes))
+        L = int(rng.choice(levels))
+        Re = float(rng.uniform(50.0, 300.0))
+        b = float(rng.uniform(0.1, 0.5))
+        aoa = float(rng.uniform(0.0, 30.0))
+        h = 8.0 / N
+        out.append(ellipse_body(nx=N, ny=N, nlevel=L, h=h, x0=(-2.0, -4.0), Re=Re, a=0.5, b=b,
+                                aoa_deg=aoa))
+    return out
+
+
 def no_body(nx=32, ny=32, nlevel=2, h=0.1, x0=(-1.6, -1.6), Re=50.0, dt=0.01):
     grid = ib.MultiDomainGrid(ib.CartesianGrid(tuple(x0), (nx, ny), h), nlevel)
     fluid = ib.Fluid(grid=grid, Re=Re, freestream_vel=lambda t: (1.0, 0.3))
