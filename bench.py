@@ -166,6 +166,54 @@ def warm_start_bench(args):
     }
 
 
+def batch_bench(args):
+    """Batched ensemble: independent stationary-ellipse problems of uneven sizes
+    (256^2..1024^2, 3-5 levels, Re 50-300, 68-400 points), advanced one graph replay per
+    step, against the same solvers advanced one after another."""
+    import immersedbodies as ib
+    import cases
+
+    probs = cases.ensemble(args.batch_problems, seed=2024)
+    t0 = time.perf_counter()
+    sol = [ib.create_solver(p, precision=args.precision, refine_sweeps=args.refine) for p in probs]
+    setup = time.perf_counter() - t0
+    log(f"batch: {len(sol)} solvers set up in {setup:.1f} s")
+    for s in sol:
+        s.advance(2)
+    t0 = time.perf_counter()
+    for s in sol:
+        s.advance(args.batch_steps)
+    t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batch = ib.Batch(sol)
+    t_cap = time.perf_counter() - t0
+    batch.advance(2)
+    t0 = time.perf_counter()
+    forces = batch.advance(args.batch_steps)
+    t_b = time.perf_counter() - t0
+    batch.close()
+    n = len(sol)
+    sizes = [p.fluid.grid.base.n[0] for p in probs]
+    cells = sum(p.fluid.grid.base.n[0] * p.fluid.grid.base.n[1] * p.fluid.grid.nlevel
+                for p in probs)
+    for s in sol:
+        s.close()
+    res = {"workload": f"{n} stationary ellipses (a 0.5, b U[0.1,0.5], AoA U[0,30] deg, points "
+                       "spaced h), N x N on [-2,6]x[-4,4] with N from {256,512,768,1024}, 3-5 "
+                       "levels, Re U[50,300], dt = h/5, numpy default_rng(2024)",
+           "problems": n, "grid_sizes": {str(k): sizes.count(k) for k in sorted(set(sizes))},
+           "panels_min_max": [min(p.bodies.npanel for p in probs), max(p.bodies.npanel for p in probs)],
+           "steps": args.batch_steps, "setup_s": round(setup, 2), "capture_s": round(t_cap, 3),
+           "problem_steps_per_s": round(n * args.batch_steps / t_b, 2),
+           "ensemble_steps_per_s": round(args.batch_steps / t_b, 3),
+           "sequential_problem_steps_per_s": round(n * args.batch_steps / t_seq, 2),
+           "speedup_vs_sequential": round(t_seq / t_b, 3),
+           "level_cell_updates_per_s": round(cells * args.batch_steps / t_b, 1),
+           "forces_finite": bool(np.all(np.isfinite(forces)))}
+    log(f"batch: {res['problem_steps_per_s']} problem-steps/s, x{res['speedup_vs_sequential']} vs sequential")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,6 +231,9 @@ def main():
     ap.add_argument("--ws-iters", type=int, default=3000)
     ap.add_argument("--ws-warm", type=int, default=200)
     ap.add_argument("--ws-eval-steps", type=int, default=300)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble section")
+    ap.add_argument("--batch-problems", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -196,6 +247,7 @@ def main():
     head, dt_h, setup_h, f_h, it_h = time_solver(prob, args.precision, args.refine, steps, warm)
     log(f"{args.precision}: {steps / dt_h:.2f} steps/s")
     roof = kernel_roofline(head, 0, 50, PEAK[args.precision])
+    roof["tiles_TB_TM"] = list(head.dst_tiles(args.levels))
     roof_nl = kernel_roofline(head, 1, 50, None)
     head.close()
     out_extra = {}
@@ -236,6 +288,8 @@ def main():
     res.update(out_extra)
     if not args.no_ws:
         res["warm_start"] = warm_start_bench(args)
+    if not args.no_batch:
+        res["batch"] = batch_bench(args)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(prob, args.cpu_steps)
     print(json.dumps(res), flush=True)

@@ -21,6 +21,9 @@
 #include <math.h>
 #include <string.h>
 
+#include <map>
+#include <mutex>
+
 #include "ibpm_internal.h"
 
 namespace ibpm {
@@ -75,30 +78,33 @@ __device__ __forceinline__ bool xcd_tile(const PassBatch& b, int& z, int& bt, in
 }
 
 // ---------------------------------------------------------------------------------
-// FP64 pass. Work-group tile: 64 values of b x (64 odd + 64 even) output rows, K chunk
-// 16. 4 waves: wave w computes parity (w>>1) on b-half (w&1), a 32 x 64 block as 2 x 4
-// MFMA 16x16x4 tiles. Input chunk and sine operands are staged through LDS; the global
-// loads of the next chunk are issued before the matrix-core work of the current one.
+// FP64 pass. Work-group tile: TB values of b x (TM odd + TM even) output rows, K chunk
+// 16; TB, TM in {32, 64} are picked per launch shape by the autotuner. 4 waves: wave w
+// computes parity (w>>1) on b-half (w&1), a TB/2 x TM block as (TB/32) x (TM/16) MFMA
+// 16x16x4 tiles. Input chunk and sine operands are staged through LDS; the global loads
+// of the next chunk are issued before the matrix-core work of the current one.
 // ---------------------------------------------------------------------------------
-constexpr int F64_TB = 64;
-constexpr int F64_TM = 64;
 constexpr int F64_KC = 16;
 
-template <typename InT, typename OutT, int EPI>
+template <typename InT, typename OutT, int EPI, int TB, int TM>
 __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch) {
+  constexpr int BS = TB / 32;                // 16-wide b subtiles per wave
+  constexpr int MS = TM / 16;                // 16-wide m subtiles per wave
+  constexpr int RA = F64_KC * TB / 256;      // input elements per thread and chunk
+  constexpr int RB = F64_KC * TM / 256;      // sine elements per thread and chunk
   int z, bt, mt;
   if (!xcd_tile(batch, z, bt, mt)) return;
   const PassPlane& d = batch.p[z];
   const int Nk = d.Nk, Nb = d.Nb;
   const int H = (Nk + 1) / 2;
-  const int b0 = bt * F64_TB;
-  const int m0 = mt * F64_TM;
+  const int b0 = bt * TB;
+  const int m0 = mt * TM;
   if (b0 >= Nb || m0 >= H) return;
 
-  __shared__ double Ps[F64_KC][F64_TB + 1];
-  __shared__ double Qs[F64_KC][F64_TB + 1];
-  __shared__ double So_s[F64_KC][F64_TM + 1];
-  __shared__ double Se_s[F64_KC][F64_TM + 1];
+  __shared__ double Ps[F64_KC][TB + 1];
+  __shared__ double Qs[F64_KC][TB + 1];
+  __shared__ double So_s[F64_KC][TM + 1];
+  __shared__ double Se_s[F64_KC][TM + 1];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -108,16 +114,16 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
   const double* SoT = (const double*)d.So;
   const double* SeT = (const double*)d.Se;
 
-  d4 acc[2][4];
+  d4 acc[BS][MS];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < BS; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < MS; ++j) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
 
-  double ra[4], rm[4], rso[4], rse[4];
+  double ra[RA], rm[RA], rso[RB], rse[RB];
   auto fetch = [&](int kc) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < RA; ++r) {
       const int e = tid + 256 * r;
       const int k1 = kc + (e & 15) + 1;  // 1-based k; 16 lanes read 16 consecutive k
       const int b = b0 + (e >> 4);
@@ -130,10 +136,10 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
       rm[r] = vm;
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < RB; ++r) {
       const int e = tid + 256 * r;
-      const int k1 = kc + (e >> 6) + 1;
-      const int m = m0 + (e & 63);
+      const int k1 = kc + e / TM + 1;
+      const int m = m0 + e % TM;
       rso[r] = (k1 <= H && m < H) ? SoT[(size_t)(k1 - 1) * H + m] : 0.0;
       rse[r] = (k1 <= H - 1 && m < H - 1) ? SeT[(size_t)(k1 - 1) * (H - 1) + m] : 0.0;
     }
@@ -144,7 +150,7 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
   for (int kc = 0; kc < H; kc += F64_KC) {
     // fold into LDS: P = x_k + x_{M-k}, Q = x_k - x_{M-k}; k = H has no partner (Q = 0)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < RA; ++r) {
       const int e = tid + 256 * r;
       const int kk = e & 15;
       const bool mid = kc + kk + 1 == H;
@@ -152,25 +158,27 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
       Qs[kk][e >> 4] = mid ? 0.0 : ra[r] - rm[r];
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < RB; ++r) {
       const int e = tid + 256 * r;
-      So_s[e >> 6][e & 63] = rso[r];
-      Se_s[e >> 6][e & 63] = rse[r];
+      So_s[e / TM][e % TM] = rso[r];
+      Se_s[e / TM][e % TM] = rse[r];
     }
     __syncthreads();
     if (kc + F64_KC < H) fetch(kc + F64_KC);
-    const double(*A)[F64_TB + 1] = parity == 0 ? Ps : Qs;
-    const double(*Bs)[F64_TM + 1] = parity == 0 ? So_s : Se_s;
+    const double(*A)[TB + 1] = parity == 0 ? Ps : Qs;
+    const double(*Bs)[TM + 1] = parity == 0 ? So_s : Se_s;
 #pragma unroll
     for (int ks = 0; ks < F64_KC / 4; ++ks) {
       const int krow = ks * 4 + kq;
-      const double a0 = A[krow][bh * 32 + col];
-      const double a1 = A[krow][bh * 32 + 16 + col];
+      double av[BS];
 #pragma unroll
-      for (int ms = 0; ms < 4; ++ms) {
+      for (int bs = 0; bs < BS; ++bs) av[bs] = A[krow][bh * (TB / 2) + bs * 16 + col];
+#pragma unroll
+      for (int ms = 0; ms < MS; ++ms) {
         const double bv = Bs[krow][ms * 16 + col];
-        acc[0][ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc[0][ms], 0, 0, 0);
-        acc[1][ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc[1][ms], 0, 0, 0);
+#pragma unroll
+        for (int bs = 0; bs < BS; ++bs)
+          acc[bs][ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[bs], bv, acc[bs][ms], 0, 0, 0);
       }
     }
     __syncthreads();
@@ -180,15 +188,15 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
   const int rq = lane >> 4;
   const int mlim = parity == 0 ? H : H - 1;
 #pragma unroll
-  for (int bs = 0; bs < 2; ++bs)
+  for (int bs = 0; bs < BS; ++bs)
 #pragma unroll
-    for (int ms = 0; ms < 4; ++ms) {
+    for (int ms = 0; ms < MS; ++ms) {
       const int m = m0 + ms * 16 + col;
       if (m >= mlim) continue;
       const int jc = parity == 0 ? 2 * m : 2 * m + 1;  // 0-based output row index j-1
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int b = b0 + bh * 32 + bs * 16 + rq + 4 * r;
+        const int b = b0 + bh * (TB / 2) + bs * 16 + rq + 4 * r;
         if (b < Nb) store_out<OutT, EPI>(d, b, jc, acc[bs][ms][r]);
       }
     }
@@ -196,12 +204,11 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
 
 // ---------------------------------------------------------------------------------
 // Mixed pass: fp32 operands split into bf16 hi + lo, three MFMA products, fp32 sums.
-// Work-group tile: 64 b x (64 odd + 64 even) rows, K chunk 32 (one MFMA k-step).
-// The sine tables are stored per output row with k contiguous and zero padding to a
-// multiple of 32 in both k and m; a chunk's 64 rows of both tables are staged in LDS.
+// Work-group tile: TB b x (TM odd + TM even) rows, K chunk 32 (one MFMA k-step), TB, TM
+// in {32, 64} picked by the autotuner. The sine tables are stored per output row with k
+// contiguous and zero padding to a multiple of 32 in both k and m; a chunk's TM rows of
+// both tables are staged in LDS.
 // ---------------------------------------------------------------------------------
-constexpr int MX_TB = 64;
-constexpr int MX_TM = 64;
 constexpr int MX_KC = 32;
 constexpr int MX_LDK = MX_KC + 8;  // padded LDS row (bf16 elements)
 
@@ -210,23 +217,26 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
   lo = (__bf16)(x - (float)hi);
 }
 
-template <typename InT, typename OutT, int EPI>
+template <typename InT, typename OutT, int EPI, int TB, int TM>
 __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch batch) {
+  constexpr int BS = TB / 32;  // 16-wide b subtiles per wave
+  constexpr int MS = TM / 16;  // 16-wide m subtiles per wave
+  constexpr int RA = TB / 16;  // input element pairs per thread and chunk
   int z, bt, mt;
   if (!xcd_tile(batch, z, bt, mt)) return;
   const PassPlane& d = batch.p[z];
   const int Nk = d.Nk, Nb = d.Nb;
   const int H = (Nk + 1) / 2;
   const int Hp = (H + 31) & ~31;
-  const int b0 = bt * MX_TB;
-  const int m0 = mt * MX_TM;
+  const int b0 = bt * TB;
+  const int m0 = mt * TM;
   if (b0 >= Nb || m0 >= H) return;
 
   // A operand (folded input, split) and B operand (sine rows, split) of one chunk
-  __shared__ __attribute__((aligned(16))) __bf16 Ah[2][MX_TB][MX_LDK];  // [P|Q] hi
-  __shared__ __attribute__((aligned(16))) __bf16 Al[2][MX_TB][MX_LDK];  // [P|Q] lo
-  __shared__ __attribute__((aligned(16))) __bf16 Bh[2][MX_TM][MX_LDK];  // [odd|even] hi
-  __shared__ __attribute__((aligned(16))) __bf16 Bl[2][MX_TM][MX_LDK];  // [odd|even] lo
+  __shared__ __attribute__((aligned(16))) __bf16 Ah[2][TB][MX_LDK];  // [P|Q] hi
+  __shared__ __attribute__((aligned(16))) __bf16 Al[2][TB][MX_LDK];  // [P|Q] lo
+  __shared__ __attribute__((aligned(16))) __bf16 Bh[2][TM][MX_LDK];  // [odd|even] hi
+  __shared__ __attribute__((aligned(16))) __bf16 Bl[2][TM][MX_LDK];  // [odd|even] lo
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -242,20 +252,20 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
   const __bf16* So = (const __bf16*)d.So;
   const __bf16* Se = (const __bf16*)d.Se;
 
-  f4 acc[2][4];
+  f4 acc[BS][MS];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < BS; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < MS; ++j) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
 
   // loader mapping for the input: element pair (2kp, 2kp+1) of row bb, 16 lanes per row
-  double ra[4][2], rm[4][2];
+  double ra[RA][2], rm[RA][2];
   // loader mapping for the sine rows: 16-byte piece (row sm, k8) of table st (4 tables)
   const int sm = tid >> 2, sk = (tid & 3) * 8;
   bf8 rs[4];
   auto fetch = [&](int kc) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < RA; ++r) {
       const int e = tid + 256 * r;
       const int kp = e & 15;
       const int b = b0 + (e >> 4);
@@ -272,7 +282,7 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
       }
     }
     const size_t off = (size_t)(m0 + sm) * Hp + kc + sk;  // m0 + sm < Mp by padding
-    if (m0 + sm < Mp) {
+    if (sm < TM && m0 + sm < Mp) {
       rs[0] = *(const bf8*)(So + off);
       rs[1] = *(const bf8*)(So + tab + off);
       rs[2] = *(const bf8*)(Se + off);
@@ -285,7 +295,7 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
   fetch(0);
   for (int kc = 0; kc < H; kc += MX_KC) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < RA; ++r) {
       const int e = tid + 256 * r;
       const int kp = e & 15, bb = e >> 4;
       __bf16 ph[2], pl[2], qh[2], ql[2];
@@ -301,25 +311,27 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
       *(bf2*)&Ah[1][bb][2 * kp] = (bf2){qh[0], qh[1]};
       *(bf2*)&Al[1][bb][2 * kp] = (bf2){ql[0], ql[1]};
     }
-    *(bf8*)&Bh[0][sm][sk] = rs[0];
-    *(bf8*)&Bl[0][sm][sk] = rs[1];
-    *(bf8*)&Bh[1][sm][sk] = rs[2];
-    *(bf8*)&Bl[1][sm][sk] = rs[3];
+    if (sm < TM) {
+      *(bf8*)&Bh[0][sm][sk] = rs[0];
+      *(bf8*)&Bl[0][sm][sk] = rs[1];
+      *(bf8*)&Bh[1][sm][sk] = rs[2];
+      *(bf8*)&Bl[1][sm][sk] = rs[3];
+    }
     __syncthreads();
     if (kc + MX_KC < H) fetch(kc + MX_KC);
-    bf8 ah[2], al[2];
+    bf8 ah[BS], al[BS];
 #pragma unroll
-    for (int bs = 0; bs < 2; ++bs) {
-      const int row = bh * 32 + bs * 16 + col;
+    for (int bs = 0; bs < BS; ++bs) {
+      const int row = bh * (TB / 2) + bs * 16 + col;
       ah[bs] = *(const bf8*)&Ah[parity][row][kq * 8];
       al[bs] = *(const bf8*)&Al[parity][row][kq * 8];
     }
 #pragma unroll
-    for (int ms = 0; ms < 4; ++ms) {
+    for (int ms = 0; ms < MS; ++ms) {
       const bf8 bhv = *(const bf8*)&Bh[parity][ms * 16 + col][kq * 8];
       const bf8 blv = *(const bf8*)&Bl[parity][ms * 16 + col][kq * 8];
 #pragma unroll
-      for (int bs = 0; bs < 2; ++bs) {
+      for (int bs = 0; bs < BS; ++bs) {
         acc[bs][ms] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bs], bhv, acc[bs][ms], 0, 0, 0);
         acc[bs][ms] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bs], blv, acc[bs][ms], 0, 0, 0);
         acc[bs][ms] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bs], bhv, acc[bs][ms], 0, 0, 0);
@@ -332,15 +344,15 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
   const int rq = lane >> 4;
   const int mlim = parity == 0 ? H : H - 1;
 #pragma unroll
-  for (int bs = 0; bs < 2; ++bs)
+  for (int bs = 0; bs < BS; ++bs)
 #pragma unroll
-    for (int ms = 0; ms < 4; ++ms) {
+    for (int ms = 0; ms < MS; ++ms) {
       const int m = m0 + ms * 16 + col;
       if (m >= mlim) continue;
       const int jc = parity == 0 ? 2 * m : 2 * m + 1;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int b = b0 + bh * 32 + bs * 16 + 4 * rq + r;
+        const int b = b0 + bh * (TB / 2) + bs * 16 + 4 * rq + r;
         if (b < Nb) store_out<OutT, EPI>(d, b, jc, (double)acc[bs][ms][r]);
       }
     }
@@ -349,13 +361,10 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
 // ---------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------
-#define IBPM_DISPATCH_EPI(KERNEL, InT, OutT)                                               \
-  switch (epi) {                                                                           \
-    case EPI_STORE: hipLaunchKernelGGL((KERNEL<InT, OutT, EPI_STORE>), grid, block, 0, st, batch); break; \
-    case EPI_DIAG: hipLaunchKernelGGL((KERNEL<InT, OutT, EPI_DIAG>), grid, block, 0, st, batch); break;   \
-    case EPI_SUB: hipLaunchKernelGGL((KERNEL<InT, OutT, EPI_SUB>), grid, block, 0, st, batch); break;     \
-    default: hipLaunchKernelGGL((KERNEL<InT, OutT, EPI_ADD>), grid, block, 0, st, batch); break;          \
-  }
+// Tile variants (TB, TM); the autotuner times them on a launch shape and keeps the best.
+constexpr int kVariants = 4;
+constexpr int kVarTB[kVariants] = {64, 64, 32, 32};
+constexpr int kVarTM[kVariants] = {64, 32, 64, 32};
 
 static dim3 tile_grid(PassBatch& b, int nplanes, int maxNb, int maxH, int TB, int TM) {
   b.nbt = (maxNb + TB - 1) / TB;
@@ -365,30 +374,153 @@ static dim3 tile_grid(PassBatch& b, int nplanes, int maxNb, int maxH, int TB, in
   return dim3(8 * ((T + 7) / 8));
 }
 
-hipError_t launch_dst_pass_f64(const PassBatch& batch_in, int nplanes, int maxNb, int maxH, int epi,
-                               hipStream_t st) {
-  PassBatch batch = batch_in;
-  dim3 grid = tile_grid(batch, nplanes, maxNb, maxH, F64_TB, F64_TM);
-  dim3 block(256);
-  IBPM_DISPATCH_EPI(dst_pass_f64_kernel, double, double)
+template <typename InT, typename OutT, int EPI, int TB, int TM>
+struct F64K {
+  static void go(dim3 g, hipStream_t st, const PassBatch& b) {
+    hipLaunchKernelGGL((dst_pass_f64_kernel<InT, OutT, EPI, TB, TM>), g, dim3(256), 0, st, b);
+  }
+};
+template <typename InT, typename OutT, int EPI, int TB, int TM>
+struct MixK {
+  static void go(dim3 g, hipStream_t st, const PassBatch& b) {
+    hipLaunchKernelGGL((dst_pass_mixed_kernel<InT, OutT, EPI, TB, TM>), g, dim3(256), 0, st, b);
+  }
+};
+
+template <template <typename, typename, int, int, int> class K, typename InT, typename OutT,
+          int TB, int TM>
+static void launch_epi(const PassBatch& b_in, int nplanes, int maxNb, int maxH, int epi,
+                       hipStream_t st) {
+  PassBatch b = b_in;
+  const dim3 g = tile_grid(b, nplanes, maxNb, maxH, TB, TM);
+  switch (epi) {
+    case EPI_STORE: K<InT, OutT, EPI_STORE, TB, TM>::go(g, st, b); break;
+    case EPI_DIAG: K<InT, OutT, EPI_DIAG, TB, TM>::go(g, st, b); break;
+    case EPI_SUB: K<InT, OutT, EPI_SUB, TB, TM>::go(g, st, b); break;
+    default: K<InT, OutT, EPI_ADD, TB, TM>::go(g, st, b); break;
+  }
+}
+
+template <template <typename, typename, int, int, int> class K, typename InT, typename OutT>
+static void launch_variant(int v, const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
+                           hipStream_t st) {
+  switch (v) {
+    case 1: launch_epi<K, InT, OutT, 64, 32>(b, nplanes, maxNb, maxH, epi, st); break;
+    case 2: launch_epi<K, InT, OutT, 32, 64>(b, nplanes, maxNb, maxH, epi, st); break;
+    case 3: launch_epi<K, InT, OutT, 32, 32>(b, nplanes, maxNb, maxH, epi, st); break;
+    default: launch_epi<K, InT, OutT, 64, 64>(b, nplanes, maxNb, maxH, epi, st); break;
+  }
+}
+
+static void launch_mixed_variant(int v, const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
+                                 bool in_f32, bool out_f32, hipStream_t st) {
+  if (!in_f32 && out_f32) launch_variant<MixK, double, float>(v, b, nplanes, maxNb, maxH, epi, st);
+  else if (in_f32 && out_f32) launch_variant<MixK, float, float>(v, b, nplanes, maxNb, maxH, epi, st);
+  else if (in_f32 && !out_f32) launch_variant<MixK, float, double>(v, b, nplanes, maxNb, maxH, epi, st);
+  else launch_variant<MixK, double, double>(v, b, nplanes, maxNb, maxH, epi, st);
+}
+
+// ---- autotuner ----
+// Process-wide table: launch shape (precision, planes, Nb and H of the largest plane) ->
+// the fastest variant measured on it. Shapes never tuned use variant 0 (64 x 64).
+namespace {
+struct TuneKey {
+  int mixed, nplanes, maxNb, maxH;
+  bool operator<(const TuneKey& o) const {
+    if (mixed != o.mixed) return mixed < o.mixed;
+    if (nplanes != o.nplanes) return nplanes < o.nplanes;
+    if (maxNb != o.maxNb) return maxNb < o.maxNb;
+    return maxH < o.maxH;
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tune;
+
+int tuned_variant(int mixed, int nplanes, int maxNb, int maxH) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(TuneKey{mixed, nplanes, maxNb, maxH});
+  return it == g_tune.end() ? 0 : it->second;
+}
+}  // namespace
+
+hipError_t launch_dst_pass_f64(const PassBatch& batch, int nplanes, int maxNb, int maxH, int epi,
+                               hipStream_t st, int variant) {
+  const int v = variant >= 0 && variant < kVariants ? variant : tuned_variant(0, nplanes, maxNb, maxH);
+  launch_variant<F64K, double, double>(v, batch, nplanes, maxNb, maxH, epi, st);
   return hipGetLastError();
 }
 
-hipError_t launch_dst_pass_mixed(const PassBatch& batch_in, int nplanes, int maxNb, int maxH, int epi,
-                                 bool in_f32, bool out_f32, hipStream_t st) {
-  PassBatch batch = batch_in;
-  dim3 grid = tile_grid(batch, nplanes, maxNb, maxH, MX_TB, MX_TM);
-  dim3 block(256);
-  if (!in_f32 && out_f32) {
-    IBPM_DISPATCH_EPI(dst_pass_mixed_kernel, double, float)
-  } else if (in_f32 && out_f32) {
-    IBPM_DISPATCH_EPI(dst_pass_mixed_kernel, float, float)
-  } else if (in_f32 && !out_f32) {
-    IBPM_DISPATCH_EPI(dst_pass_mixed_kernel, float, double)
-  } else {
-    IBPM_DISPATCH_EPI(dst_pass_mixed_kernel, double, double)
-  }
+hipError_t launch_dst_pass_mixed(const PassBatch& batch, int nplanes, int maxNb, int maxH, int epi,
+                                 bool in_f32, bool out_f32, hipStream_t st, int variant) {
+  const int v = variant >= 0 && variant < kVariants ? variant : tuned_variant(1, nplanes, maxNb, maxH);
+  launch_mixed_variant(v, batch, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
   return hipGetLastError();
+}
+
+hipError_t tune_dst_pass(const PassBatch& batch, int nplanes, int maxNb, int maxH, bool mixed,
+                         bool in_f32, bool out_f32, hipStream_t st, int* chosen, float* best_ms) {
+  const TuneKey key{mixed ? 1 : 0, nplanes, maxNb, maxH};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      if (chosen) *chosen = it->second;
+      if (best_ms) *best_ms = -1.f;
+      return hipSuccess;
+    }
+  }
+  hipEvent_t e0, e1;
+  hipError_t e = hipEventCreate(&e0);
+  if (e != hipSuccess) return e;
+  e = hipEventCreate(&e1);
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return e;
+  }
+  constexpr int reps = 5;
+  int best = 0;
+  float best_t = 1e30f;
+  for (int v = 0; v < kVariants && e == hipSuccess; ++v) {
+    auto run = [&]() {
+      if (mixed) launch_mixed_variant(v, batch, nplanes, maxNb, maxH, EPI_STORE, in_f32, out_f32, st);
+      else launch_variant<F64K, double, double>(v, batch, nplanes, maxNb, maxH, EPI_STORE, st);
+    };
+    run();  // warm-up
+    e = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) run();
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess && ms / reps < best_t) {
+      best_t = ms / reps;
+      best = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return e;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = best;
+  }
+  if (chosen) *chosen = best;
+  if (best_ms) *best_ms = best_t;
+  return hipSuccess;
+}
+
+int dst_tuned_variant(int mixed, int nplanes, int maxNb, int maxH) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(TuneKey{mixed ? 1 : 0, nplanes, maxNb, maxH});
+  return it == g_tune.end() ? -1 : it->second;
+}
+
+int dst_variant_tiles(int v, int* TB, int* TM) {
+  if (v < 0 || v >= kVariants) return 0;
+  *TB = kVarTB[v];
+  *TM = kVarTM[v];
+  return 1;
 }
 
 // ---- sine tables ----

@@ -115,9 +115,45 @@ int run_pass(ibpm_solver* s, PassBatch& pb, int np, bool xdir, int epi, bool in_
   }
   const int H = (Nk + 1) / 2;
   if (s->precision == IBPM_MIXED)
-    HIPCHK(launch_dst_pass_mixed(pb, np, Nb, H, epi, in_f32, out_f32, s->st));
+    HIPCHK(launch_dst_pass_mixed(pb, np, Nb, H, epi, in_f32, out_f32, s->st, s->dst_tiles));
   else
-    HIPCHK(launch_dst_pass_f64(pb, np, Nb, H, epi, s->st));
+    HIPCHK(launch_dst_pass_f64(pb, np, Nb, H, epi, s->st, s->dst_tiles));
+  return IBPM_OK;
+}
+
+// Autotune the sine-pass tile variant of every launch shape the step uses: the batched
+// level solves (L planes), single-plane solves, and the column groups of the B operator.
+int autotune_passes(ibpm_solver* s) {
+  const bool mixed = s->precision == IBPM_MIXED;
+  const size_t P = s->plane();
+  HIPCHK(hipMemsetAsync(s->T, 0, kMaxPlanes * P * sizeof(double), s->st));
+  int counts[3] = {1, s->L, (kMaxPlanes / s->L) * s->L};
+  for (int ci = 0; ci < 3; ++ci) {
+    const int np = counts[ci];
+    bool seen = false;
+    for (int cj = 0; cj < ci; ++cj) seen = seen || counts[cj] == np;
+    if (seen || np < 1 || np > kMaxPlanes) continue;
+    for (int xd = 0; xd < 2; ++xd) {
+      const bool xdir = xd == 0;
+      if (!xdir && s->g.Nx == s->g.Ny) continue;  // same shape as the x pass
+      PassBatch pb;
+      memset(&pb, 0, sizeof pb);
+      const int Nk = xdir ? s->g.Nx : s->g.Ny;
+      const int Nb = xdir ? s->g.Ny : s->g.Nx;
+      for (int i = 0; i < np; ++i) {
+        pb.p[i].in = s->T + i * P;
+        pb.p[i].out = s->U + i * P;
+        pb.p[i].Nk = Nk;
+        pb.p[i].Nb = Nb;
+        pb.p[i].So = tab_odd(s, xdir);
+        pb.p[i].Se = tab_even(s, xdir);
+        pb.p[i].cosb = xdir ? s->cosy : s->cosx;
+        pb.p[i].cosj = xdir ? s->cosx : s->cosy;
+        pb.p[i].dscale = s->scale;
+      }
+      HIPCHK(tune_dst_pass(pb, np, Nb, (Nk + 1) / 2, mixed, mixed, mixed, s->st, nullptr, nullptr));
+    }
+  }
   return IBPM_OK;
 }
 
@@ -774,9 +810,22 @@ void ibpm_default_options(ibpm_options* o) {
   o->cg_abstol = 0.0;
   o->cg_tol_mode = 0;
   o->device = 0;
+  o->dst_tiles = -1;
 }
 
 const char* ibpm_last_error(void) { return g_err.c_str(); }
+
+int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM) {
+  if (!s || !TB || !TM) return fail(IBPM_E_ARG, "null argument");
+  int v = s->dst_tiles;
+  if (v < 0) {
+    const int Nk = xdir ? s->g.Nx : s->g.Ny, Nb = xdir ? s->g.Ny : s->g.Nx;
+    v = dst_tuned_variant(s->precision == IBPM_MIXED, nplanes, Nb, (Nk + 1) / 2);
+    if (v < 0) v = 0;
+  }
+  dst_variant_tiles(v, TB, TM);
+  return IBPM_OK;
+}
 
 int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out) {
   if (!p || !out) return fail(IBPM_E_ARG, "null argument");
@@ -920,6 +969,11 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
     s->lvstride = std::max(nx, ny) / 2 + 1;
     CK(dalloc(&s->rlv, kMaxChains * 4 * (size_t)s->lvstride));
     (void)nbc;
+  }
+  s->dst_tiles = opt.dst_tiles >= 0 && opt.dst_tiles <= 3 ? opt.dst_tiles : -1;
+  if (s->dst_tiles < 0) {
+    int rc = autotune_passes(s);
+    if (rc) return bail(rc);
   }
   // bodies
   const int n = s->n;

@@ -71,10 +71,20 @@ struct PassBatch {
 };
 
 // Host launchers (dst_kernels.hip).
+// variant: tile variant 0..3 ((TB, TM) = (64,64), (64,32), (32,64), (32,32)); -1 takes the
+// autotuned variant of the launch shape (variant 0 when the shape was never tuned).
 hipError_t launch_dst_pass_f64(const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
-                               hipStream_t st);
+                               hipStream_t st, int variant = -1);
 hipError_t launch_dst_pass_mixed(const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
-                                 bool in_f32, bool out_f32, hipStream_t st);
+                                 bool in_f32, bool out_f32, hipStream_t st, int variant = -1);
+// Autotuner: time every tile variant of a pass launch shape (planes, largest Nb and H)
+// once, remember the fastest for later launches of that shape (process-wide). Runs real
+// passes on the given planes (EPI_STORE); not to be called under graph capture.
+hipError_t tune_dst_pass(const PassBatch& b, int nplanes, int maxNb, int maxH, bool mixed,
+                         bool in_f32, bool out_f32, hipStream_t st, int* chosen, float* best_ms);
+int dst_variant_tiles(int v, int* TB, int* TM);
+// Tuned variant of a launch shape, -1 when never tuned.
+int dst_tuned_variant(int mixed, int nplanes, int maxNb, int maxH);
 
 // ---- multi-level ring chain (ring_kernels.hip) ----
 constexpr int kMaxChains = 16;

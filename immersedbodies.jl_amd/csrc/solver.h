@@ -16,6 +16,7 @@ struct ibpm_solver {
   int index = 0;
   int n = 0, nbody = 0, moving = 0;
   int precision = IBPM_FP64, refine = 1;
+  int dst_tiles = -1;  // sine-pass tile variant, -1: autotuned
   int cg_maxiter = 5000, cg_tol_mode = 0;
   double cg_reltol = 1e-12, cg_abstol = 0.0;
   int device = 0;

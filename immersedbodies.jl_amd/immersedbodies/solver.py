@@ -42,10 +42,12 @@ class HipCnabSolver(AbstractSolver):
           5000 / 1e-12 / 0); cg_tol_mode 0 is IterativeSolvers' reltol*||r0||,
           1 is reltol*||b||
       device: HIP device ordinal
+      dst_tiles: sine-pass tile variant; -1 autotunes at creation, 0..3 fix
+                 (TB, TM) = (64,64), (64,32), (32,64), (32,32)
     """
 
     def __init__(self, state: State, *, precision="fp64", refine_sweeps=2, cg=None,
-                 cg_tol_mode=0, device=0):
+                 cg_tol_mode=0, device=0, dst_tiles=-1):
         L = _lib.load()
         self.state = state
         prob = state.prob
@@ -77,6 +79,7 @@ class HipCnabSolver(AbstractSolver):
         o.cg_abstol = float(cg.get("abstol", 0.0))
         o.cg_tol_mode = int(cg_tol_mode)
         o.device = int(device)
+        o.dst_tiles = int(dst_tiles)
         h = C.c_void_p()
         _lib.check(L.ibpm_create(C.byref(p), C.byref(o), C.byref(h)))
         self._h = h
@@ -134,6 +137,13 @@ class HipCnabSolver(AbstractSolver):
     def b_applies(self, stats=None):
         stats = self.last_stats if stats is None else stats
         return np.array([s.b_applies for s in stats])
+
+    def dst_tiles(self, nplanes: int, xdir: bool = True):
+        """(TB, TM) tile shape of the sine passes over ``nplanes`` planes (autotuned)."""
+        tb, tm = C.c_int(), C.c_int()
+        _lib.check(_lib.load().ibpm_dst_tiles(self._h, int(nplanes), int(bool(xdir)),
+                                              C.byref(tb), C.byref(tm)))
+        return tb.value, tm.value
 
     def set_warm_start(self, mode: str = "previous", net=None, record: bool = False):
         """Initial guess of the moving-body CG: "previous" (the reference's previous F̃b),

@@ -86,6 +86,11 @@ typedef struct ibpm_options {
   double cg_abstol;   /* default 0                                               */
   int cg_tol_mode;    /* 0: reltol*||r0|| (IterativeSolvers), 1: reltol*||b||    */
   int device;         /* HIP device ordinal                                      */
+  int dst_tiles;      /* sine-pass tiles: -1 (default) autotuned at creation: every
+                         variant is timed on each launch shape the step uses and the
+                         fastest kept (process-wide, per shape); 0..3 fixed
+                         (TB, TM) = (64,64), (64,32), (32,64), (32,32). Results do not
+                         depend on the variant (same summation order).             */
 } ibpm_options;
 
 typedef struct ibpm_step_stats {
@@ -142,6 +147,10 @@ int ibpm_apply(ibpm_solver* s, int op, int level, const double* in, double* out,
  * bytes: minimum HBM bytes per launch (each input read once, each output written once). */
 int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, double* flops,
                      double* bytes);
+
+/* Instrumentation: tile shape (TB values of the batch index x TM output rows per parity)
+ * that the solver's sine passes of `nplanes` planes along x (xdir != 0) or y use. */
+int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM);
 
 const char* ibpm_last_error(void);
 void ibpm_destroy(ibpm_solver* s);
