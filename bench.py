@@ -81,7 +81,7 @@ def measured_traffic(precision, tiles, levels):
     path = os.path.join(ROOT, "profiles", f"dst_pass_{precision}_cylinder1024.md")
     if levels != 5 or not os.path.exists(path):
         return None
-    key = f"traffic_bytes_per_launch[{tiles[0]},{tiles[1]}]:"
+    key = f"traffic_bytes_per_launch[{tiles[0]},{tiles[1]},{tiles[2]}]:"  # TB, TM, PF
     for line in open(path):
         if line.startswith(key):
             return float(line.split(":", 1)[1])
@@ -268,8 +268,8 @@ def main():
     for prec in paths:
         sv, dt, setup, f, it = time_solver(prob, prec, args.refine, steps, warm)
         roof = kernel_roofline(sv, 0, args.roofline_reps, PEAK[prec])
-        roof["tiles_TB_TM"] = list(sv.dst_tiles(args.levels))
-        roof["traffic"] = measured_traffic(prec, roof["tiles_TB_TM"], args.levels)
+        roof["tiles_TB_TM_PF"] = list(sv.dst_tiles(args.levels))
+        roof["traffic"] = measured_traffic(prec, roof["tiles_TB_TM_PF"], args.levels)
         roof_nl = kernel_roofline(sv, 1, args.roofline_reps, None)
         sv.close()
         runs[prec] = dict(dt=dt, setup=setup, f=f, it=it, roof=roof, roof_nl=roof_nl)

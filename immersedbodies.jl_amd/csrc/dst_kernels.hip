@@ -79,19 +79,19 @@ __device__ __forceinline__ bool xcd_tile(const PassBatch& b, int& z, int& bt, in
 
 // ---------------------------------------------------------------------------------
 // FP64 pass. Work-group tile: TB values of b x (TM odd + TM even) output rows, K chunk
-// 16; TB, TM in {32, 64} are picked per launch shape by the autotuner. 4 waves: wave w
+// KC = 16, loads issued PF chunks ahead of their use; TB, TM in {32, 64} and PF in {1, 2}
+// are picked per launch shape by the autotuner (a deeper prefetch hides more load latency
+// at the cost of registers, hence occupancy). 4 waves: wave w
 // computes parity (w>>1) on b-half (w&1), a TB/2 x TM block as (TB/32) x (TM/16) MFMA
 // 16x16x4 tiles. Input chunk and sine operands are staged through LDS; the global loads
 // of the next chunk are issued before the matrix-core work of the current one.
 // ---------------------------------------------------------------------------------
-constexpr int F64_KC = 16;
-
-template <typename InT, typename OutT, int EPI, int TB, int TM>
+template <typename InT, typename OutT, int EPI, int TB, int TM, int KC, int PF>
 __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch) {
-  constexpr int BS = TB / 32;                // 16-wide b subtiles per wave
-  constexpr int MS = TM / 16;                // 16-wide m subtiles per wave
-  constexpr int RA = F64_KC * TB / 256;      // input elements per thread and chunk
-  constexpr int RB = F64_KC * TM / 256;      // sine elements per thread and chunk
+  constexpr int BS = TB / 32;            // 16-wide b subtiles per wave
+  constexpr int MS = TM / 16;            // 16-wide m subtiles per wave
+  constexpr int RA = KC * TB / 256;      // input elements per thread and chunk
+  constexpr int RB = KC * TM / 256;      // sine elements per thread and chunk
   int z, bt, mt;
   if (!xcd_tile(batch, z, bt, mt)) return;
   const PassPlane& d = batch.p[z];
@@ -101,10 +101,10 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
   const int m0 = mt * TM;
   if (b0 >= Nb || m0 >= H) return;
 
-  __shared__ double Ps[F64_KC][TB + 1];
-  __shared__ double Qs[F64_KC][TB + 1];
-  __shared__ double So_s[F64_KC][TM + 1];
-  __shared__ double Se_s[F64_KC][TM + 1];
+  __shared__ double Ps[KC][TB + 1];
+  __shared__ double Qs[KC][TB + 1];
+  __shared__ double So_s[KC][TM + 1];
+  __shared__ double Se_s[KC][TM + 1];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -120,55 +120,55 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
 #pragma unroll
     for (int j = 0; j < MS; ++j) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
 
-  double ra[RA], rm[RA], rso[RB], rse[RB];
-  auto fetch = [&](int kc) {
+  // PF register stages: the loads of chunk c + PF are issued while chunk c is multiplied
+  double ra[PF][RA], rm[PF][RA], rso[PF][RB], rse[PF][RB];
+  auto fetch = [&](const int st, int kc) {
 #pragma unroll
     for (int r = 0; r < RA; ++r) {
       const int e = tid + 256 * r;
-      const int k1 = kc + (e & 15) + 1;  // 1-based k; 16 lanes read 16 consecutive k
-      const int b = b0 + (e >> 4);
+      const int k1 = kc + e % KC + 1;  // 1-based k; KC lanes read KC consecutive k
+      const int b = b0 + e / KC;
       double va = 0.0, vm = 0.0;
       if (k1 <= H && b < Nb) {
         va = load_in<InT>(d, k1 - 1, b);
         if (k1 < H) vm = load_in<InT>(d, Nk - k1, b);
       }
-      ra[r] = va;
-      rm[r] = vm;
+      ra[st][r] = va;
+      rm[st][r] = vm;
     }
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
       const int e = tid + 256 * r;
       const int k1 = kc + e / TM + 1;
       const int m = m0 + e % TM;
-      rso[r] = (k1 <= H && m < H) ? SoT[(size_t)(k1 - 1) * H + m] : 0.0;
-      rse[r] = (k1 <= H - 1 && m < H - 1) ? SeT[(size_t)(k1 - 1) * (H - 1) + m] : 0.0;
+      rso[st][r] = (k1 <= H && m < H) ? SoT[(size_t)(k1 - 1) * H + m] : 0.0;
+      rse[st][r] = (k1 <= H - 1 && m < H - 1) ? SeT[(size_t)(k1 - 1) * (H - 1) + m] : 0.0;
     }
   };
-  fetch(0);
   const int col = lane & 15;
   const int kq = lane >> 4;
-  for (int kc = 0; kc < H; kc += F64_KC) {
-    // fold into LDS: P = x_k + x_{M-k}, Q = x_k - x_{M-k}; k = H has no partner (Q = 0)
+  // fold into LDS: P = x_k + x_{M-k}, Q = x_k - x_{M-k}; k = H has no partner (Q = 0)
+  auto stage = [&](const int st, int kc) {
 #pragma unroll
     for (int r = 0; r < RA; ++r) {
       const int e = tid + 256 * r;
-      const int kk = e & 15;
+      const int kk = e % KC;
       const bool mid = kc + kk + 1 == H;
-      Ps[kk][e >> 4] = ra[r] + rm[r];
-      Qs[kk][e >> 4] = mid ? 0.0 : ra[r] - rm[r];
+      Ps[kk][e / KC] = ra[st][r] + rm[st][r];
+      Qs[kk][e / KC] = mid ? 0.0 : ra[st][r] - rm[st][r];
     }
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
       const int e = tid + 256 * r;
-      So_s[e / TM][e % TM] = rso[r];
-      Se_s[e / TM][e % TM] = rse[r];
+      So_s[e / TM][e % TM] = rso[st][r];
+      Se_s[e / TM][e % TM] = rse[st][r];
     }
-    __syncthreads();
-    if (kc + F64_KC < H) fetch(kc + F64_KC);
+  };
+  auto multiply = [&]() {
     const double(*A)[TB + 1] = parity == 0 ? Ps : Qs;
     const double(*Bs)[TM + 1] = parity == 0 ? So_s : Se_s;
 #pragma unroll
-    for (int ks = 0; ks < F64_KC / 4; ++ks) {
+    for (int ks = 0; ks < KC / 4; ++ks) {
       const int krow = ks * 4 + kq;
       double av[BS];
 #pragma unroll
@@ -181,7 +181,22 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
           acc[bs][ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[bs], bv, acc[bs][ms], 0, 0, 0);
       }
     }
-    __syncthreads();
+  };
+#pragma unroll
+  for (int st = 0; st < PF; ++st)
+    if (st * KC < H) fetch(st, st * KC);
+  for (int kc = 0; kc < H; kc += PF * KC) {
+#pragma unroll
+    for (int st = 0; st < PF; ++st) {
+      const int k = kc + st * KC;  // the same for every thread: barriers stay uniform
+      if (k < H) {
+        stage(st, k);
+        __syncthreads();
+        if (k + PF * KC < H) fetch(st, k + PF * KC);
+        multiply();
+        __syncthreads();
+      }
+    }
   }
 
   // epilogue: the f64 16x16 accumulator holds D[row = lane/16 + 4r][col = lane%16]
@@ -361,10 +376,13 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
 // ---------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------
-// Tile variants (TB, TM); the autotuner times them on a launch shape and keeps the best.
-constexpr int kVariants = 4;
-constexpr int kVarTB[kVariants] = {64, 64, 32, 32};
-constexpr int kVarTM[kVariants] = {64, 32, 64, 32};
+// Tile variants (TB, TM, prefetch depth); the autotuner times them on a launch shape and
+// keeps the best. The mixed pass has one prefetch stage and uses variants 0..3.
+constexpr int kVariants = 8;
+constexpr int kMixedVariants = 4;
+constexpr int kVarTB[kVariants] = {64, 64, 32, 32, 64, 64, 32, 32};
+constexpr int kVarTM[kVariants] = {64, 32, 64, 32, 64, 32, 64, 32};
+constexpr int kVarPF[kVariants] = {1, 1, 1, 1, 2, 2, 2, 2};
 
 static dim3 tile_grid(PassBatch& b, int nplanes, int maxNb, int maxH, int TB, int TM) {
   b.nbt = (maxNb + TB - 1) / TB;
@@ -374,11 +392,14 @@ static dim3 tile_grid(PassBatch& b, int nplanes, int maxNb, int maxH, int TB, in
   return dim3(8 * ((T + 7) / 8));
 }
 
-template <typename InT, typename OutT, int EPI, int TB, int TM>
-struct F64K {
-  static void go(dim3 g, hipStream_t st, const PassBatch& b) {
-    hipLaunchKernelGGL((dst_pass_f64_kernel<InT, OutT, EPI, TB, TM>), g, dim3(256), 0, st, b);
-  }
+template <int KC, int PF>
+struct F64KC {
+  template <typename InT, typename OutT, int EPI, int TB, int TM>
+  struct K {
+    static void go(dim3 g, hipStream_t st, const PassBatch& b) {
+      hipLaunchKernelGGL((dst_pass_f64_kernel<InT, OutT, EPI, TB, TM, KC, PF>), g, dim3(256), 0, st, b);
+    }
+  };
 };
 template <typename InT, typename OutT, int EPI, int TB, int TM>
 struct MixK {
@@ -410,6 +431,12 @@ static void launch_variant(int v, const PassBatch& b, int nplanes, int maxNb, in
     case 3: launch_epi<K, InT, OutT, 32, 32>(b, nplanes, maxNb, maxH, epi, st); break;
     default: launch_epi<K, InT, OutT, 64, 64>(b, nplanes, maxNb, maxH, epi, st); break;
   }
+}
+
+static void launch_f64_variant(int v, const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
+                               hipStream_t st) {
+  if (v >= 4) launch_variant<F64KC<16, 2>::K, double, double>(v - 4, b, nplanes, maxNb, maxH, epi, st);
+  else launch_variant<F64KC<16, 1>::K, double, double>(v, b, nplanes, maxNb, maxH, epi, st);
 }
 
 static void launch_mixed_variant(int v, const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
@@ -446,13 +473,14 @@ int tuned_variant(int mixed, int nplanes, int maxNb, int maxH) {
 hipError_t launch_dst_pass_f64(const PassBatch& batch, int nplanes, int maxNb, int maxH, int epi,
                                hipStream_t st, int variant) {
   const int v = variant >= 0 && variant < kVariants ? variant : tuned_variant(0, nplanes, maxNb, maxH);
-  launch_variant<F64K, double, double>(v, batch, nplanes, maxNb, maxH, epi, st);
+  launch_f64_variant(v, batch, nplanes, maxNb, maxH, epi, st);
   return hipGetLastError();
 }
 
 hipError_t launch_dst_pass_mixed(const PassBatch& batch, int nplanes, int maxNb, int maxH, int epi,
                                  bool in_f32, bool out_f32, hipStream_t st, int variant) {
-  const int v = variant >= 0 && variant < kVariants ? variant : tuned_variant(1, nplanes, maxNb, maxH);
+  const int v = variant >= 0 && variant < kMixedVariants ? variant
+                                                         : tuned_variant(1, nplanes, maxNb, maxH) % kMixedVariants;
   launch_mixed_variant(v, batch, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
   return hipGetLastError();
 }
@@ -480,13 +508,14 @@ hipError_t tune_dst_pass(const PassBatch& batch, int nplanes, int maxNb, int max
   // three rounds over the variants, best mean of 4 launches per round (interleaving the
   // rounds spreads clock and power-state drift over all variants)
   constexpr int reps = 4, rounds = 3;
+  const int nv = mixed ? kMixedVariants : kVariants;
   float t[kVariants];
   for (int v = 0; v < kVariants; ++v) t[v] = 1e30f;
   for (int round = 0; round < rounds && e == hipSuccess; ++round) {
-    for (int v = 0; v < kVariants && e == hipSuccess; ++v) {
+    for (int v = 0; v < nv && e == hipSuccess; ++v) {
       auto run = [&]() {
         if (mixed) launch_mixed_variant(v, batch, nplanes, maxNb, maxH, EPI_STORE, in_f32, out_f32, st);
-        else launch_variant<F64K, double, double>(v, batch, nplanes, maxNb, maxH, EPI_STORE, st);
+        else launch_f64_variant(v, batch, nplanes, maxNb, maxH, EPI_STORE, st);
       };
       run();  // warm-up
       e = hipEventRecord(e0, st);
@@ -501,7 +530,7 @@ hipError_t tune_dst_pass(const PassBatch& batch, int nplanes, int maxNb, int max
   }
   int best = 0;
   float best_t = t[0];
-  for (int v = 1; v < kVariants; ++v)
+  for (int v = 1; v < nv; ++v)
     if (t[v] < best_t) {
       best_t = t[v];
       best = v;
@@ -524,10 +553,12 @@ int dst_tuned_variant(int mixed, int nplanes, int maxNb, int maxH) {
   return it == g_tune.end() ? -1 : it->second;
 }
 
-int dst_variant_tiles(int v, int* TB, int* TM) {
+int dst_variant_tiles(int v, bool mixed, int* TB, int* TM, int* PF) {
   if (v < 0 || v >= kVariants) return 0;
+  if (mixed) v %= kMixedVariants;
   *TB = kVarTB[v];
   *TM = kVarTM[v];
+  *PF = kVarPF[v];
   return 1;
 }
 

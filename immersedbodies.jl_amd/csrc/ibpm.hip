@@ -815,15 +815,15 @@ void ibpm_default_options(ibpm_options* o) {
 
 const char* ibpm_last_error(void) { return g_err.c_str(); }
 
-int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM) {
-  if (!s || !TB || !TM) return fail(IBPM_E_ARG, "null argument");
+int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM, int* PF) {
+  if (!s || !TB || !TM || !PF) return fail(IBPM_E_ARG, "null argument");
   int v = s->dst_tiles;
   if (v < 0) {
     const int Nk = xdir ? s->g.Nx : s->g.Ny, Nb = xdir ? s->g.Ny : s->g.Nx;
     v = dst_tuned_variant(s->precision == IBPM_MIXED, nplanes, Nb, (Nk + 1) / 2);
     if (v < 0) v = 0;
   }
-  dst_variant_tiles(v, TB, TM);
+  dst_variant_tiles(v, s->precision == IBPM_MIXED, TB, TM, PF);
   return IBPM_OK;
 }
 
@@ -970,7 +970,7 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
     CK(dalloc(&s->rlv, kMaxChains * 4 * (size_t)s->lvstride));
     (void)nbc;
   }
-  s->dst_tiles = opt.dst_tiles >= 0 && opt.dst_tiles <= 3 ? opt.dst_tiles : -1;
+  s->dst_tiles = opt.dst_tiles >= 0 && opt.dst_tiles <= 7 ? opt.dst_tiles : -1;
   if (s->dst_tiles < 0) {
     int rc = autotune_passes(s);
     if (rc) return bail(rc);

@@ -71,8 +71,9 @@ struct PassBatch {
 };
 
 // Host launchers (dst_kernels.hip).
-// variant: tile variant 0..3 ((TB, TM) = (64,64), (64,32), (32,64), (32,32)); -1 takes the
-// autotuned variant of the launch shape (variant 0 when the shape was never tuned).
+// variant: tile variant 0..7: (TB, TM) = (64,64), (64,32), (32,64), (32,32) with loads
+// issued 1 (0..3) or 2 (4..7) K chunks ahead (FP64); the mixed pass uses v % 4. -1 takes
+// the autotuned variant of the launch shape (variant 0 when the shape was never tuned).
 hipError_t launch_dst_pass_f64(const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
                                hipStream_t st, int variant = -1);
 hipError_t launch_dst_pass_mixed(const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
@@ -82,7 +83,7 @@ hipError_t launch_dst_pass_mixed(const PassBatch& b, int nplanes, int maxNb, int
 // passes on the given planes (EPI_STORE); not to be called under graph capture.
 hipError_t tune_dst_pass(const PassBatch& b, int nplanes, int maxNb, int maxH, bool mixed,
                          bool in_f32, bool out_f32, hipStream_t st, int* chosen, float* best_ms);
-int dst_variant_tiles(int v, int* TB, int* TM);
+int dst_variant_tiles(int v, bool mixed, int* TB, int* TM, int* PF);
 // Tuned variant of a launch shape, -1 when never tuned.
 int dst_tuned_variant(int mixed, int nplanes, int maxNb, int maxH);
 

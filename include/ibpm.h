@@ -88,9 +88,10 @@ typedef struct ibpm_options {
   int device;         /* HIP device ordinal                                      */
   int dst_tiles;      /* sine-pass tiles: -1 (default) autotuned at creation: every
                          variant is timed on each launch shape the step uses and the
-                         fastest kept (process-wide, per shape); 0..3 fixed
-                         (TB, TM) = (64,64), (64,32), (32,64), (32,32). Results do not
-                         depend on the variant (same summation order).             */
+                         fastest kept (process-wide, per shape); 0..7 fixed:
+                         (TB, TM) = (64,64), (64,32), (32,64), (32,32) with FP64 loads
+                         1 (0..3) or 2 (4..7) chunks ahead; mixed uses v % 4. Results
+                         do not depend on the variant (same summation order).       */
 } ibpm_options;
 
 typedef struct ibpm_step_stats {
@@ -148,9 +149,10 @@ int ibpm_apply(ibpm_solver* s, int op, int level, const double* in, double* out,
 int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, double* flops,
                      double* bytes);
 
-/* Instrumentation: tile shape (TB values of the batch index x TM output rows per parity)
- * that the solver's sine passes of `nplanes` planes along x (xdir != 0) or y use. */
-int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM);
+/* Instrumentation: tile shape (TB values of the batch index x TM output rows per parity,
+ * loads issued PF K chunks ahead) that the solver's sine passes of `nplanes` planes along x
+ * (xdir != 0) or y use. */
+int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM, int* PF);
 
 const char* ibpm_last_error(void);
 void ibpm_destroy(ibpm_solver* s);

@@ -68,7 +68,7 @@ def test_tile_variants_give_identical_results(precision):
     and the autotuned solver agrees with all of them."""
     prob = cases.cylinder(nx=96, ny=64, nlevel=3, h=0.05, x0=(-1.6, -1.6), Re=100.0)
     ref = None
-    for v in (-1, 0, 1, 2, 3):
+    for v in (-1,) + tuple(range(8)):
         s = ib.create_solver(prob, precision=precision, dst_tiles=v)
         s.advance(4)
         got = np.concatenate([s.read("Gamma"), s.read("psi"), s.read("Ftilde")])
