@@ -60,6 +60,8 @@ def kernel_roofline(solver, kernel, reps, peak_tflops):
     _lib.check(_lib.load().ibpm_time_kernel(solver._h, kernel, reps, C.byref(ms), C.byref(fl),
                                             C.byref(by)))
     sec = ms.value * 1e-3
+    if kernel == 2:
+        return round(fl.value / sec / 1e12, 2)
     if kernel == 0:
         ach = fl.value / sec / 1e12
         return {"bound": "mfma", "achieved": round(ach, 3), "peak": peak_tflops,
@@ -269,6 +271,9 @@ def main():
         sv, dt, setup, f, it = time_solver(prob, prec, args.refine, steps, warm)
         roof = kernel_roofline(sv, 0, args.roofline_reps, PEAK[prec])
         roof["tiles_TB_TM_PF"] = list(sv.dst_tiles(args.levels))
+        ceil = kernel_roofline(sv, 2, 5, None)  # sustained MFMA rate, register-only loop
+        roof["mfma_sustained_tflops"] = ceil
+        roof["frac_of_sustained"] = round(roof["achieved"] / ceil, 4) if ceil else None
         roof["traffic"] = measured_traffic(prec, roof["tiles_TB_TM_PF"], args.levels)
         roof_nl = kernel_roofline(sv, 1, args.roofline_reps, None)
         sv.close()

@@ -161,6 +161,20 @@ __global__ __launch_bounds__(1024) void body_sort_kernel(const BodyDev b) {
     }
     b.xstart[c] = lo;
   }
+  __syncthreads();
+  // rows that panels can reach from vertex column I: a panel at (ix, iy) feeds the rot of
+  // vertex rows J in [iy - 4, iy + 3] of columns I in [ix - 4, ix + 3] (scatter_rot_kernel)
+  for (int I = threadIdx.x; I <= b.nx; I += 1024) {
+    const int c0 = max(I - 3, 0), c1 = min(I + 5, b.nx + 2);
+    int lo = 1 << 30, hi = -(1 << 30);
+    for (int s = b.xstart[c0]; s < b.xstart[c1]; ++s) {
+      const int iy = b.coords[2 * (int)(keys[s] & 8191u) + 1];
+      lo = min(lo, iy - 4);
+      hi = max(hi, iy + 3);
+    }
+    b.ywin[2 * I] = lo;
+    b.ywin[2 * I + 1] = hi;
+  }
 }
 
 hipError_t launch_body_sort(const BodyDev& b, hipStream_t st) {
@@ -171,41 +185,44 @@ hipError_t launch_body_sort(const BodyDev& b, hipStream_t st) {
 
 // ---------------------------------------------------------------------------------
 // E (regT!, coupling.jl:57-69) of the level-1 flux curl(psi1) + q0, minus ub*h
-// (couple_surface_rigid, coupling.jl:239-244). One thread per (panel, component, rhs);
-// the 7 x 7 dot product runs in the reference's column-major order.
+// (couple_surface_rigid, coupling.jl:239-244). One wave per (panel, component, rhs): lane
+// l < 49 forms the product of weight (a, b) = (l % 7, l / 7) with the flux there, and the
+// 49 products are summed by a fixed shuffle tree (deterministic).
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void gather_kernel(const GatherArgs a) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int n = a.npanel;
-  if (t >= 2 * n) return;
+  if (t >= 2 * n) return;  // whole waves leave together
   const int rhs = blockIdx.y;
   const int comp = t >= n;
   const int p = comp ? t - n : t;
   const int nx = a.g.nx, ny = a.g.ny;
   const double* p1 = a.psi1[rhs];
   const double* p2 = a.psi2[rhs];
-  const int ix = a.coords[2 * p], iy = a.coords[2 * p + 1];
-  const double* w = a.w + (size_t)p * 98 + comp * 49;
   double s = 0.0;
-  for (int bb = 0; bb < 7; ++bb) {
-    const int j = iy + bb - 3;
-    for (int aa = 0; aa < 7; ++aa) {
-      const int i = ix + aa - 3;
-      double qv;
-      if (!comp)  // u(i,j) = psi_e(i-1, j) - psi_e(i-1, j-1)
-        qv = (ext_value(p1, p2, nx, ny, i - 1, j, 1.0) - ext_value(p1, p2, nx, ny, i - 1, j - 1, 1.0)) + a.q0u;
-      else  // v(i,j) = psi_e(i-1, j-1) - psi_e(i, j-1)
-        qv = (ext_value(p1, p2, nx, ny, i - 1, j - 1, 1.0) - ext_value(p1, p2, nx, ny, i, j - 1, 1.0)) + a.q0v;
-      s += qv * w[bb * 7 + aa];
-    }
+  if (lane < 49) {
+    const int ix = a.coords[2 * p], iy = a.coords[2 * p + 1];
+    const int bb = lane / 7, aa = lane % 7;
+    const int j = iy + bb - 3, i = ix + aa - 3;
+    double qv;
+    if (!comp)  // u(i,j) = psi_e(i-1, j) - psi_e(i-1, j-1)
+      qv = (ext_value(p1, p2, nx, ny, i - 1, j, 1.0) - ext_value(p1, p2, nx, ny, i - 1, j - 1, 1.0)) + a.q0u;
+    else  // v(i,j) = psi_e(i-1, j-1) - psi_e(i, j-1)
+      qv = (ext_value(p1, p2, nx, ny, i - 1, j - 1, 1.0) - ext_value(p1, p2, nx, ny, i, j - 1, 1.0)) + a.q0v;
+    s = qv * a.w[(size_t)p * 98 + comp * 49 + lane];
   }
-  if (a.ub) s -= a.ub[t] * a.h;
-  a.out[rhs][t] = s;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) {
+    if (a.ub) s -= a.ub[t] * a.h;
+    a.out[rhs][t] = s;
+  }
 }
 
 hipError_t launch_gather(const GatherArgs& a, hipStream_t st) {
   if (a.npanel == 0) return hipSuccess;
-  hipLaunchKernelGGL(gather_kernel, dim3((2 * a.npanel + 255) / 256, a.nrhs), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(gather_kernel, dim3((2 * a.npanel + 3) / 4, a.nrhs), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
@@ -224,7 +241,9 @@ __global__ __launch_bounds__(256) void scatter_rot_kernel(const ScatterArgs a) {
   const int n = a.npanel;
   const int c0 = max(I - 3, 0);
   const int c1 = min(I + 5, g.nx + 2);  // columns ix in [I-3, I+4]
-  const int s0 = a.xstart[c0], s1 = a.xstart[c1];
+  // rows out of every panel's reach skip the panel loop (the sum is zero there)
+  const bool reach = J >= a.ywin[2 * I] && J <= a.ywin[2 * I + 1];
+  const int s0 = a.xstart[c0], s1 = reach ? a.xstart[c1] : s0;
   double ua = 0.0, ub = 0.0, va = 0.0, vb = 0.0;
   const double* f = a.f[rhs];
   for (int s = s0; s < s1; ++s) {
@@ -263,23 +282,34 @@ hipError_t launch_scatter_rot(const ScatterArgs& a, hipStream_t st) {
 // Dense B^-1 product (coupling.jl:191) and update_vars! (problem.jl:676-685):
 //   Ft = Binv * b;  fb = Ft * h / dt;  total = column sums of fb (FP64).
 // ---------------------------------------------------------------------------------
-// 16 rows per work-group; the 16 column groups of a row reduce through LDS.
+// Work-group: 64 rows (one per lane, coalesced column-major reads) x one range of
+// columns split over its 4 waves; the 4 waves reduce through LDS into a partial per column
+// range, and a second kernel sums the ranges in order (deterministic).
+constexpr int GEMV_SPLIT = 16;
+
 __global__ __launch_bounds__(256) void gemv_kernel(int m, const double* __restrict__ M,
                                                    const double* __restrict__ b,
-                                                   double* __restrict__ x) {
-  __shared__ double part[16][17];
-  const int r = threadIdx.x & 15, g = threadIdx.x >> 4;
-  const int i = blockIdx.x * 16 + r;
+                                                   double* __restrict__ part) {
+  __shared__ double red[4][64];
+  const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + r;
+  const int per = (m + GEMV_SPLIT - 1) / GEMV_SPLIT;
+  const int j0 = blockIdx.y * per, j1 = min(m, j0 + per);
   double acc = 0.0;
   if (i < m)
-    for (int j = g; j < m; j += 16) acc += M[i + (size_t)j * m] * b[j];
-  part[g][r] = acc;
+    for (int j = j0 + g; j < j1; j += 4) acc += M[i + (size_t)j * m] * b[j];
+  red[g][r] = acc;
   __syncthreads();
-  if (g == 0 && i < m) {
-    double s = 0.0;
-    for (int k = 0; k < 16; ++k) s += part[k][r];
-    x[i] = s;
-  }
+  if (g == 0 && i < m)
+    part[(size_t)blockIdx.y * m + i] = ((red[0][r] + red[1][r]) + red[2][r]) + red[3][r];
+}
+
+__global__ void gemv_sum_kernel(int m, const double* __restrict__ part, double* __restrict__ x) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  double s = 0.0;
+  for (int k = 0; k < GEMV_SPLIT; ++k) s += part[(size_t)k * m + i];
+  x[i] = s;
 }
 
 __global__ __launch_bounds__(1024) void forces_kernel(int n, const double* __restrict__ Ft,
@@ -311,9 +341,13 @@ __global__ __launch_bounds__(1024) void forces_kernel(int n, const double* __res
   }
 }
 
-hipError_t launch_gemv(int m, const double* M, const double* b, double* x, hipStream_t st) {
-  if (m == 0) return hipSuccess;
-  hipLaunchKernelGGL(gemv_kernel, dim3((m + 15) / 16), dim3(256), 0, st, m, M, b, x);
+size_t gemv_partial_count(int m) { return (size_t)GEMV_SPLIT * (size_t)(m > 0 ? m : 1); }
+
+hipError_t launch_gemv(int m, const double* M, const double* b, double* x, double* part,
+                       hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gemv_kernel, dim3((m + 63) / 64, GEMV_SPLIT), dim3(256), 0, st, m, M, b, part);
+  hipLaunchKernelGGL(gemv_sum_kernel, dim3((m + 255) / 256), dim3(256), 0, st, m, part, x);
   return hipGetLastError();
 }
 

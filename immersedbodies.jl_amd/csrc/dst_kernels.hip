@@ -562,6 +562,56 @@ int dst_variant_tiles(int v, bool mixed, int* TB, int* TM, int* PF) {
   return 1;
 }
 
+// ---- matrix-core ceiling ----
+// Register-only loops of the two MFMA instructions the passes use (8 independent
+// accumulators per wave, 8 work-groups of 4 waves per CU): the rate the matrix cores
+// sustain on this device, the ceiling the passes are measured against beside the
+// published peak.
+__global__ __launch_bounds__(256) void mfma_f64_probe_kernel(double* out, int iters) {
+  d4 acc[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) acc[i] = (d4){0.0, 0.0, 0.0, 0.0};
+  const double a = 1e-3 * threadIdx.x, b = 1e-3 * blockIdx.x;
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i], 0, 0, 0);
+  }
+  double sum = 0.0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) sum += acc[i][0] + acc[i][1] + acc[i][2] + acc[i][3];
+  out[blockIdx.x * 256 + threadIdx.x] = sum;
+}
+
+__global__ __launch_bounds__(256) void mfma_bf16_probe_kernel(double* out, int iters) {
+  f4 acc[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) acc[i] = (f4){0.f, 0.f, 0.f, 0.f};
+  bf8 a, b;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    a[i] = (__bf16)(1e-3f * threadIdx.x);
+    b[i] = (__bf16)(1e-3f * blockIdx.x);
+  }
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
+  }
+  float sum = 0.f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) sum += acc[i][0] + acc[i][1] + acc[i][2] + acc[i][3];
+  out[blockIdx.x * 256 + threadIdx.x] = sum;
+}
+
+hipError_t launch_mfma_probe(bool bf16, int iters, int blocks, double* out, hipStream_t st,
+                             double* flops) {
+  if (bf16) hipLaunchKernelGGL(mfma_bf16_probe_kernel, dim3(blocks), dim3(256), 0, st, out, iters);
+  else hipLaunchKernelGGL(mfma_f64_probe_kernel, dim3(blocks), dim3(256), 0, st, out, iters);
+  // 4 waves per block, 8 instructions per iteration: 16x16x4 f64 = 2048 flops,
+  // 16x16x32 bf16 = 16384 flops
+  *flops = (double)blocks * 4.0 * iters * 8.0 * (bf16 ? 16384.0 : 2048.0);
+  return hipGetLastError();
+}
+
 // ---- sine tables ----
 static inline double sine_entry(long long num, long long M) {
   // 2 sin(pi num / M) with exact integer reduction of the argument modulo 2M

@@ -77,6 +77,7 @@ void free_all(ibpm_solver* s) {
                   s->cosx, s->cosy, s->SoX, s->SeX, s->SoY, s->SeY, s->BfX, s->BfY,
                   s->xb, s->ub, s->xb_ref, s->Ft, s->fb, s->Ftmp, s->total, s->motion,
                   s->body_of, s->moving_d, s->coords, s->perm, s->xstart, s->err, s->w,
+                  s->ywin, s->gemv_part,
                   s->Binv, s->Gtmp, s->psitmp, s->Sc, s->cg_r, s->cg_u, s->cg_c, s->cg_scal,
                   s->edge, s->pw, s->fac, s->rpart, s->rlv, s->ws_h0, s->ws_h1, s->ws_xe,
                   s->ws_Bxe, s->ws_scal};
@@ -392,6 +393,7 @@ int apply_B(ibpm_solver* s, int ncol, const double* const* z, double* const* x) 
     sa.w = s->w;
     sa.perm = s->perm;
     sa.xstart = s->xstart;
+    sa.ywin = s->ywin;
     for (int c = 0; c < nc; ++c) {
       sa.f[c] = z[c0 + c];
       sa.out[c] = s->Sc + (size_t)(c0 + c) * P;
@@ -540,6 +542,7 @@ BodyDev body_dev(ibpm_solver* s) {
   b.w = s->w;
   b.perm = s->perm;
   b.xstart = s->xstart;
+  b.ywin = s->ywin;
   b.err = s->err;
   return b;
 }
@@ -563,7 +566,7 @@ int couple_surface(ibpm_solver* s, ibpm_step_stats* stats) {
   ga.h = s->h;
   HIPCHK(launch_gather(ga, s->st));
   if (!s->moving) {
-    HIPCHK(launch_gemv(2 * s->n, s->Binv, s->Ftmp, s->Ft, s->st));
+    HIPCHK(launch_gemv(2 * s->n, s->Binv, s->Ftmp, s->Ft, s->gemv_part, s->st));
     if (stats) {
       stats->cg_iters = 0;
       stats->cg_converged = 1;
@@ -753,6 +756,7 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
     sa.w = s->w;
     sa.perm = s->perm;
     sa.xstart = s->xstart;
+    sa.ywin = s->ywin;
     sa.f[0] = s->Ft;
     sa.out[0] = s->rhs;
     HIPCHK(launch_scatter_rot(sa, s->st));
@@ -990,6 +994,8 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   CK(dalloc(&s->coords, 2 * (size_t)n));
   CK(dalloc(&s->perm, (size_t)n));
   CK(dalloc(&s->xstart, (size_t)s->g.nx + 3));
+  CK(dalloc(&s->ywin, 2 * ((size_t)s->g.nx + 1)));
+  CK(dalloc(&s->gemv_part, gemv_partial_count(2 * n)));
   CK(dalloc(&s->err, 1));
   CK(dalloc(&s->w, 98 * (size_t)n));
   CK(dalloc(&s->cg_scal, 8));
@@ -1312,7 +1318,14 @@ int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, doubl
   const bool mixed = s->precision == IBPM_MIXED;
   NonlinArgs na;
   PassBatch pb;
+  double probe_flops = 0.0;
+  const int probe_blocks = 8 * 256;  // needs probe_blocks x 256 doubles of scratch
   auto launch = [&]() -> int {
+    if (kernel == 2) {
+      if ((size_t)probe_blocks * 256 > kMaxPlanes * P) return fail(IBPM_E_ARG, "grid too small for the probe");
+      HIPCHK(launch_mfma_probe(mixed, 1000, probe_blocks, s->T, s->st, &probe_flops));
+      return IBPM_OK;
+    }
     if (kernel == 0) {
       memset(&pb, 0, sizeof pb);
       for (int i = 0; i < L; ++i) {
@@ -1345,7 +1358,7 @@ int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, doubl
     na.beta1 = 1.5;
     na.beta2 = -0.5;
     if (s->L > kMaxPlanes) return fail(IBPM_E_ARG, "too many levels for the timing scratch");
-  } else if (kernel != 0) {
+  } else if (kernel != 0 && kernel != 2) {
     return fail(IBPM_E_ARG, "unknown kernel");
   }
   int rc = launch();  // warm-up
@@ -1365,7 +1378,10 @@ int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, doubl
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *avg_ms = ms / reps;
-  if (kernel == 0) {
+  if (kernel == 2) {
+    *flops = probe_flops;
+    *bytes = 0.0;
+  } else if (kernel == 0) {
     const double Nk = s->g.Nx, Nb = s->g.Ny, H = (Nk + 1) / 2;
     const double per_product = 2.0 * Nb * (H * H + (H - 1) * (H - 1));
     *flops = L * per_product * (mixed ? 3.0 : 1.0);

@@ -84,6 +84,10 @@ hipError_t launch_dst_pass_mixed(const PassBatch& b, int nplanes, int maxNb, int
 hipError_t tune_dst_pass(const PassBatch& b, int nplanes, int maxNb, int maxH, bool mixed,
                          bool in_f32, bool out_f32, hipStream_t st, int* chosen, float* best_ms);
 int dst_variant_tiles(int v, bool mixed, int* TB, int* TM, int* PF);
+// Register-only MFMA loop (f64 16x16x4 or bf16 16x16x32) on `blocks` x 256 threads; out
+// needs blocks x 256 doubles. flops: matrix flops of the launch.
+hipError_t launch_mfma_probe(bool bf16, int iters, int blocks, double* out, hipStream_t st,
+                             double* flops);
 // Tuned variant of a launch shape, -1 when never tuned.
 int dst_tuned_variant(int mixed, int nplanes, int maxNb, int maxH);
 
@@ -183,6 +187,7 @@ struct BodyDev {
   double* w;          // n x 98
   int* perm;          // n, panels sorted by ix
   int* xstart;        // nx + 3
+  int* ywin;          // 2 x (nx + 1): rows of vertex column I that panels can reach
   int* err;           // first offending panel + 1, 0 if none
 };
 hipError_t launch_body_update(const BodyDev& b, int only_moving, hipStream_t st);
@@ -214,14 +219,17 @@ struct ScatterArgs {
   const double* w;
   const int* perm;
   const int* xstart;
+  const int* ywin;    // [2I], [2I+1]: first and last row J of vertex column I with panels
   const double* f[64];
   void* out[64];
   int out_f32;
 };
 hipError_t launch_scatter_rot(const ScatterArgs& a, hipStream_t st);
 
-// x = M b for a dense column-major m x m matrix.
-hipError_t launch_gemv(int m, const double* M, const double* b, double* x, hipStream_t st);
+// x = M b for a dense column-major m x m matrix; part: gemv_partial_count(m) doubles.
+size_t gemv_partial_count(int m);
+hipError_t launch_gemv(int m, const double* M, const double* b, double* x, double* part,
+                       hipStream_t st);
 // fb = Ft*h/dt and the total force (column sums), FP64.
 hipError_t launch_forces(int n, const double* Ft, double* fb, double* total, double h, double dt,
                          hipStream_t st);

@@ -40,7 +40,8 @@ struct ibpm_solver {
   double *xb = nullptr, *ub = nullptr, *xb_ref = nullptr, *Ft = nullptr, *fb = nullptr;
   double *Ftmp = nullptr, *total = nullptr, *motion = nullptr;
   int *body_of = nullptr, *moving_d = nullptr, *coords = nullptr, *perm = nullptr;
-  int *xstart = nullptr, *err = nullptr;
+  int *xstart = nullptr, *err = nullptr, *ywin = nullptr;
+  double* gemv_part = nullptr;
   double* w = nullptr;
   double* Binv = nullptr;
   // B operator scratch (ncol columns)

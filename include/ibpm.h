@@ -144,6 +144,8 @@ int ibpm_apply(ibpm_solver* s, int op, int level, const double* in, double* out,
  * the step with HIP events on the solver's stream, on the solver's current fields.
  *   kernel 0: sine-contraction pass of the batched implicit solve (all levels at once)
  *   kernel 1: fused nonlinear term + CNAB right-hand side (all levels)
+ *   kernel 2: register-only loop of the MFMA instruction of the solver's precision
+ *             (f64 16x16x4 or bf16 16x16x32): the matrix-core rate the device sustains
  * avg_ms: mean launch duration; flops: MFMA flops executed per launch (0 for kernel 1);
  * bytes: minimum HBM bytes per launch (each input read once, each output written once). */
 int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, double* flops,
