@@ -307,8 +307,12 @@ __global__ __launch_bounds__(256) void gemv_kernel(int m, const double* __restri
 __global__ void gemv_sum_kernel(int m, const double* __restrict__ part, double* __restrict__ x) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= m) return;
+  double v[GEMV_SPLIT];
+#pragma unroll
+  for (int k = 0; k < GEMV_SPLIT; ++k) v[k] = part[(size_t)k * m + i];
   double s = 0.0;
-  for (int k = 0; k < GEMV_SPLIT; ++k) s += part[(size_t)k * m + i];
+#pragma unroll
+  for (int k = 0; k < GEMV_SPLIT; ++k) s += v[k];
   x[i] = s;
 }
 

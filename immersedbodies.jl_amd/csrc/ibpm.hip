@@ -79,7 +79,9 @@ void free_all(ibpm_solver* s) {
                   s->body_of, s->moving_d, s->coords, s->perm, s->xstart, s->err, s->w,
                   s->ywin, s->gemv_part,
                   s->Binv, s->Gtmp, s->psitmp, s->Sc, s->cg_r, s->cg_u, s->cg_c, s->cg_scal,
-                  s->edge, s->pw, s->fac, s->rpart, s->rlv, s->ws_h0, s->ws_h1, s->ws_xe,
+                  s->edge, s->pw, s->fac, s->rvec, s->rpart, s->Mx,
+                  s->My != s->Mx ? s->My : nullptr,
+                  s->ws_h0, s->ws_h1, s->ws_xe,
                   s->ws_Bxe, s->ws_scal};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -268,8 +270,8 @@ int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double
   }
   rc = run_pass(s, pb, np, false, EPI_STORE, mixed, mixed);
   if (rc) return rc;
-  // ring chain, coarse to fine: E_l = D . (F_l + R_l) in place and, for l >= 1, the ring
-  // factors of level l-1 from E_l
+  // ring chain, coarse to fine: E_l = D . (F_l + R_l) into T and, for l >= 1, the ring
+  // factors of level l-1 from the projections of E_l
   RingChainArgs ra;
   memset(&ra, 0, sizeof ra);
   ra.nx = s->g.nx;
@@ -282,37 +284,34 @@ int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double
   ra.scale = s->scale;
   ra.edge = s->edge;
   ra.pw = s->pw;
-  ra.SoX = (const double*)s->SoX;
-  ra.SeX = (const double*)s->SeX;
-  ra.SoY = (const double*)s->SoY;
-  ra.SeY = (const double*)s->SeY;
-  ra.lvstride = s->lvstride;
+  ra.Mx = s->Mx;
+  ra.My = s->My;
+  const size_t nv = ring_vector_count(s->g.Nx, s->g.Ny);
   const size_t npart = ring_partial_count(s->g.Nx, s->g.Ny);
-  const size_t npa = ring_partial_a_count(s->g.Nx, s->g.Ny);
+  const size_t npa = 2 * (size_t)((s->g.Ny + 63) / 64) * s->g.Nx;  // part_a: 2 x tiles_y x Nx
   for (int l = L - 1; l >= 0; --l) {
     for (int c = 0; c < ng; ++c) {
       ra.F[c] = s->U + (c * L + l) * P;
+      ra.E[c] = s->T + (c * L + l) * P;
       ra.fac_in[c] = l == L - 1 ? nullptr : s->fac + (c * L + l) * nf;
       ra.fac_out[c] = l > 0 ? s->fac + (c * L + l - 1) * nf : nullptr;
+      ra.vec[c] = s->rvec + c * nv;
       ra.part_a[c] = s->rpart + c * npart;
       ra.part_c[c] = s->rpart + c * npart + npa;
-      ra.lv[c] = s->rlv + (size_t)c * 4 * s->lvstride;
     }
-    if (l > 0)
-      HIPCHK(launch_ring_step(ra, ng, s->st));
-    else
-      HIPCHK(launch_ring_project(ra, ng, false, s->st));
+    HIPCHK(launch_ring_sweep(ra, ng, l > 0, s->st));
+    if (l > 0) HIPCHK(launch_ring_factor(ra, ng, s->st));
   }
   memset(&pb, 0, sizeof pb);
   for (int p = 0; p < np; ++p) {
-    pb.p[p].in = s->U + p * P;
-    pb.p[p].out = s->T + p * P;
+    pb.p[p].in = s->T + p * P;
+    pb.p[p].out = s->U + p * P;
   }
   rc = run_pass(s, pb, np, true, EPI_STORE, mixed, mixed);
   if (rc) return rc;
   memset(&pb, 0, sizeof pb);
   for (int p = 0; p < np; ++p) {
-    pb.p[p].in = s->T + p * P;
+    pb.p[p].in = s->U + p * P;
     pb.p[p].out = out[p];
     pb.p[p].base = out[p];
   }
@@ -969,10 +968,37 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
     CK(hipMemcpy(s->pw, pw.data(), pw.size() * 8, hipMemcpyHostToDevice));
     const size_t nf = ed.size(), nbc = 2 * (size_t)(nx + 1) + 2 * (size_t)(ny + 1);
     CK(dalloc(&s->fac, kMaxPlanes * nf));
+    CK(dalloc(&s->rvec, kMaxChains * ring_vector_count(Nx, Ny)));
     CK(dalloc(&s->rpart, kMaxChains * ring_partial_count(Nx, Ny)));
-    s->lvstride = std::max(nx, ny) / 2 + 1;
-    CK(dalloc(&s->rlv, kMaxChains * 4 * (size_t)s->lvstride));
     (void)nbc;
+    // ring operators M_n = S_n Interp Ring S_n, one FP64 sine pass each (ring_kernels.hip)
+    for (int dir = 0; dir < 2; ++dir) {
+      const int N = dir == 0 ? Nx : Ny;
+      if (dir == 1 && Ny == Nx) {
+        s->My = s->Mx;
+        break;
+      }
+      std::vector<double> Bh((size_t)N * N);
+      build_ring_operator_input(N + 1, Bh.data());
+      double* Bd = nullptr;
+      double*& Md = dir == 0 ? s->Mx : s->My;
+      CK(dalloc(&Md, (size_t)N * N));
+      CK(dalloc(&Bd, (size_t)N * N));
+      CK(hipMemcpy(Bd, Bh.data(), Bh.size() * 8, hipMemcpyHostToDevice));
+      PassBatch pb;
+      memset(&pb, 0, sizeof pb);
+      pb.p[0].in = Bd;
+      pb.p[0].out = Md;
+      pb.p[0].Nk = N;
+      pb.p[0].Nb = N;
+      pb.p[0].So = dir == 0 ? s->SoX : s->SoY;
+      pb.p[0].Se = dir == 0 ? s->SeX : s->SeY;
+      pb.p[0].cosb = dir == 0 ? s->cosx : s->cosy;
+      pb.p[0].cosj = pb.p[0].cosb;
+      CK(launch_dst_pass_f64(pb, 1, N, (N + 1) / 2, EPI_STORE, s->st, 0));
+      CK(hipStreamSynchronize(s->st));
+      (void)hipFree(Bd);
+    }
   }
   s->dst_tiles = opt.dst_tiles >= 0 && opt.dst_tiles <= 7 ? opt.dst_tiles : -1;
   if (s->dst_tiles < 0) {

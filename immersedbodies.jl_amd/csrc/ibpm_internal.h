@@ -95,30 +95,32 @@ int dst_tuned_variant(int mixed, int nplanes, int maxNb, int maxH);
 constexpr int kMaxChains = 16;
 struct RingChainArgs {
   int nx, ny, Nx, Ny;
-  void* F[kMaxChains];              // spectral plane of the level, per chain (E in place)
+  const void* F[kMaxChains];        // spectral plane of the level, per chain
+  void* E[kMaxChains];              // E = D . (F + R), written (same precision as F)
   int F_f32;
   const double* cosx;               // cos(pi kx / nx), kx = 1..Nx
   const double* cosy;
   double scale;                     // 1 / (4 nx ny)
-  double* lv[kMaxChains];           // streamfunction on the 4 ring lines (4 x lvstride)
-  int lvstride;
   const double* fac_in[kMaxChains]; // ring factors of this level (null: none)
   double* fac_out[kMaxChains];      // ring factors of the next finer level
-  double* part_a[kMaxChains];
+  double* vec[kMaxChains];          // projections a_b, a_t (Nx each), c_l, c_r (Ny each)
+  double* part_a[kMaxChains];       // per-tile partial projections (ring_partial_count)
   double* part_c[kMaxChains];
   const double* edge;               // sx1, sxN, sy1, syN
   const double* pw;                 // S_y[:, ny/4], S_y[:, 3ny/4], S_x[nx/4, :], S_x[3nx/4, :]
-  const double* SoX;                // folded FP64 sine tables (see dst_kernels.hip)
-  const double* SeX;
-  const double* SoY;
-  const double* SeY;
+  const double* Mx;                 // Nx x Nx ring operator (row-major)
+  const double* My;                 // Ny x Ny
 };
-// E = D . (F + R) in place, plus (do_proj) the ring projections of the level.
-hipError_t launch_ring_project(const RingChainArgs& a, int nchain, bool do_proj, hipStream_t st);
-// project + ring line values + factors of the next finer level.
-hipError_t launch_ring_step(const RingChainArgs& a, int nchain, hipStream_t st);
+// E = D . (F + R) of every chain and, with do_proj, the four projections (vec): a sweep
+// and a reduction launch.
+hipError_t launch_ring_sweep(const RingChainArgs& a, int nchain, bool do_proj, hipStream_t st);
+// fac_out = (M_x a_b, M_x a_t, M_y c_l, M_y c_r).
+hipError_t launch_ring_factor(const RingChainArgs& a, int nchain, hipStream_t st);
+size_t ring_vector_count(int Nx, int Ny);
 size_t ring_partial_count(int Nx, int Ny);
-size_t ring_partial_a_count(int Nx, int Ny);
+// Input plane (N x N, k contiguous per column) whose sine pass along k is the ring operator
+// M_n (row-major): projection at level l -> factor at level l-1, for an axis of n cells.
+void build_ring_operator_input(int n, double* B);
 
 // Sine tables for a DST-I of length N (N+1 even). Host builds, returns element counts.
 void build_sine_tables_f64(int N, double* So, double* Se);

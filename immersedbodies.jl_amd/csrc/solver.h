@@ -24,7 +24,6 @@ struct ibpm_solver {
   double fs[2] = {0.0, 0.0};
   std::vector<double> hc;  // grid step per level
   double scale = 0.0;      // 1 / (4 nx ny): the two unnormalised DST-I scale factors
-  int lvstride = 0;        // ring-line length: max(nx, ny)/2 + 1
 
   // fields
   double *q = nullptr, *G = nullptr, *psi = nullptr, *nl1 = nullptr, *nl2 = nullptr;
@@ -35,7 +34,8 @@ struct ibpm_solver {
   void *BfX = nullptr, *BfY = nullptr;                                  // bf16 tables
   // multi-level ring chain: sine rows/columns, per-plane factors, per-chain scratch
   double *edge = nullptr, *pw = nullptr;
-  double *fac = nullptr, *rpart = nullptr, *rlv = nullptr;
+  double *fac = nullptr, *rvec = nullptr, *rpart = nullptr;
+  double *Mx = nullptr, *My = nullptr;  // ring operators (My aliases Mx when nx == ny)
   // bodies
   double *xb = nullptr, *ub = nullptr, *xb_ref = nullptr, *Ft = nullptr, *fb = nullptr;
   double *Ftmp = nullptr, *total = nullptr, *motion = nullptr;
