@@ -192,7 +192,7 @@ def batch_bench(args):
 
     probs = cases.ensemble(args.batch_problems, seed=2024)
     t0 = time.perf_counter()
-    sol = [ib.create_solver(p, precision=args.batch_precision, refine_sweeps=args.refine)
+    sol = [ib.create_solver(p, precision=args.batch_prec, refine_sweeps=args.refine)
            for p in probs]
     setup = time.perf_counter() - t0
     log(f"batch: {len(sol)} solvers set up in {setup:.1f} s")
@@ -221,7 +221,7 @@ def batch_bench(args):
                        "levels, Re U[50,300], dt = h/5, numpy default_rng(2024)",
            "problems": n, "grid_sizes": {str(k): sizes.count(k) for k in sorted(set(sizes))},
            "panels_min_max": [min(p.bodies.npanel for p in probs), max(p.bodies.npanel for p in probs)],
-           "steps": args.batch_steps, "precision": args.batch_precision,
+           "steps": args.batch_steps, "precision": args.batch_prec,
            "setup_s": round(setup, 2), "capture_s": round(t_cap, 3),
            "problem_steps_per_s": round(n * args.batch_steps / t_b, 2),
            "ensemble_steps_per_s": round(args.batch_steps / t_b, 3),
@@ -229,7 +229,8 @@ def batch_bench(args):
            "speedup_vs_sequential": round(t_seq / t_b, 3),
            "level_cell_updates_per_s": round(cells * args.batch_steps / t_b, 1),
            "forces_finite": bool(np.all(np.isfinite(forces)))}
-    log(f"batch: {res['problem_steps_per_s']} problem-steps/s, x{res['speedup_vs_sequential']} vs sequential")
+    log(f"batch {args.batch_prec}: {res['problem_steps_per_s']} problem-steps/s, "
+        f"x{res['speedup_vs_sequential']} vs sequential")
     return res
 
 
@@ -254,7 +255,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble section")
     ap.add_argument("--batch-problems", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=50)
-    ap.add_argument("--batch-precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--batch-precision", default="both", choices=["both", "mixed", "fp64"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -324,7 +325,11 @@ def main():
     if not args.no_ws:
         res["warm_start"] = warm_start_bench(args)
     if not args.no_batch:
-        res["batch"] = batch_bench(args)
+        precs = ["mixed", "fp64"] if args.batch_precision == "both" else [args.batch_precision]
+        res["batch"] = {}
+        for prec in precs:
+            args.batch_prec = prec
+            res["batch"][prec] = batch_bench(args)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(prob, args.cpu_steps)
     print(json.dumps(res), flush=True)

@@ -86,3 +86,31 @@ def test_outside_region_error():
     prob = cases.cylinder(nx=16, ny=20, nlevel=3, h=0.1, x0=(-1.0, -1.0), Re=20.0, dt=0.01)
     with pytest.raises(Exception, match="outside allowed region"):
         ib.create_solver(prob)
+
+
+def test_cylinder_200_full_run_matches_oracle():
+    """The reference's CPU-sized workload end to end: 200 x 200 cells on [-1,3] x [-2,2],
+    3 levels, Re 100, dt 0.004, 157 points, impulsive start, 500 steps. FP64 forces and
+    fields stay within the FP64 tolerances of the oracle over the whole run; the mixed
+    path's lift/drag history stays within 1e-6. Symmetry keeps the lift at round-off in
+    the first steps."""
+    import oracle as O
+
+    O.set_threads(16)
+    prob = cases.cylinder_200()
+    orc = cases.oracle_for(prob)
+    nsteps = 500
+    f_ref = cases.oracle_advance(orc, prob, nsteps)
+    fp64 = ib.create_solver(prob)
+    f64 = fp64.forces(fp64.advance(nsteps))
+    err = np.linalg.norm(f64 - f_ref, axis=1) / np.linalg.norm(f_ref, axis=1)
+    print("fp64 force rel err max", err.max(), "final drag", f64[-1, 0], flush=True)
+    assert err.max() <= FORCE_TOL
+    _compare_state(fp64, orc, fields=["Gamma", "psi", "q", "Ftilde", "fb"])
+    mixed = ib.create_solver(prob, precision="mixed")
+    fm = mixed.forces(mixed.advance(nsteps))
+    errm = np.linalg.norm(fm - f_ref, axis=1) / np.linalg.norm(f_ref, axis=1)
+    print("mixed force rel err max", errm.max(), flush=True)
+    assert errm.max() <= 1e-6
+    # y-symmetric body and flow: lift stays at round-off early on
+    assert np.abs(f64[:20, 1]).max() <= 1e-10 * np.abs(f64[:20, 0]).max()
