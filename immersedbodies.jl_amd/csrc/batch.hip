@@ -148,6 +148,9 @@ int ibpm_batch_create(int nprob, ibpm_solver* const* solvers, const double* free
     if (s->moving)
       return set_error(IBPM_E_ARG, "ibpm_batch_create: moving bodies cannot be batched "
                                    "(their CG synchronises with the host)");
+    if (s->grid_moving)
+      return set_error(IBPM_E_ARG, "ibpm_batch_create: a moving grid's base flux changes every "
+                                   "step; batch static grids only");
     if (s->device != dev) return set_error(IBPM_E_ARG, "ibpm_batch_create: solvers on different devices");
     for (int q = 0; q < p; ++q)
       if (solvers[q] == s) return set_error(IBPM_E_ARG, "ibpm_batch_create: a solver appears twice");

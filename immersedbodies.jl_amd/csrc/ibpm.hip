@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/ibpm.h"
+#include "ibpm_device.h"
 #include "ibpm_internal.h"
 #include "solver.h"
 
@@ -435,7 +436,7 @@ int apply_B(ibpm_solver* s, int ncol, const double* const* z, double* const* x) 
       ga.psi2[c] = L > 1 ? pc[c0 + c] + P : nullptr;
       ga.out[c] = x[c0 + c];
     }
-    ga.q0u = ga.q0v = 0.0;
+    memset(&ga.bf, 0, sizeof ga.bf);  // B has no base flux
     ga.ub = nullptr;
     ga.h = s->h;
     HIPCHK(launch_gather(ga, s->st));
@@ -559,8 +560,7 @@ int couple_surface(ibpm_solver* s, ibpm_step_stats* stats) {
   ga.psi1[0] = s->psi;
   ga.psi2[0] = s->L > 1 ? s->psi + P : nullptr;
   ga.out[0] = s->Ftmp;
-  ga.q0u = s->fs[0] * s->hc[0];
-  ga.q0v = s->fs[1] * s->hc[0];
+  ga.bf = base_flux(s);
   ga.ub = s->ub;
   ga.h = s->h;
   HIPCHK(launch_gather(ga, s->st));
@@ -687,6 +687,8 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
   s->index += 1;
   s->fs[0] = fsv ? fsv[0] : 0.0;
   s->fs[1] = fsv ? fsv[1] : 0.0;
+  if (s->grid_moving)  // MovingGrid: the grid's GridVelocity at the new time
+    for (int k = 0; k < 6; ++k) s->gm[k] = fsv ? fsv[2 + k] : 0.0;
   s->q0_shadow_valid = false;
   // _update_body_motion! (bodies.jl:43-51)
   if (s->moving) {
@@ -706,10 +708,9 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
   na.nl_new = s->nl1;
   na.nl_old = s->nl2;
   na.rhs = s->rhs;
+  na.bf = base_flux(s);
   for (int l = 0; l < L; ++l) {
     const double hc = s->hc[l];
-    na.q0u[l] = s->fs[0] * hc;
-    na.q0v[l] = s->fs[1] * hc;
     na.inv_hc2[l] = 1.0 / (hc * hc);
     na.cA[l] = cA(s, l);
     na.fac_bc[l] = 0.25 * s->dt / (s->Re * (hc * hc));
@@ -796,6 +797,37 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
 }  // namespace
 
 namespace ibpm {
+BaseFlux base_flux(const ibpm_solver* s) {
+  BaseFlux b;
+  memset(&b, 0, sizeof b);
+  b.moving = s->grid_moving;
+  for (int l = 0; l < s->L && l < 16; ++l) {
+    const double hc = s->hc[l];
+    b.q0u[l] = s->fs[0] * hc;
+    b.q0v[l] = s->fs[1] * hc;
+    b.hc[l] = hc;
+    // subdomain (problem.jl:424-429): x0 + n (h - hc) / 2
+    b.x0[l] = s->x0 + (s->g.nx * (s->h - hc)) / 2;
+    b.y0[l] = s->y0 + (s->g.ny * (s->h - hc)) / 2;
+  }
+  b.ux = s->fs[0];
+  b.uy = s->fs[1];
+  b.cx = s->gm[0];
+  b.cy = s->gm[1];
+  b.vx = s->gm[2];
+  b.vy = s->gm[3];
+  const double c = cos(s->gm[4]), sn = sin(s->gm[4]), om = s->gm[5];
+  b.rx11 = c;
+  b.rx21 = sn;
+  b.rx12 = -sn;
+  b.rx22 = c;
+  b.rv11 = om * -sn;
+  b.rv12 = om * -c;
+  b.rv21 = om * c;
+  b.rv22 = om * -sn;
+  return b;
+}
+
 int solver_step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stats* stats) {
   return step(s, fsv, motion, stats);
 }
@@ -867,6 +899,7 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   s->cg_abstol = opt.cg_abstol;
   s->cg_tol_mode = opt.cg_tol_mode;
   s->device = opt.device;
+  s->grid_moving = p->grid_moving ? 1 : 0;
   s->hc.resize(s->L);
   for (int l = 0; l < s->L; ++l) s->hc[l] = pow(2.0, l) * s->h;
 
@@ -1148,7 +1181,7 @@ int ibpm_advance(ibpm_solver* s, int nsteps, const double* freestream, const dou
   for (int k = 0; k < nsteps; ++k) {
     ibpm_step_stats* sk = stats ? stats + k : nullptr;
     if (sk) memset(sk, 0, sizeof *sk);
-    int rc = step(s, freestream ? freestream + 2 * k : nullptr,
+    int rc = step(s, freestream ? freestream + (s->grid_moving ? 8 : 2) * k : nullptr,
                   motion ? motion + (size_t)6 * s->nbody * k : nullptr, sk);
     if (rc) return rc;
     if (sk && s->n > 0) HIPCHK(hipMemcpyAsync(sk->force, s->total, 16, hipMemcpyDeviceToHost, s->st));
@@ -1200,11 +1233,19 @@ int ibpm_read(ibpm_solver* s, int field, double* dst, size_t n) {
       memcpy(dst, s->q0_shadow.data(), need * 8);
       return IBPM_OK;
     }
-    // base_flux! of a static grid (fluid.jl:22-32)
-    const size_t Q = s->g.nq, nu = s->g.nu;
+    // base_flux! (fluid.jl:22-67) of the last step; zero before the first step
+    const size_t Q = s->g.nq;
+    const int nx = s->g.nx, ny = s->g.ny;
+    const BaseFlux bf = base_flux(s);
     for (int l = 0; l < s->L; ++l) {
-      for (size_t k = 0; k < nu; ++k) dst[l * Q + k] = s->index > 0 ? s->fs[0] * s->hc[l] : 0.0;
-      for (size_t k = nu; k < Q; ++k) dst[l * Q + k] = s->index > 0 ? s->fs[1] * s->hc[l] : 0.0;
+      double* qx = dst + l * Q;
+      double* qy = qx + s->g.nu;
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx + 1; ++i)
+          qx[(i - 1) + (size_t)(j - 1) * (nx + 1)] = s->index > 0 ? base_flux_u(bf, l, i, j) : 0.0;
+      for (int j = 1; j <= ny + 1; ++j)
+        for (int i = 1; i <= nx; ++i)
+          qy[(i - 1) + (size_t)(j - 1) * nx] = s->index > 0 ? base_flux_v(bf, l, i, j) : 0.0;
     }
     return IBPM_OK;
   }
@@ -1372,10 +1413,9 @@ int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, doubl
     na.nl_new = s->T;  // scratch outputs (kMaxPlanes planes), the state is untouched
     na.nl_old = s->nl2;
     na.rhs = s->rhs;
+    na.bf = base_flux(s);
     for (int l = 0; l < s->L; ++l) {
       const double hc = s->hc[l];
-      na.q0u[l] = s->fs[0] * hc;
-      na.q0v[l] = s->fs[1] * hc;
       na.inv_hc2[l] = 1.0 / (hc * hc);
       na.cA[l] = cA(s, l);
       na.fac_bc[l] = 0.25 * s->dt / (s->Re * (hc * hc));

@@ -1,5 +1,7 @@
 // Device helpers shared by the HIP translation units of libibpm.
 #pragma once
+
+#include "ibpm_internal.h"
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
@@ -40,6 +42,33 @@ __host__ __device__ __forceinline__ double ext_value(const double* __restrict__ 
   else if (i == 0) v = ring_value(rc, nx, ny, 0, j + 1);
   else v = ring_value(rc, nx, ny, 1, j + 1);
   return scale * v;
+}
+
+// Base flux on u-face (i, j) and v-face (i, j) of level l (0-based level, 1-based faces),
+// in the order of base_flux! (fluid.jl:34-67): u-faces sit at (x0 + h (i-1),
+// y0 + h/2 + h (j-1)), v-faces at (x0 + h/2 + h (i-1), y0 + h (j-1)) (problem.jl:318-334).
+__host__ __device__ inline double base_flux_u(const BaseFlux& b, int l, int i, int j) {
+#pragma clang fp contract(off)
+  if (!b.moving) return b.q0u[l];
+  const double hc = b.hc[l];
+  const double x = b.x0[l] + hc * (double)(i - 1);
+  const double y = (b.y0[l] + hc / 2) + hc * (double)(j - 1);
+  const double dx = x - b.cx, dy = y - b.cy;
+  const double wx = (b.ux - b.vx) - (b.rv11 * dx + b.rv12 * dy);
+  const double wy = (b.uy - b.vy) - (b.rv21 * dx + b.rv22 * dy);
+  return hc * (b.rx11 * wx + b.rx21 * wy);
+}
+
+__host__ __device__ inline double base_flux_v(const BaseFlux& b, int l, int i, int j) {
+#pragma clang fp contract(off)
+  if (!b.moving) return b.q0v[l];
+  const double hc = b.hc[l];
+  const double x = (b.x0[l] + hc / 2) + hc * (double)(i - 1);
+  const double y = b.y0[l] + hc * (double)(j - 1);
+  const double dx = x - b.cx, dy = y - b.cy;
+  const double wx = (b.ux - b.vx) - (b.rv11 * dx + b.rv12 * dy);
+  const double wy = (b.uy - b.vy) - (b.rv21 * dx + b.rv22 * dy);
+  return hc * (b.rx12 * wx + b.rx22 * wy);
 }
 
 }  // namespace ibpm

@@ -132,6 +132,21 @@ size_t sine_table_even_count(int N);
 size_t sine_table_bf16_odd_count(int N);
 size_t sine_table_bf16_even_count(int N);
 
+// base_flux! of one step (fluid.jl:22-67): the flux of the freestream seen in the grid's
+// frame. Static grid: uniform (ux, uy) h_l per level. Moving grid: per face,
+//   q0 = h_l Rx[:, c] . ((uinf - u0) - Rv ((x, y) - center))
+// with Rx the grid's orientation and Rv = angular_vel [-sin -cos; cos -sin]
+// (evaluated by base_flux_u / base_flux_v in ibpm_device.h).
+struct BaseFlux {
+  int moving;
+  double q0u[16], q0v[16];         // static grid
+  double hc[16], x0[16], y0[16];   // moving grid: step and lower-left corner per level
+  double ux, uy;                   // freestream velocity (global frame)
+  double cx, cy, vx, vy;           // GridVelocity: centre and velocity
+  double rx11, rx21, rx12, rx22;   // Rx = [rx11 rx12; rx21 rx22]
+  double rv11, rv12, rv21, rv22;   // Rv
+};
+
 // ---- stencil kernels (stencil_kernels.hip) ----
 // Nonlinear term and CNAB right-hand side for every level in one launch
 // (fluid.jl:441-455 and :467-497).
@@ -143,7 +158,7 @@ struct NonlinArgs {
   double* nl_new;        // nG x L, written
   const double* nl_old;  // nG x L
   double* rhs;           // nG x L, written
-  double q0u[16], q0v[16];  // uniform base flux per level (static grid)
+  BaseFlux bf;           // base flux of the step
   double inv_hc2[16];    // 1/hc^2 per level
   double cA[16];         // dt / (2 Re hc^2): A = I - cA*K
   double fac_bc[16];     // 0.25*dt/(Re hc^2)
@@ -206,7 +221,7 @@ struct GatherArgs {
   const double* psi1[64];   // level-1 streamfunction per rhs (curl on the fly)
   const double* psi2[64];   // level-2 streamfunction for boundary values (null: zero)
   double* out[64];          // 2n per rhs
-  double q0u, q0v;          // uniform base flux of level 1
+  BaseFlux bf;              // base flux (level 1 used; moving = 0 and zeros for none)
   const double* ub;         // subtract ub*h if not null
   double h;
 };

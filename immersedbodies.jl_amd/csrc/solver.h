@@ -22,6 +22,8 @@ struct ibpm_solver {
   int device = 0;
   hipStream_t st = nullptr;
   double fs[2] = {0.0, 0.0};
+  int grid_moving = 0;  // MovingGrid: gm = centre x, y, velocity x, y, angle, angular_vel
+  double gm[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   std::vector<double> hc;  // grid step per level
   double scale = 0.0;      // 1 / (4 nx ny): the two unnormalised DST-I scale factors
 
@@ -66,6 +68,8 @@ struct ibpm_solver {
 };
 
 namespace ibpm {
+// base_flux! of the solver's current freestream and grid motion (fluid.jl:22-67).
+BaseFlux base_flux(const ibpm_solver* s);
 // One advance! of a solver (solver.jl:155-190) queued on s->st: no host synchronisation
 // for static bodies. fsv: freestream at the new time; motion: 6 x nbody or null.
 int solver_step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stats* stats);

@@ -40,13 +40,16 @@ __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
   const double* qy = q + g.nu;
   const double* G = a.G + (size_t)lev * g.nG;
   const double* Gc = lev + 1 < a.nlevel ? a.G + (size_t)(lev + 1) * g.nG : nullptr;
-  const double q0u = a.q0u[lev], q0v = a.q0v[lev];
 
 #define QX(i, j) qx[((i)-1) + (size_t)((j)-1) * (nx + 1)]
 #define QY(i, j) qy[((i)-1) + (size_t)((j)-1) * nx]
 #define GE(i, j) ext_value(G, Gc, nx, ny, (i), (j), 0.25)
-  auto Qx = [&](int i, int j) { return (((QX(i, j) + QX(i, j - 1)) + q0u) + q0u) / 2; };
-  auto Qy = [&](int i, int j) { return (((QY(i, j) + QY(i - 1, j)) + q0v) + q0v) / 2; };
+  auto Qx = [&](int i, int j) {
+    return (((QX(i, j) + QX(i, j - 1)) + base_flux_u(a.bf, lev, i, j)) + base_flux_u(a.bf, lev, i, j - 1)) / 2;
+  };
+  auto Qy = [&](int i, int j) {
+    return (((QY(i, j) + QY(i - 1, j)) + base_flux_v(a.bf, lev, i, j)) + base_flux_v(a.bf, lev, i - 1, j)) / 2;
+  };
 
   const double g00 = GE(I, J);
   const double gm0 = GE(I - 1, J);

@@ -54,7 +54,8 @@ class Problem(C.Structure):
                 ("x0", C.c_double), ("y0", C.c_double), ("h", C.c_double),
                 ("Re", C.c_double), ("dt", C.c_double), ("t0", C.c_double),
                 ("npanel", C.c_int), ("xb", _dp), ("ds", _dp), ("nbody", C.c_int),
-                ("body_start", _ip), ("body_count", _ip), ("body_moving", _ip)]
+                ("body_start", _ip), ("body_count", _ip), ("body_moving", _ip),
+                ("grid_moving", C.c_int)]
 
 
 class Options(C.Structure):

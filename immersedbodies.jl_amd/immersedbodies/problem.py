@@ -6,14 +6,12 @@ package finds the same objects (paths relative to the reference repository):
 * ``CartesianGrid``, ``MultiDomainGrid``, ``fluid_grid``, ``gridstep``, ``subdomain``,
   ``coords``, ``GridIndices``                               problem.jl:269-472
 * ``Fluid``, ``default_gridstep``                            problem.jl:534-556
+* ``StaticGrid``, ``MovingGrid``, ``GridVelocity``           problem.jl:476-523
 * ``CNAB``, ``default_scheme``, ``timestep``                 problem.jl:566-596
 * ``Panels``, ``StaticBody``, ``MovingRigidBody``,
   ``RigidBodyTransform``, ``Bodies``                         problem.jl:1-261
 * ``Problem``, ``State``                                     problem.jl:598-708
 * ``boundary_total_force``                                   problem.jl:853-856
-
-Only the static grid is supported (the reference's ``MovingGrid`` is outside the
-accelerated path).
 """
 from __future__ import annotations
 
@@ -159,6 +157,32 @@ class StaticGrid:
     """A grid that is static in the global frame (problem.jl:508-513)."""
 
 
+@dataclass(frozen=True)
+class GridVelocity:
+    """Motion and orientation of the grid relative to the global frame (problem.jl:476-490):
+    centre of rotation, velocity of that centre, angle and angular velocity."""
+
+    center: Tuple[float, float] = (0.0, 0.0)
+    vel: Tuple[float, float] = (0.0, 0.0)
+    angle: float = 0.0
+    angular_vel: float = 0.0
+
+    def as_array(self):
+        return np.array([self.center[0], self.center[1], self.vel[0], self.vel[1], self.angle,
+                         self.angular_vel], dtype=np.float64)
+
+
+@dataclass
+class MovingGrid:
+    """A grid that moves in the global frame: ``motion(t)`` returns its GridVelocity
+    (problem.jl:515-523)."""
+
+    motion: Callable[[float], GridVelocity]
+
+    def __call__(self, t):
+        return self.motion(t)
+
+
 @dataclass
 class Fluid:
     """``Fluid(; grid, Re, freestream_vel, grid_motion)`` (problem.jl:534-552)."""
@@ -173,8 +197,8 @@ class Fluid:
             _, xl, yl, nl = self.grid
             self.grid = MultiDomainGrid(
                 CartesianGrid.from_limits(default_gridstep(self.Re), xl, yl), nl)
-        if not isinstance(self.grid_motion, StaticGrid):
-            raise NotImplementedError("only StaticGrid is supported by the GPU solver")
+        if not isinstance(self.grid_motion, (StaticGrid, MovingGrid)):
+            raise TypeError("grid_motion must be a StaticGrid or a MovingGrid")
 
 
 @dataclass(frozen=True)

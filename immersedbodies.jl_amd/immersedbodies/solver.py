@@ -14,7 +14,7 @@ from typing import Callable, Optional
 import numpy as np
 
 from . import _lib
-from .problem import Problem, State, timestep
+from .problem import MovingGrid, Problem, State, timestep
 
 
 class AbstractSolver:
@@ -58,6 +58,7 @@ class HipCnabSolver(AbstractSolver):
         self._n = n
         self._nbody = len(bodies.sections)
         self._moving = not bodies.static
+        self._grid_moving = isinstance(prob.fluid.grid_motion, MovingGrid)
         xb = np.ascontiguousarray(bodies.xb().T.reshape(-1))  # column-major n x 2
         ds = np.ascontiguousarray(bodies.ds())
         bs = np.array([s[2] for s in bodies.sections] or [0], dtype=np.int32)
@@ -69,7 +70,8 @@ class HipCnabSolver(AbstractSolver):
                          _dp(xb) if n else None, _dp(ds) if n else None, self._nbody,
                          bs.ctypes.data_as(C.POINTER(C.c_int)),
                          bc.ctypes.data_as(C.POINTER(C.c_int)),
-                         bm.ctypes.data_as(C.POINTER(C.c_int)))
+                         bm.ctypes.data_as(C.POINTER(C.c_int)),
+                         int(self._grid_moving))
         o = _lib.Options()
         L.ibpm_default_options(C.byref(o))
         o.precision = {"fp64": _lib.IBPM_FP64, "mixed": _lib.IBPM_MIXED}[precision]
@@ -110,11 +112,14 @@ class HipCnabSolver(AbstractSolver):
         st = self.state
         prob = st.prob
         dt = timestep(prob)
-        fs = np.zeros(2 * nsteps)
+        w = 8 if self._grid_moving else 2  # freestream, then the grid's GridVelocity
+        fs = np.zeros(w * nsteps)
         mot = np.zeros(6 * max(self._nbody, 1) * nsteps) if self._moving else None
         for k in range(nsteps):
             t = st.t0 + dt * (st.index + k)  # time after inc_time_index!
-            fs[2 * k:2 * k + 2] = prob.fluid.freestream_vel(t)
+            fs[w * k:w * k + 2] = prob.fluid.freestream_vel(t)
+            if self._grid_moving:
+                fs[w * k + 2:w * k + 8] = prob.fluid.grid_motion(t).as_array()
             if self._moving:
                 mot[6 * self._nbody * k:6 * self._nbody * (k + 1)] = \
                     prob.bodies.motion_array(t)[:6 * self._nbody]
@@ -123,7 +128,8 @@ class HipCnabSolver(AbstractSolver):
                                       _dp(mot) if mot is not None else None, stats)
         _lib.check(rc)
         st.set_time_index(st.index + nsteps)
-        st.freestream_vel = fs[-2:].copy() if nsteps else st.freestream_vel
+        st.freestream_vel = fs[w * (nsteps - 1):w * (nsteps - 1) + 2].copy() if nsteps \
+            else st.freestream_vel
         self.last_stats = stats
         return stats
 

@@ -76,6 +76,9 @@ typedef struct ibpm_problem {
   const int* body_start;    /* first panel of each body                       */
   const int* body_count;
   const int* body_moving;   /* 0: StaticBody, 1: MovingRigidBody              */
+  int grid_moving;          /* 0: StaticGrid; 1: MovingGrid (problem.jl:508-530), the
+                               grid's GridVelocity follows the freestream of every step
+                               in ibpm_advance                                  */
 } ibpm_problem;
 
 typedef struct ibpm_options {
@@ -112,7 +115,9 @@ void ibpm_default_options(ibpm_options* o);
 
 int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out);
 
-/* Advance nsteps steps. freestream: 2 x nsteps (ux, uy at each new time). motion:
+/* Advance nsteps steps. freestream: 2 x nsteps (ux, uy at each new time), or 8 x nsteps
+ * for a moving grid (ux, uy, then the grid's GridVelocity at that time: centre x, y,
+ * velocity x, y, angle, angular_vel). motion:
  * 6 x nbody x nsteps (pos x, pos y, vel x, vel y, angle, angular_vel of every body at
  * each new time; entries of static bodies are ignored), or NULL when no body moves.
  * stats: nsteps entries or NULL. */

@@ -590,6 +590,8 @@ struct ibo_solver {
   size_t nG, nq, nbc;
   /* state (problem.jl:626-665) */
   double *q, *q0, *G, *psi, *nonlin[2], *xb, *ds, *ub, *Ft, *fb, total[2], fs[2];
+  int grid_moving;
+  double gm[6];
   /* reference body description */
   double* xb_ref;
   int *body_start, *body_count, *body_moving;
@@ -765,6 +767,8 @@ ibo_solver* ibo_create(const ibo_config* c, char* err, size_t errlen) {
   s->body_moving = calloc(c->nbody + 1, sizeof(int));
   memcpy(s->xb_ref, c->xb, sizeof(double) * 2 * n);
   s->moving = 0;
+  s->grid_moving = c->grid_moving ? 1 : 0;
+  for (int k = 0; k < 6; ++k) s->gm[k] = 0.0;
   for (int b = 0; b < c->nbody; ++b) {
     s->body_start[b] = c->body_start[b];
     s->body_count[b] = c->body_count[b];
@@ -838,6 +842,10 @@ void ibo_destroy(ibo_solver* s) {
                   s->cg_r, s->cg_c};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
   free(s);
+}
+
+void ibo_set_grid_motion(ibo_solver* s, const double gm[6]) {
+  for (int k = 0; k < 6; ++k) s->gm[k] = gm[k];
 }
 
 void ibo_set_index(ibo_solver* s, int index) {
@@ -935,13 +943,48 @@ int ibo_advance(ibo_solver* s, const double fs_vel[2], const double* motion, cha
     }
     if (reg_update_range(s, st, cnt, err, errlen)) return 1;
   }
-  /* base_flux! for a static grid, fluid.jl:22-32 */
-  for (int lev = 1; lev <= L; ++lev) {
-    double hc = gridstep(g, lev);
-    double* q0 = s->q0 + (size_t)(lev - 1) * nq;
-    size_t nu = n_u(g);
-    for (size_t k = 0; k < nu; ++k) q0[k] = s->fs[0] * hc;
-    for (size_t k = nu; k < nq; ++k) q0[k] = s->fs[1] * hc;
+  if (!s->grid_moving) {
+    /* base_flux! for a static grid, fluid.jl:22-32 */
+    for (int lev = 1; lev <= L; ++lev) {
+      double hc = gridstep(g, lev);
+      double* q0 = s->q0 + (size_t)(lev - 1) * nq;
+      size_t nu = n_u(g);
+      for (size_t k = 0; k < nu; ++k) q0[k] = s->fs[0] * hc;
+      for (size_t k = nu; k < nq; ++k) q0[k] = s->fs[1] * hc;
+    }
+  } else {
+    /* base_flux! for a moving grid, fluid.jl:34-67: per face,
+       q0 = hc Rx[:, c] . ((uinf - u0) - Rv ((x, y) - centre)) */
+    const double cx = s->gm[0], cy = s->gm[1], u0x = s->gm[2], u0y = s->gm[3];
+    const double ct = cos(s->gm[4]), st = sin(s->gm[4]), om = s->gm[5];
+    const double Rx11 = ct, Rx21 = st, Rx12 = -st, Rx22 = ct;
+    const double Rv11 = om * -st, Rv12 = om * -ct, Rv21 = om * ct, Rv22 = om * -st;
+    const int nx = g->nx, ny = g->ny;
+    for (int lev = 1; lev <= L; ++lev) {
+      double hc = gridstep(g, lev);
+      /* subdomain (problem.jl:424-429) */
+      double sx0 = g->x0 + (nx * (g->h - hc)) / 2, sy0 = g->y0 + (ny * (g->h - hc)) / 2;
+      double* qx = s->q0 + (size_t)(lev - 1) * nq;
+      double* qy = qx + n_u(g);
+      /* GridU: (x0 + hc (i-1), y0 + hc/2 + hc (j-1)) (problem.jl:318-328) */
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx + 1; ++i) {
+          double x = sx0 + hc * (double)(i - 1), y = (sy0 + hc / 2) + hc * (double)(j - 1);
+          double dx = x - cx, dy = y - cy;
+          double wx = (s->fs[0] - u0x) - (Rv11 * dx + Rv12 * dy);
+          double wy = (s->fs[1] - u0y) - (Rv21 * dx + Rv22 * dy);
+          qx[(i - 1) + (size_t)(j - 1) * (nx + 1)] = hc * (Rx11 * wx + Rx21 * wy);
+        }
+      /* GridV: (x0 + hc/2 + hc (i-1), y0 + hc (j-1)) (problem.jl:330-334) */
+      for (int j = 1; j <= ny + 1; ++j)
+        for (int i = 1; i <= nx; ++i) {
+          double x = (sx0 + hc / 2) + hc * (double)(i - 1), y = sy0 + hc * (double)(j - 1);
+          double dx = x - cx, dy = y - cy;
+          double wx = (s->fs[0] - u0x) - (Rv11 * dx + Rv12 * dy);
+          double wy = (s->fs[1] - u0y) - (Rv21 * dx + Rv22 * dy);
+          qy[(i - 1) + (size_t)(j - 1) * nx] = hc * (Rx12 * wx + Rx22 * wy);
+        }
+    }
   }
   get_trial_state(s);
   /* couple_surface_rigid, coupling.jl:233-251 */

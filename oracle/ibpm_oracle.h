@@ -71,6 +71,8 @@ typedef struct ibo_config {
   int cg_tol_mode;        /* 0: reltol*||r0|| (IterativeSolvers 0.9); 1: reltol*||b|| */
   int timing_only;        /* 1: skip the dense B^-1 assembly and use the identity (same
                              per-step cost; results meaningless) for CPU step timing */
+  int grid_moving;        /* 0 StaticGrid, 1 MovingGrid: ibo_set_grid_motion before each
+                             ibo_advance gives the GridVelocity at the new time */
 } ibo_config;
 
 ibo_solver* ibo_create(const ibo_config* c, char* err, size_t errlen);
@@ -78,6 +80,9 @@ void ibo_destroy(ibo_solver* s);
 /* motion: per body (pos x, pos y, vel x, vel y, angle, angular_vel) at the new time */
 int ibo_advance(ibo_solver* s, const double fs_vel[2], const double* motion, char* err,
                 size_t errlen);
+/* GridVelocity of a MovingGrid at the next step's time: centre x, y, velocity x, y, angle,
+   angular_vel (problem.jl:476-490) */
+void ibo_set_grid_motion(ibo_solver* s, const double gm[6]);
 /* field access; names as in ibpm.h: 0 q,1 q0,2 Gamma,3 psi,4 nonlin1,5 nonlin2,6 xb,7 ub,
    8 Ftilde,9 fb,10 total force, 11 ds */
 double* ibo_field(ibo_solver* s, int which, size_t* n);

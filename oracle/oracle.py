@@ -40,7 +40,7 @@ class _Config(C.Structure):
                 ("npanel", C.c_int), ("xb", _dp), ("ds", _dp), ("nbody", C.c_int),
                 ("body_start", _ip), ("body_count", _ip), ("body_moving", _ip),
                 ("cg_maxiter", C.c_int), ("cg_reltol", C.c_double), ("cg_abstol", C.c_double),
-                ("cg_tol_mode", C.c_int), ("timing_only", C.c_int)]
+                ("cg_tol_mode", C.c_int), ("timing_only", C.c_int), ("grid_moving", C.c_int)]
 
 
 def lib():
@@ -89,6 +89,7 @@ def lib():
         L.ibo_Binv.argtypes = [C.c_void_p]
         L.ibo_Binv.restype = _dp
         L.ibo_set_threads.argtypes = [C.c_int]
+        L.ibo_set_grid_motion.argtypes = [C.c_void_p, _dp]
         _lib = L
     return _lib
 
@@ -274,7 +275,7 @@ class OracleSolver:
 
     def __init__(self, grid: Grid, Re, dt, xb, ds=None, bodies=None, t0=0.0,
                  cg_maxiter=5000, cg_reltol=1e-12, cg_abstol=0.0, cg_tol_mode=0,
-                 timing_only=False):
+                 timing_only=False, grid_moving=False):
         xb = np.asarray(xb, dtype=np.float64).reshape(-1, 2)
         n = xb.shape[0]
         self.grid, self.n = grid, n
@@ -288,7 +289,7 @@ class OracleSolver:
         self.nbody = len(bodies)
         cfg = _Config(grid.c, Re, dt, t0, n, _p(self._xb), _p(self._ds), self.nbody,
                       _pi(self._bs), _pi(self._bc), _pi(self._bm), cg_maxiter, cg_reltol,
-                      cg_abstol, cg_tol_mode, int(timing_only))
+                      cg_abstol, cg_tol_mode, int(timing_only), int(grid_moving))
         err = C.create_string_buffer(512)
         self._h = lib().ibo_create(C.byref(cfg), err, 512)
         if not self._h:
@@ -313,8 +314,13 @@ class OracleSolver:
     def next_time(self):
         return self.t0 + self.dt * self.index
 
-    def advance(self, freestream=(1.0, 0.0), motion=None):
-        fs = np.asarray(freestream, dtype=np.float64)
+    def advance(self, freestream=(1.0, 0.0), motion=None, grid_motion=None):
+        """One step. grid_motion: GridVelocity of a moving grid at the new time (centre x,
+        y, velocity x, y, angle, angular_vel)."""
+        if grid_motion is not None:
+            gm = np.ascontiguousarray(np.asarray(grid_motion, dtype=np.float64).reshape(6))
+            lib().ibo_set_grid_motion(self._h, _p(gm))
+        fs = np.ascontiguousarray(np.asarray(freestream, dtype=np.float64)[:2])
         m = np.zeros(6 * max(self.nbody, 1)) if motion is None else \
             np.ascontiguousarray(np.asarray(motion, dtype=np.float64).reshape(-1))
         err = C.create_string_buffer(512)

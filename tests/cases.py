@@ -125,6 +125,22 @@ def no_body(nx=32, ny=32, nlevel=2, h=0.1, x0=(-1.6, -1.6), Re=50.0, dt=0.01):
     return ib.Problem(fluid=fluid, bodies=ib.Bodies(), scheme=ib.CNAB(dt=dt))
 
 
+def rotating_frame(nx=64, ny=64, nlevel=3, h=0.05, x0=(-1.6, -1.6), Re=100.0, dt=0.004,
+                   omega=0.3, vel=(0.2, -0.1), U=(1.0, 0.0), body=True):
+    """MovingGrid: the grid translates with ``vel`` and turns at ``omega`` about the origin
+    (angle = omega t), a static ellipse in the grid frame (the body-fixed frame of a
+    rotating body)."""
+    grid = ib.MultiDomainGrid(ib.CartesianGrid(tuple(x0), (nx, ny), h), nlevel)
+    motion = ib.MovingGrid(lambda t: ib.GridVelocity(center=(0.0, 0.0), vel=vel,
+                                                     angle=omega * t, angular_vel=omega))
+    fluid = ib.Fluid(grid=grid, Re=Re, freestream_vel=lambda t: U, grid_motion=motion)
+    bodies = ib.Bodies()
+    if body:
+        pan = curves.partition(curves.ellipse(0.4, 0.15, angle=0.3), 2.0 * h)
+        bodies = ib.Bodies(ellipse=ib.StaticBody(pan))
+    return ib.Problem(fluid=fluid, bodies=bodies, scheme=ib.CNAB(dt=dt))
+
+
 def oracle_for(prob, **kw):
     """FP64 CPU oracle of ``prob`` (test infrastructure)."""
     import oracle as O
@@ -133,15 +149,18 @@ def oracle_for(prob, **kw):
     grid = O.Grid(g.base.n[0], g.base.n[1], g.nlevel, g.base.x0[0], g.base.x0[1], g.base.h)
     b = prob.bodies
     bodies = [(off, n, body.moving) for (_, body, off, n) in b.sections]
+    kw.setdefault("grid_moving", isinstance(prob.fluid.grid_motion, ib.MovingGrid))
     return O.OracleSolver(grid, prob.fluid.Re, prob.scheme.dt, b.xb(), b.ds(), bodies, **kw)
 
 
 def oracle_advance(orc, prob, nsteps=1):
     """Advance the oracle with the freestream and motion the host evaluates."""
     forces = []
+    gm = prob.fluid.grid_motion
     for _ in range(nsteps):
         t = orc.next_time()
-        orc.advance(prob.fluid.freestream_vel(t), prob.bodies.motion_array(t))
+        orc.advance(prob.fluid.freestream_vel(t), prob.bodies.motion_array(t),
+                    gm(t).as_array() if isinstance(gm, ib.MovingGrid) else None)
         forces.append(orc.field("total_force"))
     return np.array(forces)
 

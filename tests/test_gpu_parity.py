@@ -114,3 +114,20 @@ def test_cylinder_200_full_run_matches_oracle():
     assert errm.max() <= 1e-6
     # y-symmetric body and flow: lift stays at round-off early on
     assert np.abs(f64[:20, 1]).max() <= 1e-10 * np.abs(f64[:20, 0]).max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_moving_grid_matches_oracle(precision):
+    """MovingGrid (translating and turning frame, static ellipse in it): fields, forces and
+    the base flux against the oracle."""
+    prob = cases.rotating_frame()
+    print("case rotating_frame", precision, flush=True)
+    solver = ib.create_solver(prob, precision=precision)
+    orc = cases.oracle_for(prob)
+    stats = solver.advance(8)
+    f_ref = cases.oracle_advance(orc, prob, 8)
+    tol = FORCE_TOL if precision == "fp64" else 1e-6
+    assert cases.relerr(solver.forces(stats), f_ref) <= tol
+    assert cases.relerr(solver.read("q0"), orc.field("q0")) <= 1e-14
+    if precision == "fp64":
+        _compare_state(solver, orc, fields=["Gamma", "psi", "q", "nonlin1", "Ftilde", "fb"])

@@ -366,3 +366,82 @@ def test_time_index_convention():
     assert orc.index == 0 and abs(orc.t + prob.scheme.dt) < 1e-15
     cases.oracle_advance(orc, prob, 4)  # solve!(; t = 3 dt) runs 4 steps
     assert orc.index == 4 and abs(orc.t - 3 * prob.scheme.dt) < 1e-15
+
+
+# ---- moving grid (base_flux!, fluid.jl:34-67) ----
+
+def _q0_after_one_step(prob):
+    orc = cases.oracle_for(prob)
+    cases.oracle_advance(orc, prob, 1)
+    g = prob.fluid.grid
+    nx, ny = g.base.n
+    q0 = orc.field("q0").reshape(g.nlevel, -1)
+    return q0, nx, ny
+
+
+def test_moving_grid_translation_is_galilean():
+    """A grid translating at u0 under freestream U is the static grid under U - u0."""
+    import immersedbodies as ib
+
+    mov = cases.rotating_frame(nx=48, ny=48, omega=0.0, vel=(0.3, -0.2), U=(1.0, 0.1))
+    sta = cases.rotating_frame(nx=48, ny=48, omega=0.0, vel=(0.0, 0.0), U=(0.7, 0.3))
+    sta = ib.Problem(fluid=ib.Fluid(grid=sta.fluid.grid, Re=sta.fluid.Re,
+                                    freestream_vel=lambda t: (0.7, 0.3)),
+                     bodies=sta.bodies, scheme=sta.scheme)
+    a, b = cases.oracle_for(mov), cases.oracle_for(sta)
+    fa = cases.oracle_advance(a, mov, 5)
+    fb = cases.oracle_advance(b, sta, 5)
+    assert cases.relerr(fa, fb) < 1e-13
+    for name in ("Gamma", "psi", "q", "q0"):
+        assert cases.relerr(a.field(name), b.field(name)) < 1e-13, name
+
+
+def test_moving_grid_rotated_frame_sees_rotated_freestream():
+    """Grid turned by theta (no motion): the base flux is the freestream's components in
+    the grid's axes, h (U . e1, U . e2) with e1 = (cos, sin), e2 = (-sin, cos)."""
+    import immersedbodies as ib
+
+    th, U = 0.4, (1.0, 0.25)
+    p = cases.rotating_frame(nx=32, ny=32, nlevel=2, omega=0.0, vel=(0.0, 0.0), U=U, body=False)
+    p.fluid.grid_motion = ib.MovingGrid(lambda t: ib.GridVelocity(angle=th))
+    q0, nx, ny = _q0_after_one_step(p)
+    nu = (nx + 1) * ny
+    for lev in range(2):
+        hc = p.fluid.grid.base.h * 2 ** lev
+        assert np.allclose(q0[lev, :nu], hc * (U[0] * np.cos(th) + U[1] * np.sin(th)), rtol=1e-14)
+        assert np.allclose(q0[lev, nu:], hc * (-U[0] * np.sin(th) + U[1] * np.cos(th)), rtol=1e-14)
+
+
+def test_moving_grid_spinning_frame_sees_counter_rotation():
+    """Grid spinning at Omega about the origin, fluid at rest: in the grid's frame the fluid
+    turns the other way, velocity (Omega y, -Omega x) at each face."""
+    import immersedbodies as ib
+
+    om = 0.7
+    p = cases.rotating_frame(nx=32, ny=32, nlevel=2, omega=om, vel=(0.0, 0.0), U=(0.0, 0.0),
+                             body=False)
+    p.fluid.grid_motion = ib.MovingGrid(lambda t: ib.GridVelocity(angular_vel=om))
+    q0, nx, ny = _q0_after_one_step(p)
+    nu = (nx + 1) * ny
+    for lev in range(2):
+        sub = ib.subdomain(p.fluid.grid, lev + 1)
+        hc = sub.h
+        xu, yu = ib.coords(sub, "u")
+        xv, yv = ib.coords(sub, "v")
+        qx = q0[lev, :nu].reshape(ny, nx + 1)
+        qy = q0[lev, nu:].reshape(ny + 1, nx)
+        assert np.allclose(qx, hc * om * yu[:, None] * np.ones((1, nx + 1)), rtol=1e-12, atol=1e-15)
+        assert np.allclose(qy, -hc * om * xv[None, :] * np.ones((ny + 1, 1)), rtol=1e-12, atol=1e-15)
+
+
+def test_moving_grid_base_flux_is_divergence_free():
+    """Rigid motion: the discrete divergence of the base flux vanishes in every cell."""
+    p = cases.rotating_frame(nx=40, ny=36, nlevel=3, omega=0.5, vel=(0.2, 0.1), U=(1.0, -0.3),
+                             body=False)
+    q0, nx, ny = _q0_after_one_step(p)
+    nu = (nx + 1) * ny
+    for lev in range(3):
+        qx = q0[lev, :nu].reshape(ny, nx + 1)
+        qy = q0[lev, nu:].reshape(ny + 1, nx)
+        div = (qx[:, 1:] - qx[:, :-1]) + (qy[1:, :] - qy[:-1, :])
+        assert np.abs(div).max() < 1e-13 * np.abs(q0[lev]).max()
