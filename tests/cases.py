@@ -119,6 +119,49 @@ def ensemble(nprob=64, seed=2024, sizes=(256, 512, 768, 1024), levels=(3, 4, 5))
     return out
 
 
+def plunging_ellipses(nx=2048, ny=1024, nlevel=6, h=1.0 / 128, x0=(-4.0, -4.0), Re=200.0,
+                      dt=None, nbody=3, seed=7, amp=0.2):
+    """Several ellipses plunging in a uniform stream: body k (a = 0.5, b uniform in
+    [0.1, 0.3], 3 chords apart along x) moves as y = amp sin(2 pi f t + phase), vel =
+    (0, 2 pi f amp cos(2 pi f t + phase)), f uniform in [0.1, 0.5], phase uniform in
+    [0, 2 pi), numpy default_rng(seed). Panels spaced 2h (Panels(grid, curve)), dt from
+    CFL 0.2 at the freestream speed. The full size is 2048 x 1024 on [-4,12] x [-4,4]
+    with 6 levels."""
+    grid = ib.MultiDomainGrid(ib.CartesianGrid(tuple(x0), (nx, ny), h), nlevel)
+    fluid = ib.Fluid(grid=grid, Re=Re, freestream_vel=lambda t: (1.0, 0.0))
+    rng = np.random.default_rng(seed)
+    bodies = {}
+    for k in range(nbody):
+        b = float(rng.uniform(0.1, 0.3))
+        f = float(rng.uniform(0.1, 0.5))
+        ph = float(rng.uniform(0.0, 2 * math.pi))
+        xc = 3.0 * k
+        pan = curves.panels(grid, curves.ellipse(0.5, b))
+
+        def motion(t, xc=xc, f=f, ph=ph):
+            w = 2 * math.pi * f
+            return ib.RigidBodyTransform(pos=(xc, amp * math.sin(w * t + ph)),
+                                         vel=(0.0, w * amp * math.cos(w * t + ph)),
+                                         angle=0.0, angular_vel=0.0)
+
+        bodies[f"ellipse{k + 1}"] = ib.MovingRigidBody(motion, pan)
+    return ib.Problem(fluid=fluid, bodies=ib.Bodies(**bodies),
+                      scheme=ib.CNAB(dt=dt or ib.default_scheme(grid, Umax=1.0, cfl=0.2).dt))
+
+
+def naca_airfoil(spec="0012", aoa_deg=20.0, Re=400.0, nlevel=3):
+    """The reference's NACA example (docs/src/examples/naca.md): the section, its
+    mid-chord at the origin, turned to ``aoa_deg`` incidence, on [-1,2] x [-0.5,0.5] with
+    the Reynolds-number default step, panels spaced 2h, dt from CFL 0.2 at Umax 2."""
+    grid = ib.fluid_grid(xlims=(-1.0, 2.0), ylims=(-0.5, 0.5), nlevel=nlevel)
+    fluid = ib.Fluid(grid=grid, Re=Re, freestream_vel=lambda t: (1.0, 0.0))
+    airfoil = (curves.naca(spec) | curves.translate((-0.5, 0.0))
+               | curves.rotate(-math.radians(aoa_deg)))
+    bodies = ib.Bodies(airfoil=ib.StaticBody(curves.panels(fluid.grid, airfoil)))
+    scheme = ib.default_scheme(fluid.grid, Umax=2.0, cfl=0.2)
+    return ib.Problem(fluid=fluid, bodies=bodies, scheme=scheme)
+
+
 def no_body(nx=32, ny=32, nlevel=2, h=0.1, x0=(-1.6, -1.6), Re=50.0, dt=0.01):
     grid = ib.MultiDomainGrid(ib.CartesianGrid(tuple(x0), (nx, ny), h), nlevel)
     fluid = ib.Fluid(grid=grid, Re=Re, freestream_vel=lambda t: (1.0, 0.3))

@@ -131,3 +131,50 @@ def test_moving_grid_matches_oracle(precision):
     assert cases.relerr(solver.read("q0"), orc.field("q0")) <= 1e-14
     if precision == "fp64":
         _compare_state(solver, orc, fields=["Gamma", "psi", "q", "nonlin1", "Ftilde", "fb"])
+
+
+def test_naca_airfoil_matches_oracle():
+    """NACA 0012 at 20 degrees built by the Curves module (the reference's NACA example):
+    fields and forces against the oracle."""
+    prob = cases.naca_airfoil()
+    print("case naca_airfoil 0012 20deg", flush=True)
+    solver = ib.create_solver(prob)
+    orc = cases.oracle_for(prob)
+    stats = solver.advance(5)
+    f_ref = cases.oracle_advance(orc, prob, 5)
+    assert cases.relerr(solver.forces(stats), f_ref) <= FORCE_TOL
+    _compare_state(solver, orc, fields=["Gamma", "psi", "q", "Ftilde", "fb"])
+
+
+def test_plunging_ellipses_matches_oracle():
+    """Three plunging ellipses (moving bodies, CG) on 256 x 128 cells of [-4,12] x [-4,4],
+    3 levels: fields and forces against the oracle."""
+    prob = cases.plunging_ellipses(nx=256, ny=128, nlevel=3, h=1.0 / 16)
+    print("case plunging_ellipses 256x128 L3", flush=True)
+    solver = ib.create_solver(prob)
+    orc = cases.oracle_for(prob)
+    stats = solver.advance(5)
+    f_ref = cases.oracle_advance(orc, prob, 5)
+    assert cases.relerr(solver.forces(stats), f_ref) <= 1e-8
+    assert all(s.cg_converged for s in stats)
+    _compare_state(solver, orc, field_tol=1e-9, force_tol=1e-8, fields=["q", "Gamma", "psi", "xb"])
+
+
+def test_plunging_ellipses_full_size_warm_start_invariance():
+    """2048 x 1024 cells, 6 levels, three plunging ellipses (426 points): CG converges
+    every step, and the forces do not depend on CG's initial guess (the previous step's
+    solution or its extrapolation) beyond the CG tolerance."""
+    prob = cases.plunging_ellipses()
+    print("case plunging_ellipses 2048x1024 L6", flush=True)
+    forces, iters = {}, {}
+    for mode in ("previous", "extrapolate"):
+        s = ib.create_solver(prob, cg=dict(maxiter=500, reltol=1e-10), cg_tol_mode=1)
+        s.set_warm_start(mode)
+        st = s.advance(6)
+        assert all(x.cg_converged for x in st), mode
+        forces[mode] = s.forces(st)
+        iters[mode] = s.cg_iters(st)
+        assert np.isfinite(forces[mode]).all()
+        s.close()
+    print("cg iterations", iters, flush=True)
+    assert cases.relerr(forces["extrapolate"], forces["previous"]) <= 1e-7
