@@ -218,6 +218,195 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
 }
 
 // ---------------------------------------------------------------------------------
+// FP64 pass with a second fold. M = N+1 = 4Q (nx and ny are multiples of 4), H = 2Q.
+// Odd rows: rows j = 2m+1 and M-j = 2(H-1-m)+1 satisfy S_{M-j,k} = (-1)^{k+1} S_{j,k},
+// so with (m < Q)
+//   A_m = sum_{i<Q} T1[i][m] P_{2i+1},  B_m = sum_{i<Q} T2[i][m] P_{2i+2}
+// the outputs are Y_{2m+1} = A_m + B_m and Y_{M-1-2m} = A_m - B_m.
+// Even rows: the block is a DST-I of modulus H of Q_k and folds the same way once more:
+//   Y_{4m+2} = sum_{i<Q} T2[i][m] P'_{i+1},   P'_k = Q_k + Q_{H-k} (k < Q), P'_Q = Q_Q
+//   Y_{4m+4} = sum_{i<Q-1} T4[i][m] Q'_{i+1}, Q'_k = Q_k - Q_{H-k}
+// with T1[i][m] = 2 sin(pi (2m+1)(2i+1)/M), T2[i][m] = 2 sin(pi (2m+1)(2i+2)/M) (Q x Q)
+// and T4[i][m] = 2 sin(pi 4 (m+1)(i+1)/M) ((Q-1) x (Q-1)), stored [T1 | T2 | T4], i-major.
+// Four Q-long contractions replace two H-long ones: half the matrix-core work of the
+// single fold. Work-group: one block (odd: A and B, even: the two quarter outputs), TB
+// values of b x TM quarter rows, K chunk KC, PF register stages; wave w computes the
+// TB/2 x TM/2 block (b-half w&1, m-half w>>1) of both of the block's products, so the
+// odd block's butterfly A +- B stays in registers.
+// ---------------------------------------------------------------------------------
+template <typename InT, typename OutT, int EPI, int TB, int TM, int KC, int PF>
+__global__ __launch_bounds__(256) void dst_pass_f64q_kernel(const PassBatch batch) {
+  constexpr int BS = TB / 32;        // 16-wide b subtiles per wave
+  constexpr int MS = TM / 32;        // 16-wide m subtiles per wave
+  constexpr int RA = KC * TB / 256;  // (i, b) input slots per thread and chunk
+  constexpr int RB = KC * TM / 256;  // sine elements per thread, chunk and table
+  static_assert(BS >= 1 && MS >= 1 && RA >= 1 && RB >= 1, "tile too small");
+  int z, bt, mt2;
+  if (!xcd_tile(batch, z, bt, mt2)) return;
+  const int grp = mt2 & 1;  // 0: odd rows, 1: even rows
+  const int mt = mt2 >> 1;
+  const PassPlane& d = batch.p[z];
+  const int Nk = d.Nk, Nb = d.Nb;
+  const int M = Nk + 1, H = M / 2, Q = M / 4;
+  const int b0 = bt * TB;
+  const int m0 = mt * TM;
+  if (b0 >= Nb || m0 >= Q) return;
+
+  __shared__ double X0s[KC][TB + 1];
+  __shared__ double X1s[KC][TB + 1];
+  __shared__ double Sa_s[KC][TM + 1];
+  __shared__ double Sb_s[KC][TM + 1];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int bh = wave & 1;
+  const int mh = wave >> 1;
+  const double* T1 = (const double*)d.Sq;
+  const double* T2 = T1 + (size_t)Q * Q;
+  const double* T4 = T2 + (size_t)Q * Q;
+  const double* SA = grp == 0 ? T1 : T2;
+  const double* SB = grp == 0 ? T2 : T4;
+  const int nb_lim = grp == 0 ? Q : Q - 1;  // rows and columns of SB
+  const InT* xin = (const InT*)d.in;
+
+  d4 acc0[BS][MS], acc1[BS][MS];
+#pragma unroll
+  for (int i = 0; i < BS; ++i)
+#pragma unroll
+    for (int j = 0; j < MS; ++j) {
+      acc0[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+      acc1[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+    }
+
+  double r0[PF][RA], r1[PF][RA], r2[PF][RA], r3[PF][RA], rsa[PF][RB], rsb[PF][RB];
+  auto fetch = [&](const int st, int kc) {
+#pragma unroll
+    for (int r = 0; r < RA; ++r) {
+      const int e = tid + 256 * r;
+      const int i = kc + e % KC;  // KC lanes read KC consecutive i
+      const int b = b0 + e / KC;
+      double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+      if (i < Q && b < Nb) {
+        const InT* x = xin + (size_t)b * Nk;  // x_k at x[k-1]
+        if (grp == 0) {
+          v0 = (double)x[2 * i];               // x_{2i+1}
+          v1 = (double)x[M - 2 * i - 2];       // x_{M-2i-1}
+          v2 = (double)x[2 * i + 1];           // x_{2i+2}
+          if (2 * i + 2 < H) v3 = (double)x[M - 2 * i - 3];  // x_{M-2i-2}; k = H has no partner
+        } else {
+          v0 = (double)x[i];                   // x_{i+1}
+          v1 = (double)x[M - i - 2];           // x_{M-i-1}
+          v2 = (double)x[H - i - 2];           // x_{H-i-1}
+          v3 = (double)x[H + i];               // x_{H+i+1}
+        }
+      }
+      r0[st][r] = v0;
+      r1[st][r] = v1;
+      r2[st][r] = v2;
+      r3[st][r] = v3;
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int e = tid + 256 * r;
+      const int i = kc + e / TM;
+      const int m = m0 + e % TM;
+      rsa[st][r] = (i < Q && m < Q) ? SA[(size_t)i * Q + m] : 0.0;
+      rsb[st][r] = (i < nb_lim && m < nb_lim) ? SB[(size_t)i * nb_lim + m] : 0.0;
+    }
+  };
+  const int col = lane & 15;
+  const int kq = lane >> 4;
+  auto stage = [&](const int st, int kc) {
+#pragma unroll
+    for (int r = 0; r < RA; ++r) {
+      const int e = tid + 256 * r;
+      const int ii = e % KC;
+      double a, b;
+      if (grp == 0) {
+        a = r0[st][r] + r1[st][r];  // P_{2i+1}
+        b = r2[st][r] + r3[st][r];  // P_{2i+2}
+      } else {
+        const double qk = r0[st][r] - r1[st][r];  // Q_{i+1}
+        const double qh = r2[st][r] - r3[st][r];  // Q_{H-i-1}
+        const bool last = kc + ii == Q - 1;       // k = Q is its own partner
+        a = last ? qk : qk + qh;
+        b = last ? 0.0 : qk - qh;
+      }
+      X0s[ii][e / KC] = a;
+      X1s[ii][e / KC] = b;
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int e = tid + 256 * r;
+      Sa_s[e / TM][e % TM] = rsa[st][r];
+      Sb_s[e / TM][e % TM] = rsb[st][r];
+    }
+  };
+  auto multiply = [&]() {
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ++ks) {
+      const int krow = ks * 4 + kq;
+      double a0[BS], a1[BS];
+#pragma unroll
+      for (int bs = 0; bs < BS; ++bs) {
+        a0[bs] = X0s[krow][bh * (TB / 2) + bs * 16 + col];
+        a1[bs] = X1s[krow][bh * (TB / 2) + bs * 16 + col];
+      }
+#pragma unroll
+      for (int ms = 0; ms < MS; ++ms) {
+        const double ba = Sa_s[krow][mh * (TM / 2) + ms * 16 + col];
+        const double bb = Sb_s[krow][mh * (TM / 2) + ms * 16 + col];
+#pragma unroll
+        for (int bs = 0; bs < BS; ++bs) {
+          acc0[bs][ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[bs], ba, acc0[bs][ms], 0, 0, 0);
+          acc1[bs][ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[bs], bb, acc1[bs][ms], 0, 0, 0);
+        }
+      }
+    }
+  };
+#pragma unroll
+  for (int st = 0; st < PF; ++st)
+    if (st * KC < Q) fetch(st, st * KC);
+  for (int kc = 0; kc < Q; kc += PF * KC) {
+#pragma unroll
+    for (int st = 0; st < PF; ++st) {
+      const int k = kc + st * KC;  // the same for every thread: barriers stay uniform
+      if (k < Q) {
+        stage(st, k);
+        __syncthreads();
+        if (k + PF * KC < Q) fetch(st, k + PF * KC);
+        multiply();
+        __syncthreads();
+      }
+    }
+  }
+
+  // epilogue: the f64 16x16 accumulator holds D[row = lane/16 + 4r][col = lane%16]
+  const int rq = lane >> 4;
+#pragma unroll
+  for (int bs = 0; bs < BS; ++bs)
+#pragma unroll
+    for (int ms = 0; ms < MS; ++ms) {
+      const int m = m0 + mh * (TM / 2) + ms * 16 + col;
+      if (m >= Q) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int b = b0 + bh * (TB / 2) + bs * 16 + rq + 4 * r;
+        if (b >= Nb) continue;
+        const double u = acc0[bs][ms][r], v = acc1[bs][ms][r];
+        if (grp == 0) {
+          store_out<OutT, EPI>(d, b, 2 * m, u + v);          // row 2m+1
+          store_out<OutT, EPI>(d, b, M - 2 - 2 * m, u - v);  // row M-1-2m
+        } else {
+          store_out<OutT, EPI>(d, b, 4 * m + 1, u);                  // row 4m+2
+          if (m < Q - 1) store_out<OutT, EPI>(d, b, 4 * m + 3, v);  // row 4m+4
+        }
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------------
 // Mixed pass: fp32 operands split into bf16 hi + lo, three MFMA products, fp32 sums.
 // Work-group tile: TB b x (TM odd + TM even) rows, K chunk 32 (one MFMA k-step), TB, TM
 // in {32, 64} picked by the autotuner. The sine tables are stored per output row with k
@@ -376,13 +565,15 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
 // ---------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------
-// Tile variants (TB, TM, prefetch depth); the autotuner times them on a launch shape and
-// keeps the best. The mixed pass has one prefetch stage and uses variants 0..3.
-constexpr int kVariants = 8;
+// Tile variants (fold, TB, TM, prefetch depth); the autotuner times them on a launch
+// shape and keeps the best. FP64: 0..7 single fold, 8..15 double fold (TM counts quarter
+// rows there). The mixed pass has one prefetch stage and uses variants 0..3.
+constexpr int kVariants = kDstVariants;
 constexpr int kMixedVariants = 4;
-constexpr int kVarTB[kVariants] = {64, 64, 32, 32, 64, 64, 32, 32};
-constexpr int kVarTM[kVariants] = {64, 32, 64, 32, 64, 32, 64, 32};
-constexpr int kVarPF[kVariants] = {1, 1, 1, 1, 2, 2, 2, 2};
+constexpr int kVarTB[kVariants] = {64, 64, 32, 32, 64, 64, 32, 32, 64, 64, 32, 32, 64, 64, 32, 32};
+constexpr int kVarTM[kVariants] = {64, 32, 64, 32, 64, 32, 64, 32, 64, 32, 64, 32, 64, 32, 64, 32};
+constexpr int kVarPF[kVariants] = {1, 1, 1, 1, 2, 2, 2, 2, 1, 1, 1, 1, 2, 2, 2, 2};
+constexpr int kVarFold[kVariants] = {2, 2, 2, 2, 2, 2, 2, 2, 4, 4, 4, 4, 4, 4, 4, 4};
 
 static dim3 tile_grid(PassBatch& b, int nplanes, int maxNb, int maxH, int TB, int TM) {
   b.nbt = (maxNb + TB - 1) / TB;
@@ -408,12 +599,38 @@ struct MixK {
   }
 };
 
+template <int KC, int PF>
+struct F64QKC {
+  template <typename InT, typename OutT, int EPI, int TB, int TM>
+  struct K {
+    static constexpr bool quarter = true;
+    static void go(dim3 g, hipStream_t st, const PassBatch& b) {
+      hipLaunchKernelGGL((dst_pass_f64q_kernel<InT, OutT, EPI, TB, TM, KC, PF>), g, dim3(256), 0, st, b);
+    }
+  };
+};
+
+template <class T, class = void>
+struct is_quarter {
+  static constexpr bool value = false;
+};
+template <class T>
+struct is_quarter<T, decltype((void)T::quarter)> {
+  static constexpr bool value = T::quarter;
+};
+
 template <template <typename, typename, int, int, int> class K, typename InT, typename OutT,
           int TB, int TM>
 static void launch_epi(const PassBatch& b_in, int nplanes, int maxNb, int maxH, int epi,
                        hipStream_t st) {
   PassBatch b = b_in;
-  const dim3 g = tile_grid(b, nplanes, maxNb, maxH, TB, TM);
+  // double fold: TM quarter rows of each of the two blocks (odd, even) per tile
+  constexpr bool quarter = is_quarter<K<InT, OutT, EPI_STORE, TB, TM>>::value;
+  dim3 g = tile_grid(b, nplanes, maxNb, quarter ? maxH / 2 : maxH, TB, TM);
+  if (quarter) {
+    b.nmt *= 2;
+    g = dim3(8 * ((nplanes * b.nbt * b.nmt + 7) / 8));
+  }
   switch (epi) {
     case EPI_STORE: K<InT, OutT, EPI_STORE, TB, TM>::go(g, st, b); break;
     case EPI_DIAG: K<InT, OutT, EPI_DIAG, TB, TM>::go(g, st, b); break;
@@ -435,7 +652,9 @@ static void launch_variant(int v, const PassBatch& b, int nplanes, int maxNb, in
 
 static void launch_f64_variant(int v, const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
                                hipStream_t st) {
-  if (v >= 4) launch_variant<F64KC<16, 2>::K, double, double>(v - 4, b, nplanes, maxNb, maxH, epi, st);
+  if (v >= 12) launch_variant<F64QKC<16, 2>::K, double, double>(v - 12, b, nplanes, maxNb, maxH, epi, st);
+  else if (v >= 8) launch_variant<F64QKC<16, 1>::K, double, double>(v - 8, b, nplanes, maxNb, maxH, epi, st);
+  else if (v >= 4) launch_variant<F64KC<16, 2>::K, double, double>(v - 4, b, nplanes, maxNb, maxH, epi, st);
   else launch_variant<F64KC<16, 1>::K, double, double>(v, b, nplanes, maxNb, maxH, epi, st);
 }
 
@@ -449,7 +668,10 @@ static void launch_mixed_variant(int v, const PassBatch& b, int nplanes, int max
 
 // ---- autotuner ----
 // Process-wide table: launch shape (precision, planes, Nb and H of the largest plane) ->
-// the fastest variant measured on it. Shapes never tuned use variant 0 (64 x 64).
+// the fastest variant measured on it. The FP64 path is tuned over the double-fold
+// variants only, so every FP64 launch sums in the same order whichever tiles it gets;
+// shapes never tuned use the default variant of their precision.
+int dst_default_variant(bool mixed) { return mixed ? 0 : 10; }
 namespace {
 struct TuneKey {
   int mixed, nplanes, maxNb, maxH;
@@ -466,7 +688,7 @@ std::map<TuneKey, int> g_tune;
 int tuned_variant(int mixed, int nplanes, int maxNb, int maxH) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto it = g_tune.find(TuneKey{mixed, nplanes, maxNb, maxH});
-  return it == g_tune.end() ? 0 : it->second;
+  return it == g_tune.end() ? dst_default_variant(mixed != 0) : it->second;
 }
 }  // namespace
 
@@ -479,8 +701,8 @@ hipError_t launch_dst_pass_f64(const PassBatch& batch, int nplanes, int maxNb, i
 
 hipError_t launch_dst_pass_mixed(const PassBatch& batch, int nplanes, int maxNb, int maxH, int epi,
                                  bool in_f32, bool out_f32, hipStream_t st, int variant) {
-  const int v = variant >= 0 && variant < kMixedVariants ? variant
-                                                         : tuned_variant(1, nplanes, maxNb, maxH) % kMixedVariants;
+  const int v = (variant >= 0 && variant < kVariants ? variant : tuned_variant(1, nplanes, maxNb, maxH)) %
+                kMixedVariants;
   launch_mixed_variant(v, batch, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
   return hipGetLastError();
 }
@@ -508,11 +730,11 @@ hipError_t tune_dst_pass(const PassBatch& batch, int nplanes, int maxNb, int max
   // three rounds over the variants, best mean of 4 launches per round (interleaving the
   // rounds spreads clock and power-state drift over all variants)
   constexpr int reps = 4, rounds = 3;
-  const int nv = mixed ? kMixedVariants : kVariants;
+  const int v0 = mixed ? 0 : 8, v1 = mixed ? kMixedVariants : kVariants;
   float t[kVariants];
   for (int v = 0; v < kVariants; ++v) t[v] = 1e30f;
   for (int round = 0; round < rounds && e == hipSuccess; ++round) {
-    for (int v = 0; v < nv && e == hipSuccess; ++v) {
+    for (int v = v0; v < v1 && e == hipSuccess; ++v) {
       auto run = [&]() {
         if (mixed) launch_mixed_variant(v, batch, nplanes, maxNb, maxH, EPI_STORE, in_f32, out_f32, st);
         else launch_f64_variant(v, batch, nplanes, maxNb, maxH, EPI_STORE, st);
@@ -528,9 +750,9 @@ hipError_t tune_dst_pass(const PassBatch& batch, int nplanes, int maxNb, int max
       if (e == hipSuccess && ms / reps < t[v]) t[v] = ms / reps;
     }
   }
-  int best = 0;
-  float best_t = t[0];
-  for (int v = 1; v < nv; ++v)
+  int best = v0;
+  float best_t = t[v0];
+  for (int v = v0 + 1; v < v1; ++v)
     if (t[v] < best_t) {
       best_t = t[v];
       best = v;
@@ -553,12 +775,13 @@ int dst_tuned_variant(int mixed, int nplanes, int maxNb, int maxH) {
   return it == g_tune.end() ? -1 : it->second;
 }
 
-int dst_variant_tiles(int v, bool mixed, int* TB, int* TM, int* PF) {
+int dst_variant_tiles(int v, bool mixed, int* TB, int* TM, int* PF, int* fold) {
   if (v < 0 || v >= kVariants) return 0;
   if (mixed) v %= kMixedVariants;
   *TB = kVarTB[v];
   *TM = kVarTM[v];
   *PF = kVarPF[v];
+  if (fold) *fold = kVarFold[v];
   return 1;
 }
 
@@ -634,6 +857,25 @@ void build_sine_tables_f64(int N, double* So, double* Se) {
     for (long long m = 0; m < H; ++m) So[(k - 1) * H + m] = sine_entry((2 * m + 1) * k, M);
   for (long long k = 1; k <= H - 1; ++k)
     for (long long m = 0; m < H - 1; ++m) Se[(k - 1) * (H - 1) + m] = sine_entry(2 * (m + 1) * k, M);
+}
+
+size_t sine_table_quarter_count(int N) {
+  const size_t Q = (size_t)(N + 1) / 4;
+  return 2 * Q * Q + (Q - 1) * (Q - 1);
+}
+
+void build_sine_tables_f64q(int N, double* Sq) {
+  const long long M = N + 1, Q = M / 4;
+  double* T1 = Sq;
+  double* T2 = T1 + Q * Q;
+  double* T4 = T2 + Q * Q;
+  for (long long i = 0; i < Q; ++i)
+    for (long long m = 0; m < Q; ++m) {
+      T1[i * Q + m] = sine_entry((2 * m + 1) * (2 * i + 1), M);
+      T2[i * Q + m] = sine_entry((2 * m + 1) * (2 * i + 2), M);
+    }
+  for (long long i = 0; i < Q - 1; ++i)
+    for (long long m = 0; m < Q - 1; ++m) T4[i * (Q - 1) + m] = sine_entry(4 * (m + 1) * (i + 1), M);
 }
 
 static inline uint16_t bf16_bits_rne(float f) {

@@ -75,7 +75,7 @@ namespace {
 
 void free_all(ibpm_solver* s) {
   void* ptrs[] = {s->q, s->G, s->psi, s->nl1, s->nl2, s->Gs, s->rhs, s->T, s->U, s->R,
-                  s->cosx, s->cosy, s->SoX, s->SeX, s->SoY, s->SeY, s->BfX, s->BfY,
+                  s->cosx, s->cosy, s->SoX, s->SeX, s->SoY, s->SeY, s->SqX, s->SqY, s->BfX, s->BfY,
                   s->xb, s->ub, s->xb_ref, s->Ft, s->fb, s->Ftmp, s->total, s->motion,
                   s->body_of, s->moving_d, s->coords, s->perm, s->xstart, s->err, s->w,
                   s->ywin, s->gemv_part,
@@ -113,6 +113,7 @@ int run_pass(ibpm_solver* s, PassBatch& pb, int np, bool xdir, int epi, bool in_
     pb.p[i].Nb = Nb;
     pb.p[i].So = tab_odd(s, xdir);
     pb.p[i].Se = tab_even(s, xdir);
+    pb.p[i].Sq = xdir ? s->SqX : s->SqY;
     pb.p[i].cosb = xdir ? s->cosy : s->cosx;
     pb.p[i].cosj = xdir ? s->cosx : s->cosy;
     pb.p[i].dscale = s->scale;
@@ -151,6 +152,7 @@ int autotune_passes(ibpm_solver* s) {
         pb.p[i].Nb = Nb;
         pb.p[i].So = tab_odd(s, xdir);
         pb.p[i].Se = tab_even(s, xdir);
+        pb.p[i].Sq = xdir ? s->SqX : s->SqY;
         pb.p[i].cosb = xdir ? s->cosy : s->cosx;
         pb.p[i].cosj = xdir ? s->cosx : s->cosy;
         pb.p[i].dscale = s->scale;
@@ -850,15 +852,18 @@ void ibpm_default_options(ibpm_options* o) {
 
 const char* ibpm_last_error(void) { return g_err.c_str(); }
 
-int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM, int* PF) {
+int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM, int* PF,
+                   int* fold) {
   if (!s || !TB || !TM || !PF) return fail(IBPM_E_ARG, "null argument");
   int v = s->dst_tiles;
   if (v < 0) {
     const int Nk = xdir ? s->g.Nx : s->g.Ny, Nb = xdir ? s->g.Ny : s->g.Nx;
     v = dst_tuned_variant(s->precision == IBPM_MIXED, nplanes, Nb, (Nk + 1) / 2);
-    if (v < 0) v = 0;
+    if (v < 0) v = dst_default_variant(s->precision == IBPM_MIXED);
   }
-  dst_variant_tiles(v, s->precision == IBPM_MIXED, TB, TM, PF);
+  int f = 2;
+  dst_variant_tiles(v, s->precision == IBPM_MIXED, TB, TM, PF, &f);
+  if (fold) *fold = f;
   return IBPM_OK;
 }
 
@@ -952,10 +957,16 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   }
   // sine tables
   {
-    auto build = [&](int N, void** So, void** Se, void** Bf) -> hipError_t {
+    auto build = [&](int N, void** So, void** Se, void** Sq, void** Bf) -> hipError_t {
       std::vector<double> o(sine_table_odd_count(N)), e(std::max<size_t>(1, sine_table_even_count(N)));
       build_sine_tables_f64(N, o.data(), e.data());
-      hipError_t r = dalloc((double**)So, o.size());
+      std::vector<double> q(sine_table_quarter_count(N));
+      build_sine_tables_f64q(N, q.data());
+      hipError_t r = dalloc((double**)Sq, q.size());
+      if (r) return r;
+      r = hipMemcpy(*Sq, q.data(), q.size() * 8, hipMemcpyHostToDevice);
+      if (r) return r;
+      r = dalloc((double**)So, o.size());
       if (r) return r;
       r = dalloc((double**)Se, e.size());
       if (r) return r;
@@ -974,8 +985,8 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
       }
       return r;
     };
-    CK(build(s->g.Nx, &s->SoX, &s->SeX, &s->BfX));
-    CK(build(s->g.Ny, &s->SoY, &s->SeY, &s->BfY));
+    CK(build(s->g.Nx, &s->SoX, &s->SeX, &s->SqX, &s->BfX));
+    CK(build(s->g.Ny, &s->SoY, &s->SeY, &s->SqY, &s->BfY));
     // ring chain tables: the first and last rows of S (apply_bc! edges) and the ring-line
     // projection weights, with exact argument reduction
     const int nx = s->g.nx, ny = s->g.ny, Nx = s->g.Nx, Ny = s->g.Ny;
@@ -1026,14 +1037,15 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
       pb.p[0].Nb = N;
       pb.p[0].So = dir == 0 ? s->SoX : s->SoY;
       pb.p[0].Se = dir == 0 ? s->SeX : s->SeY;
+      pb.p[0].Sq = dir == 0 ? s->SqX : s->SqY;
       pb.p[0].cosb = dir == 0 ? s->cosx : s->cosy;
       pb.p[0].cosj = pb.p[0].cosb;
-      CK(launch_dst_pass_f64(pb, 1, N, (N + 1) / 2, EPI_STORE, s->st, 0));
+      CK(launch_dst_pass_f64(pb, 1, N, (N + 1) / 2, EPI_STORE, s->st, dst_default_variant(false)));
       CK(hipStreamSynchronize(s->st));
       (void)hipFree(Bd);
     }
   }
-  s->dst_tiles = opt.dst_tiles >= 0 && opt.dst_tiles <= 7 ? opt.dst_tiles : -1;
+  s->dst_tiles = opt.dst_tiles >= 0 && opt.dst_tiles < kDstVariants ? opt.dst_tiles : -1;
   if (s->dst_tiles < 0) {
     int rc = autotune_passes(s);
     if (rc) return bail(rc);
@@ -1448,8 +1460,14 @@ int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, doubl
     *flops = probe_flops;
     *bytes = 0.0;
   } else if (kernel == 0) {
-    const double Nk = s->g.Nx, Nb = s->g.Ny, H = (Nk + 1) / 2;
-    const double per_product = 2.0 * Nb * (H * H + (H - 1) * (H - 1));
+    // matrix-core flops of the contraction the launch runs: the single fold multiplies
+    // H x H and (H-1) x (H-1) tables, the double fold three Q x Q and one (Q-1) x (Q-1)
+    const double Nk = s->g.Nx, Nb = s->g.Ny, H = (Nk + 1) / 2, Q = (Nk + 1) / 4;
+    int v = s->dst_tiles >= 0 ? s->dst_tiles : dst_tuned_variant(mixed, L, s->g.Ny, s->g.Nx / 2 + 1);
+    int TB = 0, TM = 0, PF = 0, fold = 2;
+    dst_variant_tiles(v < 0 ? dst_default_variant(mixed) : v, mixed, &TB, &TM, &PF, &fold);
+    const double per_product = fold == 4 ? 2.0 * Nb * (3.0 * Q * Q + (Q - 1) * (Q - 1))
+                                         : 2.0 * Nb * (H * H + (H - 1) * (H - 1));
     *flops = L * per_product * (mixed ? 3.0 : 1.0);
     *bytes = L * Nk * Nb * (8.0 + (mixed ? 4.0 : 8.0));  // fp64 plane in, T plane out
   } else {

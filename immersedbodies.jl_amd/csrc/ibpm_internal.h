@@ -50,6 +50,7 @@ struct PassPlane {
   const double* cosj;  // along the transformed direction
   const void* So;      // odd-row sine table of the length-Nk transform
   const void* Se;      // even-row sine table
+  const void* Sq;      // FP64 double-fold tables [T1 | T2 | T4]
   double dscale, da, dnum, dden;
   int Nk, Nb;
 };
@@ -71,9 +72,11 @@ struct PassBatch {
 };
 
 // Host launchers (dst_kernels.hip).
-// variant: tile variant 0..7: (TB, TM) = (64,64), (64,32), (32,64), (32,32) with loads
-// issued 1 (0..3) or 2 (4..7) K chunks ahead (FP64); the mixed pass uses v % 4. -1 takes
-// the autotuned variant of the launch shape (variant 0 when the shape was never tuned).
+// variant: tile variant 0..15 (FP64): v % 4 picks (TB, TM) = (64,64), (64,32), (32,64),
+// (32,32), loads are issued 1 (v % 8 < 4) or 2 K chunks ahead, and v >= 8 folds the sine
+// matrix twice (four quarter-size contractions) instead of once; the mixed pass uses
+// v % 4. -1 takes the autotuned variant of the launch shape (dst_default_variant when the
+// shape was never tuned).
 hipError_t launch_dst_pass_f64(const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
                                hipStream_t st, int variant = -1);
 hipError_t launch_dst_pass_mixed(const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
@@ -83,13 +86,16 @@ hipError_t launch_dst_pass_mixed(const PassBatch& b, int nplanes, int maxNb, int
 // passes on the given planes (EPI_STORE); not to be called under graph capture.
 hipError_t tune_dst_pass(const PassBatch& b, int nplanes, int maxNb, int maxH, bool mixed,
                          bool in_f32, bool out_f32, hipStream_t st, int* chosen, float* best_ms);
-int dst_variant_tiles(int v, bool mixed, int* TB, int* TM, int* PF);
+int dst_variant_tiles(int v, bool mixed, int* TB, int* TM, int* PF, int* fold);
+constexpr int kDstVariants = 16;
 // Register-only MFMA loop (f64 16x16x4 or bf16 16x16x32) on `blocks` x 256 threads; out
 // needs blocks x 256 doubles. flops: matrix flops of the launch.
 hipError_t launch_mfma_probe(bool bf16, int iters, int blocks, double* out, hipStream_t st,
                              double* flops);
 // Tuned variant of a launch shape, -1 when never tuned.
 int dst_tuned_variant(int mixed, int nplanes, int maxNb, int maxH);
+// Variant of the shapes never tuned: FP64 double fold (32 x 64), mixed 64 x 64.
+int dst_default_variant(bool mixed);
 
 // ---- multi-level ring chain (ring_kernels.hip) ----
 constexpr int kMaxChains = 16;
@@ -124,6 +130,9 @@ void build_ring_operator_input(int n, double* B);
 
 // Sine tables for a DST-I of length N (N+1 even). Host builds, returns element counts.
 void build_sine_tables_f64(int N, double* So, double* Se);
+// double-fold tables of the length-N transform (N + 1 a multiple of 4)
+size_t sine_table_quarter_count(int N);
+void build_sine_tables_f64q(int N, double* Sq);
 // Split bf16 hi/lo tables, laid out for the MFMA B operand (k contiguous per column).
 void build_sine_tables_bf16(int N, uint16_t* So_hi, uint16_t* So_lo, uint16_t* Se_hi,
                             uint16_t* Se_lo);

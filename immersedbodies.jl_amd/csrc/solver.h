@@ -33,6 +33,7 @@ struct ibpm_solver {
   double *T = nullptr, *U = nullptr, *R = nullptr;  // kMaxPlanes scratch planes each
   double *cosx = nullptr, *cosy = nullptr;  // cos(pi k / n): spectral eigenvalues
   void *SoX = nullptr, *SeX = nullptr, *SoY = nullptr, *SeY = nullptr;  // FP64 tables
+  void *SqX = nullptr, *SqY = nullptr;  // FP64 double-fold tables
   void *BfX = nullptr, *BfY = nullptr;                                  // bf16 tables
   // multi-level ring chain: sine rows/columns, per-plane factors, per-chain scratch
   double *edge = nullptr, *pw = nullptr;

@@ -42,9 +42,10 @@ class HipCnabSolver(AbstractSolver):
           5000 / 1e-12 / 0); cg_tol_mode 0 is IterativeSolvers' reltol*||r0||,
           1 is reltol*||b||
       device: HIP device ordinal
-      dst_tiles: sine-pass tile variant; -1 autotunes at creation, 0..7 fix
-                 (TB, TM) = (64,64), (64,32), (32,64), (32,32) with FP64 loads issued
-                 1 (0..3) or 2 (4..7) K chunks ahead
+      dst_tiles: sine-pass tile variant; -1 autotunes at creation, 0..15 fix it:
+                 v % 4 picks (TB, TM) = (64,64), (64,32), (32,64), (32,32), FP64 loads
+                 run 1 (v % 8 < 4) or 2 K chunks ahead, v >= 8 folds the FP64 sine
+                 matrix twice
     """
 
     def __init__(self, state: State, *, precision="fp64", refine_sweeps=2, cg=None,
@@ -146,12 +147,13 @@ class HipCnabSolver(AbstractSolver):
         return np.array([s.b_applies for s in stats])
 
     def dst_tiles(self, nplanes: int, xdir: bool = True):
-        """(TB, TM, PF) tile shape and prefetch depth of the sine passes over ``nplanes``
-        planes (autotuned)."""
-        tb, tm, kc = C.c_int(), C.c_int(), C.c_int()
+        """(TB, TM, PF, fold): tile shape, prefetch depth and fold (2 or 4) of the sine
+        passes over ``nplanes`` planes (autotuned)."""
+        tb, tm, pf, fold = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         _lib.check(_lib.load().ibpm_dst_tiles(self._h, int(nplanes), int(bool(xdir)),
-                                              C.byref(tb), C.byref(tm), C.byref(kc)))
-        return tb.value, tm.value, kc.value
+                                              C.byref(tb), C.byref(tm), C.byref(pf),
+                                              C.byref(fold)))
+        return tb.value, tm.value, pf.value, fold.value
 
     def set_warm_start(self, mode: str = "previous", net=None, record: bool = False):
         """Initial guess of the moving-body CG: "previous" (the reference's previous F̃b),

@@ -89,12 +89,15 @@ typedef struct ibpm_options {
   double cg_abstol;   /* default 0                                               */
   int cg_tol_mode;    /* 0: reltol*||r0|| (IterativeSolvers), 1: reltol*||b||    */
   int device;         /* HIP device ordinal                                      */
-  int dst_tiles;      /* sine-pass tiles: -1 (default) autotuned at creation: every
-                         variant is timed on each launch shape the step uses and the
-                         fastest kept (process-wide, per shape); 0..7 fixed:
-                         (TB, TM) = (64,64), (64,32), (32,64), (32,32) with FP64 loads
-                         1 (0..3) or 2 (4..7) chunks ahead; mixed uses v % 4. Results
-                         do not depend on the variant (same summation order).       */
+  int dst_tiles;      /* sine-pass tiles: -1 (default) autotuned at creation: the
+                         variants (FP64: the double-fold ones, 8..15) are timed on
+                         each launch shape the step uses and the fastest kept
+                         (process-wide, per shape); 0..15 fixed:
+                         v % 4 picks (TB, TM) = (64,64), (64,32), (32,64), (32,32),
+                         FP64 loads run 1 (v % 8 < 4) or 2 chunks ahead, v >= 8
+                         folds the FP64 sine matrix twice instead of once; mixed uses
+                         v % 4. Variants of one fold give identical results (same
+                         summation order); the two folds agree to rounding.        */
 } ibpm_options;
 
 typedef struct ibpm_step_stats {
@@ -156,10 +159,11 @@ int ibpm_apply(ibpm_solver* s, int op, int level, const double* in, double* out,
 int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, double* flops,
                      double* bytes);
 
-/* Instrumentation: tile shape (TB values of the batch index x TM output rows per parity,
- * loads issued PF K chunks ahead) that the solver's sine passes of `nplanes` planes along x
- * (xdir != 0) or y use. */
-int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM, int* PF);
+/* Instrumentation: tile shape (TB values of the batch index x TM output rows per block,
+ * loads issued PF K chunks ahead) and fold (2: one even/odd fold, 4: two) that the
+ * solver's sine passes of `nplanes` planes along x (xdir != 0) or y use. fold may be NULL. */
+int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM, int* PF,
+                   int* fold);
 
 const char* ibpm_last_error(void);
 void ibpm_destroy(ibpm_solver* s);

@@ -1,0 +1,60 @@
+"""The double even/odd fold of the FP64 sine passes, restated with numpy and checked
+against scipy's DST-I (FFTW's RODFT00 convention: X_j = 2 sum_k x_k sin(pi j k / M)).
+
+M = N + 1 = 4Q. Odd rows pair as (2m+1, M-1-2m) through two Q x Q products of the odd-
+and even-indexed P_k = x_k + x_{M-k}; even rows are a DST-I of modulus H = 2Q in
+Q_k = x_k - x_{M-k} and fold the same way into rows 4m+2 and 4m+4.
+"""
+import numpy as np
+import pytest
+import scipy.fft
+
+
+def double_fold_dst(x):
+    N = x.shape[-1]
+    M = N + 1
+    H, Q = M // 2, M // 4
+    xx = np.concatenate([np.zeros(x.shape[:-1] + (1,)), x, np.zeros(x.shape[:-1] + (1,))], -1)
+    k = np.arange(1, H)
+    P = np.concatenate([xx[..., k] + xx[..., M - k], xx[..., H:H + 1]], -1)  # P_1..P_H
+    Qk = np.concatenate([xx[..., k] - xx[..., M - k], np.zeros(x.shape[:-1] + (1,))], -1)
+    i = np.arange(Q)
+    m = np.arange(Q)
+    T1 = 2 * np.sin(np.pi * np.outer(2 * i + 1, 2 * m + 1) / M)
+    T2 = 2 * np.sin(np.pi * np.outer(2 * i + 2, 2 * m + 1) / M)
+    T4 = 2 * np.sin(np.pi * 4 * np.outer(i[:-1] + 1, m[:-1] + 1) / M)
+    A = P[..., 0::2] @ T1          # P_{2i+1}
+    B = P[..., 1::2] @ T2          # P_{2i+2}
+    q = Qk[..., :Q]                # Q_1..Q_Q
+    qh = Qk[..., H - 2 - i]        # Q_{H-1-i}
+    Pp = np.where(i == Q - 1, q, q + qh)
+    Qp = (q - qh)[..., :Q - 1]
+    out = np.zeros(x.shape[:-1] + (N,))
+    out[..., 2 * m] = A + B                  # rows 2m+1
+    out[..., M - 2 - 2 * m] = A - B          # rows M-1-2m
+    out[..., 4 * m + 1] = Pp @ T2            # rows 4m+2
+    out[..., 4 * m[:-1] + 3] = Qp @ T4       # rows 4m+4
+    return out
+
+
+@pytest.mark.parametrize("M", [8, 12, 16, 200, 1024, 2048])
+def test_double_fold_equals_dst1(M):
+    rng = np.random.default_rng(M)
+    x = rng.standard_normal((3, M - 1))
+    ref = scipy.fft.dst(x, type=1, axis=-1)
+    got = double_fold_dst(x)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_double_fold_catches_a_sign_slip():
+    """A plausible slip (the even-k half added to the mirrored rows instead of
+    subtracted) must fail the comparison."""
+    M = 64
+    x = np.random.default_rng(1).standard_normal(M - 1)
+    ref = scipy.fft.dst(x, type=1)
+    got = double_fold_dst(x)
+    m = np.arange(M // 4)
+    bad = got.copy()
+    bad[M - 2 - 2 * m] = got[2 * m]  # A + B where A - B belongs
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    assert np.abs(bad - ref).max() > 1e-3 * np.abs(ref).max()

@@ -64,17 +64,29 @@ def test_vort2flux(precision, refine, tol, shape):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_tile_variants_give_identical_results(precision):
-    """Every sine-pass tile variant sums in the same order: identical fields after steps,
-    and the autotuned solver agrees with all of them."""
+    """Every sine-pass tile variant of one fold sums in the same order: identical fields
+    after steps. The single and double folds of the FP64 path agree to rounding."""
     prob = cases.cylinder(nx=96, ny=64, nlevel=3, h=0.05, x0=(-1.6, -1.6), Re=100.0)
-    ref = None
-    for v in (-1,) + tuple(range(8)):
+    variants = range(16) if precision == "fp64" else range(4)
+    ref = {}
+    for v in variants:
         s = ib.create_solver(prob, precision=precision, dst_tiles=v)
         s.advance(4)
         got = np.concatenate([s.read("Gamma"), s.read("psi"), s.read("Ftilde")])
-        if v == -1:
-            print("autotuned tiles", s.dst_tiles(1), s.dst_tiles(3), s.dst_tiles(3, False))
+        fold = s.dst_tiles(3)[3]
         s.close()
-        if ref is None:
-            ref = got
-        assert np.array_equal(got, ref), v
+        ref.setdefault(fold, got)
+        assert np.array_equal(got, ref[fold]), v
+    if len(ref) == 2:
+        assert cases.relerr(ref[4], ref[2]) < 1e-13
+    # autotuned: FP64 is tuned over the double fold only (mixed has one fold), so the
+    # result is bitwise that of any fixed variant of the fold
+    s = ib.create_solver(prob, precision=precision)
+    s.advance(4)
+    got = np.concatenate([s.read("Gamma"), s.read("psi"), s.read("Ftilde")])
+    tiles = [s.dst_tiles(1), s.dst_tiles(3), s.dst_tiles(3, False)]
+    print("autotuned tiles (TB, TM, PF, fold)", tiles)
+    s.close()
+    fold = 4 if precision == "fp64" else 2
+    assert all(t[3] == fold for t in tiles)
+    assert np.array_equal(got, ref[fold])
