@@ -134,9 +134,10 @@ def warm_start_bench(args):
     per = max(1, args.ws_samples // len(ks_train))
     cg = dict(maxiter=5000, reltol=1e-8)
 
-    def make(k):
+    def make(k, precond=False):
         return ib.create_solver(cases.pitching_plate_512(k), precision=args.ws_precision,
-                                refine_sweeps=args.refine, cg=cg, cg_tol_mode=1)
+                                refine_sweeps=args.refine, cg=cg, cg_tol_mode=1,
+                                cg_precond=precond)
 
     net = WarmStartNet(channels=128, layers=4, kernel=5, max_points=64,
                        capacity=per * len(ks_train), max_batch=64, seed=1)
@@ -154,36 +155,42 @@ def warm_start_bench(args):
     log(f"warm start: trained {args.ws_iters} iterations in {t_train:.1f} s")
     modes = {}
     forces = {}
-    for mode in ("previous", "extrapolate", "learned"):
-        s = make(k_test)
+    for key in ("previous", "extrapolate", "learned", "previous+precond",
+                "extrapolate+precond", "learned+precond"):
+        mode, precond = key.split("+")[0], key.endswith("+precond")
+        s = make(k_test, precond)
         s.set_warm_start("extrapolate")
-        s.advance(args.ws_warm)
+        s.advance(args.ws_warm)  # for +precond this includes the assembly (first step)
         s.set_warm_start(mode, net=net if mode == "learned" else None)
         _lib.check(_lib.load().ibpm_synchronize(s._h))
         t0 = time.perf_counter()
         st = s.advance(args.ws_eval_steps)
         dt = time.perf_counter() - t0
-        modes[mode] = {"cg_iters_per_step": round(float(s.cg_iters(st).mean()), 3),
-                       "b_applies_per_step": round(float(s.b_applies(st).mean()), 3),
-                       "steps_per_s": round(args.ws_eval_steps / dt, 2)}
+        modes[key] = {"cg_iters_per_step": round(float(s.cg_iters(st).mean()), 3),
+                      "b_applies_per_step": round(float(s.b_applies(st).mean()), 3),
+                      "steps_per_s": round(args.ws_eval_steps / dt, 2)}
         if mode == "learned":
-            modes[mode]["learned_guess_kept"] = float(np.mean([x.ws_used for x in st]))
-        forces[mode] = s.forces(st)
+            modes[key]["learned_guess_kept"] = float(np.mean([x.ws_used for x in st]))
+        forces[key] = s.forces(st)
         s.close()
-        log(f"warm start: {mode} {modes[mode]}")
+        log(f"warm start: {key} {modes[key]}")
     ref = forces["previous"]
     err = {m: float(np.max(np.linalg.norm(forces[m] - ref, axis=1) / np.linalg.norm(ref, axis=1)))
-           for m in ("extrapolate", "learned")}
+           for m in forces if m != "previous"}
     return {
         "workload": "pitching plate, 512x512 cells, 4 levels, Re 200, dt 0.002, 33 points "
                     "(spacing 2h), alpha = 20+10 sin(2 pi f t) deg; train k = "
-                    f"{list(ks_train)}, test k = {k_test}; CG reltol 1e-8 of ||b||",
+                    f"{list(ks_train)}, test k = {k_test}; CG reltol 1e-8 of ||b||; "
+                    "+precond: CG preconditioned by B^-1 of the first step, rotated with the plate",
         "train_samples": per * len(ks_train), "record_s": round(t_rec, 2),
         "train_iters": args.ws_iters, "train_s": round(t_train, 2),
         "train_loss_first_last": [float(loss[:50].mean()), float(loss[-50:].mean())],
         "eval_steps": args.ws_eval_steps, "precision": args.ws_precision, "modes": modes,
         "cg_iter_reduction_vs_previous": round(modes["previous"]["cg_iters_per_step"] /
                                                max(modes["learned"]["cg_iters_per_step"], 1e-9), 3),
+        "cg_iter_reduction_learned_precond_vs_previous": round(
+            modes["previous"]["cg_iters_per_step"] /
+            max(modes["learned+precond"]["cg_iters_per_step"], 1e-9), 3),
         "b_apply_reduction_vs_previous": round(modes["previous"]["b_applies_per_step"] /
                                                modes["learned"]["b_applies_per_step"], 3),
         "force_rel_err_vs_previous_max": err,
@@ -245,7 +252,8 @@ def moving_bench(args):
     """Moving bodies at scale: three plunging ellipses on 2048 x 1024 cells of [-4,12] x
     [-4,4], 6 levels, 426 points, FP64, CG to a 1e-8 ||b||-relative residual; the
     reference's initial guess (the previous step's solution) against its quadratic
-    extrapolation."""
+    extrapolation, each with the reference's unpreconditioned CG and with CG
+    preconditioned by B^-1 of the first step (assembly timed in first_step_s)."""
     import immersedbodies as ib
     import cases
 
@@ -255,23 +263,37 @@ def moving_bench(args):
                        "Re 200, dt 0.001, 426 points, impulsive start, numpy default_rng(7)",
            "precision": "fp64", "cg": {"reltol": 1e-8, "tol_mode": "||b||-relative"},
            "warmup_steps": args.moving_warmup, "steps": args.moving_steps, "modes": {}}
-    for mode in ("previous", "extrapolate"):
-        s = ib.create_solver(prob, cg=dict(maxiter=2000, reltol=1e-8), cg_tol_mode=1)
+    forces = {}
+    for key in ("previous", "extrapolate", "previous+precond", "extrapolate+precond"):
+        mode, precond = key.split("+")[0], key.endswith("+precond")
+        s = ib.create_solver(prob, cg=dict(maxiter=2000, reltol=1e-8), cg_tol_mode=1,
+                             cg_precond=precond)
         s.set_warm_start(mode)
-        s.advance(args.moving_warmup)
+        t0 = time.perf_counter()
+        s.advance(1)  # with +precond: assembles B (2n applications) and inverts it
+        first = time.perf_counter() - t0
+        s.advance(args.moving_warmup - 1)
         t0 = time.perf_counter()
         st = s.advance(args.moving_steps)
         dt = time.perf_counter() - t0
         it = s.cg_iters(st)
         ba = s.b_applies(st)
-        out["modes"][mode] = {"steps_per_s": round(args.moving_steps / dt, 3),
-                              "ms_per_step": round(1e3 * dt / args.moving_steps, 3),
-                              "cg_iters_per_step": round(float(it.mean()), 2),
-                              "b_applies_per_step": round(float(ba.mean()), 2),
-                              "ms_per_b_apply": round(1e3 * dt / max(1, ba.sum()), 4),
-                              "all_converged": bool(all(x.cg_converged for x in st))}
+        out["modes"][key] = {"steps_per_s": round(args.moving_steps / dt, 3),
+                             "ms_per_step": round(1e3 * dt / args.moving_steps, 3),
+                             "cg_iters_per_step": round(float(it.mean()), 2),
+                             "b_applies_per_step": round(float(ba.mean()), 2),
+                             "ms_per_b_apply": round(1e3 * dt / max(1, ba.sum()), 4),
+                             "first_step_s": round(first, 3),
+                             "all_converged": bool(all(x.cg_converged for x in st))}
+        forces[key] = s.forces(st)
         s.close()
-        log(f"moving {mode}: {out['modes'][mode]}")
+        log(f"moving {key}: {out['modes'][key]}")
+    ref = forces["previous"]
+    out["force_rel_err_vs_previous_max"] = {
+        k: float(np.max(np.linalg.norm(f - ref, axis=1) / np.linalg.norm(ref, axis=1)))
+        for k, f in forces.items() if k != "previous"}
+    out["speedup_precond_vs_previous"] = round(out["modes"]["previous"]["ms_per_step"] /
+                                               out["modes"]["extrapolate+precond"]["ms_per_step"], 3)
     return out
 
 

@@ -435,6 +435,111 @@ __global__ __launch_bounds__(1024) void cg_step_kernel(const CgDev c) {
   }
 }
 
+// ---------------------------------------------------------------------------------
+// Preconditioned CG (IterativeSolvers.jl's PCGIterable order: c = Pl \ r, rho = c.r,
+// u = c + (rho / rho_prev) u, c = A u, alpha = rho / u.c, x += alpha u, r -= alpha c,
+// stop on the true ||r||). The preconditioner is a dense inverse of B assembled at an
+// earlier body configuration, applied in each body's frame: z = R P R^T r, with R the
+// rotation of every point's (x, y) pair by its body's angle change since the assembly.
+// scal additionally holds [6] rho.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ void pc_rotation(const PcDev& p, int i, double& cs, double& sn) {
+  const int b = p.body_of[i];
+  const double d = p.motion[6 * b + 4] - p.theta0[b];
+  cs = cos(d);
+  sn = sin(d);
+}
+
+// t = R^T r (a point's pair is (i, n + i))
+__global__ __launch_bounds__(256) void pcg_rot_in_kernel(const PcDev p, const double* r, double* t) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= p.n) return;
+  double cs, sn;
+  pc_rotation(p, i, cs, sn);
+  const double rx = r[i], ry = r[p.n + i];
+  t[i] = cs * rx + sn * ry;
+  t[p.n + i] = -sn * rx + cs * ry;
+}
+
+// z = R zr, rho = r.z, u = z + (rho / rho_prev) u
+__global__ __launch_bounds__(1024) void pcg_dir_kernel(const CgDev c, const PcDev p, const double* zr) {
+  __shared__ double sh[1024];
+  if (c.scal[4] != 0.0) return;
+  const int n = p.n;
+  double rz = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    double cs, sn;
+    pc_rotation(p, i, cs, sn);
+    const double zx = cs * zr[i] - sn * zr[n + i];
+    const double zy = sn * zr[i] + cs * zr[n + i];
+    rz += c.r[i] * zx + c.r[n + i] * zy;
+  }
+  const double rho = block_sum_1024(rz, sh);
+  const double beta = rho / c.scal[6];
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    double cs, sn;
+    pc_rotation(p, i, cs, sn);
+    const double zx = cs * zr[i] - sn * zr[n + i];
+    const double zy = sn * zr[i] + cs * zr[n + i];
+    c.u[i] = zx + beta * c.u[i];
+    c.u[n + i] = zy + beta * c.u[n + i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) c.scal[6] = rho;
+}
+
+// alpha = rho / u.c, x += alpha u, r -= alpha c, residual and stopping test
+__global__ __launch_bounds__(1024) void pcg_step_kernel(const CgDev c) {
+  __shared__ double sh[1024];
+  if (c.scal[4] != 0.0) return;
+  double uc = 0.0;
+  for (int i = threadIdx.x; i < c.n; i += 1024) uc += c.u[i] * c.c[i];
+  const double alpha = c.scal[6] / block_sum_1024(uc, sh);
+  double rr = 0.0;
+  for (int i = threadIdx.x; i < c.n; i += 1024) {
+    c.x[i] += alpha * c.u[i];
+    const double r = c.r[i] - alpha * c.c[i];
+    c.r[i] = r;
+    rr += r * r;
+  }
+  const double nres = sqrt(block_sum_1024(rr, sh));
+  const double iters = c.scal[3] + 1.0;
+  const bool conv = iters >= (double)c.maxiter || nres <= c.scal[2];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    c.scal[1] = c.scal[0];
+    c.scal[0] = nres;
+    c.scal[3] = iters;
+    c.scal[4] = conv ? 1.0 : 0.0;
+  }
+}
+
+// the search direction starts at zero with rho = 1 (so the first beta multiplies zero)
+__global__ void pcg_start_kernel(const CgDev c) {
+  for (int i = threadIdx.x; i < c.n; i += blockDim.x) c.u[i] = 0.0;
+  if (threadIdx.x == 0) c.scal[6] = 1.0;
+}
+
+hipError_t launch_pcg_start(const CgDev& c, hipStream_t st) {
+  hipLaunchKernelGGL(pcg_start_kernel, dim3(1), dim3(1024), 0, st, c);
+  return hipGetLastError();
+}
+
+hipError_t launch_pcg_precond_dir(const CgDev& c, const PcDev& p, double* t, double* zr,
+                                  double* part, hipStream_t st) {
+  hipLaunchKernelGGL(pcg_rot_in_kernel, dim3((p.n + 255) / 256), dim3(256), 0, st, p,
+                     (const double*)c.r, t);
+  hipError_t e = launch_gemv(2 * p.n, p.P, t, zr, part, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(pcg_dir_kernel, dim3(1), dim3(1024), 0, st, c, p, (const double*)zr);
+  return hipGetLastError();
+}
+
+hipError_t launch_pcg_step(const CgDev& c, hipStream_t st) {
+  hipLaunchKernelGGL(pcg_step_kernel, dim3(1), dim3(1024), 0, st, c);
+  return hipGetLastError();
+}
+
 hipError_t launch_cg_init(const CgDev& c, hipStream_t st) {
   hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(1024), 0, st, c);
   return hipGetLastError();

@@ -78,7 +78,7 @@ void free_all(ibpm_solver* s) {
                   s->cosx, s->cosy, s->SoX, s->SeX, s->SoY, s->SeY, s->SqX, s->SqY, s->BfX, s->BfY,
                   s->xb, s->ub, s->xb_ref, s->Ft, s->fb, s->Ftmp, s->total, s->motion,
                   s->body_of, s->moving_d, s->coords, s->perm, s->xstart, s->err, s->w,
-                  s->ywin, s->gemv_part,
+                  s->ywin, s->gemv_part, s->pc_P, s->pc_theta0, s->pc_t, s->pc_z,
                   s->Binv, s->Gtmp, s->psitmp, s->Sc, s->cg_r, s->cg_u, s->cg_c, s->cg_scal,
                   s->edge, s->pw, s->fac, s->rvec, s->rpart, s->Mx,
                   s->My != s->Mx ? s->My : nullptr,
@@ -507,6 +507,59 @@ bool host_inverse(int n, std::vector<double>& A, std::vector<double>& Ainv) {
   return true;
 }
 
+// B of the current body configuration, column by column (apply_B on unit vectors in
+// groups of 32 columns), copied to the host (column-major 2n x 2n). Leaves the B-operator
+// scratch sized for one column.
+int assemble_B(ibpm_solver* s, std::vector<double>& Bh) {
+  const int m = 2 * s->n;
+  const int ncol = std::min(m, 32);
+  int rc = alloc_bscratch(s, ncol);
+  if (rc) return rc;
+  double *Bm = nullptr, *E = nullptr;
+  hipError_t e = hipMalloc((void**)&Bm, (size_t)m * m * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&E, (size_t)m * ncol * sizeof(double));
+  std::vector<double> eh((size_t)m * ncol);
+  for (int c0 = 0; c0 < m && e == hipSuccess && rc == IBPM_OK; c0 += ncol) {
+    const int nc = std::min(ncol, m - c0);
+    std::fill(eh.begin(), eh.end(), 0.0);
+    for (int c = 0; c < nc; ++c) eh[(size_t)c * m + c0 + c] = 1.0;
+    e = hipMemcpyAsync(E, eh.data(), eh.size() * 8, hipMemcpyHostToDevice, s->st);
+    std::vector<const double*> z(nc);
+    std::vector<double*> x(nc);
+    for (int c = 0; c < nc; ++c) {
+      z[c] = E + (size_t)c * m;
+      x[c] = Bm + (size_t)(c0 + c) * m;
+    }
+    if (e == hipSuccess) rc = apply_B(s, nc, z.data(), x.data());
+    if (e == hipSuccess && rc == IBPM_OK) e = hipStreamSynchronize(s->st);
+  }
+  if (e == hipSuccess && rc == IBPM_OK) {
+    Bh.resize((size_t)m * m);
+    e = hipMemcpy(Bh.data(), Bm, Bh.size() * 8, hipMemcpyDeviceToHost);
+  }
+  if (Bm) (void)hipFree(Bm);
+  if (E) (void)hipFree(E);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(IBPM_E_HIP, std::string("assemble_B: ") + hipGetErrorString(e));
+  return alloc_bscratch(s, 1);
+}
+
+// Preconditioner of the moving-body CG: the inverse of B at the current configuration and
+// each body's angle now (the CG applies it rotated by the angle changes since).
+int assemble_precond(ibpm_solver* s) {
+  const int m = 2 * s->n;
+  std::vector<double> Bh, Bi;
+  int rc = assemble_B(s, Bh);
+  if (rc) return rc;
+  if (!host_inverse(m, Bh, Bi)) return fail(IBPM_E_SINGULAR, "B is singular");
+  if (!s->pc_P) HIPCHK(hipMalloc((void**)&s->pc_P, (size_t)m * m * sizeof(double)));
+  HIPCHK(hipMemcpy(s->pc_P, Bi.data(), Bi.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s->pc_theta0, s->cur_theta.data(), s->nbody * sizeof(double), hipMemcpyHostToDevice));
+  s->pc_ready = true;
+  s->pc_due = false;
+  return IBPM_OK;
+}
+
 int check_body_error(ibpm_solver* s) {
   int e = 0;
   HIPCHK(hipMemcpyAsync(&e, s->err, sizeof(int), hipMemcpyDeviceToHost, s->st));
@@ -631,6 +684,12 @@ int couple_surface(ibpm_solver* s, ibpm_step_stats* stats) {
       HIPCHK(launch_copy(s->Ft, s->ws_xe, n2, s->st));
     }
   }
+  int pc_assembled = 0;
+  if (s->cg_precond && (!s->pc_ready || s->pc_due)) {
+    rc = assemble_precond(s);
+    if (rc) return rc;
+    pc_assembled = 1;
+  }
   const double* zc[1] = {s->Ft};
   rc = apply_B(s, 1, zc, xc);
   if (rc) return rc;
@@ -658,13 +717,33 @@ int couple_surface(ibpm_solver* s, ibpm_step_stats* stats) {
   }
   const double bnorm = s->h_scal[5];
   const double res0 = s->h_scal[0];
-  while (s->h_scal[4] == 0.0) {
-    const double* zu[1] = {s->cg_u};
-    rc = apply_B(s, 1, zu, xc);
-    if (rc) return rc;
-    HIPCHK(launch_cg_step(c, s->st));
-    HIPCHK(hipMemcpyAsync(s->h_scal, s->cg_scal, 6 * sizeof(double), hipMemcpyDeviceToHost, s->st));
-    HIPCHK(hipStreamSynchronize(s->st));
+  if (s->cg_precond) {
+    PcDev pd;
+    pd.n = s->n;
+    pd.P = s->pc_P;
+    pd.body_of = s->body_of;
+    pd.motion = s->motion;
+    pd.theta0 = s->pc_theta0;
+    if (s->h_scal[4] == 0.0) HIPCHK(launch_pcg_start(c, s->st));
+    while (s->h_scal[4] == 0.0) {
+      HIPCHK(launch_pcg_precond_dir(c, pd, s->pc_t, s->pc_z, s->gemv_part, s->st));
+      const double* zu[1] = {s->cg_u};
+      rc = apply_B(s, 1, zu, xc);
+      if (rc) return rc;
+      HIPCHK(launch_pcg_step(c, s->st));
+      HIPCHK(hipMemcpyAsync(s->h_scal, s->cg_scal, 6 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+      HIPCHK(hipStreamSynchronize(s->st));
+    }
+    if (s->cg_precond_refresh > 0 && s->h_scal[3] > s->cg_precond_refresh) s->pc_due = true;
+  } else {
+    while (s->h_scal[4] == 0.0) {
+      const double* zu[1] = {s->cg_u};
+      rc = apply_B(s, 1, zu, xc);
+      if (rc) return rc;
+      HIPCHK(launch_cg_step(c, s->st));
+      HIPCHK(hipMemcpyAsync(s->h_scal, s->cg_scal, 6 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+      HIPCHK(hipStreamSynchronize(s->st));
+    }
   }
   if (s->ws_record) {
     rc = net_record(s->net, wa, s->st);
@@ -678,6 +757,7 @@ int couple_surface(ibpm_solver* s, ibpm_step_stats* stats) {
     stats->ws_used = ws_used;
     stats->ws_base_residual = bnorm > 0.0 ? base_res / bnorm : 0.0;
     stats->ws_residual = bnorm > 0.0 ? res0 / bnorm : 0.0;
+    stats->pc_assembled = pc_assembled;
   }
   return IBPM_OK;
 }
@@ -694,6 +774,7 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
   s->q0_shadow_valid = false;
   // _update_body_motion! (bodies.jl:43-51)
   if (s->moving) {
+    for (int b = 0; b < s->nbody; ++b) s->cur_theta[b] = motion[6 * b + 4];
     HIPCHK(hipMemcpyAsync(s->motion, motion, 6 * s->nbody * sizeof(double), hipMemcpyHostToDevice, s->st));
     BodyDev b = body_dev(s);
     HIPCHK(launch_body_update(b, 1, s->st));
@@ -848,6 +929,8 @@ void ibpm_default_options(ibpm_options* o) {
   o->cg_tol_mode = 0;
   o->device = 0;
   o->dst_tiles = -1;
+  o->cg_precond = 0;
+  o->cg_precond_refresh = 0;
 }
 
 const char* ibpm_last_error(void) { return g_err.c_str(); }
@@ -903,6 +986,8 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   s->cg_reltol = opt.cg_reltol > 0 ? opt.cg_reltol : 1e-12;
   s->cg_abstol = opt.cg_abstol;
   s->cg_tol_mode = opt.cg_tol_mode;
+  s->cg_precond = opt.cg_precond ? 1 : 0;
+  s->cg_precond_refresh = std::max(0, opt.cg_precond_refresh);
   s->device = opt.device;
   s->grid_moving = p->grid_moving ? 1 : 0;
   s->hc.resize(s->L);
@@ -1135,36 +1220,13 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   if (n > 0) {
     if (!s->moving) {
       const int m = 2 * n;
-      const int ncol = std::min(m, 32);
-      int rc = alloc_bscratch(s, ncol);
+      std::vector<double> Bh, Bi;
+      int rc = assemble_B(s, Bh);
       if (rc) return bail(rc);
-      double *Bm = nullptr, *E = nullptr;
-      CK(dalloc(&Bm, (size_t)m * m));
-      CK(dalloc(&E, (size_t)m * ncol));
-      std::vector<double> eh((size_t)m * ncol);
-      for (int c0 = 0; c0 < m; c0 += ncol) {
-        const int nc = std::min(ncol, m - c0);
-        std::fill(eh.begin(), eh.end(), 0.0);
-        for (int c = 0; c < nc; ++c) eh[(size_t)c * m + c0 + c] = 1.0;
-        CK(hipMemcpyAsync(E, eh.data(), eh.size() * 8, hipMemcpyHostToDevice, s->st));
-        std::vector<const double*> z(nc);
-        std::vector<double*> x(nc);
-        for (int c = 0; c < nc; ++c) {
-          z[c] = E + (size_t)c * m;
-          x[c] = Bm + (size_t)(c0 + c) * m;
-        }
-        rc = apply_B(s, nc, z.data(), x.data());
-        if (rc) { (void)hipFree(Bm); (void)hipFree(E); return bail(rc); }
-        CK(hipStreamSynchronize(s->st));
-      }
-      std::vector<double> Bh((size_t)m * m), Bi;
-      CK(hipMemcpy(Bh.data(), Bm, Bh.size() * 8, hipMemcpyDeviceToHost));
-      (void)hipFree(Bm);
-      (void)hipFree(E);
       if (!host_inverse(m, Bh, Bi)) return bail(fail(IBPM_E_SINGULAR, "B is singular"));
       CK(dalloc(&s->Binv, (size_t)m * m));
       CK(hipMemcpy(s->Binv, Bi.data(), Bi.size() * 8, hipMemcpyHostToDevice));
-      rc = alloc_bscratch(s, 0);
+      rc = alloc_bscratch(s, 0);  // static bodies never apply B again
       if (rc) return bail(rc);
     } else {
       int rc = alloc_bscratch(s, 1);
@@ -1177,6 +1239,12 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
       CK(dalloc(&s->ws_xe, 2 * (size_t)n));
       CK(dalloc(&s->ws_Bxe, 2 * (size_t)n));
       CK(dalloc(&s->ws_scal, 4));
+      s->cur_theta.assign(std::max(1, s->nbody), 0.0);
+      if (s->cg_precond) {
+        CK(dalloc(&s->pc_theta0, (size_t)std::max(1, s->nbody)));
+        CK(dalloc(&s->pc_t, 2 * (size_t)n));
+        CK(dalloc(&s->pc_z, 2 * (size_t)n));
+      }
     }
   }
   CK(hipStreamSynchronize(s->st));

@@ -268,13 +268,28 @@ struct CgDev {
   double* u;
   double* c;
   const double* b;
-  double* scal;  // [0] residual, [1] prev_residual, [2] tol, [3] iters, [4] converged
+  double* scal;  // [0] residual, [1] prev_residual, [2] tol, [3] iters, [4] converged,
+                 // [5] ||b||, [6] rho (preconditioned)
   int maxiter;
   double reltol, abstol;
   int tol_mode;
 };
 hipError_t launch_cg_init(const CgDev& c, hipStream_t st);
 hipError_t launch_cg_step(const CgDev& c, hipStream_t st);
+// Preconditioned CG: the dense inverse P (2n x 2n, column-major) of B at an earlier body
+// configuration, applied in each body's frame (angle change motion[6b+4] - theta0[b]).
+struct PcDev {
+  int n;                 // points (vectors hold x then y: 2n)
+  const double* P;
+  const int* body_of;
+  const double* motion;  // 6 per body: pos, vel, angle, angular velocity (this step)
+  const double* theta0;  // per body: angle when P was assembled
+};
+hipError_t launch_pcg_start(const CgDev& c, hipStream_t st);
+// u = z + beta u with z = R P R^T r; t, zr: 2n scratch; part: gemv_partial_count(2n)
+hipError_t launch_pcg_precond_dir(const CgDev& c, const PcDev& p, double* t, double* zr,
+                                  double* part, hipStream_t st);
+hipError_t launch_pcg_step(const CgDev& c, hipStream_t st);
 
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t st);
 hipError_t launch_copy(double* dst, const double* src, size_t n, hipStream_t st);

@@ -19,6 +19,12 @@ struct ibpm_solver {
   int dst_tiles = -1;  // sine-pass tile variant, -1: autotuned
   int cg_maxiter = 5000, cg_tol_mode = 0;
   double cg_reltol = 1e-12, cg_abstol = 0.0;
+  // preconditioned CG: inverse of B at an earlier configuration (pc_P), the bodies'
+  // angles then (pc_theta0, device), scratch vectors and GEMV partials
+  int cg_precond = 0, cg_precond_refresh = 0;
+  bool pc_ready = false, pc_due = false;
+  double *pc_P = nullptr, *pc_theta0 = nullptr, *pc_t = nullptr, *pc_z = nullptr;
+  std::vector<double> cur_theta;  // host: each body's angle at the current step
   int device = 0;
   hipStream_t st = nullptr;
   double fs[2] = {0.0, 0.0};

@@ -61,13 +61,15 @@ class Problem(C.Structure):
 class Options(C.Structure):
     _fields_ = [("precision", C.c_int), ("refine_sweeps", C.c_int), ("cg_maxiter", C.c_int),
                 ("cg_reltol", C.c_double), ("cg_abstol", C.c_double), ("cg_tol_mode", C.c_int),
-                ("device", C.c_int), ("dst_tiles", C.c_int)]
+                ("device", C.c_int), ("dst_tiles", C.c_int), ("cg_precond", C.c_int),
+                ("cg_precond_refresh", C.c_int)]
 
 
 class StepStats(C.Structure):
     _fields_ = [("cg_iters", C.c_int), ("cg_converged", C.c_int), ("cg_residual", C.c_double),
                 ("force", C.c_double * 2), ("b_applies", C.c_int), ("ws_used", C.c_int),
-                ("ws_base_residual", C.c_double), ("ws_residual", C.c_double)]
+                ("ws_base_residual", C.c_double), ("ws_residual", C.c_double),
+                ("pc_assembled", C.c_int)]
 
 
 class NetConfig(C.Structure):

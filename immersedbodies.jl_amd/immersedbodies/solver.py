@@ -41,6 +41,10 @@ class HipCnabSolver(AbstractSolver):
       cg: dict of maxiter / reltol / abstol for moving bodies (solver.jl:137 defaults
           5000 / 1e-12 / 0); cg_tol_mode 0 is IterativeSolvers' reltol*||r0||,
           1 is reltol*||b||
+      cg_precond: moving bodies: precondition cg! with the inverse of B assembled at the
+                  first step (2n applications of B), applied in each body's frame; CG
+                  still stops on the true residual. cg_precond_refresh > 0 reassembles
+                  after a step that needed more iterations than that.
       device: HIP device ordinal
       dst_tiles: sine-pass tile variant; -1 autotunes at creation, 0..15 fix it:
                  v % 4 picks (TB, TM) = (64,64), (64,32), (32,64), (32,32), FP64 loads
@@ -49,7 +53,8 @@ class HipCnabSolver(AbstractSolver):
     """
 
     def __init__(self, state: State, *, precision="fp64", refine_sweeps=2, cg=None,
-                 cg_tol_mode=0, device=0, dst_tiles=-1):
+                 cg_tol_mode=0, device=0, dst_tiles=-1, cg_precond=False,
+                 cg_precond_refresh=0):
         L = _lib.load()
         self.state = state
         prob = state.prob
@@ -84,6 +89,8 @@ class HipCnabSolver(AbstractSolver):
         o.cg_tol_mode = int(cg_tol_mode)
         o.device = int(device)
         o.dst_tiles = int(dst_tiles)
+        o.cg_precond = int(bool(cg_precond))
+        o.cg_precond_refresh = int(cg_precond_refresh)
         h = C.c_void_p()
         _lib.check(L.ibpm_create(C.byref(p), C.byref(o), C.byref(h)))
         self._h = h
@@ -145,6 +152,10 @@ class HipCnabSolver(AbstractSolver):
     def b_applies(self, stats=None):
         stats = self.last_stats if stats is None else stats
         return np.array([s.b_applies for s in stats])
+
+    def pc_assembled(self, stats=None):
+        stats = self.last_stats if stats is None else stats
+        return np.array([s.pc_assembled for s in stats])
 
     def dst_tiles(self, nplanes: int, xdir: bool = True):
         """(TB, TM, PF, fold): tile shape, prefetch depth and fold (2 or 4) of the sine

@@ -98,6 +98,13 @@ typedef struct ibpm_options {
                          folds the FP64 sine matrix twice instead of once; mixed uses
                          v % 4. Variants of one fold give identical results (same
                          summation order); the two folds agree to rounding.        */
+  int cg_precond;     /* moving bodies: 0 (default) the reference's unpreconditioned
+                         cg!; 1 preconditions it with the inverse of B assembled at
+                         an earlier body configuration, applied in each body's frame
+                         (its rotation since the assembly). CG still stops on the
+                         true residual of the same tolerance.                       */
+  int cg_precond_refresh; /* with cg_precond: reassemble at the next step when a
+                         step needed more iterations than this (0: never)          */
 } ibpm_options;
 
 typedef struct ibpm_step_stats {
@@ -111,6 +118,8 @@ typedef struct ibpm_step_stats {
   double ws_base_residual;  /* ||b - B x_e|| / ||b|| of the extrapolated guess x_e
                                (0 when not computed)                              */
   double ws_residual;  /* ||b - B x0|| / ||b|| of the guess CG started from       */
+  int pc_assembled;    /* cg_precond: 1 when the preconditioner was assembled (2n
+                          applications of B) in this step                         */
 } ibpm_step_stats;
 
 /* Defaults of the reference: FP64, cg maxiter 5000, reltol 1e-12. */
