@@ -1172,6 +1172,9 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
     s->body_start.assign(p->body_start, p->body_start + s->nbody);
     s->body_count.assign(p->body_count, p->body_count + s->nbody);
     s->body_moving.assign(p->body_moving, p->body_moving + s->nbody);
+    s->staged_motion.assign(6 * (size_t)s->nbody, 0.0);
+    s->staged_t.assign(s->nbody, 0.0);
+    s->staged_set.assign(s->nbody, 0);
     for (int b = 0; b < s->nbody; ++b) {
       const int st0 = p->body_start[b], cnt = p->body_count[b];
       if (st0 < 0 || cnt < 0 || st0 + cnt > n) return bail(fail(IBPM_E_ARG, "body partition out of range"));
@@ -1256,7 +1259,23 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
 int ibpm_advance(ibpm_solver* s, int nsteps, const double* freestream, const double* motion,
                  ibpm_step_stats* stats) {
   if (!s || nsteps < 0) return fail(IBPM_E_ARG, "bad arguments");
-  if (s->moving && !motion) return fail(IBPM_E_ARG, "moving bodies need motion");
+  if (s->moving && !motion && nsteps > 0) {
+    // staged motion (ibpm_set_body_motion): one step, every moving body at its time
+    if (nsteps != 1)
+      return fail(IBPM_E_ARG, "moving bodies need the motion array (staged motion covers one step)");
+    const double t_next = s->t0 + s->dt * s->index;  // time after inc_time_index!
+    for (int b = 0; b < s->nbody; ++b) {
+      if (!s->body_moving[b]) continue;
+      if (!s->staged_set[b])
+        return fail(IBPM_E_ARG, "moving body " + std::to_string(b) +
+                                    " has no motion for the next step (ibpm_set_body_motion)");
+      if (fabs(s->staged_t[b] - t_next) > 1e-9 * std::max(1.0, fabs(t_next)))
+        return fail(IBPM_E_ARG, "motion of body " + std::to_string(b) + " was evaluated at t = " +
+                                    std::to_string(s->staged_t[b]) + ", the next step is at t = " +
+                                    std::to_string(t_next));
+    }
+    motion = s->staged_motion.data();
+  }
   HIPCHK(hipSetDevice(s->device));
   for (int k = 0; k < nsteps; ++k) {
     ibpm_step_stats* sk = stats ? stats + k : nullptr;
@@ -1267,7 +1286,25 @@ int ibpm_advance(ibpm_solver* s, int nsteps, const double* freestream, const dou
     if (sk && s->n > 0) HIPCHK(hipMemcpyAsync(sk->force, s->total, 16, hipMemcpyDeviceToHost, s->st));
     if (sk && s->n == 0) sk->force[0] = sk->force[1] = 0.0;
   }
+  std::fill(s->staged_set.begin(), s->staged_set.end(), 0);
   HIPCHK(hipStreamSynchronize(s->st));
+  return IBPM_OK;
+}
+
+int ibpm_set_body_motion(ibpm_solver* s, int body, double t, const double pos[2],
+                         const double vel[2], double angle, double angular_vel) {
+  if (!s || !pos || !vel) return fail(IBPM_E_ARG, "null argument");
+  if (body < 0 || body >= s->nbody) return fail(IBPM_E_ARG, "body index out of range");
+  if (!s->body_moving[body]) return fail(IBPM_E_ARG, "body " + std::to_string(body) + " is static");
+  double* m = &s->staged_motion[6 * (size_t)body];
+  m[0] = pos[0];
+  m[1] = pos[1];
+  m[2] = vel[0];
+  m[3] = vel[1];
+  m[4] = angle;
+  m[5] = angular_vel;
+  s->staged_t[body] = t;
+  s->staged_set[body] = 1;
   return IBPM_OK;
 }
 

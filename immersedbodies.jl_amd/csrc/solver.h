@@ -70,6 +70,9 @@ struct ibpm_solver {
   std::vector<double> q0_shadow;
   bool q0_shadow_valid = false;
   std::vector<int> body_start, body_count, body_moving;
+  std::vector<double> staged_motion;  // ibpm_set_body_motion: 6 per body, and its time
+  std::vector<double> staged_t;
+  std::vector<char> staged_set;
 
   size_t plane() const { return (size_t)g.nG; }
 };

@@ -33,7 +33,7 @@ EXPORTS = ("ibpm_default_options", "ibpm_create", "ibpm_advance", "ibpm_read", "
            "ibpm_net_destroy", "ibpm_net_dataset_size", "ibpm_net_clear_dataset",
            "ibpm_net_train", "ibpm_net_param_count", "ibpm_net_get_params",
            "ibpm_net_set_params", "ibpm_net_forward", "ibpm_net_gradients", "ibpm_net_step",
-           "ibpm_batch_create", "ibpm_batch_advance", "ibpm_batch_destroy", "ibpm_dst_tiles")
+           "ibpm_batch_create", "ibpm_batch_advance", "ibpm_batch_destroy", "ibpm_dst_tiles", "ibpm_set_body_motion")
 
 IBPM_WS_PREVIOUS = 0
 IBPM_WS_EXTRAPOLATE = 1
@@ -159,6 +159,9 @@ def load():
     L.ibpm_batch_destroy.restype = None
     L.ibpm_dst_tiles.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _ip, _ip]
     L.ibpm_dst_tiles.restype = C.c_int
+    L.ibpm_set_body_motion.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp, _dp, C.c_double,
+                                       C.c_double]
+    L.ibpm_set_body_motion.restype = C.c_int
     _lib = L
     return L
 

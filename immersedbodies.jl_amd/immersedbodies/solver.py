@@ -141,6 +141,33 @@ class HipCnabSolver(AbstractSolver):
         self.last_stats = stats
         return stats
 
+    def set_body_motion(self, body: int, t: float, transform):
+        """Stage body ``body``'s RigidBodyTransform at time ``t`` (the next step's time) for
+        the next ``advance_staged``; the library checks the time."""
+        pos = (C.c_double * 2)(*transform.pos)
+        vel = (C.c_double * 2)(*transform.vel)
+        _lib.check(_lib.load().ibpm_set_body_motion(self._h, int(body), float(t), pos, vel,
+                                                    float(transform.angle),
+                                                    float(transform.angular_vel)))
+
+    def advance_staged(self):
+        """One advance! with the motion staged by set_body_motion (the freestream and grid
+        motion still come from the problem)."""
+        st = self.state
+        prob = st.prob
+        t = st.t0 + timestep(prob) * st.index
+        w = 8 if self._grid_moving else 2
+        fs = np.zeros(w)
+        fs[:2] = prob.fluid.freestream_vel(t)
+        if self._grid_moving:
+            fs[2:] = prob.fluid.grid_motion(t).as_array()
+        stats = (_lib.StepStats * 1)()
+        _lib.check(_lib.load().ibpm_advance(self._h, 1, _dp(fs), None, stats))
+        st.set_time_index(st.index + 1)
+        st.freestream_vel = fs[:2].copy()
+        self.last_stats = stats
+        return stats
+
     def forces(self, stats=None):
         stats = self.last_stats if stats is None else stats
         return np.array([[s.force[0], s.force[1]] for s in stats])

@@ -14,6 +14,7 @@
  *                        (includes _init_body_motion!, bodies.jl:33-41, and Binv_operator,
  *                        coupling.jl:179-202)
  *   ibpm_advance      <- advance!(solver::CnabSolver), solver.jl:155-190, nsteps times;
+ *   ibpm_set_body_motion <- _update_body_motion!'s motion(t) (bodies.jl:43-51), staged;
  *                        freestream_vel(t) and body motion(t) are evaluated by the caller
  *                        (problem.jl:546, :259) and passed in per step
  *   ibpm_read/_write  <- the State fields read by getstate/savers and written by
@@ -131,10 +132,18 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
  * for a moving grid (ux, uy, then the grid's GridVelocity at that time: centre x, y,
  * velocity x, y, angle, angular_vel). motion:
  * 6 x nbody x nsteps (pos x, pos y, vel x, vel y, angle, angular_vel of every body at
- * each new time; entries of static bodies are ignored), or NULL when no body moves.
- * stats: nsteps entries or NULL. */
+ * each new time; entries of static bodies are ignored), or NULL: when no body moves, or
+ * for one step (nsteps = 1) whose motion every moving body received through
+ * ibpm_set_body_motion. stats: nsteps entries or NULL. */
 int ibpm_advance(ibpm_solver* s, int nsteps, const double* freestream, const double* motion,
                  ibpm_step_stats* stats);
+
+/* Stage the motion of moving body `body` (0-based, in Bodies order) for the next step:
+ * its RigidBodyTransform evaluated by the caller at time t (motion(t), problem.jl:241-261,
+ * bodies.jl:43-51). t must be the time of the next step (ibpm_get_time after it); the
+ * next ibpm_advance with motion = NULL uses the staged values and clears them. */
+int ibpm_set_body_motion(ibpm_solver* s, int body, double t, const double pos[2],
+                         const double vel[2], double angle, double angular_vel);
 
 int ibpm_read(ibpm_solver* s, int field, double* dst, size_t n);
 int ibpm_write(ibpm_solver* s, int field, const double* src, size_t n);

@@ -178,3 +178,31 @@ def test_plunging_ellipses_full_size_warm_start_invariance():
         s.close()
     print("cg iterations", iters, flush=True)
     assert cases.relerr(forces["extrapolate"], forces["previous"]) <= 1e-7
+
+
+def test_staged_body_motion_matches_motion_array():
+    """ibpm_set_body_motion + ibpm_advance(motion = NULL) equals the motion array, and
+    the staging is checked: the time, a static body, a missing body, several steps."""
+    prob = cases.moving_cylinder()
+    a = ib.create_solver(prob)
+    b = ib.create_solver(prob)
+    dt = prob.scheme.dt
+    body = prob.bodies["cyl"]
+    for k in range(3):
+        a.advance(1)
+        t = b.state.t0 + dt * b.state.index
+        b.set_body_motion(0, t, body.motion(t))
+        b.advance_staged()
+    for name in ("Gamma", "psi", "Ftilde", "xb", "ub"):
+        assert np.array_equal(a.read(name), b.read(name)), name
+    with pytest.raises(RuntimeError, match="has no motion"):
+        b.advance_staged()
+    t = b.state.t0 + dt * b.state.index
+    b.set_body_motion(0, t + dt, body.motion(t + dt))
+    with pytest.raises(RuntimeError, match="evaluated at t"):
+        b.advance_staged()
+    with pytest.raises(RuntimeError, match="out of range"):
+        b.set_body_motion(1, t, body.motion(t))
+    s = ib.create_solver(cases.cylinder())
+    with pytest.raises(RuntimeError, match="is static"):
+        s.set_body_motion(0, 0.0, body.motion(0.0))
