@@ -540,6 +540,102 @@ hipError_t launch_pcg_step(const CgDev& c, hipStream_t st) {
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------------
+// Dense inverse by Gauss-Jordan elimination with partial pivoting on the augmented
+// [A | X], X = I at the start (the role of LAPACK getrf/getri in inv(), coupling.jl:190).
+// Column-major m x m. Step k: the pivot is the first row of largest |A[i,k]|, i >= k
+// (idamax); rows k and p are swapped in A (columns >= k: the earlier columns are already
+// unit vectors) and in X, the pivot row is divided by the pivot into `rowk`, and every
+// other row i subtracts A[i,k] times it. At the end A = I and X = A^-1.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void gj_pivot_kernel(int m, const double* A, int k, int* piv,
+                                                        double* pval, int* singular) {
+  __shared__ double sv[1024];
+  __shared__ int si[1024];
+  double best = -1.0;
+  int bi = m;
+  const double* col = A + (size_t)k * m;
+  for (int i = k + threadIdx.x; i < m; i += 1024) {
+    const double v = fabs(col[i]);
+    if (v > best) {  // strided scan in increasing i: keeps the first of equal values
+      best = v;
+      bi = i;
+    }
+  }
+  sv[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const double v2 = sv[threadIdx.x + w];
+      const int i2 = si[threadIdx.x + w];
+      if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && i2 < si[threadIdx.x])) {
+        sv[threadIdx.x] = v2;
+        si[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int p = si[0] < m ? si[0] : k;
+    piv[0] = p;
+    pval[0] = col[p];
+    if (col[p] == 0.0) singular[0] = 1;
+  }
+}
+
+// swap rows k and piv in A (columns >= k) and X; rowk = the new row k / pivot
+__global__ __launch_bounds__(256) void gj_swap_kernel(int m, double* A, double* X, int k,
+                                                      const int* piv, const double* pval,
+                                                      double* rowk) {
+  const int t = blockIdx.x * 256 + threadIdx.x;  // over (m - k) columns of A, then m of X
+  const int na = m - k;
+  if (t >= na + m) return;
+  const int p = piv[0];
+  const double d = pval[0];
+  double* col = t < na ? A + (size_t)(k + t) * m : X + (size_t)(t - na) * m;
+  const double a = col[k], b = col[p];
+  col[k] = b;
+  col[p] = a;
+  rowk[t] = d != 0.0 ? b / d : 0.0;
+}
+
+// M[i, j] -= A[i,k] * rowk[j] (i != k), M[k, j] = rowk[j], over A's columns > k and X
+__global__ __launch_bounds__(256) void gj_elim_kernel(int m, double* A, double* X, int k,
+                                                      const double* rowk) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int t = blockIdx.y * 4 + (threadIdx.x >> 6) + 1;  // skips A's column k (t = 0)
+  const int na = m - k;
+  if (i >= m || t >= na + m) return;
+  double* col = t < na ? A + (size_t)(k + t) * m : X + (size_t)(t - na) * m;
+  const double r = rowk[t];
+  if (i == k) col[i] = r;
+  else col[i] -= A[(size_t)k * m + i] * r;
+}
+
+__global__ void gj_identity_kernel(int m, double* X, int* singular) {
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx == 0) singular[0] = 0;
+  if (idx >= (size_t)m * m) return;
+  X[idx] = (idx % (m + 1) == 0) ? 1.0 : 0.0;
+}
+
+hipError_t launch_gauss_jordan_inverse(int m, double* A, double* X, double* rowk, int* ipiv,
+                                       double* pval, int* singular, hipStream_t st) {
+  hipLaunchKernelGGL(gj_identity_kernel, dim3((unsigned)(((size_t)m * m + 255) / 256)), dim3(256), 0,
+                     st, m, X, singular);
+  for (int k = 0; k < m; ++k) {
+    hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(1024), 0, st, m, (const double*)A, k, ipiv,
+                       pval, singular);
+    const int ncol = (m - k) + m;
+    hipLaunchKernelGGL(gj_swap_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st, m, A, X, k,
+                       (const int*)ipiv, (const double*)pval, rowk);
+    hipLaunchKernelGGL(gj_elim_kernel, dim3((m + 63) / 64, (ncol - 1 + 3) / 4), dim3(256), 0, st,
+                       m, A, X, k, (const double*)rowk);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_cg_init(const CgDev& c, hipStream_t st) {
   hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(1024), 0, st, c);
   return hipGetLastError();

@@ -462,62 +462,23 @@ int alloc_bscratch(ibpm_solver* s, int ncol) {
   return IBPM_OK;
 }
 
-// LU with partial pivoting and inverse (LAPACK getrf/getri in inv(), coupling.jl:190).
-bool host_inverse(int n, std::vector<double>& A, std::vector<double>& Ainv) {
-  std::vector<int> piv(n);
-  auto a = [&](int i, int j) -> double& { return A[i + (size_t)j * n]; };
-  for (int k = 0; k < n; ++k) {
-    int p = k;
-    double mx = fabs(a(k, k));
-    for (int i = k + 1; i < n; ++i)
-      if (fabs(a(i, k)) > mx) { mx = fabs(a(i, k)); p = i; }
-    if (mx == 0.0) return false;
-    piv[k] = p;
-    if (p != k)
-      for (int j = 0; j < n; ++j) std::swap(a(k, j), a(p, j));
-    const double inv = 1.0 / a(k, k);
-    for (int i = k + 1; i < n; ++i) a(i, k) *= inv;
-    for (int j = k + 1; j < n; ++j) {
-      const double akj = a(k, j);
-      if (akj != 0.0) {
-        double* cj = &A[(size_t)j * n];
-        const double* ck = &A[(size_t)k * n];
-        for (int i = k + 1; i < n; ++i) cj[i] -= ck[i] * akj;
-      }
-    }
-  }
-  Ainv.assign((size_t)n * n, 0.0);
-  for (int col = 0; col < n; ++col) {
-    double* x = &Ainv[(size_t)col * n];
-    x[col] = 1.0;
-    for (int k = 0; k < n; ++k)
-      if (piv[k] != k) std::swap(x[k], x[piv[k]]);
-    for (int j = 0; j < n; ++j) {  // forward, unit lower
-      const double xj = x[j];
-      if (xj != 0.0)
-        for (int i = j + 1; i < n; ++i) x[i] -= a(i, j) * xj;
-    }
-    for (int j = n - 1; j >= 0; --j) {  // backward
-      x[j] /= a(j, j);
-      const double xj = x[j];
-      if (xj != 0.0)
-        for (int i = 0; i < j; ++i) x[i] -= a(i, j) * xj;
-    }
-  }
-  return true;
-}
-
 // B of the current body configuration, column by column (apply_B on unit vectors in
-// groups of 32 columns), copied to the host (column-major 2n x 2n). Leaves the B-operator
-// scratch sized for one column.
-int assemble_B(ibpm_solver* s, std::vector<double>& Bh) {
+// groups of 32 columns), then its inverse on the device (Gauss-Jordan with partial
+// pivoting, the role of inv() in coupling.jl:190) into `inv` (2n x 2n, column-major,
+// allocated here when null). Leaves the B-operator scratch sized for one column.
+int assemble_B_inverse(ibpm_solver* s, double*& inv) {
   const int m = 2 * s->n;
   const int ncol = std::min(m, 32);
   int rc = alloc_bscratch(s, ncol);
   if (rc) return rc;
-  double *Bm = nullptr, *E = nullptr;
+  double *Bm = nullptr, *E = nullptr, *rowk = nullptr, *pval = nullptr;
+  int* ip = nullptr;
   hipError_t e = hipMalloc((void**)&Bm, (size_t)m * m * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&E, (size_t)m * ncol * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&rowk, 2 * (size_t)m * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&pval, sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&ip, 2 * sizeof(int));
+  if (e == hipSuccess && !inv) e = hipMalloc((void**)&inv, (size_t)m * m * sizeof(double));
   std::vector<double> eh((size_t)m * ncol);
   for (int c0 = 0; c0 < m && e == hipSuccess && rc == IBPM_OK; c0 += ncol) {
     const int nc = std::min(ncol, m - c0);
@@ -531,29 +492,27 @@ int assemble_B(ibpm_solver* s, std::vector<double>& Bh) {
       x[c] = Bm + (size_t)(c0 + c) * m;
     }
     if (e == hipSuccess) rc = apply_B(s, nc, z.data(), x.data());
-    if (e == hipSuccess && rc == IBPM_OK) e = hipStreamSynchronize(s->st);
+    if (e == hipSuccess && rc == IBPM_OK) e = hipStreamSynchronize(s->st);  // eh is reused
   }
-  if (e == hipSuccess && rc == IBPM_OK) {
-    Bh.resize((size_t)m * m);
-    e = hipMemcpy(Bh.data(), Bm, Bh.size() * 8, hipMemcpyDeviceToHost);
-  }
-  if (Bm) (void)hipFree(Bm);
-  if (E) (void)hipFree(E);
+  int singular = 0;
+  if (e == hipSuccess && rc == IBPM_OK)
+    e = launch_gauss_jordan_inverse(m, Bm, inv, rowk, ip, pval, ip + 1, s->st);
+  if (e == hipSuccess && rc == IBPM_OK)
+    e = hipMemcpyAsync(&singular, ip + 1, sizeof(int), hipMemcpyDeviceToHost, s->st);
+  if (e == hipSuccess && rc == IBPM_OK) e = hipStreamSynchronize(s->st);
+  for (void* p : {(void*)Bm, (void*)E, (void*)rowk, (void*)pval, (void*)ip})
+    if (p) (void)hipFree(p);
   if (rc) return rc;
-  if (e != hipSuccess) return fail(IBPM_E_HIP, std::string("assemble_B: ") + hipGetErrorString(e));
+  if (e != hipSuccess) return fail(IBPM_E_HIP, std::string("assemble_B_inverse: ") + hipGetErrorString(e));
+  if (singular) return fail(IBPM_E_SINGULAR, "B is singular");
   return alloc_bscratch(s, 1);
 }
 
 // Preconditioner of the moving-body CG: the inverse of B at the current configuration and
 // each body's angle now (the CG applies it rotated by the angle changes since).
 int assemble_precond(ibpm_solver* s) {
-  const int m = 2 * s->n;
-  std::vector<double> Bh, Bi;
-  int rc = assemble_B(s, Bh);
+  int rc = assemble_B_inverse(s, s->pc_P);
   if (rc) return rc;
-  if (!host_inverse(m, Bh, Bi)) return fail(IBPM_E_SINGULAR, "B is singular");
-  if (!s->pc_P) HIPCHK(hipMalloc((void**)&s->pc_P, (size_t)m * m * sizeof(double)));
-  HIPCHK(hipMemcpy(s->pc_P, Bi.data(), Bi.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(s->pc_theta0, s->cur_theta.data(), s->nbody * sizeof(double), hipMemcpyHostToDevice));
   s->pc_ready = true;
   s->pc_due = false;
@@ -1222,13 +1181,8 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   // B^-1 (static) or CG buffers (moving): coupling.jl:179-202
   if (n > 0) {
     if (!s->moving) {
-      const int m = 2 * n;
-      std::vector<double> Bh, Bi;
-      int rc = assemble_B(s, Bh);
+      int rc = assemble_B_inverse(s, s->Binv);
       if (rc) return bail(rc);
-      if (!host_inverse(m, Bh, Bi)) return bail(fail(IBPM_E_SINGULAR, "B is singular"));
-      CK(dalloc(&s->Binv, (size_t)m * m));
-      CK(hipMemcpy(s->Binv, Bi.data(), Bi.size() * 8, hipMemcpyHostToDevice));
       rc = alloc_bscratch(s, 0);  // static bodies never apply B again
       if (rc) return bail(rc);
     } else {

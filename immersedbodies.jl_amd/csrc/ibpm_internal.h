@@ -285,6 +285,10 @@ struct PcDev {
   const double* motion;  // 6 per body: pos, vel, angle, angular velocity (this step)
   const double* theta0;  // per body: angle when P was assembled
 };
+// X = A^-1 by Gauss-Jordan with partial pivoting (A is destroyed); column-major m x m.
+// rowk: 2m doubles; ipiv, pval, singular: one element each (singular = 1 on a zero pivot).
+hipError_t launch_gauss_jordan_inverse(int m, double* A, double* X, double* rowk, int* ipiv,
+                                       double* pval, int* singular, hipStream_t st);
 hipError_t launch_pcg_start(const CgDev& c, hipStream_t st);
 // u = z + beta u with z = R P R^T r; t, zr: 2n scratch; part: gemv_partial_count(2n)
 hipError_t launch_pcg_precond_dir(const CgDev& c, const PcDev& p, double* t, double* zr,
