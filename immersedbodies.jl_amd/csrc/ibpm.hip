@@ -291,7 +291,7 @@ int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double
   ra.My = s->My;
   const size_t nv = ring_vector_count(s->g.Nx, s->g.Ny);
   const size_t npart = ring_partial_count(s->g.Nx, s->g.Ny);
-  const size_t npa = 2 * (size_t)((s->g.Ny + 63) / 64) * s->g.Nx;  // part_a: 2 x tiles_y x Nx
+  const size_t npa = ring_part_a_count(s->g.Nx, s->g.Ny);
   for (int l = L - 1; l >= 0; --l) {
     for (int c = 0; c < ng; ++c) {
       ra.F[c] = s->U + (c * L + l) * P;

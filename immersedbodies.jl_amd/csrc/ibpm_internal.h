@@ -124,6 +124,8 @@ hipError_t launch_ring_sweep(const RingChainArgs& a, int nchain, bool do_proj, h
 hipError_t launch_ring_factor(const RingChainArgs& a, int nchain, hipStream_t st);
 size_t ring_vector_count(int Nx, int Ny);
 size_t ring_partial_count(int Nx, int Ny);
+// leading part_a share of the partials (part_c follows it)
+size_t ring_part_a_count(int Nx, int Ny);
 // Input plane (N x N, k contiguous per column) whose sine pass along k is the ring operator
 // M_n (row-major): projection at level l -> factor at level l-1, for an axis of n cells.
 void build_ring_operator_input(int n, double* B);

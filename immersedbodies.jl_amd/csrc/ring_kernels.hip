@@ -34,27 +34,29 @@
 
 namespace ibpm {
 
-constexpr int RT = 64;  // tile edge (kx and ky) of the sweep
+constexpr int RT = 64;   // tile width (kx) of the sweep
+constexpr int RTY = 16;  // tile height (ky): 4 rows per wave
 
-// One 64 x 64 tile of the spectral plane per block (lanes along kx, each wave 16 rows):
+// One 64 x 16 tile of the spectral plane per block (lanes along kx, each wave 4 rows):
 // E = D . (F + R) written to E and, with do_proj, the tile's partial projections
 //   part_a[by][side][kx] = sum over the tile's ky of E S_y[ky, ny/4 | 3ny/4]
 //   part_c[bx][side][ky] = sum over the tile's kx of S_x[nx/4 | 3nx/4, kx] E
-// (completed by ring_reduce_kernel; every sum runs in a fixed order).
+// (completed by ring_reduce_kernel; every sum runs in a fixed order). Short tiles give
+// four blocks per CU on a 1023^2 plane, enough loads in flight to cover the latency.
 __global__ __launch_bounds__(256) void ring_sweep_kernel(const RingChainArgs a, int do_proj) {
-  __shared__ double yv[7][RT];           // cosy, sy1, syN, vL, vR, wb, wt of the tile's rows
-  __shared__ double rs[2][RT][RT + 1];   // S_x-weighted E of the tile, per row, for the row sums
+  __shared__ double yv[7][RTY];          // cosy, sy1, syN, vL, vR, wb, wt of the tile's rows
+  __shared__ double rs[2][RTY][RT + 1];  // S_x-weighted E of the tile, per row, for the row sums
   __shared__ double pa[2][4][RT];        // per-wave partial a_b, a_t
-  __shared__ double half[2][2][RT];      // row sums over each half of the tile's kx
+  __shared__ double part[2][8][RTY];     // row sums over eighths of the tile's kx
   const int c = blockIdx.z;
   const int Nx = a.Nx, Ny = a.Ny;
   const int tid = threadIdx.x, tx = tid & 63, w = tid >> 6;
   const int bx = blockIdx.x, by = blockIdx.y;
   const int kx = bx * RT + tx + 1;
-  const int ky0 = by * RT + 1;
+  const int ky0 = by * RTY + 1;
   const bool okx = kx <= Nx;
   const double* __restrict__ fac = a.fac_in[c];
-  if (tid < RT) {
+  if (tid < RTY) {
     const int ky = ky0 + tid;
     const bool ok = ky <= Ny;
     yv[0][tid] = ok ? a.cosy[ky - 1] : 0.0;
@@ -73,9 +75,9 @@ __global__ __launch_bounds__(256) void ring_sweep_kernel(const RingChainArgs a, 
   const void* __restrict__ F = a.F[c];
   void* __restrict__ E = a.E[c];
   double ab = 0.0, at = 0.0;
-#pragma unroll 4
-  for (int m = 0; m < RT / 4; ++m) {
-    const int yy = w * (RT / 4) + m;
+#pragma unroll
+  for (int m = 0; m < RTY / 4; ++m) {
+    const int yy = w * (RTY / 4) + m;
     const int ky = ky0 + yy;
     double e = 0.0;
     if (okx && ky <= Ny) {
@@ -98,18 +100,21 @@ __global__ __launch_bounds__(256) void ring_sweep_kernel(const RingChainArgs a, 
   __syncthreads();
   if (w < 2 && okx)
     a.part_a[c][((size_t)by * 2 + w) * Nx + (kx - 1)] = ((pa[w][0][tx] + pa[w][1][tx]) + pa[w][2][tx]) + pa[w][3][tx];
-  {  // row sums: thread -> (row, side, half of the kx range)
-    const int row = tid & 63, side = (tid >> 6) & 1, h = tid >> 7;
+  {  // row sums: thread -> (row, side, eighth of the kx range)
+    const int row = tid & (RTY - 1), side = (tid >> 4) & 1, e8 = tid >> 5;
     double sum = 0.0;
-    for (int q = 0; q < RT / 2; ++q) sum += rs[side][row][h * (RT / 2) + q];
-    half[side][h][row] = sum;
+#pragma unroll
+    for (int q = 0; q < RT / 8; ++q) sum += rs[side][row][e8 * (RT / 8) + q];
+    part[side][e8][row] = sum;
   }
   __syncthreads();
-  if (tid < 2 * RT) {
-    const int row = tid & 63, side = tid >> 6;
+  if (tid < 2 * RTY) {
+    const int row = tid & (RTY - 1), side = tid >> 4;
     const int ky = ky0 + row;
-    if (ky <= Ny)
-      a.part_c[c][((size_t)bx * 2 + side) * Ny + (ky - 1)] = half[side][0][row] + half[side][1][row];
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) sum += part[side][q][row];
+    if (ky <= Ny) a.part_c[c][((size_t)bx * 2 + side) * Ny + (ky - 1)] = sum;
   }
 }
 
@@ -179,7 +184,7 @@ __global__ __launch_bounds__(256) void ring_factor_kernel(const RingChainArgs a)
 }
 
 hipError_t launch_ring_sweep(const RingChainArgs& a, int nchain, bool do_proj, hipStream_t st) {
-  const int ntx = (a.Nx + RT - 1) / RT, nty = (a.Ny + RT - 1) / RT;
+  const int ntx = (a.Nx + RT - 1) / RT, nty = (a.Ny + RTY - 1) / RTY;
   hipLaunchKernelGGL(ring_sweep_kernel, dim3(ntx, nty, nchain), dim3(256), 0, st, a, do_proj ? 1 : 0);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !do_proj) return e;
@@ -188,9 +193,11 @@ hipError_t launch_ring_sweep(const RingChainArgs& a, int nchain, bool do_proj, h
   return hipGetLastError();
 }
 
+size_t ring_part_a_count(int Nx, int Ny) { return (size_t)2 * ((Ny + RTY - 1) / RTY) * Nx; }
+
 size_t ring_partial_count(int Nx, int Ny) {
-  const int ntx = (Nx + RT - 1) / RT, nty = (Ny + RT - 1) / RT;
-  return (size_t)2 * nty * Nx + (size_t)2 * ntx * Ny;
+  const int ntx = (Nx + RT - 1) / RT;
+  return ring_part_a_count(Nx, Ny) + (size_t)2 * ntx * Ny;
 }
 
 hipError_t launch_ring_factor(const RingChainArgs& a, int nchain, hipStream_t st) {

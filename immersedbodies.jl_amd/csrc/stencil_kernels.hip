@@ -28,13 +28,16 @@ namespace ibpm {
 // Dirichlet values, the operator the DST diagonalises) and rhsbc = apply_bc! of the
 // unscaled ring with fac = 0.25 dt / (Re hc^2).
 // ---------------------------------------------------------------------------------
+constexpr int NL_RJ = 4;  // rows J per thread: a sliding window over the column
+
 __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
   const GridDims g = a.g;
   const int lev = blockIdx.z;
   const int I = blockIdx.x * 64 + (threadIdx.x & 63) + 1;
-  const int J = blockIdx.y * 4 + (threadIdx.x >> 6) + 1;
-  if (I > g.Nx || J > g.Ny) return;
+  const int J0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * NL_RJ + 1;
+  if (I > g.Nx || J0 > g.Ny) return;
   const int nx = g.nx, ny = g.ny;
+  const int nrow = min(NL_RJ, g.Ny - J0 + 1);
   const double* q = a.q + (size_t)lev * g.nq;
   const double* qx = q;
   const double* qy = q + g.nu;
@@ -43,58 +46,80 @@ __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
 
 #define QX(i, j) qx[((i)-1) + (size_t)((j)-1) * (nx + 1)]
 #define QY(i, j) qy[((i)-1) + (size_t)((j)-1) * nx]
-#define GE(i, j) ext_value(G, Gc, nx, ny, (i), (j), 0.25)
-  auto Qx = [&](int i, int j) {
-    return (((QX(i, j) + QX(i, j - 1)) + base_flux_u(a.bf, lev, i, j)) + base_flux_u(a.bf, lev, i, j - 1)) / 2;
-  };
-  auto Qy = [&](int i, int j) {
-    return (((QY(i, j) + QY(i - 1, j)) + base_flux_v(a.bf, lev, i, j)) + base_flux_v(a.bf, lev, i - 1, j)) / 2;
-  };
-
-  const double g00 = GE(I, J);
-  const double gm0 = GE(I - 1, J);
-  const double gp0 = GE(I + 1, J);
-  const double g0m = GE(I, J - 1);
-  const double g0p = GE(I, J + 1);
-  const double qy1 = Qy(I + 1, J), qy2 = Qy(I + 1, J + 1), qy3 = Qy(I + 1, J + 2);
-  const double qxa = Qx(I, J + 1), qxb = Qx(I + 1, J + 1), qxc = Qx(I + 2, J + 1);
-
-  const double fqx_a = (qy2 * g00 + qy1 * g0m) / 2;   // fqx(I+1, J)
-  const double fqx_b = (qy3 * g0p + qy2 * g00) / 2;   // fqx(I+1, J+1)
-  const double fqy_a = -(qxc * gp0 + qxb * g00) / 2;  // fqy(I+1, J+1)
-  const double fqy_b = -(qxb * g00 + qxa * gm0) / 2;  // fqy(I, J+1)
-  const double nl = ((fqx_a - fqx_b) + (fqy_a - fqy_b)) * a.inv_hc2[lev];
+  // Gamma with the 0.25-scaled coarse ring outside (ge) on columns I-1..I+1, rows
+  // J0-1..J0+NL_RJ; rows past the top edge (j > ny) are never used
+  double ge[3][NL_RJ + 2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < NL_RJ + 2; ++r) {
+      const int j = J0 - 1 + r;
+      ge[c][r] = (r <= nrow + 1 && j <= ny) ? ext_value(G, Gc, nx, ny, I - 1 + c, j, 0.25) : 0.0;
+    }
+  // Qy(I+1, j), j = J0 .. J0+NL_RJ+1 (v-faces up to ny+1)
+  double qyv[NL_RJ + 2];
+#pragma unroll
+  for (int r = 0; r < NL_RJ + 2; ++r) {
+    const int j = J0 + r;
+    qyv[r] = r <= nrow + 1
+                 ? (((QY(I + 1, j) + QY(I, j)) + base_flux_v(a.bf, lev, I + 1, j)) + base_flux_v(a.bf, lev, I, j)) / 2
+                 : 0.0;
+  }
+  // Qx(I + c, J + 1) for the window's rows J (u-faces up to ny)
+  double qxr[3][NL_RJ + 1];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < NL_RJ + 1; ++r) qxr[c][r] = r <= nrow ? QX(I + c, J0 + r) : 0.0;
 #undef QX
 #undef QY
-#undef GE
 
-  const size_t idx = (size_t)(I - 1) + (size_t)(J - 1) * g.Nx + (size_t)lev * g.nG;
-  a.nl_new[idx] = nl;
+  const double fac = Gc ? a.fac_bc[lev] : 0.0;
+#pragma unroll
+  for (int r = 0; r < NL_RJ; ++r) {
+    if (r >= nrow) break;
+    const int J = J0 + r;
+    auto Qx = [&](int c) {
+      return (((qxr[c][r + 1] + qxr[c][r]) + base_flux_u(a.bf, lev, I + c, J + 1)) + base_flux_u(a.bf, lev, I + c, J)) / 2;
+    };
+    const double g00 = ge[1][r + 1];
+    const double gm0 = ge[0][r + 1];
+    const double gp0 = ge[2][r + 1];
+    const double g0m = ge[1][r];
+    const double g0p = ge[1][r + 2];
+    const double qy1 = qyv[r], qy2 = qyv[r + 1], qy3 = qyv[r + 2];
+    const double qxa = Qx(0), qxb = Qx(1), qxc = Qx(2);
 
-  // A Gamma with zero Dirichlet neighbours
-  const double* Gl = G;
-  auto Gz = [&](int i, int j) -> double {
-    return (i >= 1 && i <= g.Nx && j >= 1 && j <= g.Ny) ? Gl[(i - 1) + (size_t)(j - 1) * g.Nx] : 0.0;
-  };
-  const double gc = Gz(I, J);
-  const double Kg = 4.0 * gc - Gz(I - 1, J) - Gz(I + 1, J) - Gz(I, J - 1) - Gz(I, J + 1);
-  double v = gc - a.cA[lev] * Kg;
-  v = v + (a.dt * a.beta1) * nl;
-  v = v + (a.dt * a.beta2) * a.nl_old[idx];
-  if (Gc) {
-    const double fac = a.fac_bc[lev];
-    double rb = 0.0;
-    if (J == 1) rb += fac * ring_value(Gc, nx, ny, 2, I + 1);
-    if (J == g.Ny) rb += fac * ring_value(Gc, nx, ny, 3, I + 1);
-    if (I == 1) rb += fac * ring_value(Gc, nx, ny, 0, J + 1);
-    if (I == g.Nx) rb += fac * ring_value(Gc, nx, ny, 1, J + 1);
-    v = v + rb;
+    const double fqx_a = (qy2 * g00 + qy1 * g0m) / 2;   // fqx(I+1, J)
+    const double fqx_b = (qy3 * g0p + qy2 * g00) / 2;   // fqx(I+1, J+1)
+    const double fqy_a = -(qxc * gp0 + qxb * g00) / 2;  // fqy(I+1, J+1)
+    const double fqy_b = -(qxb * g00 + qxa * gm0) / 2;  // fqy(I, J+1)
+    const double nl = ((fqx_a - fqx_b) + (fqy_a - fqy_b)) * a.inv_hc2[lev];
+
+    const size_t idx = (size_t)(I - 1) + (size_t)(J - 1) * g.Nx + (size_t)lev * g.nG;
+    a.nl_new[idx] = nl;
+
+    // A Gamma with zero Dirichlet neighbours: the window's values inside the plane
+    const double gl = I > 1 ? gm0 : 0.0, gr = I < g.Nx ? gp0 : 0.0;
+    const double gb = J > 1 ? g0m : 0.0, gt = J < g.Ny ? g0p : 0.0;
+    const double Kg = 4.0 * g00 - gl - gr - gb - gt;
+    double v = g00 - a.cA[lev] * Kg;
+    v = v + (a.dt * a.beta1) * nl;
+    v = v + (a.dt * a.beta2) * a.nl_old[idx];
+    if (Gc) {
+      double rb = 0.0;
+      if (J == 1) rb += fac * ring_value(Gc, nx, ny, 2, I + 1);
+      if (J == g.Ny) rb += fac * ring_value(Gc, nx, ny, 3, I + 1);
+      if (I == 1) rb += fac * ring_value(Gc, nx, ny, 0, J + 1);
+      if (I == g.Nx) rb += fac * ring_value(Gc, nx, ny, 1, J + 1);
+      v = v + rb;
+    }
+    a.rhs[idx] = v;
   }
-  a.rhs[idx] = v;
 }
 
 hipError_t launch_nonlin_rhs(const NonlinArgs& a, hipStream_t st) {
-  dim3 grid((a.g.Nx + 63) / 64, (a.g.Ny + 3) / 4, a.nlevel);
+  dim3 grid((a.g.Nx + 63) / 64, (a.g.Ny + 4 * NL_RJ - 1) / (4 * NL_RJ), a.nlevel);
   hipLaunchKernelGGL(nonlin_rhs_kernel, grid, dim3(256), 0, st, a);
   return hipGetLastError();
 }
