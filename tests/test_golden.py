@@ -38,3 +38,18 @@ def test_gpu_matches_golden(name):
     assert cases.relerr(solver.read("Gamma"), FIX[f"{name}/Gamma"]) < 1e-10 * (1 if prob.bodies.static else 10)
     assert cases.relerr(solver.read("xb"), FIX[f"{name}/xb"]) < 1e-14
     solver.close()
+
+
+def test_cylinder1024_fixture_is_symmetric():
+    """The frozen 1024^2 oracle run keeps the y-symmetric cylinder's lift at round-off
+    against its drag, and its sampled indices are those the generator draws."""
+    import os
+
+    from golden.make_golden_cylinder1024 import sample_indices
+
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "oracle_cylinder1024.npz"))
+    f = ref["forces"]
+    assert np.all(np.isfinite(f)) and np.all(f[:, 0] > 0)
+    assert np.abs(f[:, 1]).max() < 1e-9 * np.abs(f[:, 0]).max()
+    n = ref["Gamma/norms"].size * 1023 * 1023
+    assert np.array_equal(ref["Gamma/idx"], sample_indices(n))

@@ -206,3 +206,27 @@ def test_staged_body_motion_matches_motion_array():
     s = ib.create_solver(cases.cylinder())
     with pytest.raises(RuntimeError, match="is static"):
         s.set_body_motion(0, 0.0, body.motion(0.0))
+
+
+def test_cylinder_1024_matches_oracle_fixture():
+    """The full-size 1024 x 1024, 5-level cylinder for 5 steps against the oracle frozen in
+    tests/golden/oracle_cylinder1024.npz: forces and F~b within 1e-9; per-level norms of
+    Gamma, psi and q within 1e-10, and the error over 4096 sampled values within 1e-10 of
+    the field's norm (what a full-field relative error of 1e-10 implies)."""
+    import os
+
+    path = os.path.join(os.path.dirname(__file__), "golden", "oracle_cylinder1024.npz")
+    ref = np.load(path)
+    prob = cases.cylinder_1024()
+    print("case cylinder_1024 fixture", flush=True)
+    solver = ib.create_solver(prob)
+    stats = solver.advance(len(ref["forces"]))
+    assert cases.relerr(solver.forces(stats), ref["forces"]) <= FORCE_TOL
+    assert cases.relerr(solver.read("Ftilde"), ref["Ftilde"]) <= FORCE_TOL
+    L = prob.fluid.grid.nlevel
+    for name in ("Gamma", "psi", "q"):
+        f = solver.read(name)
+        norms = np.linalg.norm(f.reshape(L, -1), axis=1)
+        assert np.allclose(norms, ref[f"{name}/norms"], rtol=1e-10, atol=0), name
+        err = np.linalg.norm(f[ref[f"{name}/idx"]] - ref[f"{name}/values"])
+        assert err <= 1e-10 * np.linalg.norm(ref[f"{name}/norms"]), (name, err)
