@@ -9,7 +9,8 @@ package finds the same objects (paths relative to the reference repository):
 * ``StaticGrid``, ``MovingGrid``, ``GridVelocity``           problem.jl:476-523
 * ``CNAB``, ``default_scheme``, ``timestep``                 problem.jl:566-596
 * ``Panels``, ``StaticBody``, ``MovingRigidBody``,
-  ``RigidBodyTransform``, ``Bodies``                         problem.jl:1-261
+  ``RigidBodyTransform``, ``Bodies``, ``any_fsi``,
+  ``panel_range``, ``eachbody``                              problem.jl:1-261
 * ``Problem``, ``State``                                     problem.jl:598-708
 * ``boundary_total_force``                                   problem.jl:853-856
 """
@@ -322,6 +323,23 @@ class Bodies:
             if b.moving:
                 out[6 * k:6 * k + 6] = b.motion(t).as_array()
         return out
+
+
+def any_fsi(bodies: Bodies) -> bool:
+    """Whether any body is an FSI body (problem.jl:186-191). The reference declares the
+    abstract FsiBody (problem.jl:75-92) but defines no concrete one, so this is False."""
+    return False
+
+
+def panel_range(bodies: Bodies, name) -> range:
+    """0-based index range of body ``name`` in the concatenated panels (problem.jl:103-111;
+    the reference's 1-based range shifted by one)."""
+    return bodies.panel_range(name)
+
+
+def eachbody(bodies: Bodies):
+    """Mapping of body names to bodies (problem.jl:180-184)."""
+    return dict(bodies.byname)
 
 
 def n_panels(x) -> int:

@@ -92,3 +92,32 @@ def test_state_time_index():
     assert st.index == 0 and st.t == pytest.approx(-0.01)
     st.inc_time_index()
     assert st.t == 0.0
+
+
+def test_bodies_accessors():
+    """test/problem.jl:30-40: two lines, their panel ranges (0-based here) and no FSI."""
+    line1 = ib.StaticBody(ib.Panels(np.stack([np.arange(11) * 0.1, np.zeros(11)], 1), np.full(11, 0.1)))
+    line2 = ib.StaticBody(ib.Panels(np.stack([np.zeros(21), np.arange(21) * 0.1], 1), np.full(21, 0.1)))
+    bodies = ib.Bodies(line1=line1, line2=line2)
+    assert not ib.any_fsi(bodies)
+    assert ib.panel_range(bodies, "line1") == range(0, 11)
+    assert ib.panel_range(bodies, "line2") == range(11, 32)
+    assert bodies["line1"] is line1 and ib.eachbody(bodies)["line2"] is line2
+
+
+def test_problem_with_moving_grid():
+    """test/problem.jl:52-70: a Problem on a MovingGrid with a circle and a segment
+    discretised by Panels(grid, curve)."""
+    from immersedbodies import curves
+
+    fluid = ib.Fluid(grid=ib.fluid_grid(h=0.2, xlims=(-1.0, 2.0), ylims=(-1.5, 1.5), nlevel=5),
+                     Re=10.0, freestream_vel=lambda t: (1.0, 0.0),
+                     grid_motion=ib.MovingGrid(lambda t: ib.GridVelocity(
+                         center=(0.0, 1.0), vel=(-1.0, 0.0), angle=t, angular_vel=1.0)))
+    bodies = ib.Bodies(
+        cyl=ib.StaticBody(curves.panels(fluid.grid, curves.Circle(r=0.5, center=(0.0, 0.0)))),
+        line=ib.StaticBody(curves.panels(fluid.grid, curves.LineSegment((1.0, -0.3), (1.0, 0.3)))))
+    prob = ib.Problem(fluid=fluid, bodies=bodies, scheme=ib.CNAB(dt=0.1))
+    assert ib.gridstep(prob) == 0.2
+    assert ib.timestep(prob) == 0.1
+    assert prob.fluid.grid_motion(0.5).angle == 0.5
