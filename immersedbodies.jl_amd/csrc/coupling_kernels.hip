@@ -207,9 +207,9 @@ __global__ __launch_bounds__(256) void gather_kernel(const GatherArgs a) {
     const int j = iy + bb - 3, i = ix + aa - 3;
     double qv;
     if (!comp)  // u(i,j) = psi_e(i-1, j) - psi_e(i-1, j-1)
-      qv = (ext_value(p1, p2, nx, ny, i - 1, j, 1.0) - ext_value(p1, p2, nx, ny, i - 1, j - 1, 1.0)) + base_flux_u(a.bf, 0, i, j);
+      qv = curl_u(p1, p2, nx, ny, i, j) + base_flux_u(a.bf, 0, i, j);
     else  // v(i,j) = psi_e(i-1, j-1) - psi_e(i, j-1)
-      qv = (ext_value(p1, p2, nx, ny, i - 1, j - 1, 1.0) - ext_value(p1, p2, nx, ny, i, j - 1, 1.0)) + base_flux_v(a.bf, 0, i, j);
+      qv = curl_v(p1, p2, nx, ny, i, j) + base_flux_v(a.bf, 0, i, j);
     s = qv * a.w[(size_t)p * 98 + comp * 49 + lane];
   }
 #pragma unroll

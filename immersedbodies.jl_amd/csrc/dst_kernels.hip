@@ -6,12 +6,13 @@
 // of its input with the sine matrix and writes the result transposed, so the second
 // pass again contracts a contiguous index and the plane returns to its original layout.
 //
-// Even/odd folding. With M = N+1 (even) and H = M/2, row j of S (S_jk = 2 sin(pi j k/M))
+// Even/odd folding. With M = N+1 and H = floor(M/2), row j of S (S_jk = 2 sin(pi j k/M))
 // satisfies S_{j,M-k} = (-1)^{j+1} S_{j,k}. Folding the input as
-//   P_k = x_k + x_{M-k} (k < H), P_H = x_H        -> odd rows  j = 2m+1, m = 0..H-1
-//   Q_k = x_k - x_{M-k} (k < H)                    -> even rows j = 2m,   m = 1..H-1
+//   P_k = x_k + x_{M-k} (k <= H; for even M, P_H = x_H)  -> odd rows  j = 2m+1, m < H
+//   Q_k = x_k - x_{M-k} (k < H, or k <= H for odd M)     -> even rows j = 2m+2, m < E
 // halves the contraction: odd rows use So[m][k] = 2 sin(pi (2m+1) k / M) (H x H) and
-// even rows use Se[m][k] = 2 sin(pi m k / H) ((H-1) x (H-1)).
+// even rows use Se[m][k] = 2 sin(pi 2(m+1) k / M) (E x E, E = H - 1 for even M and H for
+// odd M).
 //
 // The fold and the transpose are staged through LDS; the products run on
 //   FP64 path : v_mfma_f64_16x16x4_f64
@@ -97,6 +98,8 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
   const PassPlane& d = batch.p[z];
   const int Nk = d.Nk, Nb = d.Nb;
   const int H = (Nk + 1) / 2;
+  const bool modd = (Nk + 1) & 1;  // odd M: every k <= H has a partner, H even rows
+  const int E = modd ? H : H - 1;  // even output rows (and their contraction length)
   const int b0 = bt * TB;
   const int m0 = mt * TM;
   if (b0 >= Nb || m0 >= H) return;
@@ -131,7 +134,7 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
       double va = 0.0, vm = 0.0;
       if (k1 <= H && b < Nb) {
         va = load_in<InT>(d, k1 - 1, b);
-        if (k1 < H) vm = load_in<InT>(d, Nk - k1, b);
+        if (k1 < H || modd) vm = load_in<InT>(d, Nk - k1, b);
       }
       ra[st][r] = va;
       rm[st][r] = vm;
@@ -142,18 +145,19 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
       const int k1 = kc + e / TM + 1;
       const int m = m0 + e % TM;
       rso[st][r] = (k1 <= H && m < H) ? SoT[(size_t)(k1 - 1) * H + m] : 0.0;
-      rse[st][r] = (k1 <= H - 1 && m < H - 1) ? SeT[(size_t)(k1 - 1) * (H - 1) + m] : 0.0;
+      rse[st][r] = (k1 <= E && m < E) ? SeT[(size_t)(k1 - 1) * E + m] : 0.0;
     }
   };
   const int col = lane & 15;
   const int kq = lane >> 4;
-  // fold into LDS: P = x_k + x_{M-k}, Q = x_k - x_{M-k}; k = H has no partner (Q = 0)
+  // fold into LDS: P = x_k + x_{M-k}, Q = x_k - x_{M-k}; for even M, k = H has no
+  // partner (Q = 0)
   auto stage = [&](const int st, int kc) {
 #pragma unroll
     for (int r = 0; r < RA; ++r) {
       const int e = tid + 256 * r;
       const int kk = e % KC;
-      const bool mid = kc + kk + 1 == H;
+      const bool mid = !modd && kc + kk + 1 == H;
       Ps[kk][e / KC] = ra[st][r] + rm[st][r];
       Qs[kk][e / KC] = mid ? 0.0 : ra[st][r] - rm[st][r];
     }
@@ -201,7 +205,7 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
 
   // epilogue: the f64 16x16 accumulator holds D[row = lane/16 + 4r][col = lane%16]
   const int rq = lane >> 4;
-  const int mlim = parity == 0 ? H : H - 1;
+  const int mlim = parity == 0 ? H : E;
 #pragma unroll
   for (int bs = 0; bs < BS; ++bs)
 #pragma unroll
@@ -431,6 +435,8 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
   const PassPlane& d = batch.p[z];
   const int Nk = d.Nk, Nb = d.Nb;
   const int H = (Nk + 1) / 2;
+  const bool modd = (Nk + 1) & 1;  // odd M: every k <= H has a partner, H even rows
+  const int E = modd ? H : H - 1;
   const int Hp = (H + 31) & ~31;
   const int b0 = bt * TB;
   const int m0 = mt * TM;
@@ -479,7 +485,7 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
         double va = 0.0, vm = 0.0;
         if (k1 <= H && b < Nb) {
           va = load_in<InT>(d, k1 - 1, b);
-          if (k1 < H) vm = load_in<InT>(d, Nk - k1, b);
+          if (k1 < H || modd) vm = load_in<InT>(d, Nk - k1, b);
         }
         ra[r][j] = va;
         rm[r][j] = vm;
@@ -505,7 +511,7 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
       __bf16 ph[2], pl[2], qh[2], ql[2];
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        const bool mid = kc + 2 * kp + j + 1 == H;  // k = H has no mirror partner
+        const bool mid = !modd && kc + 2 * kp + j + 1 == H;  // even M: k = H has no partner
         split_bf16((float)(ra[r][j] + rm[r][j]), ph[j], pl[j]);
         split_bf16(mid ? 0.f : (float)(ra[r][j] - rm[r][j]), qh[j], ql[j]);
       }
@@ -546,7 +552,7 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
 
   // the fp32 16x16 accumulator of the bf16 MFMA holds D[row = 4*(lane/16) + r][col = lane%16]
   const int rq = lane >> 4;
-  const int mlim = parity == 0 ? H : H - 1;
+  const int mlim = parity == 0 ? H : E;
 #pragma unroll
   for (int bs = 0; bs < BS; ++bs)
 #pragma unroll
@@ -692,9 +698,20 @@ int tuned_variant(int mixed, int nplanes, int maxNb, int maxH) {
 }
 }  // namespace
 
+// The double fold needs M = N + 1 to be a multiple of 4 on every plane of the launch;
+// elsewhere a double-fold variant runs as the single-fold variant of the same tiles.
+static bool quarter_ok(const PassBatch& b, int nplanes) {
+  for (int i = 0; i < nplanes; ++i)
+    if ((b.p[i].Nk + 1) % 4) return false;
+  return true;
+}
+
+int dst_effective_variant(int v, int N) { return v >= 8 && (N + 1) % 4 ? v - 8 : v; }
+
 hipError_t launch_dst_pass_f64(const PassBatch& batch, int nplanes, int maxNb, int maxH, int epi,
                                hipStream_t st, int variant) {
-  const int v = variant >= 0 && variant < kVariants ? variant : tuned_variant(0, nplanes, maxNb, maxH);
+  int v = variant >= 0 && variant < kVariants ? variant : tuned_variant(0, nplanes, maxNb, maxH);
+  if (v >= 8 && !quarter_ok(batch, nplanes)) v -= 8;
   launch_f64_variant(v, batch, nplanes, maxNb, maxH, epi, st);
   return hipGetLastError();
 }
@@ -730,7 +747,9 @@ hipError_t tune_dst_pass(const PassBatch& batch, int nplanes, int maxNb, int max
   // three rounds over the variants, best mean of 4 launches per round (interleaving the
   // rounds spreads clock and power-state drift over all variants)
   constexpr int reps = 4, rounds = 3;
-  const int v0 = mixed ? 0 : 8, v1 = mixed ? kMixedVariants : kVariants;
+  // FP64: the double-fold variants where the length allows them, else the single fold
+  const bool q = !mixed && quarter_ok(batch, nplanes);
+  const int v0 = mixed || !q ? 0 : 8, v1 = mixed ? kMixedVariants : (q ? kVariants : 8);
   float t[kVariants];
   for (int v = 0; v < kVariants; ++v) t[v] = 1e30f;
   for (int round = 0; round < rounds && e == hipSuccess; ++round) {
@@ -847,16 +866,17 @@ size_t sine_table_odd_count(int N) {
   return H * H;
 }
 size_t sine_table_even_count(int N) {
-  const size_t H = (size_t)(N + 1) / 2;
-  return (H - 1) * (H - 1);
+  const size_t H = (size_t)(N + 1) / 2, E = ((N + 1) & 1) ? H : H - 1;
+  return E * E;
 }
 
 void build_sine_tables_f64(int N, double* So, double* Se) {
   const long long M = N + 1, H = M / 2;
   for (long long k = 1; k <= H; ++k)
     for (long long m = 0; m < H; ++m) So[(k - 1) * H + m] = sine_entry((2 * m + 1) * k, M);
-  for (long long k = 1; k <= H - 1; ++k)
-    for (long long m = 0; m < H - 1; ++m) Se[(k - 1) * (H - 1) + m] = sine_entry(2 * (m + 1) * k, M);
+  const long long E = (M & 1) ? H : H - 1;
+  for (long long k = 1; k <= E; ++k)
+    for (long long m = 0; m < E; ++m) Se[(k - 1) * E + m] = sine_entry(2 * (m + 1) * k, M);
 }
 
 size_t sine_table_quarter_count(int N) {
@@ -919,8 +939,9 @@ void build_sine_tables_bf16(int N, uint16_t* So_hi, uint16_t* So_lo, uint16_t* S
       So_hi[m * Hp + (k - 1)] = hi;
       So_lo[m * Hp + (k - 1)] = lo;
     }
-  for (long long m = 0; m < H - 1; ++m)
-    for (long long k = 1; k <= H - 1; ++k) {
+  const long long E = (M & 1) ? H : H - 1;
+  for (long long m = 0; m < E; ++m)
+    for (long long k = 1; k <= E; ++k) {
       const double v = sine_entry(2 * (m + 1) * k, M);
       uint16_t hi = bf16_bits_rne((float)v);
       uint16_t lo = bf16_bits_rne((float)(v - (double)bf16_to_float(hi)));

@@ -903,6 +903,7 @@ int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM
     v = dst_tuned_variant(s->precision == IBPM_MIXED, nplanes, Nb, (Nk + 1) / 2);
     if (v < 0) v = dst_default_variant(s->precision == IBPM_MIXED);
   }
+  v = dst_effective_variant(v, xdir ? s->g.Nx : s->g.Ny);
   int f = 2;
   dst_variant_tiles(v, s->precision == IBPM_MIXED, TB, TM, PF, &f);
   if (fold) *fold = f;
@@ -915,8 +916,7 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   ibpm_options opt;
   ibpm_default_options(&opt);
   if (o) opt = *o;
-  if (p->nx % 4 || p->ny % 4 || p->nx < 8 || p->ny < 8)
-    return fail(IBPM_E_ARG, "nx and ny must be multiples of 4 and at least 8");
+  if (p->nx < 8 || p->ny < 8) return fail(IBPM_E_ARG, "nx and ny must be at least 8");
   if (p->nlevel < 1 || p->nlevel > 16) return fail(IBPM_E_ARG, "nlevel must be in 1..16");
   if (p->npanel < 0 || p->npanel > 8191) return fail(IBPM_E_ARG, "npanel must be in 0..8191");
   if (p->npanel > 0 && (!p->xb || p->nbody < 1 || !p->body_start || !p->body_count || !p->body_moving))
@@ -1521,12 +1521,15 @@ int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, doubl
   } else if (kernel == 0) {
     // matrix-core flops of the contraction the launch runs: the single fold multiplies
     // H x H and (H-1) x (H-1) tables, the double fold three Q x Q and one (Q-1) x (Q-1)
-    const double Nk = s->g.Nx, Nb = s->g.Ny, H = (Nk + 1) / 2, Q = (Nk + 1) / 4;
-    int v = s->dst_tiles >= 0 ? s->dst_tiles : dst_tuned_variant(mixed, L, s->g.Ny, s->g.Nx / 2 + 1);
+    const double Nk = s->g.Nx, Nb = s->g.Ny;
+    const int Hi = (s->g.Nx + 1) / 2;
+    const double H = Hi, Q = (s->g.Nx + 1) / 4, E = ((s->g.Nx + 1) & 1) ? Hi : Hi - 1;
+    int v = s->dst_tiles >= 0 ? s->dst_tiles : dst_tuned_variant(mixed, L, s->g.Ny, (s->g.Nx + 1) / 2);
+    if (v < 0) v = dst_default_variant(mixed);
     int TB = 0, TM = 0, PF = 0, fold = 2;
-    dst_variant_tiles(v < 0 ? dst_default_variant(mixed) : v, mixed, &TB, &TM, &PF, &fold);
+    dst_variant_tiles(dst_effective_variant(v, s->g.Nx), mixed, &TB, &TM, &PF, &fold);
     const double per_product = fold == 4 ? 2.0 * Nb * (3.0 * Q * Q + (Q - 1) * (Q - 1))
-                                         : 2.0 * Nb * (H * H + (H - 1) * (H - 1));
+                                         : 2.0 * Nb * (H * H + E * E);
     *flops = L * per_product * (mixed ? 3.0 : 1.0);
     *bytes = L * Nk * Nb * (8.0 + (mixed ? 4.0 : 8.0));  // fp64 plane in, T plane out
   } else {

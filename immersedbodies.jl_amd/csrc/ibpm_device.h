@@ -29,8 +29,9 @@ __host__ __device__ __forceinline__ double ring_value(const double* __restrict__
 // Extended-plane value of a level: interior vertices (1..nx-1, 1..ny-1) come from the
 // plane p, boundary vertices (i = 0, nx or j = 0, ny) from the get_bc! ring of the
 // coarser plane rc (zero when rc is null), times scale. A corner vertex belongs to two
-// sides of the boundary vector; both entries interpolate the same coarse vertex, so
-// the side chosen here does not change the value.
+// sides of the boundary vector; for even nx and ny both entries interpolate the same
+// coarse vertex. The callers that read corners (the curl's boundary rows and columns)
+// choose the side the reference reads instead (curl_all_kernel).
 __host__ __device__ __forceinline__ double ext_value(const double* __restrict__ p,
                                             const double* __restrict__ rc, int nx, int ny,
                                             int i, int j, double scale) {
@@ -42,6 +43,41 @@ __host__ __device__ __forceinline__ double ext_value(const double* __restrict__ 
   else if (i == 0) v = ring_value(rc, nx, ny, 0, j + 1);
   else v = ring_value(rc, nx, ny, 1, j + 1);
   return scale * v;
+}
+
+// Entry k (1-based) of side `side` (0 left, 1 right, 2 bottom, 3 top) of the get_bc!
+// boundary vector of the coarser plane rc: zero when rc is null, and zero at k = n + 1 of
+// a side of odd length n, which get_bc! never writes (fluid.jl:278-306) and vort2flux
+// zeroes before use (fluid.jl:188).
+__host__ __device__ __forceinline__ double bc_entry(const double* __restrict__ rc, int nx, int ny,
+                                                    int side, int k) {
+  if (!rc) return 0.0;
+  if (side < 2 ? (k == ny + 1 && (ny & 1)) : (k == nx + 1 && (nx & 1))) return 0.0;
+  return ring_value(rc, nx, ny, side, k);
+}
+
+// curl! (fluid.jl:230-276) on u-face (i, j) and v-face (i, j): the u-faces of the left and
+// right boundary columns take both values from the left / right side of the boundary
+// vector, the v-faces of the bottom and top rows from the bottom / top side (corners
+// included); elsewhere the extended plane.
+__host__ __device__ __forceinline__ double curl_u(const double* __restrict__ p,
+                                                  const double* __restrict__ rc, int nx, int ny,
+                                                  int i, int j) {
+  if (i == 1 || i == nx + 1) {
+    const int side = i == 1 ? 0 : 1;
+    return bc_entry(rc, nx, ny, side, j + 1) - bc_entry(rc, nx, ny, side, j);
+  }
+  return ext_value(p, rc, nx, ny, i - 1, j, 1.0) - ext_value(p, rc, nx, ny, i - 1, j - 1, 1.0);
+}
+
+__host__ __device__ __forceinline__ double curl_v(const double* __restrict__ p,
+                                                  const double* __restrict__ rc, int nx, int ny,
+                                                  int i, int j) {
+  if (j == 1 || j == ny + 1) {
+    const int side = j == 1 ? 2 : 3;
+    return bc_entry(rc, nx, ny, side, i) - bc_entry(rc, nx, ny, side, i + 1);
+  }
+  return ext_value(p, rc, nx, ny, i - 1, j - 1, 1.0) - ext_value(p, rc, nx, ny, i, j - 1, 1.0);
 }
 
 // Base flux on u-face (i, j) and v-face (i, j) of level l (0-based level, 1-based faces),

@@ -96,6 +96,9 @@ hipError_t launch_mfma_probe(bool bf16, int iters, int blocks, double* out, hipS
 int dst_tuned_variant(int mixed, int nplanes, int maxNb, int maxH);
 // Variant of the shapes never tuned: FP64 double fold (32 x 64), mixed 64 x 64.
 int dst_default_variant(bool mixed);
+// The variant a launch along a length-N transform runs: a double-fold variant becomes its
+// single-fold counterpart unless N + 1 is a multiple of 4.
+int dst_effective_variant(int v, int N);
 
 // ---- multi-level ring chain (ring_kernels.hip) ----
 constexpr int kMaxChains = 16;

@@ -125,31 +125,49 @@ hipError_t launch_nonlin_rhs(const NonlinArgs& a, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------------
-// coarsify! (fluid.jl:208-228): coarse vertex (Ic, Jc) of the centre region
-// [nx/4+1, 3nx/4-1] x [ny/4+1, 3ny/4-1] takes the 1-1/2-1/4 full weighting of fine
-// vertex (2(Ic-nx/4), 2(Jc-ny/4)). With `outer`, the rest of the coarse plane is
+// coarsify! (fluid.jl:208-228): coarse vertex ic0 + a, jc0 + c of the centre region takes
+// the 1-1/2-1/4 full weighting of fine vertex (2 + 2a, 2 + 2c), with the reference's
+// ranges: ic0 = nx÷2 - nx÷4 + 1 and a < min(nx÷2 - 1, 2 (nx÷4) - 1) (its zip of the
+// fine and coarse index ranges stops at the shorter), likewise in y. For nx divisible by 4
+// that is the region [nx/4+1, 3nx/4-1]. With `outer`, the rest of the coarse plane is
 // copied from it (project_circ's state.Gamma .= Gamma_s fused into the final pass).
 // ---------------------------------------------------------------------------------
+struct CoarsifyRange {
+  int ic0, ni, jc0, nj;
+};
+__host__ __device__ inline CoarsifyRange coarsify_range(int nx, int ny) {
+  CoarsifyRange r;
+  r.ic0 = nx / 2 - nx / 4 + 1;
+  r.jc0 = ny / 2 - ny / 4 + 1;
+  const int fi = nx / 2 - 1, ci = 2 * (nx / 4) - 1, fj = ny / 2 - 1, cj = 2 * (ny / 4) - 1;
+  r.ni = fi < ci ? fi : ci;
+  r.nj = fj < cj ? fj : cj;
+  if (r.ni < 0) r.ni = 0;
+  if (r.nj < 0) r.nj = 0;
+  return r;
+}
+
 __global__ __launch_bounds__(256) void coarsify_kernel(GridDims g, double* __restrict__ coarse,
                                                        const double* __restrict__ fine,
                                                        const double* __restrict__ outer) {
-  const int nx = g.nx, ny = g.ny;
+  const CoarsifyRange cr = coarsify_range(g.nx, g.ny);
   int Ic, Jc;
   if (outer) {
     Ic = blockIdx.x * 64 + (threadIdx.x & 63) + 1;
     Jc = blockIdx.y * 4 + (threadIdx.x >> 6) + 1;
   } else {
-    Ic = nx / 4 + 1 + blockIdx.x * 64 + (threadIdx.x & 63);
-    Jc = ny / 4 + 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
+    Ic = cr.ic0 + blockIdx.x * 64 + (threadIdx.x & 63);
+    Jc = cr.jc0 + blockIdx.y * 4 + (threadIdx.x >> 6);
   }
   if (Ic > g.Nx || Jc > g.Ny) return;
   const size_t idx = (size_t)(Ic - 1) + (size_t)(Jc - 1) * g.Nx;
-  const bool centre = Ic >= nx / 4 + 1 && Ic <= 3 * nx / 4 - 1 && Jc >= ny / 4 + 1 && Jc <= 3 * ny / 4 - 1;
+  const int a = Ic - cr.ic0, c = Jc - cr.jc0;
+  const bool centre = a >= 0 && a < cr.ni && c >= 0 && c < cr.nj;
   if (!centre) {
     if (outer) coarse[idx] = outer[idx];
     return;
   }
-  const int i = 2 * (Ic - nx / 4), j = 2 * (Jc - ny / 4);
+  const int i = 2 + 2 * a, j = 2 + 2 * c;
 #define GF(ii, jj) fine[((ii)-1) + (size_t)((jj)-1) * g.Nx]
   coarse[idx] = GF(i, j) + 0.5 * (GF(i + 1, j) + GF(i, j + 1) + GF(i - 1, j) + GF(i, j - 1)) +
                 0.25 * (GF(i + 1, j + 1) + GF(i + 1, j - 1) + GF(i - 1, j - 1) + GF(i - 1, j + 1));
@@ -162,8 +180,9 @@ hipError_t launch_coarsify(GridDims g, double* coarse, const double* fine, const
   if (outer) {
     grid = dim3((g.Nx + 63) / 64, (g.Ny + 3) / 4, 1);
   } else {
-    const int ni = g.nx / 2 - 1, nj = g.ny / 2 - 1;
-    grid = dim3((ni + 63) / 64, (nj + 3) / 4, 1);
+    const CoarsifyRange cr = coarsify_range(g.nx, g.ny);
+    if (cr.ni == 0 || cr.nj == 0) return hipSuccess;
+    grid = dim3((cr.ni + 63) / 64, (cr.nj + 3) / 4, 1);
   }
   hipLaunchKernelGGL(coarsify_kernel, grid, dim3(256), 0, st, g, coarse, fine, outer);
   return hipGetLastError();
@@ -173,7 +192,8 @@ hipError_t launch_coarsify(GridDims g, double* coarse, const double* fine, const
 // curl! with boundary streamfunction (fluid.jl:230-276) for all levels at once:
 //   u(i,j) = psi_e(i-1, j) - psi_e(i-1, j-1),  v(i,j) = psi_e(i-1, j-1) - psi_e(i, j-1)
 // psi_e = psi on interior vertices, get_bc! ring of the next coarser level on the
-// boundary (zero on the coarsest level, fluid.jl:187-190).
+// boundary (zero on the coarsest level, fluid.jl:187-190), with the reference's choice of
+// side at the corners (curl_u / curl_v).
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void curl_all_kernel(GridDims g, int nlevel, double* __restrict__ q,
                                                        const double* __restrict__ psi) {
@@ -186,11 +206,11 @@ __global__ __launch_bounds__(256) void curl_all_kernel(GridDims g, int nlevel, d
   double v;
   if (e < (size_t)g.nu) {
     const int i = (int)(e % (nx + 1)) + 1, j = (int)(e / (nx + 1)) + 1;
-    v = ext_value(p, pc, nx, ny, i - 1, j, 1.0) - ext_value(p, pc, nx, ny, i - 1, j - 1, 1.0);
+    v = curl_u(p, pc, nx, ny, i, j);
   } else {
     const size_t f = e - g.nu;
     const int i = (int)(f % nx) + 1, j = (int)(f / nx) + 1;
-    v = ext_value(p, pc, nx, ny, i - 1, j - 1, 1.0) - ext_value(p, pc, nx, ny, i, j - 1, 1.0);
+    v = curl_v(p, pc, nx, ny, i, j);
   }
   q[(size_t)lev * g.nq + e] = v;
 }

@@ -66,7 +66,7 @@ enum ibpm_field {
 
 /* Problem description (Problem + State(prob; t0) of problem.jl). */
 typedef struct ibpm_problem {
-  int nx, ny, nlevel;       /* cells per level (multiples of 4), levels      */
+  int nx, ny, nlevel;       /* cells per level (>= 8), levels                */
   double x0, y0, h;         /* lower-left corner and step of level 1          */
   double Re, dt;            /* Reynolds number, CNAB time step                */
   double t0;                /* time at index 1                                */

@@ -314,7 +314,8 @@ void ibo_curl(const ibo_grid* g, double* q, const double* psi, const double* bc)
   for (int i = 1; i <= nx; ++i) QYA(qy, i, ny + 1) = BCA(bc, T + i) - BCA(bc, (T + 1) + i);
 }
 
-/* coarsify!, fluid.jl:208-228 (requires nx, ny divisible by 4) */
+/* coarsify!, fluid.jl:208-228 (the ranges use Julia's integer division; their zip stops at
+ * the shorter one, which matters when nx or ny is not divisible by 4) */
 void ibo_coarsify(const ibo_grid* g, double* Gc, const double* G) {
   int nx = g->nx, ny = g->ny;
   /* is = nx/2 + (-nx/2+2 : 2 : nx/2-2), ics = nx/2 + (-nx/4+1 : nx/4-1) */
@@ -745,8 +746,8 @@ static int reg_update_range(ibo_solver* s, int start, int count, char* err, size
 
 ibo_solver* ibo_create(const ibo_config* c, char* err, size_t errlen) {
   const ibo_grid* g = &c->grid;
-  if (g->nx % 4 || g->ny % 4 || g->nx < 8 || g->ny < 8 || g->nlevel < 1) {
-    seterr(err, errlen, "nx and ny must be multiples of 4 (>= 8) and nlevel >= 1");
+  if (g->nx < 8 || g->ny < 8 || g->nlevel < 1) {
+    seterr(err, errlen, "nx and ny must be at least 8 and nlevel >= 1");
     return NULL;
   }
   ibo_solver* s = (ibo_solver*)calloc(1, sizeof(ibo_solver));

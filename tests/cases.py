@@ -149,6 +149,43 @@ def plunging_ellipses(nx=2048, ny=1024, nlevel=6, h=1.0 / 128, x0=(-4.0, -4.0), 
                       scheme=ib.CNAB(dt=dt or ib.default_scheme(grid, Umax=1.0, cfl=0.2).dt))
 
 
+def reference_solver_problem(grid_motion="static grid", bodies="single cyl"):
+    """The problems of the reference's solver tests (test/solver.jl:1-63): a 3-level grid
+    on [-1,2] x [-1.5,1.5] with the Re 100 default step (h = 0.02: 150 x 150 cells, not
+    divisible by 4), a static or moving grid, and one cylinder, two segments, or a moving
+    cylinder; dt from CFL 0.2 at Umax 2."""
+    if grid_motion == "static grid":
+        gm = ib.StaticGrid()
+    else:
+        gm = ib.MovingGrid(lambda t: ib.GridVelocity(center=(-1.0, 0.0), vel=(0.0, 0.2),
+                                                     angle=0.1 * math.sin(t),
+                                                     angular_vel=0.1 * math.cos(t)))
+    fluid = ib.Fluid(grid=ib.fluid_grid(xlims=(-1.0, 2.0), ylims=(-1.5, 1.5), nlevel=3),
+                     Re=100.0, freestream_vel=lambda t: (1.0, 0.0), grid_motion=gm)
+    circle = curves.panels(fluid.grid, curves.Circle(r=0.5, center=(0.0, 0.0)))
+    if bodies == "single cyl":
+        b = ib.Bodies(cyl=ib.StaticBody(circle))
+    elif bodies == "double line":
+        b = ib.Bodies(
+            line1=ib.StaticBody(curves.panels(fluid.grid, curves.LineSegment((-0.2, 0.4), (0.2, 0.1)))),
+            line2=ib.StaticBody(curves.panels(fluid.grid, curves.LineSegment((-0.2, -0.1), (0.2, -0.4)))))
+    else:
+        motion = lambda t: ib.RigidBodyTransform(pos=(0.3 * math.cos(t), 0.1 * math.sin(t)),
+                                                 vel=(-0.3 * math.sin(t), 0.1 * math.cos(t)),
+                                                 angle=0.1 * t, angular_vel=0.1)
+        b = ib.Bodies(cyl=ib.MovingRigidBody(motion, circle))
+    return ib.Problem(fluid=fluid, bodies=b,
+                      scheme=ib.default_scheme(fluid.grid, Umax=2.0, cfl=0.2))
+
+
+def reference_outside_region_problem():
+    """test/solver.jl:66-79: a cylinder too close to the edge of a 15 x 20 grid."""
+    fluid = ib.Fluid(grid=ib.fluid_grid(h=0.1, xlims=(-1.0, 0.5), ylims=(-1.0, 1.0), nlevel=3),
+                     Re=20.0, freestream_vel=lambda t: (1.0, 0.0))
+    b = ib.Bodies(cyl=ib.StaticBody(curves.panels(fluid.grid, curves.Circle(r=0.5, center=(0.0, 0.0)))))
+    return ib.Problem(fluid=fluid, bodies=b, scheme=ib.default_scheme(fluid.grid, Umax=2.0, cfl=0.2))
+
+
 def naca_airfoil(spec="0012", aoa_deg=20.0, Re=400.0, nlevel=3):
     """The reference's NACA example (docs/src/examples/naca.md): the section, its
     mid-chord at the origin, turned to ``aoa_deg`` incidence, on [-1,2] x [-0.5,0.5] with

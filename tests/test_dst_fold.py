@@ -58,3 +58,34 @@ def test_double_fold_catches_a_sign_slip():
     bad[M - 2 - 2 * m] = got[2 * m]  # A + B where A - B belongs
     assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
     assert np.abs(bad - ref).max() > 1e-3 * np.abs(ref).max()
+
+
+def single_fold_dst(x):
+    """The single fold for any M = N + 1: H = floor(M/2) pairs (k, M - k); for even M the
+    middle element x_H has no partner. Odd rows from P_k, even rows from Q_k."""
+    N = x.shape[-1]
+    M = N + 1
+    H = M // 2
+    E = H if M % 2 else H - 1
+    xx = np.concatenate([np.zeros(x.shape[:-1] + (1,)), x, np.zeros(x.shape[:-1] + (1,))], -1)
+    k = np.arange(1, H + 1)
+    partner = np.where((M % 2 == 1) | (k < H), xx[..., M - k], 0.0)
+    P = xx[..., k] + partner
+    Q = (xx[..., k] - partner)[..., :E]
+    m = np.arange(H)
+    So = 2 * np.sin(np.pi * np.outer(k, 2 * m + 1) / M)
+    Se = 2 * np.sin(np.pi * np.outer(k[:E], 2 * (m[:E] + 1)) / M)
+    out = np.zeros(x.shape[:-1] + (N,))
+    out[..., 2 * m] = P @ So            # rows 2m+1
+    out[..., 2 * m[:E] + 1] = Q @ Se    # rows 2m+2
+    return out
+
+
+@pytest.mark.parametrize("M", [8, 9, 10, 15, 16, 61, 150, 151])
+def test_single_fold_equals_dst1_any_length(M):
+    """Odd lengths and lengths not divisible by 4 take the single fold (the double fold
+    needs M divisible by 4)."""
+    rng = np.random.default_rng(M)
+    x = rng.standard_normal((2, M - 1))
+    ref = scipy.fft.dst(x, type=1, axis=-1)
+    assert np.abs(single_fold_dst(x) - ref).max() <= 1e-12 * np.abs(ref).max()

@@ -44,9 +44,24 @@ def test_poisson_and_helmholtz(precision, refine, tol):
     s.close()
 
 
+@pytest.mark.parametrize("precision,refine,tol", [("fp64", 0, 1e-12), ("mixed", 2, 1e-9)])
+@pytest.mark.parametrize("nx,ny", [(42, 38), (41, 37), (45, 52)])
+def test_poisson_and_helmholtz_any_length(precision, refine, tol, nx, ny):
+    """Lengths not divisible by 4 (single fold) and odd lengths (single fold without a
+    middle element)."""
+    prob, og, s = _grid_and_solver(precision, refine, nx, ny, 2)
+    print("poisson/helmholtz", nx, ny, precision, flush=True)
+    b = np.random.default_rng(nx).standard_normal(og.nG)
+    assert cases.relerr(s.apply("poisson", b), O.lap_inv(og, O.diag_poisson(og), b)) < tol
+    ref = O.lap_inv(og, O.diag_A(og, prob.fluid.Re, prob.scheme.dt, 2, True), b)
+    assert cases.relerr(s.apply("helmholtz", b, level=2), ref) < tol
+    s.close()
+
+
 @pytest.mark.parametrize("precision,refine,tol", [("fp64", 0, 1e-11), ("mixed", 2, 1e-9),
                                                   ("mixed", 0, 1e-4)])
-@pytest.mark.parametrize("shape", [(64, 48, 3), (32, 32, 1), (40, 56, 4)])
+@pytest.mark.parametrize("shape", [(64, 48, 3), (32, 32, 1), (40, 56, 4), (42, 38, 3), (41, 37, 3),
+                                   (45, 52, 2)])
 def test_vort2flux(precision, refine, tol, shape):
     nx, ny, L = shape
     prob, og, s = _grid_and_solver(precision, refine, nx, ny, L)
