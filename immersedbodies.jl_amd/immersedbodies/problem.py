@@ -130,10 +130,15 @@ def subdomain(grid: MultiDomainGrid, level: int) -> CartesianGrid:
     return CartesianGrid(x0, b.n, h)
 
 
-def coords(grid, pts: str = "vertices"):
-    """Coordinates of grid points: ``vertices``, ``u``, ``v`` or ``gamma`` (problem.jl:311-338)."""
+def coords(grid, pts="vertices", levels=None):
+    """Coordinates of grid points: ``vertices``, ``u``, ``v`` or ``gamma``, or those of an
+    accessor (x_velocity, y_velocity, vorticity) (problem.jl:311-338, 432-441, 776-787).
+    For a MultiDomainGrid, one (x, y) pair per level in ``levels`` (1-based, all by
+    default)."""
+    if callable(pts):
+        pts = GRID_POINTS[pts]
     if isinstance(grid, MultiDomainGrid):
-        return [coords(subdomain(grid, l), pts) for l in range(1, grid.nlevel + 1)]
+        return [coords(subdomain(grid, l), pts) for l in _level_list(grid, levels)]
     xs = grid.x0[0] + grid.h * np.arange(grid.n[0] + 1)
     ys = grid.x0[1] + grid.h * np.arange(grid.n[1] + 1)
     mid = lambda r: r[:-1] + grid.h / 2
@@ -146,6 +151,34 @@ def coords(grid, pts: str = "vertices"):
     if pts == "gamma":
         return xs[1:-1], ys[1:-1]
     raise ValueError(pts)
+
+
+def extents(grid: CartesianGrid):
+    """((xmin, xmax), (ymin, ymax)) of a grid (problem.jl:290-295)."""
+    return tuple((grid.x0[i], grid.x0[i] + grid.n[i] * grid.h) for i in range(2))
+
+
+def gridsize(grid, pts: str = "vertices", levels=None):
+    """gridsize(grid, pts, levels) as the reference defines it (problem.jl:304-340,
+    443-448): n for vertices, (n1, n2 - 1) for u, (n1 - 1, n2) for v and n - 2 for gamma,
+    plus the number of levels for a MultiDomainGrid. These are one less in each direction
+    than the arrays ``coords`` describes (n + 1 vertices, the flux and circulation arrays),
+    exactly as in the reference; the accessors below size their arrays from ``coords``."""
+    if isinstance(grid, MultiDomainGrid):
+        lev = _level_list(grid, levels)
+        return gridsize(grid.base, pts) + (len(lev),)
+    n1, n2 = grid.n
+    return {"vertices": (n1, n2), "u": (n1, n2 - 1), "v": (n1 - 1, n2),
+            "gamma": (n1 - 2, n2 - 2)}[pts]
+
+
+def _level_list(grid, levels):
+    """1-based levels: None (all), an int, or an iterable."""
+    if levels is None:
+        return list(range(1, grid.nlevel + 1))
+    if isinstance(levels, (int, np.integer)):
+        return [int(levels)]
+    return [int(l) for l in levels]
 
 
 def default_gridstep(Re: float) -> float:
@@ -391,7 +424,135 @@ class State:
         self.set_time_index(self.index + 1)
 
 
-def boundary_total_force(state: State, inds=None):
-    """Total [x, y] force on the panels (problem.jl:853-856)."""
-    fb = state.fb if inds is None else state.fb[inds]
-    return fb.sum(axis=0)
+# ---- state accessors (problem.jl:710-866) ----
+# Arrays keep the reference's index order: u[i, j, level] (i along x), level 0-based in
+# the last axis; ``levels`` (1-based) selects which level slices are filled.
+
+def _flux_level(state, lev, axis):
+    g = state.prob.fluid.grid
+    nx, ny = g.base.n
+    nu = (nx + 1) * ny
+    q = state.q[lev - 1] + state.q0[lev - 1]
+    if axis == 0:
+        return q[:nu].reshape((nx + 1, ny), order="F")
+    return q[nu:].reshape((nx, ny + 1), order="F")
+
+
+def _velocity_(out, state, axis, levels=None):
+    g = state.prob.fluid.grid
+    for lev in _level_list(g, levels):
+        out[:, :, lev - 1] = _flux_level(state, lev, axis) / gridstep(g, lev)
+    return out
+
+
+def x_velocity_(out, state, levels=None):
+    """x_velocity!(u, state, [levels]): (q + q0) / h_level on the u-faces (problem.jl:716-748)."""
+    return _velocity_(out, state, 0, levels)
+
+
+def y_velocity_(out, state, levels=None):
+    """y_velocity!(v, state, [levels]) on the v-faces (problem.jl:726-748)."""
+    return _velocity_(out, state, 1, levels)
+
+
+def x_velocity(state, levels=None):
+    """x velocity, (nx + 1) x ny x nlevel (problem.jl:710-748)."""
+    g = state.prob.fluid.grid
+    nx, ny = g.base.n
+    return x_velocity_(np.zeros((nx + 1, ny, g.nlevel), order="F"), state, levels)
+
+
+def y_velocity(state, levels=None):
+    """y velocity, nx x (ny + 1) x nlevel (problem.jl:720-748)."""
+    g = state.prob.fluid.grid
+    nx, ny = g.base.n
+    return y_velocity_(np.zeros((nx, ny + 1, g.nlevel), order="F"), state, levels)
+
+
+def vorticity_(out, state, levels=None):
+    """vorticity!(w, state, [levels]): Gamma / h_level^2 (problem.jl:761-771)."""
+    g = state.prob.fluid.grid
+    nx, ny = g.base.n
+    for lev in _level_list(g, levels):
+        out[:, :, lev - 1] = state.Gamma[lev - 1].reshape((nx - 1, ny - 1), order="F") / gridstep(g, lev) ** 2
+    return out
+
+
+def vorticity(state, levels=None):
+    """Vorticity, (nx - 1) x (ny - 1) x nlevel (problem.jl:750-759)."""
+    g = state.prob.fluid.grid
+    nx, ny = g.base.n
+    return vorticity_(np.zeros((nx - 1, ny - 1, g.nlevel), order="F"), state, levels)
+
+
+GRID_POINTS = {x_velocity: "u", x_velocity_: "u", y_velocity: "v", y_velocity_: "v",
+               vorticity: "gamma", vorticity_: "gamma"}
+
+
+def _select(state, sel):
+    """Panel selection: all (None), 0-based indices or range, or a body name."""
+    if sel is None:
+        return slice(None)
+    if isinstance(sel, str):
+        r = state.prob.bodies.panel_range(sel)
+        return slice(r.start, r.stop)
+    return sel
+
+
+def boundary_pos(state, sel=None):
+    """Panel positions, npanel x 2 (problem.jl:789-803)."""
+    return state.xb[_select(state, sel)]
+
+
+def boundary_len(state, sel=None):
+    """Panel lengths (problem.jl:805-815)."""
+    return state.ds[_select(state, sel)]
+
+
+def boundary_vel(state, sel=None):
+    """Panel velocities, npanel x 2 (problem.jl:817-827)."""
+    return state.ub[_select(state, sel)]
+
+
+def boundary_force(state, sel=None):
+    """Force on each panel by the fluid, npanel x 2 (problem.jl:829-839)."""
+    return state.fb[_select(state, sel)]
+
+
+def boundary_total_force(state: State, sel=None):
+    """Total [x, y] force on the panels, all or a selection (problem.jl:845-856)."""
+    return boundary_force(state, sel).sum(axis=0)
+
+
+def _copy_into(out, val):
+    out[...] = val
+    return out
+
+
+def boundary_pos_(out, state, sel=None):
+    return _copy_into(out, boundary_pos(state, sel))
+
+
+def boundary_len_(out, state, sel=None):
+    return _copy_into(out, boundary_len(state, sel))
+
+
+def boundary_vel_(out, state, sel=None):
+    return _copy_into(out, boundary_vel(state, sel))
+
+
+def boundary_force_(out, state, sel=None):
+    return _copy_into(out, boundary_force(state, sel))
+
+
+def boundary_total_force_(out, state, sel=None):
+    return _copy_into(out, boundary_total_force(state, sel))
+
+
+def arraysize(func, npanel: int):
+    """Array size of a boundary accessor for ``npanel`` panels (problem.jl:858-862)."""
+    if func in (boundary_len, boundary_len_):
+        return (npanel,)
+    if func in (boundary_total_force, boundary_total_force_):
+        return (2,)
+    return (npanel, 2)

@@ -285,25 +285,49 @@ def getstate(solver: AbstractSolver) -> State:
     return solver.getstate()
 
 
-def solve_(f: Optional[Callable], solver: HipCnabSolver, *, t: float, log: bool = False,
-           chunk: int = 1):
-    """solve!(f, solver; t) (solver.jl:37-62): n = round((t - state.t)/dt) steps, calling
-    f(state, i, n) after each step. With chunk > 1 the steps run in groups of ``chunk``
-    between callbacks (forces of every step are still returned)."""
-    st = solver.state
+def solve_(f: Optional[Callable], solver: AbstractSolver, *, t: float, save=None,
+           log: bool = False, chunk: int = 1):
+    """solve!(f, solver; t, save, log) (solver.jl:31-63): n = round((t - state.t)/dt) steps,
+    calling f(state, i, n) after each step and handing the state to the ``save`` saver
+    (a SolutionSaver, saving.py) at the timesteps it wants. ``log`` prints every tenth step
+    as the reference does (its default is on; here off).
+
+    For the GPU solver the host State is refreshed from the device only where a saver
+    wants it (call ``solver.sync_state()`` inside ``f`` to read fields there), and
+    ``chunk`` > 1 advances groups of steps between callbacks (stopping early at a saved
+    timestep). Returns the force history of the GPU solver (one row per step)."""
+    from .saving import SaveNothing
+
+    hip = isinstance(solver, HipCnabSolver)
+    st = solver.state if hip else solver.getstate()
     dt = timestep(st.prob)
     n = int(round((t - st.t) / dt))
+    saver = (save if save is not None else SaveNothing()).init(st.index + n)
     forces = []
-    i = 0
-    while i < n:
-        k = min(chunk, n - i)
-        stats = solver.advance(k)
-        forces.append(solver.forces(stats))
-        i += k
-        if f is not None:
-            f(st, i, n)
-        if log and (i - 1) % 10 == 0:
-            print("i = ", st.index, "  t = ", st.t)
+    try:
+        i = 0
+        while i < n:
+            if hip:
+                k = min(max(1, int(chunk)), n - i)
+                for idx in range(st.index + 1, st.index + k):
+                    if saver.wants(idx):
+                        k = idx - st.index
+                        break
+                forces.append(solver.forces(solver.advance(k)))
+            else:
+                k = 1
+                solver.advance()
+            i += k
+            if saver.wants(st.index):
+                if hip:
+                    solver.sync_state()
+                saver.update(st)
+            if f is not None:
+                f(st, i, n)
+            if log and i % 10 == 1:
+                print("i = ", st.index, "  t = ", st.t)
+    finally:
+        saver.finalize()
     return np.concatenate(forces) if forces else np.zeros((0, 2))
 
 

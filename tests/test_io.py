@@ -86,3 +86,30 @@ def test_gpu_resume_matches_uninterrupted_run():
     fc = c.forces(c.advance(3))
     assert np.array_equal(fa[3:], fc)
     assert np.array_equal(a.read("Gamma"), c.read("Gamma"))
+
+
+@pytest.mark.gpu
+def test_gpu_solve_with_saver(tmp_path):
+    """solve_ with SaveNpz on the GPU solver: the host state is refreshed from the device
+    at the saved steps (also inside a chunk of steps), so the saved vorticity and forces
+    are the solver's."""
+    prob = cases.cylinder()
+    solver = ib.create_solver(prob)
+    filename = str(tmp_path / "run.npz")
+    save = ib.SaveNpz(filename, ("prob", prob),
+                      ("fluid", ib.fluid_group(prob, ib.TimestepRange(start=2, step=2), [ib.vorticity_])),
+                      ("totals", ib.body_group(prob, ib.TimestepRange(step=1), [ib.boundary_total_force_])))
+    dt = ib.timestep(prob)
+    forces = ib.solve_(None, solver, t=5 * dt, save=save, chunk=4)  # 6 steps from t = -dt
+    assert solver.state.index == 6
+    f = np.load(filename, allow_pickle=False)
+    assert np.array_equal(f["fluid/_timestep_index"], [2, 4, 6])
+    L = prob.fluid.grid.nlevel
+    g = solver.read("Gamma").reshape(L, -1)
+    w = f["fluid/vorticity"][..., 2]
+    for lev in range(L):
+        assert np.allclose(w[:, :, lev].ravel(order="F"), g[lev] / ib.gridstep(prob, lev + 1) ** 2,
+                           rtol=1e-14, atol=0)
+    # the host sums fb in another order than the GPU's force kernel: round-off of the drag
+    scale = np.abs(forces).max()
+    assert np.allclose(f["totals/cyl/boundary_total_force"].T, forces, rtol=0, atol=1e-13 * scale)
