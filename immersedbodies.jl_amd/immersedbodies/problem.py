@@ -105,12 +105,29 @@ class MultiDomainGrid:
         return self.base.h
 
 
+@dataclass(frozen=True)
+class MultiDomainExtents:
+    """The region of a MultiDomainGrid without its step (problem.jl:450-457); Fluid picks
+    the step from the Reynolds number."""
+
+    xlims: Tuple[float, float]
+    ylims: Tuple[float, float]
+    nlevel: int
+
+
 def fluid_grid(*, h=None, xlims, ylims, nlevel):
-    """``fluid_grid(; h, xlims, ylims, nlevel)`` (problem.jl:459-464). Without ``h`` the
+    """``fluid_grid(; h, xlims, ylims, nlevel)`` (problem.jl:459-475). Without ``h`` the
     step is chosen by ``Fluid`` from the Reynolds number."""
     if h is None:
-        return ("extents", tuple(xlims), tuple(ylims), int(nlevel))
+        return MultiDomainExtents(tuple(xlims), tuple(ylims), int(nlevel))
     return MultiDomainGrid(CartesianGrid.from_limits(h, xlims, ylims), int(nlevel))
+
+
+# grid point kinds (GridPoints, problem.jl:297-301) as the names coords / gridsize take
+GridVertices = "vertices"
+GridU = "u"
+GridV = "v"
+GridΓ = GridGamma = "gamma"
 
 
 def gridstep(grid, level: int = 1) -> float:
@@ -186,8 +203,12 @@ def default_gridstep(Re: float) -> float:
     return _floor_sig1(2.0 / Re)
 
 
+class GridMotion:
+    """Motion of the grid in the global frame: StaticGrid or MovingGrid (problem.jl:492-502)."""
+
+
 @dataclass
-class StaticGrid:
+class StaticGrid(GridMotion):
     """A grid that is static in the global frame (problem.jl:508-513)."""
 
 
@@ -207,7 +228,7 @@ class GridVelocity:
 
 
 @dataclass
-class MovingGrid:
+class MovingGrid(GridMotion):
     """A grid that moves in the global frame: ``motion(t)`` returns its GridVelocity
     (problem.jl:515-523)."""
 
@@ -227,16 +248,20 @@ class Fluid:
     grid_motion: object = field(default_factory=StaticGrid)
 
     def __post_init__(self):
-        if isinstance(self.grid, tuple) and self.grid and self.grid[0] == "extents":
-            _, xl, yl, nl = self.grid
+        if isinstance(self.grid, MultiDomainExtents):
+            e = self.grid
             self.grid = MultiDomainGrid(
-                CartesianGrid.from_limits(default_gridstep(self.Re), xl, yl), nl)
+                CartesianGrid.from_limits(default_gridstep(self.Re), e.xlims, e.ylims), e.nlevel)
         if not isinstance(self.grid_motion, (StaticGrid, MovingGrid)):
             raise TypeError("grid_motion must be a StaticGrid or a MovingGrid")
 
 
+class AbstractScheme:
+    """A time-stepping scheme (problem.jl:559-564)."""
+
+
 @dataclass(frozen=True)
-class CNAB:
+class CNAB(AbstractScheme):
     """2-step Crank-Nicolson / Adams-Bashforth scheme (problem.jl:566-577)."""
 
     dt: float
@@ -292,8 +317,35 @@ class RigidBodyTransform:
                          self.angular_vel], dtype=np.float64)
 
 
+class AbstractBody:
+    """A body immersed in the fluid (problem.jl:47-53)."""
+
+
+class PresetBody(AbstractBody):
+    """A body whose motion is prescribed (problem.jl:70-75)."""
+
+
+class FsiBody(AbstractBody):
+    """A body moved by the fluid (problem.jl:77-92). The reference defines no concrete
+    one; neither does this package."""
+
+
 @dataclass
-class StaticBody:
+class PanelState:
+    """Positions xb (n x 2), lengths ds and velocities ub (n x 2) of panels
+    (problem.jl:21-45)."""
+
+    xb: np.ndarray
+    ds: np.ndarray
+    ub: np.ndarray
+
+    @property
+    def n(self):
+        return self.xb.shape[0]
+
+
+@dataclass
+class StaticBody(PresetBody):
     """A body fixed at ``panels`` (problem.jl:220-226)."""
 
     panels: Panels
@@ -304,7 +356,7 @@ class StaticBody:
 
 
 @dataclass
-class MovingRigidBody:
+class MovingRigidBody(PresetBody):
     """A rigid body at ``motion(t)`` applied to ``panels`` (problem.jl:241-261)."""
 
     motion: Callable[[float], RigidBodyTransform]
@@ -356,6 +408,24 @@ class Bodies:
             if b.moving:
                 out[6 * k:6 * k + 6] = b.motion(t).as_array()
         return out
+
+
+@dataclass(frozen=True)
+class PanelSection:
+    """A body's place among the concatenated panels (problem.jl:94-130): its 1-based
+    index among the bodies, the body, and its 0-based panel range."""
+
+    index: int
+    body: object
+    panel_range: range
+
+
+def panel_section(bodies: Bodies, name) -> PanelSection:
+    """The PanelSection of body ``name`` (problem.jl:198-203)."""
+    for k, (nm, body, off, n) in enumerate(bodies.sections):
+        if nm == name:
+            return PanelSection(k + 1, body, range(off, off + n))
+    raise KeyError(name)
 
 
 def any_fsi(bodies: Bodies) -> bool:
@@ -414,6 +484,19 @@ class State:
         self.fb = np.zeros((n, 2))
         self.freestream_vel = np.zeros(2)
         self.set_time_index(0)
+
+    @property
+    def panels(self) -> PanelState:
+        """state.panels of the reference: the panel arrays of this State."""
+        return PanelState(self.xb, self.ds, self.ub)
+
+    @property
+    def Γ(self):
+        return self.Gamma
+
+    @property
+    def ψ(self):
+        return self.psi
 
     def set_time_index(self, index: int):
         """t = t0 + dt (index - 1) (problem.jl:703-708)."""
@@ -524,6 +607,9 @@ def boundary_total_force(state: State, sel=None):
     return boundary_force(state, sel).sum(axis=0)
 
 
+boundary_ds = boundary_len  # the name the reference exports for boundary_len
+
+
 def _copy_into(out, val):
     out[...] = val
     return out
@@ -535,6 +621,9 @@ def boundary_pos_(out, state, sel=None):
 
 def boundary_len_(out, state, sel=None):
     return _copy_into(out, boundary_len(state, sel))
+
+
+boundary_ds_ = boundary_len_
 
 
 def boundary_vel_(out, state, sel=None):
@@ -551,7 +640,7 @@ def boundary_total_force_(out, state, sel=None):
 
 def arraysize(func, npanel: int):
     """Array size of a boundary accessor for ``npanel`` panels (problem.jl:858-862)."""
-    if func in (boundary_len, boundary_len_):
+    if func in (boundary_len, boundary_len_, boundary_ds, boundary_ds_):
         return (npanel,)
     if func in (boundary_total_force, boundary_total_force_):
         return (2,)

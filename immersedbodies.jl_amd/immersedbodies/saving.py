@@ -49,7 +49,11 @@ class MultiDomainValue(SolutionValue):
         self.dtype = np.dtype(np.float64)
 
 
-class TimestepRange:
+class Timesteps:
+    """The timesteps to save at (io/io.jl:115-120)."""
+
+
+class TimestepRange(Timesteps):
     """Every ``step``-th timestep from ``start``; at most ``length`` of them
     (io/io.jl:115-143)."""
 

@@ -270,6 +270,9 @@ class HipCnabSolver(AbstractSolver):
         return self.sync_state()
 
 
+CnabSolver = HipCnabSolver  # the reference's name for the CNAB solver (solver.jl:78)
+
+
 def create_solver(x, **kw) -> HipCnabSolver:
     """create_solver(prob_or_state; kw...) (solver.jl:28,152)."""
     state = State(x) if isinstance(x, Problem) else x
