@@ -238,8 +238,12 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
 // TB/2 x TM/2 block (b-half w&1, m-half w>>1) of both of the block's products, so the
 // odd block's butterfly A +- B stays in registers.
 // ---------------------------------------------------------------------------------
+// The 32-row tiles with one register stage run at 3 waves per SIMD or more: the epilogues
+// that scale or combine (EPI_DIAG, EPI_SUB, EPI_ADD) would otherwise take enough
+// registers to drop to 2.
 template <typename InT, typename OutT, int EPI, int TB, int TM, int KC, int PF>
-__global__ __launch_bounds__(256) void dst_pass_f64q_kernel(const PassBatch batch) {
+__global__ __launch_bounds__(256, (TB == 32 && PF == 1) ? 3 : 1) void dst_pass_f64q_kernel(
+    const PassBatch batch) {
   constexpr int BS = TB / 32;        // 16-wide b subtiles per wave
   constexpr int MS = TM / 32;        // 16-wide m subtiles per wave
   constexpr int RA = KC * TB / 256;  // (i, b) input slots per thread and chunk
