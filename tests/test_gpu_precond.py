@@ -77,3 +77,16 @@ def test_precond_refresh():
     assert pa[0] == 1
     for k in range(1, 4):
         assert pa[k] == (1 if it[k - 1] > 1 else 0)
+
+
+def test_precond_mixed_precision_moving_cylinder():
+    """Mixed precision with a moving body and the preconditioned CG: forces within the
+    mixed path's 1e-6 of the oracle."""
+    prob = cases.moving_cylinder()
+    print("case moving_cylinder mixed cg_precond", flush=True)
+    s = ib.create_solver(prob, precision="mixed", cg_precond=True)
+    orc = cases.oracle_for(prob)
+    st = s.advance(6)
+    f_ref = cases.oracle_advance(orc, prob, 6)
+    assert all(x.cg_converged for x in st)
+    assert cases.relerr(s.forces(st), f_ref) <= 1e-6
