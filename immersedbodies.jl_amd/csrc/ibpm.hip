@@ -132,8 +132,12 @@ int autotune_passes(ibpm_solver* s) {
   const bool mixed = s->precision == IBPM_MIXED;
   const size_t P = s->plane();
   HIPCHK(hipMemsetAsync(s->T, 0, kMaxPlanes * P * sizeof(double), s->st));
-  int counts[3] = {1, s->L, (kMaxPlanes / s->L) * s->L};
-  for (int ci = 0; ci < 3; ++ci) {
+  // single planes, the levels of one column, the column groups of the B operator, and the
+  // two finest levels of one column or of a B-operator group (inverse passes of vort2flux
+  // when only they are read)
+  const int g = std::max(1, kMaxPlanes / s->L), two = std::min(2, s->L);
+  const int counts[5] = {1, s->L, g * s->L, two, g * two};
+  for (int ci = 0; ci < 5; ++ci) {
     const int np = counts[ci];
     bool seen = false;
     for (int cj = 0; cj < ci; ++cj) seen = seen || counts[cj] == np;
@@ -253,7 +257,7 @@ int solve(ibpm_solver* s, const SolveJob* jobs, int nj) {
 // one batched forward pass pair, the FP64 ring chain level L..2, one batched inverse pass
 // pair whose first pass adds the ring terms. in[p], out[p]: plane p = column*L + level.
 int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double* const* out,
-              int epi) {
+              int epi, int nout) {
   const size_t P = s->plane();
   const int L = s->L, np = ng * L;
   const bool mixed = s->precision == IBPM_MIXED;
@@ -305,30 +309,42 @@ int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double
     HIPCHK(launch_ring_sweep(ra, ng, l > 0, s->st));
     if (l > 0) HIPCHK(launch_ring_factor(ra, ng, s->st));
   }
+  // inverse passes of the levels the caller reads (l < nout of every column)
   memset(&pb, 0, sizeof pb);
-  for (int p = 0; p < np; ++p) {
-    pb.p[p].in = s->T + p * P;
-    pb.p[p].out = s->U + p * P;
-  }
-  rc = run_pass(s, pb, np, true, EPI_STORE, mixed, mixed);
+  int no = 0;
+  for (int c = 0; c < ng; ++c)
+    for (int l = 0; l < nout; ++l) {
+      pb.p[no].in = s->T + (c * L + l) * P;
+      pb.p[no].out = s->U + (c * L + l) * P;
+      ++no;
+    }
+  rc = run_pass(s, pb, no, true, EPI_STORE, mixed, mixed);
   if (rc) return rc;
   memset(&pb, 0, sizeof pb);
-  for (int p = 0; p < np; ++p) {
-    pb.p[p].in = s->U + p * P;
-    pb.p[p].out = out[p];
-    pb.p[p].base = out[p];
-  }
-  return run_pass(s, pb, np, false, epi, mixed, false);
+  no = 0;
+  for (int c = 0; c < ng; ++c)
+    for (int l = 0; l < nout; ++l) {
+      const int p = c * L + l;
+      pb.p[no].in = s->U + p * P;
+      pb.p[no].out = out[p];
+      pb.p[no].base = out[p];
+      ++no;
+    }
+  return run_pass(s, pb, no, false, epi, mixed, false);
 }
 
 // vort2flux (fluid.jl:175-206) for ncol independent columns of L planes each.
 // With `outer`, the coarse levels outside the centre come from outer (final pass of the
 // step); otherwise coarsify! writes the centre region in place. q: curl all levels.
+// nout: psi is written on the nout finest levels only (the others are only needed for
+// their boundary values, which the spectral ring chain carries); the mixed path's
+// refinement reads every level and always writes all of them.
 int vort2flux(ibpm_solver* s, int ncol, double* const* Gc, double* const* psic,
-              double* const* outer, double* const* qc) {
+              double* const* outer, double* const* qc, int nout = 0) {
   const size_t P = s->plane();
   const int L = s->L;
   const bool mixed = s->precision == IBPM_MIXED;
+  if (nout <= 0 || nout > L || mixed || qc) nout = L;
   for (int c = 0; c < ncol; ++c)
     for (int l = 1; l < L; ++l)
       HIPCHK(launch_coarsify(s->g, Gc[c] + l * P, Gc[c] + (l - 1) * P,
@@ -343,7 +359,7 @@ int vort2flux(ibpm_solver* s, int ncol, double* const* Gc, double* const* psic,
         in[c * L + l] = Gc[c0 + c] + l * P;
         out[c * L + l] = psic[c0 + c] + l * P;
       }
-    int rc = v2f_group(s, ng, in, false, out, EPI_STORE);
+    int rc = v2f_group(s, ng, in, false, out, EPI_STORE, nout);
     if (rc) return rc;
     if (mixed) {
       // refinement of the coupled multi-level system: r_l = Gamma_l + apply_bc(get_bc(
@@ -368,7 +384,7 @@ int vort2flux(ibpm_solver* s, int ncol, double* const* Gc, double* const* psic,
             rin[p] = s->R + p * P;
           }
         HIPCHK(launch_residual(ra, s->st));
-        rc = v2f_group(s, ng, rin, true, out, EPI_ADD);
+        rc = v2f_group(s, ng, rin, true, out, EPI_ADD, L);
         if (rc) return rc;
       }
     }
@@ -422,7 +438,7 @@ int apply_B(ibpm_solver* s, int ncol, const double* const* z, double* const* x) 
     Gc[c] = s->Gtmp + (size_t)c * L * P;
     pc[c] = s->psitmp + (size_t)c * L * P;
   }
-  rc = vort2flux(s, ncol, Gc.data(), pc.data(), nullptr, nullptr);
+  rc = vort2flux(s, ncol, Gc.data(), pc.data(), nullptr, nullptr, std::min(2, L));  // E reads levels 1, 2
   if (rc) return rc;
   for (int c0 = 0; c0 < ncol; c0 += 64) {
     const int nc = std::min(64, ncol - c0);
@@ -777,10 +793,13 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
   int rc = solve(s, jobs.data(), L);
   if (rc) return rc;
   std::swap(s->nl1, s->nl2);  // _cycle!
-  {
+  if (s->n > 0) {
+    // trial psi: only couple_surface's E reads it (levels 1 and 2); the final vort2flux
+    // below writes every level. Without bodies that final call is the same computation
+    // (coarsify! is idempotent), so the trial one is skipped.
     double* Gc[1] = {s->Gs};
     double* pc[1] = {s->psi};
-    rc = vort2flux(s, 1, Gc, pc, nullptr, nullptr);
+    rc = vort2flux(s, 1, Gc, pc, nullptr, nullptr, std::min(2, L));
     if (rc) return rc;
   }
   // couple_surface!, then project_circ! (coupling.jl:217-231): Gamma = Gamma_s,
