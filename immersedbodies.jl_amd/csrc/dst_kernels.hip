@@ -54,7 +54,9 @@ __device__ __forceinline__ void store_out(const PassPlane& d, int b, int jc, dou
   OutT* out = (OutT*)d.out;
   if (EPI == EPI_DIAG) {
     const double lam = -2.0 * ((d.cosb[b] + d.cosj[jc]) - 2.0);
-    val *= d.dscale / (d.da + lam * d.dnum / d.dden);
+    const double den = d.da + lam * d.dnum / d.dden;
+    if (d.spec) d.spec[idx] = val / den;
+    val *= d.dscale / den;
   }
   if (EPI == EPI_SUB) val = d.base[idx] - val;
   if (EPI == EPI_ADD) val = d.base[idx] + val;

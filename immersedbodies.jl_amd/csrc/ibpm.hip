@@ -66,6 +66,7 @@ struct SolveJob {
   int epi;             // EPI_STORE, EPI_SUB (x = base - solve)
   const double* base;
   double a, c;
+  double* spec = nullptr;  // FP64: the solve's spectral plane, unscaled (PassPlane::spec)
 };
 
 }  // namespace
@@ -80,6 +81,7 @@ void free_all(ibpm_solver* s) {
                   s->body_of, s->moving_d, s->coords, s->perm, s->xstart, s->err, s->w,
                   s->ywin, s->gemv_part, s->pc_P, s->pc_theta0, s->pc_t, s->pc_z,
                   s->Binv, s->Gtmp, s->psitmp, s->Sc, s->cg_r, s->cg_u, s->cg_c, s->cg_scal,
+                  s->F1, s->G1,
                   s->edge, s->pw, s->fac, s->rvec, s->rpart, s->Mx,
                   s->My != s->Mx ? s->My : nullptr,
                   s->ws_h0, s->ws_h1, s->ws_xe,
@@ -189,6 +191,7 @@ int solve_chunk(ibpm_solver* s, const SolveJob* jobs, int nj) {
       pb.p[i].da = jobs[i].da;
       pb.p[i].dnum = jobs[i].dnum;
       pb.p[i].dden = jobs[i].dden;
+      pb.p[i].spec = correction || mixed ? nullptr : jobs[i].spec;
     }
     rc = run_pass(s, pb, nj, false, EPI_DIAG, mixed, mixed);
     if (rc) return rc;
@@ -257,25 +260,27 @@ int solve(ibpm_solver* s, const SolveJob* jobs, int nj) {
 // one batched forward pass pair, the FP64 ring chain level L..2, one batched inverse pass
 // pair whose first pass adds the ring terms. in[p], out[p]: plane p = column*L + level.
 int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double* const* out,
-              int epi, int nout) {
+              int epi, int nout, const double* spec0 = nullptr) {
   const size_t P = s->plane();
   const int L = s->L, np = ng * L;
   const bool mixed = s->precision == IBPM_MIXED;
   const size_t nf = 2 * (size_t)s->g.Nx + 2 * (size_t)s->g.Ny;
+  // forward passes of every plane but plane 0 when its spectral plane is given (spec0)
   PassBatch pb;
   memset(&pb, 0, sizeof pb);
-  for (int p = 0; p < np; ++p) {
-    pb.p[p].in = in[p];
-    pb.p[p].out = s->T + p * P;
+  const int p0 = spec0 ? 1 : 0;
+  for (int p = p0; p < np; ++p) {
+    pb.p[p - p0].in = in[p];
+    pb.p[p - p0].out = s->T + p * P;
   }
-  int rc = run_pass(s, pb, np, true, EPI_STORE, in_f32, mixed);
+  int rc = np > p0 ? run_pass(s, pb, np - p0, true, EPI_STORE, in_f32, mixed) : IBPM_OK;
   if (rc) return rc;
   memset(&pb, 0, sizeof pb);
-  for (int p = 0; p < np; ++p) {
-    pb.p[p].in = s->T + p * P;
-    pb.p[p].out = s->U + p * P;
+  for (int p = p0; p < np; ++p) {
+    pb.p[p - p0].in = s->T + p * P;
+    pb.p[p - p0].out = s->U + p * P;
   }
-  rc = run_pass(s, pb, np, false, EPI_STORE, mixed, mixed);
+  rc = np > p0 ? run_pass(s, pb, np - p0, false, EPI_STORE, mixed, mixed) : IBPM_OK;
   if (rc) return rc;
   // ring chain, coarse to fine: E_l = D . (F_l + R_l) into T and, for l >= 1, the ring
   // factors of level l-1 from the projections of E_l
@@ -298,7 +303,7 @@ int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double
   const size_t npa = ring_part_a_count(s->g.Nx, s->g.Ny);
   for (int l = L - 1; l >= 0; --l) {
     for (int c = 0; c < ng; ++c) {
-      ra.F[c] = s->U + (c * L + l) * P;
+      ra.F[c] = spec0 && c == 0 && l == 0 ? (const void*)spec0 : (const void*)(s->U + (c * L + l) * P);
       ra.E[c] = s->T + (c * L + l) * P;
       ra.fac_in[c] = l == L - 1 ? nullptr : s->fac + (c * L + l) * nf;
       ra.fac_out[c] = l > 0 ? s->fac + (c * L + l - 1) * nf : nullptr;
@@ -338,13 +343,18 @@ int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double
 // step); otherwise coarsify! writes the centre region in place. q: curl all levels.
 // nout: psi is written on the nout finest levels only (the others are only needed for
 // their boundary values, which the spectral ring chain carries); the mixed path's
-// refinement reads every level and always writes all of them.
+// refinement reads every level and always writes all of them. spec0 (FP64): the forward
+// transform S Gamma_1 S of column 0's finest level, known from the solve that produced
+// it (S S = 2 n I makes a sine solve's spectral plane that transform), so its two
+// forward passes are skipped.
 int vort2flux(ibpm_solver* s, int ncol, double* const* Gc, double* const* psic,
-              double* const* outer, double* const* qc, int nout = 0) {
+              double* const* outer, double* const* qc, int nout = 0,
+              const double* spec0 = nullptr) {
   const size_t P = s->plane();
   const int L = s->L;
   const bool mixed = s->precision == IBPM_MIXED;
   if (nout <= 0 || nout > L || mixed || qc) nout = L;
+  if (mixed) spec0 = nullptr;
   for (int c = 0; c < ncol; ++c)
     for (int l = 1; l < L; ++l)
       HIPCHK(launch_coarsify(s->g, Gc[c] + l * P, Gc[c] + (l - 1) * P,
@@ -359,7 +369,7 @@ int vort2flux(ibpm_solver* s, int ncol, double* const* Gc, double* const* psic,
         in[c * L + l] = Gc[c0 + c] + l * P;
         out[c * L + l] = psic[c0 + c] + l * P;
       }
-    int rc = v2f_group(s, ng, in, false, out, EPI_STORE, nout);
+    int rc = v2f_group(s, ng, in, false, out, EPI_STORE, nout, c0 == 0 ? spec0 : nullptr);
     if (rc) return rc;
     if (mixed) {
       // refinement of the coupled multi-level system: r_l = Gamma_l + apply_bc(get_bc(
@@ -790,6 +800,7 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
     j.a = 1.0;
     j.c = cA(s, l);
   }
+  jobs[0].spec = s->F1;  // null in mixed precision
   int rc = solve(s, jobs.data(), L);
   if (rc) return rc;
   std::swap(s->nl1, s->nl2);  // _cycle!
@@ -799,7 +810,7 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
     // (coarsify! is idempotent), so the trial one is skipped.
     double* Gc[1] = {s->Gs};
     double* pc[1] = {s->psi};
-    rc = vort2flux(s, 1, Gc, pc, nullptr, nullptr, std::min(2, L));
+    rc = vort2flux(s, 1, Gc, pc, nullptr, nullptr, std::min(2, L), s->F1);
     if (rc) return rc;
   }
   // couple_surface!, then project_circ! (coupling.jl:217-231): Gamma = Gamma_s,
@@ -831,20 +842,23 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
     j.base = s->Gs;
     j.a = 1.0;
     j.c = cA(s, 0);
+    j.spec = s->G1;
     rc = solve(s, &j, 1);
     if (rc) return rc;
+    // S Gamma_1 S of the projected circulation: that of Gamma_s minus the correction's
+    if (s->F1) HIPCHK(launch_sub(s->F1, s->F1, s->G1, P, s->st));
     double* Gc[1] = {s->G};
     double* pc[1] = {s->psi};
     double* oc[1] = {s->Gs};
     double* qc[1] = {s->q};
-    rc = vort2flux(s, 1, Gc, pc, L > 1 ? oc : nullptr, qc);
+    rc = vort2flux(s, 1, Gc, pc, L > 1 ? oc : nullptr, qc, 0, s->F1);
     if (rc) return rc;
   } else {
     HIPCHK(launch_copy(s->G, s->Gs, (size_t)L * P, s->st));
     double* Gc[1] = {s->G};
     double* pc[1] = {s->psi};
     double* qc[1] = {s->q};
-    rc = vort2flux(s, 1, Gc, pc, nullptr, qc);
+    rc = vort2flux(s, 1, Gc, pc, nullptr, qc, 0, s->F1);
     if (rc) return rc;
     if (stats) {
       stats->cg_iters = 0;
@@ -1075,6 +1089,10 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
     CK(hipMemcpy(s->pw, pw.data(), pw.size() * 8, hipMemcpyHostToDevice));
     const size_t nf = ed.size(), nbc = 2 * (size_t)(nx + 1) + 2 * (size_t)(ny + 1);
     CK(dalloc(&s->fac, kMaxPlanes * nf));
+    if (s->precision != IBPM_MIXED) {  // level-1 spectral planes of the step (FP64)
+      CK(dalloc(&s->F1, s->plane()));
+      CK(dalloc(&s->G1, s->plane()));
+    }
     CK(dalloc(&s->rvec, kMaxChains * ring_vector_count(Nx, Ny)));
     CK(dalloc(&s->rpart, kMaxChains * ring_partial_count(Nx, Ny)));
     (void)nbc;

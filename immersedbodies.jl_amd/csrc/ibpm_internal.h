@@ -51,6 +51,8 @@ struct PassPlane {
   const void* So;      // odd-row sine table of the length-Nk transform
   const void* Se;      // even-row sine table
   const void* Sq;      // FP64 double-fold tables [T1 | T2 | T4]
+  double* spec;        // EPI_DIAG, FP64: also store val / (da + lambda dnum / dden) here
+                       // (the plane a forward transform of the solve's result would give)
   double dscale, da, dnum, dden;
   int Nk, Nb;
 };
@@ -302,6 +304,8 @@ hipError_t launch_pcg_step(const CgDev& c, hipStream_t st);
 
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t st);
 hipError_t launch_copy(double* dst, const double* src, size_t n, hipStream_t st);
+// d = a - b elementwise
+hipError_t launch_sub(double* d, const double* a, const double* b, size_t n, hipStream_t st);
 hipError_t launch_axpy_f32(double* x, const float* y, size_t n, hipStream_t st);
 
 // ---- warm-start network (net_kernels.hip, net.hip) ----

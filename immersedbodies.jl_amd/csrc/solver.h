@@ -40,6 +40,7 @@ struct ibpm_solver {
   double *cosx = nullptr, *cosy = nullptr;  // cos(pi k / n): spectral eigenvalues
   void *SoX = nullptr, *SeX = nullptr, *SoY = nullptr, *SeY = nullptr;  // FP64 tables
   void *SqX = nullptr, *SqY = nullptr;  // FP64 double-fold tables
+  double *F1 = nullptr, *G1 = nullptr;  // FP64: level-1 spectral planes of the step
   void *BfX = nullptr, *BfY = nullptr;                                  // bf16 tables
   // multi-level ring chain: sine rows/columns, per-plane factors, per-chain scratch
   double *edge = nullptr, *pw = nullptr;

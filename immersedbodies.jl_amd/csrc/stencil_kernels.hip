@@ -279,6 +279,18 @@ hipError_t launch_fill(double* p, size_t n, double v, hipStream_t st) {
   hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
   return hipGetLastError();
 }
+__global__ void sub_kernel(double* __restrict__ d, const double* __restrict__ a,
+                           const double* __restrict__ b, size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) d[i] = a[i] - b[i];
+}
+
+hipError_t launch_sub(double* d, const double* a, const double* b, size_t n, hipStream_t st) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(sub_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, a, b, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_copy(double* d, const double* s, size_t n, hipStream_t st) {
   if (!n) return hipSuccess;
   hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, s, n);
