@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <string>
+#include <cmath>
 #include <vector>
 
 #include "../../include/ibpm.h"
@@ -243,13 +244,77 @@ int solve_chunk(ibpm_solver* s, const SolveJob* jobs, int nj) {
   return IBPM_OK;
 }
 
+// ---- Helmholtz solves by red-black Gauss-Seidel (gs_kernels.hip) ----
+constexpr int kGsMaxSweeps = 64;
+
+// Half-sweeps m with rho^(m-1) <= 2^-56 for (da I + k (-L)) x = b, k = dnum / dden, where
+// rho = 4k / (da + 4k) bounds the error contraction per half-sweep in the max norm; 0 when
+// the job is no Helmholtz solve (da = 0: the Poisson solve) or m would exceed kGsMaxSweeps.
+int gs_sweeps(const SolveJob& j) {
+  if (!(j.da > 0.0) || !(j.dnum > 0.0) || !(j.dden > 0.0)) return 0;
+  const double k = j.dnum / j.dden;
+  const double rho = 4.0 * k / (j.da + 4.0 * k);
+  if (!(rho > 0.0)) return 1;
+  const double m = 1.0 + std::ceil(56.0 * std::log(2.0) / -std::log(rho));
+  return m > kGsMaxSweeps ? 0 : (int)m;
+}
+
+bool use_gs(const ibpm_solver* s, const SolveJob& j) {
+  return s->helmholtz == 0 && (j.epi == EPI_STORE || j.epi == EPI_SUB) && gs_sweeps(j) > 0;
+}
+
+// Up to kMaxPlanes Gauss-Seidel solves: launches of at most kGsHalo half-sweeps each,
+// the iterate passed between launches through the scratch planes T[i] and U[i].
+int solve_gs(ibpm_solver* s, const SolveJob* jobs, int nj) {
+  const size_t P = s->plane();
+  int m[kMaxPlanes], done[kMaxPlanes];
+  for (int i = 0; i < nj; ++i) {
+    m[i] = gs_sweeps(jobs[i]);
+    done[i] = 0;
+  }
+  for (int round = 0;; ++round) {
+    GsBatch bt;
+    memset(&bt, 0, sizeof bt);
+    bt.Nx = s->g.Nx;
+    bt.Ny = s->g.Ny;
+    int np = 0;
+    for (int i = 0; i < nj; ++i) {
+      if (done[i] >= m[i]) continue;
+      const SolveJob& j = jobs[i];
+      const double k = j.dnum / j.dden, d = j.da + 4.0 * k;
+      GsPlane& q = bt.p[np++];
+      q.b = j.b;
+      q.c = k / d;
+      q.inv_d = 1.0 / d;
+      q.nsweep = std::min(kGsHalo, m[i] - done[i]);
+      q.color0 = done[i] & 1;
+      q.xin = done[i] ? ((round - 1) & 1 ? s->U : s->T) + i * P : nullptr;
+      const bool last = done[i] + q.nsweep == m[i];
+      q.out = last ? j.x : ((round & 1) ? s->U : s->T) + i * P;
+      q.base = last && j.epi == EPI_SUB ? j.base : nullptr;
+      done[i] += q.nsweep;
+    }
+    if (!np) break;
+    HIPCHK(launch_gs_helmholtz(bt, np, s->st));
+  }
+  return IBPM_OK;
+}
+
+// Helmholtz solves that Gauss-Seidel brings to rounding go to solve_gs, the others (and
+// the Poisson solves) to the sine transforms; a chunk of the latter shares one epilogue.
 int solve(ibpm_solver* s, const SolveJob* jobs, int nj) {
-  // all jobs of a chunk must agree on the epilogue kind
+  std::vector<SolveJob> gs, rest;
+  for (int i = 0; i < nj; ++i) (use_gs(s, jobs[i]) ? gs : rest).push_back(jobs[i]);
+  for (size_t i = 0; i < gs.size(); i += kMaxPlanes) {
+    int rc = solve_gs(s, gs.data() + i, (int)std::min<size_t>(kMaxPlanes, gs.size() - i));
+    if (rc) return rc;
+  }
+  const int nr = (int)rest.size();
   int i = 0;
-  while (i < nj) {
+  while (i < nr) {
     int k = i + 1;
-    while (k < nj && k - i < kMaxPlanes && jobs[k].epi == jobs[i].epi) ++k;
-    int rc = solve_chunk(s, jobs + i, k - i);
+    while (k < nr && k - i < kMaxPlanes && rest[k].epi == rest[i].epi) ++k;
+    int rc = solve_chunk(s, rest.data() + i, k - i);
     if (rc) return rc;
     i = k;
   }
@@ -800,7 +865,10 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
     j.a = 1.0;
     j.c = cA(s, l);
   }
-  jobs[0].spec = s->F1;  // null in mixed precision
+  // level 1's spectral plane S Gamma_s S, kept when the sine transforms solve level 1
+  // (FP64); both vort2flux calls of the step then skip its forward passes
+  const bool spec1 = s->F1 && !use_gs(s, jobs[0]);
+  jobs[0].spec = spec1 ? s->F1 : nullptr;
   int rc = solve(s, jobs.data(), L);
   if (rc) return rc;
   std::swap(s->nl1, s->nl2);  // _cycle!
@@ -810,7 +878,7 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
     // (coarsify! is idempotent), so the trial one is skipped.
     double* Gc[1] = {s->Gs};
     double* pc[1] = {s->psi};
-    rc = vort2flux(s, 1, Gc, pc, nullptr, nullptr, std::min(2, L), s->F1);
+    rc = vort2flux(s, 1, Gc, pc, nullptr, nullptr, std::min(2, L), jobs[0].spec);
     if (rc) return rc;
   }
   // couple_surface!, then project_circ! (coupling.jl:217-231): Gamma = Gamma_s,
@@ -842,23 +910,23 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
     j.base = s->Gs;
     j.a = 1.0;
     j.c = cA(s, 0);
-    j.spec = s->G1;
+    j.spec = spec1 ? s->G1 : nullptr;  // the same operator as level 1's solve above
     rc = solve(s, &j, 1);
     if (rc) return rc;
     // S Gamma_1 S of the projected circulation: that of Gamma_s minus the correction's
-    if (s->F1) HIPCHK(launch_sub(s->F1, s->F1, s->G1, P, s->st));
+    if (spec1) HIPCHK(launch_sub(s->F1, s->F1, s->G1, P, s->st));
     double* Gc[1] = {s->G};
     double* pc[1] = {s->psi};
     double* oc[1] = {s->Gs};
     double* qc[1] = {s->q};
-    rc = vort2flux(s, 1, Gc, pc, L > 1 ? oc : nullptr, qc, 0, s->F1);
+    rc = vort2flux(s, 1, Gc, pc, L > 1 ? oc : nullptr, qc, 0, spec1 ? s->F1 : nullptr);
     if (rc) return rc;
   } else {
     HIPCHK(launch_copy(s->G, s->Gs, (size_t)L * P, s->st));
     double* Gc[1] = {s->G};
     double* pc[1] = {s->psi};
     double* qc[1] = {s->q};
-    rc = vort2flux(s, 1, Gc, pc, nullptr, qc, 0, s->F1);
+    rc = vort2flux(s, 1, Gc, pc, nullptr, qc, 0, spec1 ? s->F1 : nullptr);
     if (rc) return rc;
     if (stats) {
       stats->cg_iters = 0;
@@ -923,6 +991,7 @@ void ibpm_default_options(ibpm_options* o) {
   o->dst_tiles = -1;
   o->cg_precond = 0;
   o->cg_precond_refresh = 0;
+  o->helmholtz = 0;
 }
 
 const char* ibpm_last_error(void) { return g_err.c_str(); }
@@ -980,6 +1049,7 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   s->cg_tol_mode = opt.cg_tol_mode;
   s->cg_precond = opt.cg_precond ? 1 : 0;
   s->cg_precond_refresh = std::max(0, opt.cg_precond_refresh);
+  s->helmholtz = opt.helmholtz == 1 ? 1 : 0;
   s->device = opt.device;
   s->grid_moving = p->grid_moving ? 1 : 0;
   s->hc.resize(s->L);

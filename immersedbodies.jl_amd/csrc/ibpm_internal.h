@@ -304,6 +304,26 @@ hipError_t launch_pcg_step(const CgDev& c, hipStream_t st);
 
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t st);
 hipError_t launch_copy(double* dst, const double* src, size_t n, hipStream_t st);
+// ---- red-black Gauss-Seidel Helmholtz solves (gs_kernels.hip) ----
+// One launch runs nsweep <= kGsHalo half-sweeps of x = inv_d b + c (sum of the 4
+// neighbours) on each plane, starting from xin (zero when null) with colour color0
+// ((i + j) & 1 of the points updated first), and writes x, or base - x when base is set,
+// to out (which must not alias b, xin or base).
+constexpr int kGsHalo = 16;
+struct GsPlane {
+  const double* b;
+  const double* xin;
+  double* out;
+  const double* base;
+  double c, inv_d;
+  int nsweep, color0;
+};
+struct GsBatch {
+  GsPlane p[kMaxPlanes];
+  int Nx, Ny, ntx, nty;  // plane shape; tile grid (filled by the launcher)
+};
+hipError_t launch_gs_helmholtz(GsBatch& bt, int nplanes, hipStream_t st);
+
 // d = a - b elementwise
 hipError_t launch_sub(double* d, const double* a, const double* b, size_t n, hipStream_t st);
 hipError_t launch_axpy_f32(double* x, const float* y, size_t n, hipStream_t st);

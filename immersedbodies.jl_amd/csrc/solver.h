@@ -22,6 +22,7 @@ struct ibpm_solver {
   // preconditioned CG: inverse of B at an earlier configuration (pc_P), the bodies'
   // angles then (pc_theta0, device), scratch vectors and GEMV partials
   int cg_precond = 0, cg_precond_refresh = 0;
+  int helmholtz = 0;  // 0: Gauss-Seidel where it converges fast, else sine transforms; 1: sine
   bool pc_ready = false, pc_due = false;
   double *pc_P = nullptr, *pc_theta0 = nullptr, *pc_t = nullptr, *pc_z = nullptr;
   std::vector<double> cur_theta;  // host: each body's angle at the current step

@@ -62,7 +62,7 @@ class Options(C.Structure):
     _fields_ = [("precision", C.c_int), ("refine_sweeps", C.c_int), ("cg_maxiter", C.c_int),
                 ("cg_reltol", C.c_double), ("cg_abstol", C.c_double), ("cg_tol_mode", C.c_int),
                 ("device", C.c_int), ("dst_tiles", C.c_int), ("cg_precond", C.c_int),
-                ("cg_precond_refresh", C.c_int)]
+                ("cg_precond_refresh", C.c_int), ("helmholtz", C.c_int)]
 
 
 class StepStats(C.Structure):

@@ -106,6 +106,11 @@ typedef struct ibpm_options {
                          true residual of the same tolerance.                       */
   int cg_precond_refresh; /* with cg_precond: reassemble at the next step when a
                          step needed more iterations than this (0: never)          */
+  int helmholtz;      /* implicit Helmholtz solves (da I - k L) x = b: 0 (default)
+                         red-black Gauss-Seidel in FP64 where 64 half-sweeps bring
+                         the error below 2^-56 max|x| (Jacobi contraction
+                         rho = 4k/(da+4k) <= 0.54; the sine transforms elsewhere);
+                         1 always the sine transforms (the reference's method)     */
 } ibpm_options;
 
 typedef struct ibpm_step_stats {

@@ -58,6 +58,52 @@ def test_poisson_and_helmholtz_any_length(precision, refine, tol, nx, ny):
     s.close()
 
 
+@pytest.mark.parametrize("dt", [0.004, 0.1, 0.5])
+def test_helmholtz_gauss_seidel_and_sine_transforms(dt):
+    """The two Helmholtz methods against the oracle's sine-transform solve at 1e-12: with
+    h = 0.05 and Re = 100, k = dt / (2 Re h^2) on level 1 is 0.008 (13 Gauss-Seidel
+    half-sweeps), 0.2 (49) and 1.0 (beyond 64: the sine transforms; level 2's 0.25 still
+    Gauss-Seidel)."""
+    nx, ny, L, h = 64, 48, 3, 0.05
+    prob = cases.no_body(nx=nx, ny=ny, nlevel=L, h=h, x0=(-nx * h / 2, -ny * h / 2), Re=100.0,
+                         dt=dt)
+    og = O.Grid(nx, ny, L, -nx * h / 2, -ny * h / 2, h)
+    b = np.random.default_rng(5).standard_normal(og.nG)
+    got = {}
+    for mode in ("auto", "sine"):
+        s = ib.create_solver(prob, helmholtz=mode)
+        for lev in (1, 2, 3):
+            got[mode, lev] = s.apply("helmholtz", b, level=lev)
+        s.close()
+    for lev in (1, 2, 3):
+        ref = O.lap_inv(og, O.diag_A(og, prob.fluid.Re, dt, lev, True), b)
+        for mode in ("auto", "sine"):
+            e = cases.relerr(got[mode, lev], ref)
+            print("dt", dt, "level", lev, mode, "rel err", e)
+            assert e < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_step_gauss_seidel_matches_sine_transforms(precision):
+    """Steps with the Gauss-Seidel Helmholtz solves end within rounding of those with the
+    sine transforms (FP64: 1e-12 on fields, 1e-11 on forces; mixed: the forces of both
+    within 1e-7 of each other)."""
+    prob = cases.cylinder(nx=96, ny=64, nlevel=3, h=0.05, x0=(-1.6, -1.6), Re=100.0)
+    out = {}
+    for mode in ("auto", "sine"):
+        s = ib.create_solver(prob, precision=precision, helmholtz=mode)
+        st = s.advance(6)
+        out[mode] = (s.forces(st), s.read("Gamma"), s.read("psi"), s.read("q"))
+        s.close()
+    fa, fs = out["auto"][0], out["sine"][0]
+    if precision == "fp64":
+        assert cases.relerr(fa, fs) < 1e-11
+        for a, b in zip(out["auto"][1:], out["sine"][1:]):
+            assert cases.relerr(a, b) < 1e-12
+    else:
+        assert cases.relerr(fa, fs) < 1e-7
+
+
 @pytest.mark.parametrize("precision,refine,tol", [("fp64", 0, 1e-11), ("mixed", 2, 1e-9),
                                                   ("mixed", 0, 1e-4)])
 @pytest.mark.parametrize("shape", [(64, 48, 3), (32, 32, 1), (40, 56, 4), (42, 38, 3), (41, 37, 3),
