@@ -9,10 +9,11 @@
 // 2^-56, below FP64 rounding) and uses the sine transforms where m would be large.
 //
 // Temporal blocking: a work-group owns a 128 x 128 tile of one plane (8 waves of 16 rows;
-// lane l holds columns 2l, 2l+1 of its wave's rows in registers) and runs up to kGsHalo
-// half-sweeps on it without leaving registers. Values on the tile's edge use wrong
-// neighbours for those outside the tile; a half-sweep moves that error one cell inward, so
-// the centre 96 x 96, kGsHalo cells from the edge, is exact and is what the tile writes.
+// lane l holds columns 2l, 2l+1 of its wave's rows in registers) and runs up to H <=
+// kGsHalo half-sweeps on it without leaving registers. Values on the tile's edge use
+// wrong neighbours for those outside the tile; a half-sweep moves that error one cell
+// inward, so the centre (128 - 2H)^2, H cells from the edge, is exact and is what the
+// tile writes (H: the launch's largest half-sweep count, rounded up to even).
 // Horizontal neighbours come from the adjacent lane by DPP wave shifts, vertical ones
 // from the thread's own registers, and across waves through a small LDS row exchange.
 // The colour of vertex (i, j) is (i + j) & 1 in plane indices; tile origins are even,
@@ -22,35 +23,37 @@
 namespace ibpm {
 namespace {
 
-constexpr int kRows = 16;                 // rows per wave
-constexpr int kWaves = 8;                 // waves per work-group
-constexpr int kTile = 128;                // tile edge: 64 lanes x 2 columns, 8 x 16 rows
-constexpr int kOut = kTile - 2 * kGsHalo; // exact centre written per tile
+constexpr int kRows = 16;   // rows per wave
+constexpr int kWaves = 8;   // waves per work-group
+constexpr int kTile = 128;  // tile edge: 64 lanes x 2 columns, 8 waves x 16 rows
 static_assert(kRows * kWaves == kTile, "tile rows");
-static_assert((kOut % 2) == 0 && (kGsHalo % 2) == 0, "tile origins must stay even");
+static_assert(kGsHalo % 2 == 0 && 2 * kGsHalo < kTile, "halo");
 
-// value of the lane below (lane l reads lane l - 1); lane 0 keeps its own (a tile edge)
+// value of the lane below (lane l reads lane l - 1; lane 0 reads 0, a tile edge)
 __device__ __forceinline__ double from_left(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), 0x138, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), 0x138, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 
-// value of the lane above (lane l reads lane l + 1); lane 63 keeps its own (a tile edge)
+// value of the lane above (lane l reads lane l + 1; lane 63 reads 0, a tile edge)
 __device__ __forceinline__ double from_right(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), 0x130, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), 0x130, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 
-// One half-sweep over the points of colour C held by this thread. rok: bit r set when
-// row r of the wave lies in the plane; cok[cp]: column cp of the lane does.
-template <int C>
+// One half-sweep over the points of colour C held by this thread. With ROWS, rok
+// (wave-uniform) has bit r set when row r of the wave lies in the plane, and rows outside
+// keep x = 0 (waves on the plane's lower or upper edge); without, every row does.
+// cl[cp]: c for a column in the plane, 0 outside, where bb = 0 too, so x stays 0 there.
+template <int C, bool ROWS>
 __device__ __forceinline__ void half_sweep(double (&x)[kRows][2], const double (&bb)[kRows][2],
                                            const double above[2], const double below[2],
-                                           double c, unsigned rok, const bool cok[2]) {
+                                           const double cl[2], unsigned rok) {
 #pragma unroll
   for (int r = 0; r < kRows; ++r) {
+    if (ROWS && !((rok >> r) & 1u)) continue;
     const int cp = (C + r) & 1;
     const double up = r > 0 ? x[r > 0 ? r - 1 : 0][cp] : above[cp];
     const double dn = r < kRows - 1 ? x[r < kRows - 1 ? r + 1 : 0][cp] : below[cp];
@@ -62,19 +65,19 @@ __device__ __forceinline__ void half_sweep(double (&x)[kRows][2], const double (
       left = x[r][0];
       right = from_right(x[r][0]);
     }
-    const double v = bb[r][cp] + c * ((up + dn) + (left + right));
-    x[r][cp] = (((rok >> r) & 1u) && cok[cp]) ? v : 0.0;
+    x[r][cp] = bb[r][cp] + cl[cp] * ((up + dn) + (left + right));
   }
 }
 
-__global__ __launch_bounds__(kWaves * 64) void gs_helmholtz_kernel(const GsBatch bt) {
+__global__ __launch_bounds__(kWaves * 64) void gs_helmholtz_kernel(const GsBatch bt, int H) {
   const GsPlane& pl = bt.p[blockIdx.y];
-  const int Nx = bt.Nx, Ny = bt.Ny;
+  const int Nx = bt.Nx, Ny = bt.Ny, out_w = kTile - 2 * H;
   const int tx = blockIdx.x % bt.ntx, ty = blockIdx.x / bt.ntx;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int i0 = tx * kOut - kGsHalo + 2 * lane;  // this lane's first column
-  const int j0 = ty * kOut - kGsHalo + kRows * w;  // this wave's first row
-  __shared__ double2 edge[2][kWaves][2][64];       // [buffer][wave][first/last row][lane]
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i0 = tx * out_w - H + 2 * lane;  // this lane's first column (even)
+  const int j0 = ty * out_w - H + kRows * w;  // this wave's first row (even)
+  __shared__ double2 edge[2][kWaves][2][64];  // [buffer][wave][first/last row][lane]
 
   const bool cok[2] = {i0 >= 0 && i0 < Nx, i0 + 1 >= 0 && i0 + 1 < Nx};
   unsigned rok = 0;
@@ -82,17 +85,32 @@ __global__ __launch_bounds__(kWaves * 64) void gs_helmholtz_kernel(const GsBatch
   for (int r = 0; r < kRows; ++r)
     if (j0 + r >= 0 && j0 + r < Ny) rok |= 1u << r;
 
-  const double c = pl.c, inv_d = pl.inv_d;
+  const double inv_d = pl.inv_d;
+  const double cl[2] = {cok[0] ? pl.c : 0.0, cok[1] ? pl.c : 0.0};
+  // clamped indices: every load is in bounds and issues unconditionally (points outside
+  // the plane are zeroed by the selects)
+  const int ic0 = min(max(i0, 0), Nx - 1), ic1 = min(max(i0 + 1, 0), Nx - 1);
   double x[kRows][2], bb[kRows][2];
 #pragma unroll
   for (int r = 0; r < kRows; ++r) {
+    const size_t row = (size_t)min(max(j0 + r, 0), Ny - 1) * Nx;
+    const double v0 = pl.b[row + ic0], v1 = pl.b[row + ic1];
+    const bool rk = (rok >> r) & 1u;
+    bb[r][0] = rk && cok[0] ? inv_d * v0 : 0.0;
+    bb[r][1] = rk && cok[1] ? inv_d * v1 : 0.0;
+  }
+  if (pl.xin) {
 #pragma unroll
-    for (int cp = 0; cp < 2; ++cp) {
-      const bool ok = ((rok >> r) & 1u) && cok[cp];
-      const size_t idx = (size_t)(i0 + cp) + (size_t)(j0 + r) * Nx;
-      bb[r][cp] = ok ? inv_d * pl.b[idx] : 0.0;
-      x[r][cp] = ok && pl.xin ? pl.xin[idx] : 0.0;
+    for (int r = 0; r < kRows; ++r) {
+      const size_t row = (size_t)min(max(j0 + r, 0), Ny - 1) * Nx;
+      const double v0 = pl.xin[row + ic0], v1 = pl.xin[row + ic1];
+      const bool rk = (rok >> r) & 1u;
+      x[r][0] = rk && cok[0] ? v0 : 0.0;
+      x[r][1] = rk && cok[1] ? v1 : 0.0;
     }
+  } else {
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) x[r][0] = x[r][1] = 0.0;
   }
   edge[0][w][0][lane] = make_double2(x[0][0], x[0][1]);
   edge[0][w][1][lane] = make_double2(x[kRows - 1][0], x[kRows - 1][1]);
@@ -110,40 +128,58 @@ __global__ __launch_bounds__(kWaves * 64) void gs_helmholtz_kernel(const GsBatch
       below[0] = b.x;
       below[1] = b.y;
     }
-    if (((pl.color0 + s) & 1) == 0)
-      half_sweep<0>(x, bb, above, below, c, rok, cok);
-    else
-      half_sweep<1>(x, bb, above, below, c, rok, cok);
+    const bool all_rows = rok == (1u << kRows) - 1;
+    if (((pl.color0 + s) & 1) == 0) {
+      if (all_rows)
+        half_sweep<0, false>(x, bb, above, below, cl, rok);
+      else
+        half_sweep<0, true>(x, bb, above, below, cl, rok);
+    } else {
+      if (all_rows)
+        half_sweep<1, false>(x, bb, above, below, cl, rok);
+      else
+        half_sweep<1, true>(x, bb, above, below, cl, rok);
+    }
     edge[rb ^ 1][w][0][lane] = make_double2(x[0][0], x[0][1]);
     edge[rb ^ 1][w][1][lane] = make_double2(x[kRows - 1][0], x[kRows - 1][1]);
     __syncthreads();
   }
-  // the exact centre: tile rows and columns kGsHalo .. kTile - kGsHalo - 1
+  // the exact centre: tile rows and columns H .. kTile - H - 1
+  if (kRows * (w + 1) <= H || kRows * w >= kTile - H) return;
+  if (pl.base) {
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const size_t row = (size_t)min(max(j0 + r, 0), Ny - 1) * Nx;
+      x[r][0] = pl.base[row + ic0] - x[r][0];
+      x[r][1] = pl.base[row + ic1] - x[r][1];
+    }
+  }
+  const bool cw[2] = {cok[0] && 2 * lane >= H && 2 * lane < kTile - H,
+                      cok[1] && 2 * lane + 1 >= H && 2 * lane + 1 < kTile - H};
 #pragma unroll
   for (int r = 0; r < kRows; ++r) {
     const int lr = kRows * w + r;
-    if (lr < kGsHalo || lr >= kTile - kGsHalo || !((rok >> r) & 1u)) continue;
-#pragma unroll
-    for (int cp = 0; cp < 2; ++cp) {
-      const int lc = 2 * lane + cp;
-      if (lc < kGsHalo || lc >= kTile - kGsHalo || !cok[cp]) continue;
-      const size_t idx = (size_t)(i0 + cp) + (size_t)(j0 + r) * Nx;
-      pl.out[idx] = pl.base ? pl.base[idx] - x[r][cp] : x[r][cp];
-    }
+    if (!((rok >> r) & 1u) || lr < H || lr >= kTile - H) continue;
+    const size_t row = (size_t)(j0 + r) * Nx;
+    if (cw[0]) pl.out[row + i0] = x[r][0];
+    if (cw[1]) pl.out[row + i0 + 1] = x[r][1];
   }
 }
 
 }  // namespace
 
-int gs_tiles(int n) { return (n + kOut - 1) / kOut; }
-
 hipError_t launch_gs_helmholtz(GsBatch& bt, int nplanes, hipStream_t st) {
-  bt.ntx = gs_tiles(bt.Nx);
-  bt.nty = gs_tiles(bt.Ny);
-  for (int p = 0; p < nplanes; ++p)
+  int H = 0;
+  for (int p = 0; p < nplanes; ++p) {
     if (bt.p[p].nsweep < 1 || bt.p[p].nsweep > kGsHalo) return hipErrorInvalidValue;
+    H = bt.p[p].nsweep > H ? bt.p[p].nsweep : H;
+  }
+  H += H & 1;  // even: tile origins stay even, so colours are (column + row) & 1 in a tile
+  const int out_w = kTile - 2 * H;
+  bt.ntx = (bt.Nx + out_w - 1) / out_w;
+  bt.nty = (bt.Ny + out_w - 1) / out_w;
   hipLaunchKernelGGL(gs_helmholtz_kernel, dim3(bt.ntx * bt.nty, nplanes), dim3(kWaves * 64), 0, st,
-                     bt);
+                     bt, H);
   return hipGetLastError();
 }
 

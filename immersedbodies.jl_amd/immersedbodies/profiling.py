@@ -20,10 +20,12 @@ from collections import defaultdict
 def step_breakdown(db_path: str, steps: int = 20, marker: str = "nonlin_rhs_kernel"):
     db = sqlite3.connect(db_path)
     rows = list(db.execute("select name, start, end from kernels order by start"))
-    # a step's marker is followed by the step's first sine-contraction pass (isolated
-    # marker launches, such as a kernel-timing loop, are not step boundaries)
+    # a step's marker is followed by the step's first Helmholtz-solve kernel, a sine
+    # contraction pass or a Gauss-Seidel launch (isolated marker launches, such as a
+    # kernel-timing loop, are not step boundaries)
     marks = [i for i, r in enumerate(rows)
-             if marker in r[0] and i + 1 < len(rows) and "dst_pass" in rows[i + 1][0]]
+             if marker in r[0] and i + 1 < len(rows)
+             and ("dst_pass" in rows[i + 1][0] or "gs_helmholtz" in rows[i + 1][0])]
     if len(marks) < 2:
         raise ValueError("fewer than two marker dispatches")
     sel = marks[-(steps + 1):]
@@ -33,7 +35,8 @@ def step_breakdown(db_path: str, steps: int = 20, marker: str = "nonlin_rhs_kern
     for i in range(sel[0], sel[-1]):
         name, s, e = rows[i]
         d = (e - s) * 1e-3  # ns -> us
-        short = name.split("(")[0].replace("void ", "").replace("ibpm::", "")
+        short = name.replace("(anonymous namespace)::", "").split("(")[0]
+        short = short.replace("void ", "").replace("ibpm::", "")
         per[short][0] += 1
         per[short][1] += d
         busy += d
