@@ -82,7 +82,7 @@ void free_all(ibpm_solver* s) {
                   s->body_of, s->moving_d, s->coords, s->perm, s->xstart, s->err, s->w,
                   s->ywin, s->gemv_part, s->pc_P, s->pc_theta0, s->pc_t, s->pc_z,
                   s->Binv, s->Gtmp, s->psitmp, s->Sc, s->cg_r, s->cg_u, s->cg_c, s->cg_scal,
-                  s->F1, s->G1,
+                  s->F1, s->G1, s->Sfx, s->Sfy != s->Sfx ? s->Sfy : nullptr, s->Ftr, s->Tbox,
                   s->edge, s->pw, s->fac, s->rvec, s->rpart, s->Mx,
                   s->My != s->Mx ? s->My : nullptr,
                   s->ws_h0, s->ws_h1, s->ws_xe,
@@ -324,28 +324,40 @@ int solve(ibpm_solver* s, const SolveJob* jobs, int nj) {
 // Multi-level Poisson solve of vort2flux for ng columns of L planes (ng*L <= kMaxPlanes):
 // one batched forward pass pair, the FP64 ring chain level L..2, one batched inverse pass
 // pair whose first pass adds the ring terms. in[p], out[p]: plane p = column*L + level.
+// spec_in[p] (FP64, may be null): plane p's spectral plane S Gamma S is known, and its
+// forward passes are skipped; spec_out[p] (may be null): where the forward passes put it
+// (else the scratch plane U[p]).
 int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double* const* out,
-              int epi, int nout, const double* spec0 = nullptr) {
+              int epi, int nout, const double* const* spec_in = nullptr,
+              double* const* spec_out = nullptr) {
   const size_t P = s->plane();
   const int L = s->L, np = ng * L;
   const bool mixed = s->precision == IBPM_MIXED;
   const size_t nf = 2 * (size_t)s->g.Nx + 2 * (size_t)s->g.Ny;
-  // forward passes of every plane but plane 0 when its spectral plane is given (spec0)
+  const void* F[kMaxPlanes];  // spectral plane of each plane
+  int fw[kMaxPlanes], nfw = 0;  // planes that need the forward passes
+  for (int p = 0; p < np; ++p) {
+    if (spec_in && spec_in[p]) {
+      F[p] = spec_in[p];
+    } else {
+      F[p] = spec_out && spec_out[p] ? spec_out[p] : s->U + p * P;
+      fw[nfw++] = p;
+    }
+  }
   PassBatch pb;
   memset(&pb, 0, sizeof pb);
-  const int p0 = spec0 ? 1 : 0;
-  for (int p = p0; p < np; ++p) {
-    pb.p[p - p0].in = in[p];
-    pb.p[p - p0].out = s->T + p * P;
+  for (int i = 0; i < nfw; ++i) {
+    pb.p[i].in = in[fw[i]];
+    pb.p[i].out = s->T + fw[i] * P;
   }
-  int rc = np > p0 ? run_pass(s, pb, np - p0, true, EPI_STORE, in_f32, mixed) : IBPM_OK;
+  int rc = nfw ? run_pass(s, pb, nfw, true, EPI_STORE, in_f32, mixed) : IBPM_OK;
   if (rc) return rc;
   memset(&pb, 0, sizeof pb);
-  for (int p = p0; p < np; ++p) {
-    pb.p[p - p0].in = s->T + p * P;
-    pb.p[p - p0].out = s->U + p * P;
+  for (int i = 0; i < nfw; ++i) {
+    pb.p[i].in = s->T + fw[i] * P;
+    pb.p[i].out = (void*)F[fw[i]];
   }
-  rc = np > p0 ? run_pass(s, pb, np - p0, false, EPI_STORE, mixed, mixed) : IBPM_OK;
+  rc = nfw ? run_pass(s, pb, nfw, false, EPI_STORE, mixed, mixed) : IBPM_OK;
   if (rc) return rc;
   // ring chain, coarse to fine: E_l = D . (F_l + R_l) into T and, for l >= 1, the ring
   // factors of level l-1 from the projections of E_l
@@ -368,7 +380,7 @@ int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double
   const size_t npa = ring_part_a_count(s->g.Nx, s->g.Ny);
   for (int l = L - 1; l >= 0; --l) {
     for (int c = 0; c < ng; ++c) {
-      ra.F[c] = spec0 && c == 0 && l == 0 ? (const void*)spec0 : (const void*)(s->U + (c * L + l) * P);
+      ra.F[c] = F[c * L + l];
       ra.E[c] = s->T + (c * L + l) * P;
       ra.fac_in[c] = l == L - 1 ? nullptr : s->fac + (c * L + l) * nf;
       ra.fac_out[c] = l > 0 ? s->fac + (c * L + l - 1) * nf : nullptr;
@@ -412,18 +424,35 @@ int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double
 // transform S Gamma_1 S of column 0's finest level, known from the solve that produced
 // it (S S = 2 n I makes a sine solve's spectral plane that transform), so its two
 // forward passes are skipped.
+int box_update(ibpm_solver* s, const double* G, const double* Gs, double* F);
+
+// fwd (FP64, column 0): the L spectral planes of column 0; with fwd_given they hold those
+// of the trial vort2flux of the step, which box_update turns into this call's (Gc[0] minus
+// outer[0] vanishes outside the boxes), else the forward passes write them there.
 int vort2flux(ibpm_solver* s, int ncol, double* const* Gc, double* const* psic,
               double* const* outer, double* const* qc, int nout = 0,
-              const double* spec0 = nullptr) {
+              const double* spec0 = nullptr, double* fwd = nullptr, bool fwd_given = false) {
   const size_t P = s->plane();
   const int L = s->L;
   const bool mixed = s->precision == IBPM_MIXED;
   if (nout <= 0 || nout > L || mixed || qc) nout = L;
-  if (mixed) spec0 = nullptr;
+  if (mixed) spec0 = nullptr, fwd = nullptr;
+  if (!fwd) fwd_given = false;
   for (int c = 0; c < ncol; ++c)
     for (int l = 1; l < L; ++l)
       HIPCHK(launch_coarsify(s->g, Gc[c] + l * P, Gc[c] + (l - 1) * P,
                              outer && outer[c] ? outer[c] + l * P : nullptr, s->st));
+  if (fwd_given) {
+    int rc = box_update(s, Gc[0], outer[0], fwd);
+    if (rc) return rc;
+  }
+  const double* sin0[kMaxPlanes] = {};
+  double* sout0[kMaxPlanes] = {};
+  for (int l = 0; l < L; ++l) {
+    if (fwd_given) sin0[l] = fwd + l * P;
+    else if (fwd) sout0[l] = fwd + l * P;
+  }
+  if (spec0) sin0[0] = spec0;
   const int gmax = std::max(1, kMaxPlanes / L);
   for (int c0 = 0; c0 < ncol; c0 += gmax) {
     const int ng = std::min(gmax, ncol - c0);
@@ -434,7 +463,8 @@ int vort2flux(ibpm_solver* s, int ncol, double* const* Gc, double* const* psic,
         in[c * L + l] = Gc[c0 + c] + l * P;
         out[c * L + l] = psic[c0 + c] + l * P;
       }
-    int rc = v2f_group(s, ng, in, false, out, EPI_STORE, nout, c0 == 0 ? spec0 : nullptr);
+    int rc = v2f_group(s, ng, in, false, out, EPI_STORE, nout, c0 == 0 ? sin0 : nullptr,
+                       c0 == 0 ? sout0 : nullptr);
     if (rc) return rc;
     if (mixed) {
       // refinement of the coupled multi-level system: r_l = Gamma_l + apply_bc(get_bc(
@@ -470,6 +500,151 @@ int vort2flux(ibpm_solver* s, int ncol, double* const* Gc, double* const* psic,
 }
 
 double cA(const ibpm_solver* s, int l) { return s->dt / (2.0 * s->Re * (s->hc[l] * s->hc[l])); }
+
+// Forward transform of the change D_l = G_l - Gs_l of each level, which vanishes outside
+// box l, added to F_l: F_l += S_x[:, I] D_l[I, J] S_y[J, :], as two products with the sine
+// columns of the box, in the cheaper order (the x contraction first: an Nx x |J|
+// intermediate; else a |I| x Ny one). All levels' first products in one launch, then all
+// second ones.
+int box_update(ibpm_solver* s, const double* G, const double* Gs, double* F) {
+  const size_t P = s->plane();
+  const int Nx = s->g.Nx, Ny = s->g.Ny;
+  GemmBatch g1, g2;
+  memset(&g1, 0, sizeof g1);
+  memset(&g2, 0, sizeof g2);
+  int nj = 0;
+  for (int l = 0; l < s->L; ++l) {
+    if (s->bw[l] <= 0 || s->bh[l] <= 0) continue;
+    const size_t o = (size_t)s->bi0[l] + (size_t)s->bj0[l] * Nx;
+    const double *Gl = G + l * P + o, *Gsl = Gs + l * P + o;
+    double* T = s->Tbox + s->btoff[l];
+    GemmJob& a = g1.j[nj];
+    GemmJob& b = g2.j[nj];
+    ++nj;
+    if (s->bxfirst[l]) {
+      // T (Nx x bh) = S_x[:, I] D (bw x bh);  F += T S_y[J, :]
+      a.A = s->Sfx + (size_t)s->bi0[l] * Nx, a.lda = Nx;
+      a.B = Gl, a.B2 = Gsl, a.ldb = Nx;
+      a.C = T, a.ldc = Nx;
+      a.M = Nx, a.N = s->bh[l], a.K = s->bw[l];
+      b.A = T, b.lda = Nx;
+      b.B = s->Sfy + s->bj0[l], b.ldb = Ny;
+      b.K = s->bh[l];
+    } else {
+      // T (bw x Ny) = D S_y[J, :];  F += S_x[:, I] T
+      a.A = Gl, a.A2 = Gsl, a.lda = Nx;
+      a.B = s->Sfy + s->bj0[l], a.ldb = Ny;
+      a.C = T, a.ldc = s->bw[l];
+      a.M = s->bw[l], a.N = Ny, a.K = s->bh[l];
+      b.A = s->Sfx + (size_t)s->bi0[l] * Nx, b.lda = Nx;
+      b.B = T, b.ldb = s->bw[l];
+      b.K = s->bw[l];
+    }
+    b.C = F + l * P, b.ldc = Nx;
+    b.M = Nx, b.N = Ny;
+    b.accumulate = 1;
+  }
+  HIPCHK(launch_gemm_f64(g1, nj, s->st));
+  HIPCHK(launch_gemm_f64(g2, nj, s->st));
+  return IBPM_OK;
+}
+
+// The boxes of box_update (static bodies, FP64, Gauss-Seidel on level 1). A panel in flux
+// cell (ix, iy) feeds rot(E^T f) on vertices [ix - 4, ix + 3] x [iy - 4, iy + 3]
+// (scatter_rot_kernel); m Gauss-Seidel half-sweeps from zero move the support of
+// Ainv1 rot(E^T f), hence of Gamma_1 - Gamma_s1, by at most m cells; coarsify! maps fine
+// vertices 2 + 2a + {-1, 0, 1} to the coarse vertex ic0 + a of the centre, and outside the
+// centre Gamma_l = Gamma_s,l. Each box gets one more cell on every side.
+int setup_box_update(ibpm_solver* s, const ibpm_problem* p) {
+  s->box_on = false;
+  if (s->box_opt != 0 || s->precision == IBPM_MIXED || s->n == 0 || s->moving) return IBPM_OK;
+  SolveJob j;
+  j.b = nullptr;
+  j.x = nullptr;
+  j.da = 1.0;
+  j.dnum = s->dt;
+  j.dden = 2.0 * s->Re * (s->hc[0] * s->hc[0]);
+  j.epi = EPI_SUB;
+  j.base = nullptr;
+  j.a = 1.0;
+  j.c = cA(s, 0);
+  if (!use_gs(s, j)) return IBPM_OK;
+  const int m = gs_sweeps(j);
+  const int Nx = s->g.Nx, Ny = s->g.Ny, L = s->L;
+  int ilo = 1 << 30, ihi = -(1 << 30), jlo = 1 << 30, jhi = -(1 << 30);
+  for (int k = 0; k < s->n; ++k) {
+    const int ix = (int)rint((p->xb[k] - s->x0) / s->h + 1.0 - 0.25);
+    const int iy = (int)rint((p->xb[s->n + k] - s->y0) / s->h + 1.0 - 0.25);
+    ilo = std::min(ilo, ix - 4);
+    ihi = std::max(ihi, ix + 3);
+    jlo = std::min(jlo, iy - 4);
+    jhi = std::max(jhi, iy + 3);
+  }
+  // 1-based vertex boxes per level
+  int lo[2] = {ilo - m - 1, jlo - m - 1}, hi[2] = {ihi + m + 1, jhi + m + 1};
+  const CoarsifyRange cr = coarsify_range(s->g.nx, s->g.ny);
+  const int c0[2] = {cr.ic0, cr.jc0}, cn[2] = {cr.ni, cr.nj}, N[2] = {Nx, Ny};
+  auto floor_div2 = [](int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); };
+  double cost = 0.0;
+  size_t toff = 0;
+  for (int l = 0; l < L; ++l) {
+    if (l > 0) {
+      for (int d = 0; d < 2; ++d) {
+        int alo = floor_div2(lo[d] - 3), ahi = floor_div2(hi[d] - 1) + 1;
+        alo = std::max(alo, 0);
+        ahi = std::min(ahi, cn[d] - 1);
+        if (alo > ahi) {
+          lo[d] = 1, hi[d] = 0;  // empty
+        } else {
+          lo[d] = c0[d] + alo - 1;
+          hi[d] = c0[d] + ahi + 1;
+        }
+      }
+    }
+    const int i0 = std::max(lo[0], 1), i1 = std::min(hi[0], Nx);
+    const int j0 = std::max(lo[1], 1), j1 = std::min(hi[1], Ny);
+    s->bi0[l] = i0 - 1;
+    s->bj0[l] = j0 - 1;
+    s->bw[l] = std::max(0, i1 - i0 + 1);
+    s->bh[l] = std::max(0, j1 - j0 + 1);
+    if (s->bw[l] == 0 || s->bh[l] == 0) {
+      s->bw[l] = s->bh[l] = 0;
+      continue;
+    }
+    const double bw = s->bw[l], bh = s->bh[l];
+    const double cx = (double)Nx * bw * bh + (double)Nx * Ny * bh;  // x contraction first
+    const double cy = bw * (double)Ny * bh + (double)Nx * Ny * bw;
+    s->bxfirst[l] = cx <= cy ? 1 : 0;
+    cost += std::min(cx, cy);
+    s->btoff[l] = toff;
+    toff += s->bxfirst[l] ? (size_t)Nx * s->bh[l] : (size_t)s->bw[l] * Ny;
+  }
+  // the forward pass pair it replaces: about Nx Ny (Nx + Ny) / 4 multiply-adds per plane
+  // with the twice-folded sine matrices; take the box update when it needs at most half
+  const double passes = (double)L * Nx * Ny * (double)(Nx + Ny) / 4.0;
+  if (!(cost <= 0.5 * passes)) return IBPM_OK;
+  const size_t P = s->plane();
+  hipError_t e = dalloc(&s->Ftr, (size_t)L * P);
+  if (!e) e = dalloc(&s->Tbox, std::max<size_t>(toff, 1));
+  auto sine_matrix = [](int Nn, double** out) -> hipError_t {
+    const long long n = Nn + 1;
+    std::vector<double> tab(2 * n), S((size_t)Nn * Nn);
+    for (long long a = 0; a < 2 * n; ++a) tab[a] = 2.0 * sin(M_PI * (double)a / (double)n);
+    for (long long c = 1; c <= Nn; ++c)
+      for (long long r = 1; r <= Nn; ++r) S[(size_t)(r - 1) + (size_t)(c - 1) * Nn] = tab[(r * c) % (2 * n)];
+    hipError_t r = dalloc(out, S.size());
+    if (!r) r = hipMemcpy(*out, S.data(), S.size() * 8, hipMemcpyHostToDevice);
+    return r;
+  };
+  if (!e) e = sine_matrix(Nx, &s->Sfx);
+  if (!e) {
+    if (Ny == Nx) s->Sfy = s->Sfx;
+    else e = sine_matrix(Ny, &s->Sfy);
+  }
+  if (e) return fail(IBPM_E_HIP, hipGetErrorString(e));
+  s->box_on = true;
+  return IBPM_OK;
+}
 
 // B operator (coupling.jl:150-177) on ncol columns: x_c = E vort2flux(Ainv1 rot(E^T z_c))[1]
 int apply_B(ibpm_solver* s, int ncol, const double* const* z, double* const* x) {
@@ -878,7 +1053,8 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
     // (coarsify! is idempotent), so the trial one is skipped.
     double* Gc[1] = {s->Gs};
     double* pc[1] = {s->psi};
-    rc = vort2flux(s, 1, Gc, pc, nullptr, nullptr, std::min(2, L), jobs[0].spec);
+    rc = vort2flux(s, 1, Gc, pc, nullptr, nullptr, std::min(2, L), jobs[0].spec,
+                   s->box_on && !spec1 ? s->Ftr : nullptr, false);
     if (rc) return rc;
   }
   // couple_surface!, then project_circ! (coupling.jl:217-231): Gamma = Gamma_s,
@@ -919,7 +1095,10 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
     double* pc[1] = {s->psi};
     double* oc[1] = {s->Gs};
     double* qc[1] = {s->q};
-    rc = vort2flux(s, 1, Gc, pc, L > 1 ? oc : nullptr, qc, 0, spec1 ? s->F1 : nullptr);
+    // with the box update, only the change of Gamma since the trial vort2flux is
+    // transformed (outer = Gamma_s: G and Gs differ inside the boxes only)
+    rc = vort2flux(s, 1, Gc, pc, oc, qc, 0, spec1 ? s->F1 : nullptr,
+                   s->box_on && !spec1 ? s->Ftr : nullptr, s->box_on && !spec1);
     if (rc) return rc;
   } else {
     HIPCHK(launch_copy(s->G, s->Gs, (size_t)L * P, s->st));
@@ -992,9 +1171,21 @@ void ibpm_default_options(ibpm_options* o) {
   o->cg_precond = 0;
   o->cg_precond_refresh = 0;
   o->helmholtz = 0;
+  o->box_update = 0;
 }
 
 const char* ibpm_last_error(void) { return g_err.c_str(); }
+
+int ibpm_box_update(const ibpm_solver* s, int level, int box[4]) {
+  if (!s || !box || level < 1 || level > s->L) return fail(IBPM_E_ARG, "bad arguments");
+  box[0] = box[1] = box[2] = box[3] = 0;
+  if (!s->box_on) return 0;
+  box[0] = s->bi0[level - 1];
+  box[1] = s->bw[level - 1];
+  box[2] = s->bj0[level - 1];
+  box[3] = s->bh[level - 1];
+  return 1;
+}
 
 int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM, int* PF,
                    int* fold) {
@@ -1050,6 +1241,7 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   s->cg_precond = opt.cg_precond ? 1 : 0;
   s->cg_precond_refresh = std::max(0, opt.cg_precond_refresh);
   s->helmholtz = opt.helmholtz == 1 ? 1 : 0;
+  s->box_opt = opt.box_update == 1 ? 1 : 0;
   s->device = opt.device;
   s->grid_moving = p->grid_moving ? 1 : 0;
   s->hc.resize(s->L);
@@ -1291,6 +1483,8 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
       int rc = assemble_B_inverse(s, s->Binv);
       if (rc) return bail(rc);
       rc = alloc_bscratch(s, 0);  // static bodies never apply B again
+      if (rc) return bail(rc);
+      rc = setup_box_update(s, p);
       if (rc) return bail(rc);
     } else {
       int rc = alloc_bscratch(s, 1);

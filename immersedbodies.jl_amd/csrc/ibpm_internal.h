@@ -324,6 +324,38 @@ struct GsBatch {
 };
 hipError_t launch_gs_helmholtz(GsBatch& bt, int nplanes, hipStream_t st);
 
+// coarsify! centre region (stencil_kernels.hip): coarse vertices ic0 + a, jc0 + c (1-based,
+// a < ni, c < nj) take the full weighting of fine vertex (2 + 2a, 2 + 2c)
+struct CoarsifyRange {
+  int ic0, ni, jc0, nj;
+};
+__host__ __device__ inline CoarsifyRange coarsify_range(int nx, int ny) {
+  CoarsifyRange r;
+  r.ic0 = nx / 2 - nx / 4 + 1;
+  r.jc0 = ny / 2 - ny / 4 + 1;
+  const int fi = nx / 2 - 1, ci = 2 * (nx / 4) - 1, fj = ny / 2 - 1, cj = 2 * (ny / 4) - 1;
+  r.ni = fi < ci ? fi : ci;
+  r.nj = fj < cj ? fj : cj;
+  if (r.ni < 0) r.ni = 0;
+  if (r.nj < 0) r.nj = 0;
+  return r;
+}
+
+// ---- FP64 matrix-core products (gemm_kernels.hip) ----
+// C (+)= (A - A2) (B - B2), column-major, M x K times K x N; A2 / B2 may be null.
+struct GemmJob {
+  const double *A, *A2;
+  const double *B, *B2;
+  double* C;
+  int lda, ldb, ldc;
+  int M, N, K;
+  int accumulate;  // 1: C += A B, 0: C = A B
+};
+struct GemmBatch {
+  GemmJob j[kMaxPlanes];
+};
+hipError_t launch_gemm_f64(const GemmBatch& gb, int njobs, hipStream_t st);
+
 // d = a - b elementwise
 hipError_t launch_sub(double* d, const double* a, const double* b, size_t n, hipStream_t st);
 hipError_t launch_axpy_f32(double* x, const float* y, size_t n, hipStream_t st);

@@ -42,6 +42,16 @@ struct ibpm_solver {
   void *SoX = nullptr, *SeX = nullptr, *SoY = nullptr, *SeY = nullptr;  // FP64 tables
   void *SqX = nullptr, *SqY = nullptr;  // FP64 double-fold tables
   double *F1 = nullptr, *G1 = nullptr;  // FP64: level-1 spectral planes of the step
+  // box update of the final vort2flux (FP64, static bodies): the change of Gamma after the
+  // trial vort2flux lies in box [bi0, bi0 + bw) x [bj0, bj0 + bh) of each level (0-based
+  // plane indices; bw = 0: none), and its forward transform is added to the trial's
+  int box_opt = 0;  // 0: when it pays, 1: off
+  bool box_on = false;
+  int bi0[16] = {}, bw[16] = {}, bj0[16] = {}, bh[16] = {}, bxfirst[16] = {};
+  size_t btoff[16] = {};
+  double *Sfx = nullptr, *Sfy = nullptr;  // full sine matrices 2 sin(pi k i / n)
+  double *Ftr = nullptr;                   // L spectral planes of the trial vort2flux
+  double *Tbox = nullptr;                  // per-level intermediate products
   void *BfX = nullptr, *BfY = nullptr;                                  // bf16 tables
   // multi-level ring chain: sine rows/columns, per-plane factors, per-chain scratch
   double *edge = nullptr, *pw = nullptr;

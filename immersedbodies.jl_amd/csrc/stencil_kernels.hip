@@ -132,20 +132,7 @@ hipError_t launch_nonlin_rhs(const NonlinArgs& a, hipStream_t st) {
 // that is the region [nx/4+1, 3nx/4-1]. With `outer`, the rest of the coarse plane is
 // copied from it (project_circ's state.Gamma .= Gamma_s fused into the final pass).
 // ---------------------------------------------------------------------------------
-struct CoarsifyRange {
-  int ic0, ni, jc0, nj;
-};
-__host__ __device__ inline CoarsifyRange coarsify_range(int nx, int ny) {
-  CoarsifyRange r;
-  r.ic0 = nx / 2 - nx / 4 + 1;
-  r.jc0 = ny / 2 - ny / 4 + 1;
-  const int fi = nx / 2 - 1, ci = 2 * (nx / 4) - 1, fj = ny / 2 - 1, cj = 2 * (ny / 4) - 1;
-  r.ni = fi < ci ? fi : ci;
-  r.nj = fj < cj ? fj : cj;
-  if (r.ni < 0) r.ni = 0;
-  if (r.nj < 0) r.nj = 0;
-  return r;
-}
+// (CoarsifyRange, coarsify_range: ibpm_internal.h)
 
 __global__ __launch_bounds__(256) void coarsify_kernel(GridDims g, double* __restrict__ coarse,
                                                        const double* __restrict__ fine,

@@ -33,7 +33,8 @@ EXPORTS = ("ibpm_default_options", "ibpm_create", "ibpm_advance", "ibpm_read", "
            "ibpm_net_destroy", "ibpm_net_dataset_size", "ibpm_net_clear_dataset",
            "ibpm_net_train", "ibpm_net_param_count", "ibpm_net_get_params",
            "ibpm_net_set_params", "ibpm_net_forward", "ibpm_net_gradients", "ibpm_net_step",
-           "ibpm_batch_create", "ibpm_batch_advance", "ibpm_batch_destroy", "ibpm_dst_tiles", "ibpm_set_body_motion")
+           "ibpm_batch_create", "ibpm_batch_advance", "ibpm_batch_destroy", "ibpm_dst_tiles", "ibpm_set_body_motion",
+           "ibpm_box_update")
 
 IBPM_WS_PREVIOUS = 0
 IBPM_WS_EXTRAPOLATE = 1
@@ -62,7 +63,8 @@ class Options(C.Structure):
     _fields_ = [("precision", C.c_int), ("refine_sweeps", C.c_int), ("cg_maxiter", C.c_int),
                 ("cg_reltol", C.c_double), ("cg_abstol", C.c_double), ("cg_tol_mode", C.c_int),
                 ("device", C.c_int), ("dst_tiles", C.c_int), ("cg_precond", C.c_int),
-                ("cg_precond_refresh", C.c_int), ("helmholtz", C.c_int)]
+                ("cg_precond_refresh", C.c_int), ("helmholtz", C.c_int),
+                ("box_update", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -162,6 +164,8 @@ def load():
     L.ibpm_set_body_motion.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp, _dp, C.c_double,
                                        C.c_double]
     L.ibpm_set_body_motion.restype = C.c_int
+    L.ibpm_box_update.argtypes = [C.c_void_p, C.c_int, _ip]
+    L.ibpm_box_update.restype = C.c_int
     _lib = L
     return L
 

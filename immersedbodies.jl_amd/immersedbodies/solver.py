@@ -54,11 +54,14 @@ class HipCnabSolver(AbstractSolver):
                  in FP64 where 64 half-sweeps bring the error below 2^-56 max|x| (Jacobi
                  contraction 4k/(1+4k) <= 0.54, k = dt / (2 Re h^2)), by the sine
                  transforms elsewhere; "sine" always by the sine transforms
+      box_update: "auto" (FP64, static bodies, Gauss-Seidel on level 1): the final
+                  vort2flux of a step transforms only the change of Gamma since the
+                  trial one, which is zero outside a box around the bodies; "off"
     """
 
     def __init__(self, state: State, *, precision="fp64", refine_sweeps=2, cg=None,
                  cg_tol_mode=0, device=0, dst_tiles=-1, cg_precond=False,
-                 cg_precond_refresh=0, helmholtz="auto"):
+                 cg_precond_refresh=0, helmholtz="auto", box_update="auto"):
         L = _lib.load()
         self.state = state
         prob = state.prob
@@ -96,10 +99,12 @@ class HipCnabSolver(AbstractSolver):
         o.cg_precond = int(bool(cg_precond))
         o.cg_precond_refresh = int(cg_precond_refresh)
         o.helmholtz = {"auto": 0, "sine": 1}[helmholtz]
+        o.box_update = {"auto": 0, "off": 1}[box_update]
         h = C.c_void_p()
         _lib.check(L.ibpm_create(C.byref(p), C.byref(o), C.byref(h)))
         self._h = h
         self.precision = precision
+
         self.last_stats = None
         # a State created before the solver may carry a non-zero index (resume)
         if state.index != 0:
@@ -197,6 +202,13 @@ class HipCnabSolver(AbstractSolver):
                                               C.byref(tb), C.byref(tm), C.byref(pf),
                                               C.byref(fold)))
         return tb.value, tm.value, pf.value, fold.value
+
+    def box_update(self, level: int = 1):
+        """None when the steps transform Gamma again in the final vort2flux; else the box
+        (first column, columns, first row, rows) of the change of Gamma on ``level``."""
+        box = (C.c_int * 4)()
+        on = _lib.load().ibpm_box_update(self._h, int(level), box)
+        return tuple(box) if on else None
 
     def set_warm_start(self, mode: str = "previous", net=None, record: bool = False):
         """Initial guess of the moving-body CG: "previous" (the reference's previous F̃b),

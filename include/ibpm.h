@@ -111,6 +111,12 @@ typedef struct ibpm_options {
                          the error below 2^-56 max|x| (Jacobi contraction
                          rho = 4k/(da+4k) <= 0.54; the sine transforms elsewhere);
                          1 always the sine transforms (the reference's method)     */
+  int box_update;     /* FP64 with static bodies whose level-1 Helmholtz solve is
+                         Gauss-Seidel: 0 (default) the final vort2flux of a step adds
+                         the forward transform of the change of Gamma since the trial
+                         one (zero outside a box around the bodies) to the trial's,
+                         instead of transforming Gamma again, when that is cheaper;
+                         1 off                                                       */
 } ibpm_options;
 
 typedef struct ibpm_step_stats {
@@ -187,6 +193,12 @@ int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, doubl
  * solver's sine passes of `nplanes` planes along x (xdir != 0) or y use. fold may be NULL. */
 int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM, int* PF,
                    int* fold);
+
+/* Instrumentation: 1 when the solver's steps use the box update of the final
+ * vort2flux (ibpm_options.box_update), with box[0..3] = first column, columns, first row,
+ * rows (0-based interior-vertex indices) of the change of Gamma on `level` (1-based; 0
+ * columns: none on that level); 0 when they do not. */
+int ibpm_box_update(const ibpm_solver* s, int level, int box[4]);
 
 const char* ibpm_last_error(void);
 void ibpm_destroy(ibpm_solver* s);
