@@ -6,7 +6,7 @@
 // Gamma read from the two planes it lies between).
 //
 // Tile: 64 x 64 of C per work-group, 4 waves of 32 x 32 (2 x 2 v_mfma_f64_16x16x4), K in
-// chunks of 16 staged through LDS with the next chunk's loads in flight. The matrix core
+// chunks of 32 staged through LDS with the next chunk's loads in flight. The matrix core
 // takes B^T as its first operand so that a lane's accumulator entries run along the
 // column of C (16 consecutive rows per store). Each C entry sums its K terms in one fixed
 // order, so results do not depend on scheduling.
@@ -17,11 +17,21 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int GT = 64;  // C tile edge
-constexpr int GK = 16;  // K chunk
+constexpr int GK = 32;  // K chunk (the next chunk's loads fly during this one's products)
+constexpr int GR = GT * GK / 256;  // elements of A (and of B) each thread stages per chunk
 
 __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
+  // the job's fields in registers (uniform), read once; a missing difference term reads
+  // the matrix itself with weight 0 (a - 0 * a = a exactly), so no load is conditional
   const GemmJob& jb = gb.j[blockIdx.y];
-  const int ntm = (jb.M + GT - 1) / GT, ntn = (jb.N + GT - 1) / GT;
+  const double *A = jb.A, *B = jb.B;
+  const double* A2 = jb.A2 ? jb.A2 : jb.A;
+  const double* B2 = jb.B2 ? jb.B2 : jb.B;
+  const double wa = jb.A2 ? 1.0 : 0.0, wb = jb.B2 ? 1.0 : 0.0;
+  double* C = jb.C;
+  const int lda = jb.lda, ldb = jb.ldb, ldc = jb.ldc, M = jb.M, N = jb.N, K = jb.K;
+  const bool accumulate = jb.accumulate != 0;
+  const int ntm = (M + GT - 1) / GT, ntn = (N + GT - 1) / GT;
   if ((int)blockIdx.x >= ntm * ntn) return;  // whole work-group: barriers stay uniform
   const int m0 = (blockIdx.x % ntm) * GT, n0 = (blockIdx.x / ntm) * GT;
   __shared__ double As[GK][GT + 1];  // As[k][m]
@@ -36,44 +46,57 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
 
-  // A chunk: 64 m x 16 k, thread -> m = tid % 64, k = tid / 64 + 4r (m contiguous)
-  // B chunk: 16 k x 64 n, thread -> k = tid % 16, n = tid / 16 + 16r (k contiguous)
-  double ra[4], rb[4];
-  auto fetch = [&](int k0) {
+  // A chunk: 64 m x GK k, thread -> m = tid % 64, k = tid / 64 + 4r (m contiguous)
+  // B chunk: GK k x 64 n, thread -> k = tid % GK, n = tid / GK + (256 / GK) r (k contiguous)
+  // Every load is unconditional, its indices clamped into the matrix (the value is zeroed
+  // when staged), so the loads issue back to back; the two terms of a difference are
+  // subtracted when staged, after the current chunk's products.
+  constexpr int NS = 256 / GK;
+  static_assert(256 % GK == 0 && GT * GK == 256 * GR && GT == 64, "chunk mapping");
+  const int tm = tid & 63, tq = tid >> 6, tkb = tid % GK, tnb = tid / GK;
+  const size_t am = (size_t)min(m0 + tm, M - 1);
+  const bool mok = m0 + tm < M;
+  double ra[GR], ra2[GR], rb[GR], rb2[GR];
+#define GEMM_FETCH(k0)                                                            \
+  {                                                                               \
+    const size_t bk_ = (size_t)min((k0) + tkb, K - 1);                            \
+    _Pragma("unroll") for (int r = 0; r < GR; ++r) {                              \
+      const size_t o = am + (size_t)min((k0) + tq + 4 * r, K - 1) * lda;          \
+      ra[r] = A[o];                                                               \
+      ra2[r] = A2[o];                                                             \
+    }                                                                             \
+    _Pragma("unroll") for (int r = 0; r < GR; ++r) {                              \
+      const size_t o = bk_ + (size_t)min(n0 + tnb + NS * r, N - 1) * ldb;         \
+      rb[r] = B[o];                                                               \
+      rb2[r] = B2[o];                                                             \
+    }                                                                             \
+  }
+  // C entries of this thread (accumulate): loaded now, in flight during the products
+  double cin[2][2][4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int m = m0 + (tid & 63), k = k0 + (tid >> 6) + 4 * r;
-      double v = 0.0;
-      if (m < jb.M && k < jb.K) {
-        const size_t o = (size_t)m + (size_t)k * jb.lda;
-        v = jb.A[o];
-        if (jb.A2) v -= jb.A2[o];
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = min(m0 + wm * 32 + j * 16 + col, M - 1);
+        const int n = min(n0 + wn * 32 + i * 16 + kq + 4 * r, N - 1);
+        cin[i][j][r] = accumulate ? C[(size_t)m + (size_t)n * ldc] : 0.0;
       }
-      ra[r] = v;
-    }
+  GEMM_FETCH(0)
+  for (int k0 = 0; k0 < K; k0 += GK) {
+    const bool kbok = k0 + tkb < K;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int k = k0 + (tid & 15), n = n0 + (tid >> 4) + 16 * r;
-      double v = 0.0;
-      if (k < jb.K && n < jb.N) {
-        const size_t o = (size_t)k + (size_t)n * jb.ldb;
-        v = jb.B[o];
-        if (jb.B2) v -= jb.B2[o];
-      }
-      rb[r] = v;
-    }
-  };
-  fetch(0);
-  for (int k0 = 0; k0 < jb.K; k0 += GK) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      As[(tid >> 6) + 4 * r][tid & 63] = ra[r];
-      Bs[tid & 15][(tid >> 4) + 16 * r] = rb[r];
+    for (int r = 0; r < GR; ++r) {
+      const bool aok = mok && k0 + tq + 4 * r < K, bok = kbok && n0 + tnb + NS * r < N;
+      const double a = fma(-wa, ra2[r], ra[r]), b = fma(-wb, rb2[r], rb[r]);
+      As[tq + 4 * r][tm] = aok ? a : 0.0;
+      Bs[tkb][tnb + NS * r] = bok ? b : 0.0;
     }
     __syncthreads();
-    if (k0 + GK < jb.K) fetch(k0 + GK);
-#pragma unroll
-    for (int ks = 0; ks < GK / 4; ++ks) {
+    if (k0 + GK < K) GEMM_FETCH(k0 + GK)
+    const int ksteps = (min(K - k0, GK) + 3) / 4;  // zero-padded past K: skip those steps
+    for (int ks = 0; ks < ksteps; ++ks) {
       const int kr = ks * 4 + kq;
       double av[2], bv[2];
 #pragma unroll
@@ -88,24 +111,48 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
     }
     __syncthreads();
   }
+#undef GEMM_FETCH
   // acc[i][j][r] at lane: C(m = m0 + wm*32 + j*16 + col, n = n0 + wn*32 + i*16 + kq + 4r)
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int m = m0 + wm * 32 + j * 16 + col;
-      if (m >= jb.M) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int n = n0 + wn * 32 + i * 16 + kq + 4 * r;
-        if (n >= jb.N) continue;
-        double* c = jb.C + (size_t)m + (size_t)n * jb.ldc;
-        *c = jb.accumulate ? *c + acc[i][j][r] : acc[i][j][r];
+        if (m < M && n < N) C[(size_t)m + (size_t)n * ldc] = cin[i][j][r] + acc[i][j][r];
       }
     }
 }
 
+// out[m + n ldo] = sum over s of P[s stride + m + n ldp], the slices in order
+__global__ __launch_bounds__(256) void sum_slices_kernel(const double* __restrict__ P, size_t stride,
+                                                         int nslice, int ldp, int M, int N,
+                                                         double* __restrict__ out, int ldo) {
+  const int m = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int n = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (m >= M || n >= N) return;
+  const double* p = P + (size_t)m + (size_t)n * ldp;
+  double v[16];
+#pragma unroll
+  for (int s = 0; s < 16; ++s) v[s] = s < nslice ? p[(size_t)s * stride] : 0.0;
+  double sum = v[0];
+#pragma unroll
+  for (int s = 1; s < 16; ++s)
+    if (s < nslice) sum += v[s];
+  out[(size_t)m + (size_t)n * ldo] = sum;
+}
+
 }  // namespace
+
+hipError_t launch_sum_slices(const double* P, size_t stride, int nslice, int ldp, int M, int N,
+                             double* out, int ldo, hipStream_t st) {
+  if (nslice < 1 || nslice > 16 || M <= 0 || N <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(sum_slices_kernel, dim3((M + 63) / 64, (N + 3) / 4), dim3(256), 0, st, P, stride,
+                     nslice, ldp, M, N, out, ldo);
+  return hipGetLastError();
+}
 
 hipError_t launch_gemm_f64(const GemmBatch& gb, int njobs, hipStream_t st) {
   int tiles = 0;

@@ -82,7 +82,7 @@ void free_all(ibpm_solver* s) {
                   s->body_of, s->moving_d, s->coords, s->perm, s->xstart, s->err, s->w,
                   s->ywin, s->gemv_part, s->pc_P, s->pc_theta0, s->pc_t, s->pc_z,
                   s->Binv, s->Gtmp, s->psitmp, s->Sc, s->cg_r, s->cg_u, s->cg_c, s->cg_scal,
-                  s->F1, s->G1, s->Sfx, s->Sfy != s->Sfx ? s->Sfy : nullptr, s->Ftr, s->Tbox,
+                  s->F1, s->G1, s->Sfx, s->Sfy != s->Sfx ? s->Sfy : nullptr, s->Ftr, s->Tbox, s->Pa, s->Pb, s->T2,
                   s->edge, s->pw, s->fac, s->rvec, s->rpart, s->Mx,
                   s->My != s->Mx ? s->My : nullptr,
                   s->ws_h0, s->ws_h1, s->ws_xe,
@@ -246,6 +246,7 @@ int solve_chunk(ibpm_solver* s, const SolveJob* jobs, int nj) {
 
 // ---- Helmholtz solves by red-black Gauss-Seidel (gs_kernels.hip) ----
 constexpr int kGsMaxSweeps = 64;
+constexpr int kPsiSlicesA = 8, kPsiSlicesB = 16;  // K slices of the trial psi box products
 
 // Half-sweeps m with rho^(m-1) <= 2^-56 for (da I + k (-L)) x = b, k = dnum / dden, where
 // rho = 4k / (da + 4k) bounds the error contraction per half-sweep in the max norm; 0 when
@@ -327,9 +328,13 @@ int solve(ibpm_solver* s, const SolveJob* jobs, int nj) {
 // spec_in[p] (FP64, may be null): plane p's spectral plane S Gamma S is known, and its
 // forward passes are skipped; spec_out[p] (may be null): where the forward passes put it
 // (else the scratch plane U[p]).
+int psi_box_product(ibpm_solver* s, const double* E, double* psi);
+
+// psi_box (FP64, ng = 1): instead of the inverse passes, only level 1's psi on the trial
+// box (psi_box_product); the rest of out[] is left as it was.
 int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double* const* out,
               int epi, int nout, const double* const* spec_in = nullptr,
-              double* const* spec_out = nullptr) {
+              double* const* spec_out = nullptr, bool psi_box = false) {
   const size_t P = s->plane();
   const int L = s->L, np = ng * L;
   const bool mixed = s->precision == IBPM_MIXED;
@@ -391,6 +396,7 @@ int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double
     HIPCHK(launch_ring_sweep(ra, ng, l > 0, s->st));
     if (l > 0) HIPCHK(launch_ring_factor(ra, ng, s->st));
   }
+  if (psi_box) return psi_box_product(s, s->T, out[0]);
   // inverse passes of the levels the caller reads (l < nout of every column)
   memset(&pb, 0, sizeof pb);
   int no = 0;
@@ -431,13 +437,15 @@ int box_update(ibpm_solver* s, const double* G, const double* Gs, double* F);
 // outer[0] vanishes outside the boxes), else the forward passes write them there.
 int vort2flux(ibpm_solver* s, int ncol, double* const* Gc, double* const* psic,
               double* const* outer, double* const* qc, int nout = 0,
-              const double* spec0 = nullptr, double* fwd = nullptr, bool fwd_given = false) {
+              const double* spec0 = nullptr, double* fwd = nullptr, bool fwd_given = false,
+              bool psi_box = false) {
   const size_t P = s->plane();
   const int L = s->L;
   const bool mixed = s->precision == IBPM_MIXED;
   if (nout <= 0 || nout > L || mixed || qc) nout = L;
   if (mixed) spec0 = nullptr, fwd = nullptr;
   if (!fwd) fwd_given = false;
+  if (mixed || ncol != 1 || qc || !s->pbox_on) psi_box = false;
   for (int c = 0; c < ncol; ++c)
     for (int l = 1; l < L; ++l)
       HIPCHK(launch_coarsify(s->g, Gc[c] + l * P, Gc[c] + (l - 1) * P,
@@ -464,7 +472,7 @@ int vort2flux(ibpm_solver* s, int ncol, double* const* Gc, double* const* psic,
         out[c * L + l] = psic[c0 + c] + l * P;
       }
     int rc = v2f_group(s, ng, in, false, out, EPI_STORE, nout, c0 == 0 ? sin0 : nullptr,
-                       c0 == 0 ? sout0 : nullptr);
+                       c0 == 0 ? sout0 : nullptr, psi_box);
     if (rc) return rc;
     if (mixed) {
       // refinement of the coupled multi-level system: r_l = Gamma_l + apply_bc(get_bc(
@@ -643,6 +651,63 @@ int setup_box_update(ibpm_solver* s, const ibpm_problem* p) {
   }
   if (e) return fail(IBPM_E_HIP, hipGetErrorString(e));
   s->box_on = true;
+  // the trial psi box: gather_kernel reads faces [ix - 3, ix + 3] x [iy - 3, iy + 3] of
+  // each panel, whose curl reads psi on vertices [ix - 4, ix + 3] x [iy - 4, iy + 3], i.e.
+  // [ilo, ihi] x [jlo, jhi]; with those and one more vertex inside the level (and so no
+  // face on its first row or column) no boundary value, hence no coarser level, is read.
+  const int nx = s->g.nx, ny = s->g.ny;
+  if (ilo >= 2 && ihi <= nx - 2 && jlo >= 2 && jhi <= ny - 2) {
+    const int ia = ilo - 1, ib = ihi + 1, ja = jlo - 1, jb = jhi + 1;  // vertices read +- 1
+    s->pi0 = ia - 1;
+    s->pbw = ib - ia + 1;
+    s->pj0 = ja - 1;
+    s->pbh = jb - ja + 1;
+    // E S_y[:, J] and S_x[I, :] (.) against the inverse pass pairs over the trial's
+    // planes (levels 1 and 2)
+    const double pcost = (double)Nx * Ny * s->pbh + (double)s->pbw * s->pbh * Nx;
+    if (pcost <= 0.5 * std::min(2, L) * (double)Nx * Ny * (double)(Nx + Ny) / 4.0) {
+      e = dalloc(&s->Pa, (size_t)kPsiSlicesA * Nx * s->pbh);
+      if (!e) e = dalloc(&s->T2, (size_t)Nx * s->pbh);
+      if (!e) e = dalloc(&s->Pb, (size_t)kPsiSlicesB * s->pbw * s->pbh);
+      if (e) return fail(IBPM_E_HIP, hipGetErrorString(e));
+      s->pbox_on = true;
+    }
+  }
+  return IBPM_OK;
+}
+
+// Trial psi of level 1 on the box [pi0, pi0 + pbw) x [pj0, pj0 + pbh) from the level's
+// ring-chain spectral plane E: psi[I, J] = S_x[I, :] (E S_y[:, J]). Both products contract
+// a full length (Ny, then Nx) into a small output, so K is split into slices computed by
+// separate work-groups and summed in slice order (deterministic).
+int psi_box_product(ibpm_solver* s, const double* E, double* psi) {
+  const int Nx = s->g.Nx, Ny = s->g.Ny, bw = s->pbw, bh = s->pbh;
+  GemmBatch g;
+  memset(&g, 0, sizeof g);
+  const int ka = ((Ny + kPsiSlicesA - 1) / kPsiSlicesA + 31) / 32 * 32;
+  int na = 0;
+  for (int k0 = 0; k0 < Ny && na < kPsiSlicesA; k0 += ka, ++na) {
+    GemmJob& j = g.j[na];
+    j.A = E + (size_t)k0 * Nx, j.lda = Nx;
+    j.B = s->Sfy + k0 + (size_t)s->pj0 * Ny, j.ldb = Ny;
+    j.C = s->Pa + (size_t)na * Nx * bh, j.ldc = Nx;
+    j.M = Nx, j.N = bh, j.K = std::min(ka, Ny - k0);
+  }
+  HIPCHK(launch_gemm_f64(g, na, s->st));
+  HIPCHK(launch_sum_slices(s->Pa, (size_t)Nx * bh, na, Nx, Nx, bh, s->T2, Nx, s->st));
+  memset(&g, 0, sizeof g);
+  const int kb = ((Nx + kPsiSlicesB - 1) / kPsiSlicesB + 31) / 32 * 32;
+  int nb = 0;
+  for (int k0 = 0; k0 < Nx && nb < kPsiSlicesB; k0 += kb, ++nb) {
+    GemmJob& j = g.j[nb];
+    j.A = s->Sfx + s->pi0 + (size_t)k0 * Nx, j.lda = Nx;
+    j.B = s->T2 + k0, j.ldb = Nx;
+    j.C = s->Pb + (size_t)nb * bw * bh, j.ldc = bw;
+    j.M = bw, j.N = bh, j.K = std::min(kb, Nx - k0);
+  }
+  HIPCHK(launch_gemm_f64(g, nb, s->st));
+  HIPCHK(launch_sum_slices(s->Pb, (size_t)bw * bh, nb, bw, bw, bh,
+                           psi + s->pi0 + (size_t)s->pj0 * Nx, Nx, s->st));
   return IBPM_OK;
 }
 
@@ -1054,7 +1119,7 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
     double* Gc[1] = {s->Gs};
     double* pc[1] = {s->psi};
     rc = vort2flux(s, 1, Gc, pc, nullptr, nullptr, std::min(2, L), jobs[0].spec,
-                   s->box_on && !spec1 ? s->Ftr : nullptr, false);
+                   s->box_on && !spec1 ? s->Ftr : nullptr, false, s->box_on && !spec1);
     if (rc) return rc;
   }
   // couple_surface!, then project_circ! (coupling.jl:217-231): Gamma = Gamma_s,
@@ -1177,8 +1242,16 @@ void ibpm_default_options(ibpm_options* o) {
 const char* ibpm_last_error(void) { return g_err.c_str(); }
 
 int ibpm_box_update(const ibpm_solver* s, int level, int box[4]) {
-  if (!s || !box || level < 1 || level > s->L) return fail(IBPM_E_ARG, "bad arguments");
+  if (!s || !box || level < 0 || level > s->L) return fail(IBPM_E_ARG, "bad arguments");
   box[0] = box[1] = box[2] = box[3] = 0;
+  if (level == 0) {
+    if (!s->pbox_on) return 0;
+    box[0] = s->pi0;
+    box[1] = s->pbw;
+    box[2] = s->pj0;
+    box[3] = s->pbh;
+    return 1;
+  }
   if (!s->box_on) return 0;
   box[0] = s->bi0[level - 1];
   box[1] = s->bw[level - 1];

@@ -52,6 +52,11 @@ struct ibpm_solver {
   double *Sfx = nullptr, *Sfy = nullptr;  // full sine matrices 2 sin(pi k i / n)
   double *Ftr = nullptr;                   // L spectral planes of the trial vort2flux
   double *Tbox = nullptr;                  // per-level intermediate products
+  // trial psi on level 1 only where couple_surface's E reads it (a box around the panels
+  // clear of the level's edge): S_x[I, :] E_1 S_y[:, J] by K-split products
+  bool pbox_on = false;
+  int pi0 = 0, pbw = 0, pj0 = 0, pbh = 0;
+  double *Pa = nullptr, *Pb = nullptr, *T2 = nullptr;
   void *BfX = nullptr, *BfY = nullptr;                                  // bf16 tables
   // multi-level ring chain: sine rows/columns, per-plane factors, per-chain scratch
   double *edge = nullptr, *pw = nullptr;

@@ -205,7 +205,9 @@ class HipCnabSolver(AbstractSolver):
 
     def box_update(self, level: int = 1):
         """None when the steps transform Gamma again in the final vort2flux; else the box
-        (first column, columns, first row, rows) of the change of Gamma on ``level``."""
+        (first column, columns, first row, rows) of the change of Gamma on ``level``.
+        level 0: the box of level 1 on which the trial vort2flux computes psi (None when it
+        runs the inverse transforms)."""
         box = (C.c_int * 4)()
         on = _lib.load().ibpm_box_update(self._h, int(level), box)
         return tuple(box) if on else None

@@ -197,7 +197,9 @@ int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM
 /* Instrumentation: 1 when the solver's steps use the box update of the final
  * vort2flux (ibpm_options.box_update), with box[0..3] = first column, columns, first row,
  * rows (0-based interior-vertex indices) of the change of Gamma on `level` (1-based; 0
- * columns: none on that level); 0 when they do not. */
+ * columns: none on that level); 0 when they do not. level 0: 1 when the trial vort2flux
+ * computes level 1's psi only on the box that the interpolation to the panels reads
+ * (given in box), 0 when it runs the inverse transforms. */
 int ibpm_box_update(const ibpm_solver* s, int level, int box[4]);
 
 const char* ibpm_last_error(void);

@@ -235,8 +235,8 @@ def test_cylinder_1024_matches_oracle_fixture():
 @pytest.mark.parametrize("nlevel", [4, 1])
 def test_box_update_matches_full_transform(nlevel):
     """The final vort2flux by the box update (the trial's spectral planes plus the transform
-    of the change of Gamma, zero outside boxes around the body) against the full forward
-    transforms: 512 x 512 cells, h = 0.01, dt = 0.001 (23 Gauss-Seidel half-sweeps on level
+    of the change of Gamma, zero outside boxes around the body) and the trial psi on the
+    panels' box only, against the full transforms: 512 x 512 cells, h = 0.01, dt = 0.001 (23 Gauss-Seidel half-sweeps on level
     1), cylinder of radius 0.5 off centre. Fields within 1e-12, forces within 1e-12; a box
     that missed part of the change would leave errors of the change's size."""
     prob = cases.cylinder(nx=512, ny=512, nlevel=nlevel, h=0.01, x0=(-1.6, -2.6), Re=100.0,
@@ -252,6 +252,8 @@ def test_box_update_matches_full_transform(nlevel):
         assert box is None
         return
     assert box is not None and 0 < box[1] < 511 and 0 < box[3] < 511
+    pbox = a.box_update(0)  # the trial psi only around the panels
+    assert pbox is not None and pbox[1] < box[1] and pbox[3] < box[3]
     fa = a.forces(a.advance(6))
     fb = b.forces(b.advance(6))
     assert cases.relerr(fa, fb) <= 1e-12
