@@ -118,19 +118,20 @@ __global__ __launch_bounds__(256) void ring_sweep_kernel(const RingChainArgs a, 
   }
 }
 
-// Complete projections from the per-tile partials, tiles summed in order.
+// Complete projections from the per-tile partials. Four lanes share one projection entry,
+// each summing a quarter of the tiles in order (all its loads in flight together), and
+// the quarters are added in order: a fixed summation order.
 __global__ __launch_bounds__(256) void ring_reduce_kernel(const RingChainArgs a, int ntx, int nty) {
   const int c = blockIdx.y;
   const int Nx = a.Nx, Ny = a.Ny;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= 2 * Nx + 2 * Ny) return;
-  // gather the partials of every tile first (independent loads in flight together), then
-  // sum them in tile order
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int i = t >> 2, qt = t & 3;  // entry, quarter of the tiles
+  const bool ok = i < 2 * Nx + 2 * Ny;
   const double* src;
   size_t stride;
   int nt;
-  if (i < 2 * Nx) {
-    const int side = i / Nx, x = i % Nx;
+  if (!ok || i < 2 * Nx) {
+    const int ii = ok ? i : 0, side = ii / Nx, x = ii % Nx;
     src = a.part_a[c] + (size_t)side * Nx + x;
     stride = 2 * (size_t)Nx;
     nt = nty;
@@ -140,15 +141,18 @@ __global__ __launch_bounds__(256) void ring_reduce_kernel(const RingChainArgs a,
     stride = 2 * (size_t)Ny;
     nt = ntx;
   }
+  const int per = (nt + 3) / 4, q0 = qt * per, q1 = min(nt, q0 + per);
   double sum = 0.0;
-  for (int q0 = 0; q0 < nt; q0 += 16) {
+  for (int b = q0; b < q1; b += 16) {
     double v[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = q0 + q < nt ? src[(size_t)(q0 + q) * stride] : 0.0;
+    for (int q = 0; q < 16; ++q) v[q] = ok && b + q < q1 ? src[(size_t)(b + q) * stride] : 0.0;
 #pragma unroll
     for (int q = 0; q < 16; ++q) sum += v[q];
   }
-  a.vec[c][i] = sum;
+  // quarters 0..3 of the entry sit in consecutive lanes: ((s0 + s1) + s2) + s3
+  const double s1 = __shfl_down(sum, 1, 64), s2 = __shfl_down(sum, 2, 64), s3 = __shfl_down(sum, 3, 64);
+  if (ok && qt == 0) a.vec[c][i] = ((sum + s1) + s2) + s3;
 }
 
 // Factors of the next finer level: uB = M_x a_b, uT = M_x a_t, vL = M_y c_l, vR = M_y c_r.
@@ -165,11 +169,24 @@ __global__ __launch_bounds__(256) void ring_factor_kernel(const RingChainArgs a)
   const double* v0 = a.vec[c] + (ydir ? 2 * (size_t)a.Nx : 0);
   const double* v1 = v0 + N;
   double s0 = 0.0, s1 = 0.0;
-#pragma unroll 4
-  for (int k = lane; k < N; k += 64) {
-    const double m = M[k];
-    s0 += m * v0[k];
-    s1 += m * v1[k];
+  // 16 elements per lane per chunk, every load of the chunk issued before the products
+  for (int k0 = 0; k0 < N; k0 += 1024) {
+    double m[16], x0[16], x1[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = k0 + lane + 64 * q;
+      const int kc = k < N ? k : 0;
+      m[q] = M[kc];
+      x0[q] = v0[kc];
+      x1[q] = v1[kc];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      if (k0 + lane + 64 * q < N) {
+        s0 += m[q] * x0[q];
+        s1 += m[q] * x1[q];
+      }
+    }
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -189,7 +206,7 @@ hipError_t launch_ring_sweep(const RingChainArgs& a, int nchain, bool do_proj, h
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !do_proj) return e;
   const int nv = 2 * a.Nx + 2 * a.Ny;
-  hipLaunchKernelGGL(ring_reduce_kernel, dim3((nv + 255) / 256, nchain), dim3(256), 0, st, a, ntx, nty);
+  hipLaunchKernelGGL(ring_reduce_kernel, dim3((4 * nv + 255) / 256, nchain), dim3(256), 0, st, a, ntx, nty);
   return hipGetLastError();
 }
 
