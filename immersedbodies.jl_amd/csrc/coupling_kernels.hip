@@ -235,9 +235,9 @@ hipError_t launch_gather(const GatherArgs& a, hipStream_t st) {
 __global__ __launch_bounds__(256) void scatter_rot_kernel(const ScatterArgs a) {
   const GridDims g = a.g;
   const int rhs = blockIdx.z;
-  const int I = blockIdx.x * 64 + (threadIdx.x & 63) + 1;
-  const int J = blockIdx.y * 4 + (threadIdx.x >> 6) + 1;
-  if (I > g.Nx || J > g.Ny) return;
+  const int I = a.ri0 + blockIdx.x * 64 + (threadIdx.x & 63) + 1;
+  const int J = a.rj0 + blockIdx.y * 4 + (threadIdx.x >> 6) + 1;
+  if (I > a.ri0 + a.rw || J > a.rj0 + a.rh) return;
   const int n = a.npanel;
   const int c0 = max(I - 3, 0);
   const int c1 = min(I + 5, g.nx + 2);  // columns ix in [I-3, I+4]
@@ -272,8 +272,12 @@ __global__ __launch_bounds__(256) void scatter_rot_kernel(const ScatterArgs a) {
     ((double*)a.out[rhs])[idx] = v;
 }
 
-hipError_t launch_scatter_rot(const ScatterArgs& a, hipStream_t st) {
-  dim3 grid((a.g.Nx + 63) / 64, (a.g.Ny + 3) / 4, a.nrhs);
+hipError_t launch_scatter_rot(const ScatterArgs& a0, hipStream_t st) {
+  ScatterArgs a = a0;
+  if (a.rw <= 0) a.ri0 = 0, a.rj0 = 0, a.rw = a.g.Nx, a.rh = a.g.Ny;
+  if (a.ri0 < 0 || a.rj0 < 0 || a.ri0 + a.rw > a.g.Nx || a.rj0 + a.rh > a.g.Ny || a.rh <= 0)
+    return hipErrorInvalidValue;
+  dim3 grid((a.rw + 63) / 64, (a.rh + 3) / 4, a.nrhs);
   hipLaunchKernelGGL(scatter_rot_kernel, grid, dim3(256), 0, st, a);
   return hipGetLastError();
 }

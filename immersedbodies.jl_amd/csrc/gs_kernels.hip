@@ -16,8 +16,10 @@
 // tile writes (H: the launch's largest half-sweep count, rounded up to even).
 // Horizontal neighbours come from the adjacent lane by DPP wave shifts, vertical ones
 // from the thread's own registers, and across waves through a small LDS row exchange.
-// The colour of vertex (i, j) is (i + j) & 1 in plane indices; tile origins are even,
-// so within a tile it is (column + row) & 1 and each row updates one column per lane.
+// The colour of vertex (i, j) is (i + j) & 1 in the indices of the region solved; tile
+// origins are even, so within a tile it is (column + row) & 1 and each row updates one
+// column per lane. A region smaller than the plane (row pitch ld) is solved with x = 0
+// around it.
 #include "ibpm_device.h"
 
 namespace ibpm {
@@ -71,7 +73,7 @@ __device__ __forceinline__ void half_sweep(double (&x)[kRows][2], const double (
 
 __global__ __launch_bounds__(kWaves * 64) void gs_helmholtz_kernel(const GsBatch bt, int H) {
   const GsPlane& pl = bt.p[blockIdx.y];
-  const int Nx = bt.Nx, Ny = bt.Ny, out_w = kTile - 2 * H;
+  const int Nx = bt.Nx, Ny = bt.Ny, ld = bt.ld, out_w = kTile - 2 * H;
   const int tx = blockIdx.x % bt.ntx, ty = blockIdx.x / bt.ntx;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -93,7 +95,7 @@ __global__ __launch_bounds__(kWaves * 64) void gs_helmholtz_kernel(const GsBatch
   double x[kRows][2], bb[kRows][2];
 #pragma unroll
   for (int r = 0; r < kRows; ++r) {
-    const size_t row = (size_t)min(max(j0 + r, 0), Ny - 1) * Nx;
+    const size_t row = (size_t)min(max(j0 + r, 0), Ny - 1) * ld;
     const double v0 = pl.b[row + ic0], v1 = pl.b[row + ic1];
     const bool rk = (rok >> r) & 1u;
     bb[r][0] = rk && cok[0] ? inv_d * v0 : 0.0;
@@ -102,7 +104,7 @@ __global__ __launch_bounds__(kWaves * 64) void gs_helmholtz_kernel(const GsBatch
   if (pl.xin) {
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
-      const size_t row = (size_t)min(max(j0 + r, 0), Ny - 1) * Nx;
+      const size_t row = (size_t)min(max(j0 + r, 0), Ny - 1) * ld;
       const double v0 = pl.xin[row + ic0], v1 = pl.xin[row + ic1];
       const bool rk = (rok >> r) & 1u;
       x[r][0] = rk && cok[0] ? v0 : 0.0;
@@ -149,7 +151,7 @@ __global__ __launch_bounds__(kWaves * 64) void gs_helmholtz_kernel(const GsBatch
   if (pl.base) {
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
-      const size_t row = (size_t)min(max(j0 + r, 0), Ny - 1) * Nx;
+      const size_t row = (size_t)min(max(j0 + r, 0), Ny - 1) * ld;
       x[r][0] = pl.base[row + ic0] - x[r][0];
       x[r][1] = pl.base[row + ic1] - x[r][1];
     }
@@ -160,7 +162,7 @@ __global__ __launch_bounds__(kWaves * 64) void gs_helmholtz_kernel(const GsBatch
   for (int r = 0; r < kRows; ++r) {
     const int lr = kRows * w + r;
     if (!((rok >> r) & 1u) || lr < H || lr >= kTile - H) continue;
-    const size_t row = (size_t)(j0 + r) * Nx;
+    const size_t row = (size_t)(j0 + r) * ld;
     if (cw[0]) pl.out[row + i0] = x[r][0];
     if (cw[1]) pl.out[row + i0 + 1] = x[r][1];
   }
@@ -175,6 +177,8 @@ hipError_t launch_gs_helmholtz(GsBatch& bt, int nplanes, hipStream_t st) {
     H = bt.p[p].nsweep > H ? bt.p[p].nsweep : H;
   }
   H += H & 1;  // even: tile origins stay even, so colours are (column + row) & 1 in a tile
+  if (bt.ld == 0) bt.ld = bt.Nx;
+  if (bt.ld < bt.Nx || bt.Nx < 1 || bt.Ny < 1) return hipErrorInvalidValue;
   const int out_w = kTile - 2 * H;
   bt.ntx = (bt.Nx + out_w - 1) / out_w;
   bt.nty = (bt.Ny + out_w - 1) / out_w;

@@ -68,6 +68,9 @@ struct SolveJob {
   const double* base;
   double a, c;
   double* spec = nullptr;  // FP64: the solve's spectral plane, unscaled (PassPlane::spec)
+  // Gauss-Seidel only: solve on vertices [ri0, ri0 + rw) x [rj0, rj0 + rh) with x = 0
+  // around them, writing only there (rw = 0: the whole plane)
+  int ri0 = 0, rj0 = 0, rw = 0, rh = 0;
 };
 
 }  // namespace
@@ -272,27 +275,35 @@ int solve_gs(ibpm_solver* s, const SolveJob* jobs, int nj) {
   for (int i = 0; i < nj; ++i) {
     m[i] = gs_sweeps(jobs[i]);
     done[i] = 0;
+    if (jobs[i].rw != jobs[0].rw || jobs[i].rh != jobs[0].rh || jobs[i].ri0 != jobs[0].ri0 ||
+        jobs[i].rj0 != jobs[0].rj0)
+      return fail(IBPM_E_ARG, "Gauss-Seidel solves of one launch share their region");
   }
+  // a region: plane pointers offset to its first vertex, colours kept those of the plane
+  const bool region = jobs[0].rw > 0;
+  const size_t o = region ? (size_t)jobs[0].ri0 + (size_t)jobs[0].rj0 * s->g.Nx : 0;
+  const int coff = region ? (jobs[0].ri0 + jobs[0].rj0) & 1 : 0;
   for (int round = 0;; ++round) {
     GsBatch bt;
     memset(&bt, 0, sizeof bt);
-    bt.Nx = s->g.Nx;
-    bt.Ny = s->g.Ny;
+    bt.Nx = region ? jobs[0].rw : s->g.Nx;
+    bt.Ny = region ? jobs[0].rh : s->g.Ny;
+    bt.ld = s->g.Nx;
     int np = 0;
     for (int i = 0; i < nj; ++i) {
       if (done[i] >= m[i]) continue;
       const SolveJob& j = jobs[i];
       const double k = j.dnum / j.dden, d = j.da + 4.0 * k;
       GsPlane& q = bt.p[np++];
-      q.b = j.b;
+      q.b = j.b + o;
       q.c = k / d;
       q.inv_d = 1.0 / d;
       q.nsweep = std::min(kGsHalo, m[i] - done[i]);
-      q.color0 = done[i] & 1;
-      q.xin = done[i] ? ((round - 1) & 1 ? s->U : s->T) + i * P : nullptr;
+      q.color0 = (done[i] & 1) ^ coff;
+      q.xin = done[i] ? ((round - 1) & 1 ? s->U : s->T) + i * P + o : nullptr;
       const bool last = done[i] + q.nsweep == m[i];
-      q.out = last ? j.x : ((round & 1) ? s->U : s->T) + i * P;
-      q.base = last && j.epi == EPI_SUB ? j.base : nullptr;
+      q.out = (last ? j.x : ((round & 1) ? s->U : s->T) + i * P) + o;
+      q.base = last && j.epi == EPI_SUB ? j.base + o : nullptr;
       done[i] += q.nsweep;
     }
     if (!np) break;
@@ -1140,6 +1151,14 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
     sa.ywin = s->ywin;
     sa.f[0] = s->Ft;
     sa.out[0] = s->rhs;
+    // with the box update, Ainv1 rot(E^T Ft) from zero vanishes outside the level-1 box
+    // (the Gauss-Seidel support argument): rot and the solve run on the box only, and
+    // Gamma_1 = Gamma_s1 elsewhere
+    const bool pbox = s->box_on && !spec1;
+    if (pbox) {
+      sa.ri0 = s->bi0[0], sa.rj0 = s->bj0[0], sa.rw = s->bw[0], sa.rh = s->bh[0];
+      HIPCHK(launch_copy(s->G, s->Gs, P, s->st));
+    }
     HIPCHK(launch_scatter_rot(sa, s->st));
     SolveJob j;
     j.b = s->rhs;
@@ -1152,6 +1171,7 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
     j.a = 1.0;
     j.c = cA(s, 0);
     j.spec = spec1 ? s->G1 : nullptr;  // the same operator as level 1's solve above
+    if (pbox) j.ri0 = s->bi0[0], j.rj0 = s->bj0[0], j.rw = s->bw[0], j.rh = s->bh[0];
     rc = solve(s, &j, 1);
     if (rc) return rc;
     // S Gamma_1 S of the projected circulation: that of Gamma_s minus the correction's

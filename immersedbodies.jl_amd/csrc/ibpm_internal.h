@@ -256,6 +256,8 @@ struct ScatterArgs {
   const double* f[64];
   void* out[64];
   int out_f32;
+  int ri0, rj0, rw, rh;  // vertices written: [ri0, ri0 + rw) x [rj0, rj0 + rh) (0-based);
+                         // rw = 0: the whole plane
 };
 hipError_t launch_scatter_rot(const ScatterArgs& a, hipStream_t st);
 
@@ -320,7 +322,8 @@ struct GsPlane {
 };
 struct GsBatch {
   GsPlane p[kMaxPlanes];
-  int Nx, Ny, ntx, nty;  // plane shape; tile grid (filled by the launcher)
+  int Nx, Ny, ntx, nty;  // region solved (x = 0 outside it); tile grid (by the launcher)
+  int ld;                // row pitch of every plane pointer (0: Nx)
 };
 hipError_t launch_gs_helmholtz(GsBatch& bt, int nplanes, hipStream_t st);
 
