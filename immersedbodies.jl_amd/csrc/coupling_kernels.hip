@@ -246,23 +246,49 @@ __global__ __launch_bounds__(256) void scatter_rot_kernel(const ScatterArgs a) {
   const int s0 = a.xstart[c0], s1 = reach ? a.xstart[c1] : s0;
   double ua = 0.0, ub = 0.0, va = 0.0, vb = 0.0;
   const double* f = a.f[rhs];
-  for (int s = s0; s < s1; ++s) {
-    const int p = a.perm[s];
-    const int ix = a.coords[2 * p], iy = a.coords[2 * p + 1];
-    const int db = J - iy + 3;  // b index of row J; row J+1 has db+1
-    if (db < -1 || db > 6) continue;
-    const double* w = a.w + (size_t)p * 98;
-    const double fx = f[p], fy = f[n + p];
-    const int au = I + 1 - ix + 3;  // a index of column I+1
-    const int av = I - ix + 3;      // a index of column I
-    if (au >= 0 && au <= 6) {
-      if (db >= 0) ua += w[db * 7 + au] * fx;
-      if (db + 1 <= 6) {
-        ub += w[(db + 1) * 7 + au] * fx;
-        va += w[49 + (db + 1) * 7 + au] * fy;
-      }
+  // panels in batches of 8: each batch's loads (sort order, cells, forces, weights) are
+  // issued together, then the terms are added in panel order as one at a time would
+  constexpr int PB = 8;
+  for (int sb = s0; sb < s1; sb += PB) {
+    int pp[PB];
+#pragma unroll
+    for (int q = 0; q < PB; ++q) pp[q] = a.perm[sb + q < s1 ? sb + q : s0];
+    int ix[PB], iy[PB];
+    double fx[PB], fy[PB];
+#pragma unroll
+    for (int q = 0; q < PB; ++q) {
+      ix[q] = a.coords[2 * pp[q]];
+      iy[q] = a.coords[2 * pp[q] + 1];
+      fx[q] = f[pp[q]];
+      fy[q] = f[n + pp[q]];
     }
-    if (av >= 0 && av <= 6 && db + 1 <= 6) vb += w[49 + (db + 1) * 7 + av] * fy;
+    double wua[PB], wub[PB], wva[PB], wvb[PB];
+    bool tua[PB], tub[PB], tvb[PB];
+#pragma unroll
+    for (int q = 0; q < PB; ++q) {
+      const int db = J - iy[q] + 3;   // b index of row J; row J+1 has db+1
+      const int au = I + 1 - ix[q] + 3;  // a index of column I+1
+      const int av = I - ix[q] + 3;      // a index of column I
+      const bool in = sb + q < s1 && db >= -1 && db <= 6;
+      const bool cu = in && au >= 0 && au <= 6;
+      tua[q] = cu && db >= 0;
+      tub[q] = cu && db + 1 <= 6;  // also the va term
+      tvb[q] = in && av >= 0 && av <= 6 && db + 1 <= 6;
+      const double* w = a.w + (size_t)pp[q] * 98;
+      wua[q] = w[tua[q] ? db * 7 + au : 0];
+      wub[q] = w[tub[q] ? (db + 1) * 7 + au : 0];
+      wva[q] = w[tub[q] ? 49 + (db + 1) * 7 + au : 0];
+      wvb[q] = w[tvb[q] ? 49 + (db + 1) * 7 + av : 0];
+    }
+#pragma unroll
+    for (int q = 0; q < PB; ++q) {
+      if (tua[q]) ua += wua[q] * fx[q];
+      if (tub[q]) {
+        ub += wub[q] * fx[q];
+        va += wva[q] * fy[q];
+      }
+      if (tvb[q]) vb += wvb[q] * fy[q];
+    }
   }
   const double v = (ua - ub) + (va - vb);
   const size_t idx = (size_t)(I - 1) + (size_t)(J - 1) * g.Nx;
