@@ -576,7 +576,7 @@ int box_update(ibpm_solver* s, const double* G, const double* Gs, double* F) {
 // centre Gamma_l = Gamma_s,l. Each box gets one more cell on every side.
 int setup_box_update(ibpm_solver* s, const ibpm_problem* p) {
   s->box_on = false;
-  if (s->box_opt != 0 || s->precision == IBPM_MIXED || s->n == 0 || s->moving) return IBPM_OK;
+  if (s->box_opt == 1 || s->precision == IBPM_MIXED || s->n == 0 || s->moving) return IBPM_OK;
   SolveJob j;
   j.b = nullptr;
   j.x = nullptr;
@@ -641,7 +641,7 @@ int setup_box_update(ibpm_solver* s, const ibpm_problem* p) {
   // the forward pass pair it replaces: about Nx Ny (Nx + Ny) / 4 multiply-adds per plane
   // with the twice-folded sine matrices; take the box update when it needs at most half
   const double passes = (double)L * Nx * Ny * (double)(Nx + Ny) / 4.0;
-  if (!(cost <= 0.5 * passes)) return IBPM_OK;
+  if (s->box_opt != 2 && !(cost <= 0.5 * passes)) return IBPM_OK;
   const size_t P = s->plane();
   hipError_t e = dalloc(&s->Ftr, (size_t)L * P);
   if (!e) e = dalloc(&s->Tbox, std::max<size_t>(toff, 1));
@@ -676,7 +676,7 @@ int setup_box_update(ibpm_solver* s, const ibpm_problem* p) {
     // E S_y[:, J] and S_x[I, :] (.) against the inverse pass pairs over the trial's
     // planes (levels 1 and 2)
     const double pcost = (double)Nx * Ny * s->pbh + (double)s->pbw * s->pbh * Nx;
-    if (pcost <= 0.5 * std::min(2, L) * (double)Nx * Ny * (double)(Nx + Ny) / 4.0) {
+    if (s->box_opt == 2 || pcost <= 0.5 * std::min(2, L) * (double)Nx * Ny * (double)(Nx + Ny) / 4.0) {
       e = dalloc(&s->Pa, (size_t)kPsiSlicesA * Nx * s->pbh);
       if (!e) e = dalloc(&s->T2, (size_t)Nx * s->pbh);
       if (!e) e = dalloc(&s->Pb, (size_t)kPsiSlicesB * s->pbw * s->pbh);
@@ -1334,7 +1334,7 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   s->cg_precond = opt.cg_precond ? 1 : 0;
   s->cg_precond_refresh = std::max(0, opt.cg_precond_refresh);
   s->helmholtz = opt.helmholtz == 1 ? 1 : 0;
-  s->box_opt = opt.box_update == 1 ? 1 : 0;
+  s->box_opt = opt.box_update == 1 || opt.box_update == 2 ? opt.box_update : 0;
   s->device = opt.device;
   s->grid_moving = p->grid_moving ? 1 : 0;
   s->hc.resize(s->L);

@@ -45,7 +45,7 @@ struct ibpm_solver {
   // box update of the final vort2flux (FP64, static bodies): the change of Gamma after the
   // trial vort2flux lies in box [bi0, bi0 + bw) x [bj0, bj0 + bh) of each level (0-based
   // plane indices; bw = 0: none), and its forward transform is added to the trial's
-  int box_opt = 0;  // 0: when it pays, 1: off
+  int box_opt = 0;  // 0: when it pays, 1: off, 2: wherever valid
   bool box_on = false;
   int bi0[16] = {}, bw[16] = {}, bj0[16] = {}, bh[16] = {}, bxfirst[16] = {};
   size_t btoff[16] = {};

@@ -184,26 +184,28 @@ hipError_t launch_coarsify(GridDims g, double* coarse, const double* fine, const
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void curl_all_kernel(GridDims g, int nlevel, double* __restrict__ q,
                                                        const double* __restrict__ psi) {
-  const int lev = blockIdx.y;
-  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (e >= (size_t)g.nq) return;
+  // block row r: u-faces of row j = r + 1 (r < ny, nx + 1 faces) or v-faces of row
+  // j = r - ny + 1 (nx faces); no index division
+  const int lev = blockIdx.z;
+  const int r = blockIdx.y;
   const int nx = g.nx, ny = g.ny;
+  const int i = blockIdx.x * 256 + threadIdx.x + 1;
+  const bool u = r < ny;
+  if (i > (u ? nx + 1 : nx)) return;
   const double* p = psi + (size_t)lev * g.nG;
   const double* pc = lev + 1 < nlevel ? psi + (size_t)(lev + 1) * g.nG : nullptr;
-  double v;
-  if (e < (size_t)g.nu) {
-    const int i = (int)(e % (nx + 1)) + 1, j = (int)(e / (nx + 1)) + 1;
-    v = curl_u(p, pc, nx, ny, i, j);
+  double* ql = q + (size_t)lev * g.nq;
+  if (u) {
+    const int j = r + 1;
+    ql[(size_t)(i - 1) + (size_t)(j - 1) * (nx + 1)] = curl_u(p, pc, nx, ny, i, j);
   } else {
-    const size_t f = e - g.nu;
-    const int i = (int)(f % nx) + 1, j = (int)(f / nx) + 1;
-    v = curl_v(p, pc, nx, ny, i, j);
+    const int j = r - ny + 1;
+    ql[(size_t)g.nu + (size_t)(i - 1) + (size_t)(j - 1) * nx] = curl_v(p, pc, nx, ny, i, j);
   }
-  q[(size_t)lev * g.nq + e] = v;
 }
 
 hipError_t launch_curl_all(GridDims g, int nlevel, double* q, const double* psi, hipStream_t st) {
-  dim3 grid((g.nq + 255) / 256, nlevel, 1);
+  dim3 grid((g.nx + 1 + 255) / 256, 2 * g.ny + 1, nlevel);
   hipLaunchKernelGGL(curl_all_kernel, grid, dim3(256), 0, st, g, nlevel, q, psi);
   return hipGetLastError();
 }

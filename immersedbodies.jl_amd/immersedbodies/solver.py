@@ -56,7 +56,8 @@ class HipCnabSolver(AbstractSolver):
                  transforms elsewhere; "sine" always by the sine transforms
       box_update: "auto" (FP64, static bodies, Gauss-Seidel on level 1): the final
                   vort2flux of a step transforms only the change of Gamma since the
-                  trial one, which is zero outside a box around the bodies; "off"
+                  trial one, which is zero outside a box around the bodies; "off";
+                  "force": wherever valid, cheaper or not (tests)
     """
 
     def __init__(self, state: State, *, precision="fp64", refine_sweeps=2, cg=None,
@@ -99,7 +100,7 @@ class HipCnabSolver(AbstractSolver):
         o.cg_precond = int(bool(cg_precond))
         o.cg_precond_refresh = int(cg_precond_refresh)
         o.helmholtz = {"auto": 0, "sine": 1}[helmholtz]
-        o.box_update = {"auto": 0, "off": 1}[box_update]
+        o.box_update = {"auto": 0, "off": 1, "force": 2}[box_update]
         h = C.c_void_p()
         _lib.check(L.ibpm_create(C.byref(p), C.byref(o), C.byref(h)))
         self._h = h

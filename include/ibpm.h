@@ -116,7 +116,7 @@ typedef struct ibpm_options {
                          the forward transform of the change of Gamma since the trial
                          one (zero outside a box around the bodies) to the trial's,
                          instead of transforming Gamma again, when that is cheaper;
-                         1 off                                                       */
+                         1 off; 2 wherever valid, cheaper or not (tests)           */
 } ibpm_options;
 
 typedef struct ibpm_step_stats {

@@ -28,11 +28,16 @@ def _compare_state(solver, orc, field_tol=FIELD_TOL, force_tol=FORCE_TOL, fields
     return errs
 
 
+@pytest.mark.parametrize("box", ["auto", "force"])
 @pytest.mark.parametrize("builder", [cases.cylinder, cases.two_lines])
-def test_static_fp64_matches_oracle(builder):
+def test_static_fp64_matches_oracle(builder, box):
+    """box "force": the box updates of the final vort2flux and the trial psi on the panels'
+    box run on these small grids too (where they would not pay)."""
     prob = builder()
-    print("case", builder.__name__, flush=True)
-    solver = ib.create_solver(prob)
+    print("case", builder.__name__, box, flush=True)
+    solver = ib.create_solver(prob, box_update=box)
+    if box == "force":
+        assert solver.box_update(1) is not None
     orc = cases.oracle_for(prob)
     nsteps = 6
     stats = solver.advance(nsteps)
@@ -107,6 +112,14 @@ def test_cylinder_200_full_run_matches_oracle():
     print("fp64 force rel err max", err.max(), "final drag", f64[-1, 0], flush=True)
     assert err.max() <= FORCE_TOL
     _compare_state(fp64, orc, fields=["Gamma", "psi", "q", "Ftilde", "fb"])
+    # the box updates forced on (they do not pay at this size): the same history
+    boxed = ib.create_solver(prob, box_update="force")
+    assert boxed.box_update(1) is not None and boxed.box_update(0) is not None
+    fbx = boxed.forces(boxed.advance(nsteps))
+    errb = np.linalg.norm(fbx - f_ref, axis=1) / np.linalg.norm(f_ref, axis=1)
+    print("box-update force rel err max", errb.max(), flush=True)
+    assert errb.max() <= FORCE_TOL
+    _compare_state(boxed, orc, fields=["Gamma", "psi", "q", "Ftilde", "fb"])
     mixed = ib.create_solver(prob, precision="mixed")
     fm = mixed.forces(mixed.advance(nsteps))
     errm = np.linalg.norm(fm - f_ref, axis=1) / np.linalg.norm(f_ref, axis=1)
@@ -122,7 +135,9 @@ def test_moving_grid_matches_oracle(precision):
     the base flux against the oracle."""
     prob = cases.rotating_frame()
     print("case rotating_frame", precision, flush=True)
-    solver = ib.create_solver(prob, precision=precision)
+    # FP64: the box updates forced (a static ellipse in the moving frame)
+    solver = ib.create_solver(prob, precision=precision,
+                              box_update="force" if precision == "fp64" else "auto")
     orc = cases.oracle_for(prob)
     stats = solver.advance(8)
     f_ref = cases.oracle_advance(orc, prob, 8)
@@ -133,12 +148,13 @@ def test_moving_grid_matches_oracle(precision):
         _compare_state(solver, orc, fields=["Gamma", "psi", "q", "nonlin1", "Ftilde", "fb"])
 
 
-def test_naca_airfoil_matches_oracle():
+@pytest.mark.parametrize("box", ["auto", "force"])
+def test_naca_airfoil_matches_oracle(box):
     """NACA 0012 at 20 degrees built by the Curves module (the reference's NACA example):
-    fields and forces against the oracle."""
+    fields and forces against the oracle, with and without the forced box updates."""
     prob = cases.naca_airfoil()
-    print("case naca_airfoil 0012 20deg", flush=True)
-    solver = ib.create_solver(prob)
+    print("case naca_airfoil 0012 20deg", box, flush=True)
+    solver = ib.create_solver(prob, box_update=box)
     orc = cases.oracle_for(prob)
     stats = solver.advance(5)
     f_ref = cases.oracle_advance(orc, prob, 5)
