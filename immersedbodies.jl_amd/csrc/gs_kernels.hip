@@ -13,7 +13,8 @@
 // kGsHalo half-sweeps on it without leaving registers. Values on the tile's edge use
 // wrong neighbours for those outside the tile; a half-sweep moves that error one cell
 // inward, so the centre (128 - 2H)^2, H cells from the edge, is exact and is what the
-// tile writes (H: the launch's largest half-sweep count, rounded up to even).
+// tile writes (H: the plane's half-sweep count in this launch, rounded up to even; each
+// plane of a launch has its own tile grid).
 // Horizontal neighbours come from the adjacent lane by DPP wave shifts, vertical ones
 // from the thread's own registers, and across waves through a small LDS row exchange.
 // The colour of vertex (i, j) is (i + j) & 1 in the indices of the region solved; tile
@@ -71,10 +72,11 @@ __device__ __forceinline__ void half_sweep(double (&x)[kRows][2], const double (
   }
 }
 
-__global__ __launch_bounds__(kWaves * 64) void gs_helmholtz_kernel(const GsBatch bt, int H) {
+__global__ __launch_bounds__(kWaves * 64) void gs_helmholtz_kernel(const GsBatch bt) {
   const GsPlane& pl = bt.p[blockIdx.y];
-  const int Nx = bt.Nx, Ny = bt.Ny, ld = bt.ld, out_w = kTile - 2 * H;
-  const int tx = blockIdx.x % bt.ntx, ty = blockIdx.x / bt.ntx;
+  const int Nx = bt.Nx, Ny = bt.Ny, ld = bt.ld, H = pl.H, out_w = kTile - 2 * H;
+  if ((int)blockIdx.x >= pl.ntx * pl.nty) return;  // whole work-group: barriers stay uniform
+  const int tx = blockIdx.x % pl.ntx, ty = blockIdx.x / pl.ntx;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i0 = tx * out_w - H + 2 * lane;  // this lane's first column (even)
@@ -171,19 +173,21 @@ __global__ __launch_bounds__(kWaves * 64) void gs_helmholtz_kernel(const GsBatch
 }  // namespace
 
 hipError_t launch_gs_helmholtz(GsBatch& bt, int nplanes, hipStream_t st) {
-  int H = 0;
-  for (int p = 0; p < nplanes; ++p) {
-    if (bt.p[p].nsweep < 1 || bt.p[p].nsweep > kGsHalo) return hipErrorInvalidValue;
-    H = bt.p[p].nsweep > H ? bt.p[p].nsweep : H;
-  }
-  H += H & 1;  // even: tile origins stay even, so colours are (column + row) & 1 in a tile
   if (bt.ld == 0) bt.ld = bt.Nx;
-  if (bt.ld < bt.Nx || bt.Nx < 1 || bt.Ny < 1) return hipErrorInvalidValue;
-  const int out_w = kTile - 2 * H;
-  bt.ntx = (bt.Nx + out_w - 1) / out_w;
-  bt.nty = (bt.Ny + out_w - 1) / out_w;
-  hipLaunchKernelGGL(gs_helmholtz_kernel, dim3(bt.ntx * bt.nty, nplanes), dim3(kWaves * 64), 0, st,
-                     bt, H);
+  if (bt.ld < bt.Nx || bt.Nx < 1 || bt.Ny < 1 || nplanes < 1) return hipErrorInvalidValue;
+  int tiles = 0;
+  for (int p = 0; p < nplanes; ++p) {
+    GsPlane& q = bt.p[p];
+    if (q.nsweep < 1 || q.nsweep > kGsHalo) return hipErrorInvalidValue;
+    // even: tile origins stay even, so colours are (column + row) & 1 in a tile
+    q.H = q.nsweep + (q.nsweep & 1);
+    const int out_w = kTile - 2 * q.H;
+    q.ntx = (bt.Nx + out_w - 1) / out_w;
+    q.nty = (bt.Ny + out_w - 1) / out_w;
+    tiles = q.ntx * q.nty > tiles ? q.ntx * q.nty : tiles;
+  }
+  bt.ntx = bt.nty = 0;
+  hipLaunchKernelGGL(gs_helmholtz_kernel, dim3(tiles, nplanes), dim3(kWaves * 64), 0, st, bt);
   return hipGetLastError();
 }
 

@@ -311,7 +311,7 @@ hipError_t launch_copy(double* dst, const double* src, size_t n, hipStream_t st)
 // neighbours) on each plane, starting from xin (zero when null) with colour color0
 // ((i + j) & 1 of the points updated first), and writes x, or base - x when base is set,
 // to out (which must not alias b, xin or base).
-constexpr int kGsHalo = 16;
+constexpr int kGsHalo = 24;
 struct GsPlane {
   const double* b;
   const double* xin;
@@ -319,10 +319,11 @@ struct GsPlane {
   const double* base;
   double c, inv_d;
   int nsweep, color0;
+  int H, ntx, nty;  // halo (nsweep rounded up to even) and tile grid (by the launcher)
 };
 struct GsBatch {
   GsPlane p[kMaxPlanes];
-  int Nx, Ny, ntx, nty;  // region solved (x = 0 outside it); tile grid (by the launcher)
+  int Nx, Ny, ntx, nty;  // region solved (x = 0 outside it); ntx, nty unused
   int ld;                // row pitch of every plane pointer (0: Nx)
 };
 hipError_t launch_gs_helmholtz(GsBatch& bt, int nplanes, hipStream_t st);
