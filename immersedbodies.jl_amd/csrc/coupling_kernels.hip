@@ -177,6 +177,40 @@ __global__ __launch_bounds__(1024) void body_sort_kernel(const BodyDev b) {
   }
 }
 
+// Range of the panels' flux cells: out = (min ix, max ix, min iy, max iy), one work-group.
+__global__ __launch_bounds__(256) void body_bbox_kernel(const int* __restrict__ coords, int n,
+                                                        double* __restrict__ out) {
+  __shared__ int red[4][256];
+  int v[4] = {1 << 30, -(1 << 30), 1 << 30, -(1 << 30)};
+  for (int p = threadIdx.x; p < n; p += 256) {
+    const int ix = coords[2 * p], iy = coords[2 * p + 1];
+    v[0] = min(v[0], ix);
+    v[1] = max(v[1], ix);
+    v[2] = min(v[2], iy);
+    v[3] = max(v[3], iy);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const int o = threadIdx.x + w;
+      red[0][threadIdx.x] = min(red[0][threadIdx.x], red[0][o]);
+      red[1][threadIdx.x] = max(red[1][threadIdx.x], red[1][o]);
+      red[2][threadIdx.x] = min(red[2][threadIdx.x], red[2][o]);
+      red[3][threadIdx.x] = max(red[3][threadIdx.x], red[3][o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[threadIdx.x] = (double)red[threadIdx.x][0];
+}
+
+hipError_t launch_body_bbox(const int* coords, int n, double* out, hipStream_t st) {
+  if (n <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(body_bbox_kernel, dim3(1), dim3(256), 0, st, coords, n, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_body_sort(const BodyDev& b, hipStream_t st) {
   if (b.npanel == 0) return hipSuccess;
   hipLaunchKernelGGL(body_sort_kernel, dim3(1), dim3(1024), 0, st, b);

@@ -85,7 +85,7 @@ void free_all(ibpm_solver* s) {
                   s->body_of, s->moving_d, s->coords, s->perm, s->xstart, s->err, s->w,
                   s->ywin, s->gemv_part, s->pc_P, s->pc_theta0, s->pc_t, s->pc_z,
                   s->Binv, s->Gtmp, s->psitmp, s->Sc, s->cg_r, s->cg_u, s->cg_c, s->cg_scal,
-                  s->F1, s->G1, s->Sfx, s->Sfy != s->Sfx ? s->Sfy : nullptr, s->Ftr, s->Tbox, s->Pa, s->Pb, s->T2,
+                  s->F1, s->G1, s->Sfx, s->Sfy != s->Sfx ? s->Sfy : nullptr, s->Ftr, s->Tbox, s->Pa, s->Pb, s->T2, s->bbox_d,
                   s->edge, s->pw, s->fac, s->rvec, s->rpart, s->Mx,
                   s->My != s->Mx ? s->My : nullptr,
                   s->ws_h0, s->ws_h1, s->ws_xe,
@@ -341,8 +341,8 @@ int solve(ibpm_solver* s, const SolveJob* jobs, int nj) {
 // (else the scratch plane U[p]).
 int psi_box_product(ibpm_solver* s, const double* E, double* psi);
 
-// psi_box (FP64, ng = 1): instead of the inverse passes, only level 1's psi on the trial
-// box (psi_box_product); the rest of out[] is left as it was.
+// psi_box (FP64): instead of the inverse passes, only level 1's psi of every column on the
+// psi box (psi_box_product); the rest of out[] is left as it was.
 int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double* const* out,
               int epi, int nout, const double* const* spec_in = nullptr,
               double* const* spec_out = nullptr, bool psi_box = false) {
@@ -407,7 +407,13 @@ int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double
     HIPCHK(launch_ring_sweep(ra, ng, l > 0, s->st));
     if (l > 0) HIPCHK(launch_ring_factor(ra, ng, s->st));
   }
-  if (psi_box) return psi_box_product(s, s->T, out[0]);
+  if (psi_box) {
+    for (int c = 0; c < ng; ++c) {
+      rc = psi_box_product(s, s->T + (size_t)c * L * P, out[c * L]);
+      if (rc) return rc;
+    }
+    return IBPM_OK;
+  }
   // inverse passes of the levels the caller reads (l < nout of every column)
   memset(&pb, 0, sizeof pb);
   int no = 0;
@@ -441,7 +447,7 @@ int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double
 // transform S Gamma_1 S of column 0's finest level, known from the solve that produced
 // it (S S = 2 n I makes a sine solve's spectral plane that transform), so its two
 // forward passes are skipped.
-int box_update(ibpm_solver* s, const double* G, const double* Gs, double* F);
+int box_update(ibpm_solver* s, const double* G, const double* Gs, double* F, bool accumulate);
 
 // fwd (FP64, column 0): the L spectral planes of column 0; with fwd_given they hold those
 // of the trial vort2flux of the step, which box_update turns into this call's (Gc[0] minus
@@ -462,7 +468,7 @@ int vort2flux(ibpm_solver* s, int ncol, double* const* Gc, double* const* psic,
       HIPCHK(launch_coarsify(s->g, Gc[c] + l * P, Gc[c] + (l - 1) * P,
                              outer && outer[c] ? outer[c] + l * P : nullptr, s->st));
   if (fwd_given) {
-    int rc = box_update(s, Gc[0], outer[0], fwd);
+    int rc = box_update(s, Gc[0], outer[0], fwd, true);
     if (rc) return rc;
   }
   const double* sin0[kMaxPlanes] = {};
@@ -520,12 +526,13 @@ int vort2flux(ibpm_solver* s, int ncol, double* const* Gc, double* const* psic,
 
 double cA(const ibpm_solver* s, int l) { return s->dt / (2.0 * s->Re * (s->hc[l] * s->hc[l])); }
 
-// Forward transform of the change D_l = G_l - Gs_l of each level, which vanishes outside
-// box l, added to F_l: F_l += S_x[:, I] D_l[I, J] S_y[J, :], as two products with the sine
+// Forward transform of the change D_l = G_l - Gs_l of each level (of G_l when Gs is null),
+// which vanishes outside box l, added to F_l (or stored there when not accumulate):
+// F_l += S_x[:, I] D_l[I, J] S_y[J, :], as two products with the sine
 // columns of the box, in the cheaper order (the x contraction first: an Nx x |J|
 // intermediate; else a |I| x Ny one). All levels' first products in one launch, then all
 // second ones.
-int box_update(ibpm_solver* s, const double* G, const double* Gs, double* F) {
+int box_update(ibpm_solver* s, const double* G, const double* Gs, double* F, bool accumulate) {
   const size_t P = s->plane();
   const int Nx = s->g.Nx, Ny = s->g.Ny;
   GemmBatch g1, g2;
@@ -535,7 +542,7 @@ int box_update(ibpm_solver* s, const double* G, const double* Gs, double* F) {
   for (int l = 0; l < s->L; ++l) {
     if (s->bw[l] <= 0 || s->bh[l] <= 0) continue;
     const size_t o = (size_t)s->bi0[l] + (size_t)s->bj0[l] * Nx;
-    const double *Gl = G + l * P + o, *Gsl = Gs + l * P + o;
+    const double *Gl = G + l * P + o, *Gsl = Gs ? Gs + l * P + o : nullptr;
     double* T = s->Tbox + s->btoff[l];
     GemmJob& a = g1.j[nj];
     GemmJob& b = g2.j[nj];
@@ -561,22 +568,25 @@ int box_update(ibpm_solver* s, const double* G, const double* Gs, double* F) {
     }
     b.C = F + l * P, b.ldc = Nx;
     b.M = Nx, b.N = Ny;
-    b.accumulate = 1;
+    b.accumulate = accumulate ? 1 : 0;
   }
   HIPCHK(launch_gemm_f64(g1, nj, s->st));
   HIPCHK(launch_gemm_f64(g2, nj, s->st));
   return IBPM_OK;
 }
 
-// The boxes of box_update (static bodies, FP64, Gauss-Seidel on level 1). A panel in flux
-// cell (ix, iy) feeds rot(E^T f) on vertices [ix - 4, ix + 3] x [iy - 4, iy + 3]
+// The boxes of the box updates (FP64, Gauss-Seidel on level 1). A panel in flux cell
+// (ix, iy) feeds rot(E^T f) on vertices [ix - 4, ix + 3] x [iy - 4, iy + 3]
 // (scatter_rot_kernel); m Gauss-Seidel half-sweeps from zero move the support of
-// Ainv1 rot(E^T f), hence of Gamma_1 - Gamma_s1, by at most m cells; coarsify! maps fine
-// vertices 2 + 2a + {-1, 0, 1} to the coarse vertex ic0 + a of the centre, and outside the
-// centre Gamma_l = Gamma_s,l. Each box gets one more cell on every side.
-int setup_box_update(ibpm_solver* s, const ibpm_problem* p) {
-  s->box_on = false;
-  if (s->box_opt == 1 || s->precision == IBPM_MIXED || s->n == 0 || s->moving) return IBPM_OK;
+// Ainv1 rot(E^T f), hence of Gamma_1 - Gamma_s1 and of B's input, by at most m cells;
+// coarsify! maps fine vertices 2 + 2a + {-1, 0, 1} to the coarse vertex ic0 + a of the
+// centre, and outside the centre Gamma_l = Gamma_s,l (B's input: zero). Each box gets
+// one more cell on every side.
+
+// the level-1 Helmholtz job of project_circ and B; its Gauss-Seidel sweep count (0: the
+// box updates do not apply)
+int box_sweeps(const ibpm_solver* s) {
+  if (s->box_opt == 1 || s->precision == IBPM_MIXED || s->n == 0) return 0;
   SolveJob j;
   j.b = nullptr;
   j.x = nullptr;
@@ -587,22 +597,34 @@ int setup_box_update(ibpm_solver* s, const ibpm_problem* p) {
   j.base = nullptr;
   j.a = 1.0;
   j.c = cA(s, 0);
-  if (!use_gs(s, j)) return IBPM_OK;
-  const int m = gs_sweeps(j);
-  const int Nx = s->g.Nx, Ny = s->g.Ny, L = s->L;
-  int ilo = 1 << 30, ihi = -(1 << 30), jlo = 1 << 30, jhi = -(1 << 30);
-  for (int k = 0; k < s->n; ++k) {
-    const int ix = (int)rint((p->xb[k] - s->x0) / s->h + 1.0 - 0.25);
-    const int iy = (int)rint((p->xb[s->n + k] - s->y0) / s->h + 1.0 - 0.25);
-    ilo = std::min(ilo, ix - 4);
-    ihi = std::max(ihi, ix + 3);
-    jlo = std::min(jlo, iy - 4);
-    jhi = std::max(jhi, iy + 3);
+  return use_gs(s, j) ? gs_sweeps(j) : 0;
+}
+
+// grow-only device buffer (callers are outside graph capture: creation, moving-body steps)
+hipError_t ensure_buffer(double** p, size_t* cap, size_t n) {
+  if (n <= *cap && *p) return hipSuccess;
+  if (*p) {
+    hipError_t e = hipFree(*p);
+    if (e != hipSuccess) return e;
+    *p = nullptr;
   }
+  *cap = 0;
+  hipError_t e = dalloc(p, std::max<size_t>(n, 1));
+  if (e == hipSuccess) *cap = std::max<size_t>(n, 1);
+  return e;
+}
+
+// Boxes for panel cells whose rot reaches vertices [ilo, ihi] x [jlo, jhi]; sets box_on
+// and pbox_on by the cost rule (or box_opt 2) and sizes the scratch.
+int compute_boxes(ibpm_solver* s, int ilo, int ihi, int jlo, int jhi) {
+  s->box_on = s->pbox_on = false;
+  const int m = box_sweeps(s);
+  if (m == 0) return IBPM_OK;
+  const int Nx = s->g.Nx, Ny = s->g.Ny, L = s->L;
   // 1-based vertex boxes per level
   int lo[2] = {ilo - m - 1, jlo - m - 1}, hi[2] = {ihi + m + 1, jhi + m + 1};
   const CoarsifyRange cr = coarsify_range(s->g.nx, s->g.ny);
-  const int c0[2] = {cr.ic0, cr.jc0}, cn[2] = {cr.ni, cr.nj}, N[2] = {Nx, Ny};
+  const int c0[2] = {cr.ic0, cr.jc0}, cn[2] = {cr.ni, cr.nj};
   auto floor_div2 = [](int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); };
   double cost = 0.0;
   size_t toff = 0;
@@ -643,8 +665,8 @@ int setup_box_update(ibpm_solver* s, const ibpm_problem* p) {
   const double passes = (double)L * Nx * Ny * (double)(Nx + Ny) / 4.0;
   if (s->box_opt != 2 && !(cost <= 0.5 * passes)) return IBPM_OK;
   const size_t P = s->plane();
-  hipError_t e = dalloc(&s->Ftr, (size_t)L * P);
-  if (!e) e = dalloc(&s->Tbox, std::max<size_t>(toff, 1));
+  hipError_t e = s->Ftr ? hipSuccess : dalloc(&s->Ftr, (size_t)L * P);
+  if (!e) e = ensure_buffer(&s->Tbox, &s->tbox_cap, toff);
   auto sine_matrix = [](int Nn, double** out) -> hipError_t {
     const long long n = Nn + 1;
     std::vector<double> tab(2 * n), S((size_t)Nn * Nn);
@@ -655,15 +677,15 @@ int setup_box_update(ibpm_solver* s, const ibpm_problem* p) {
     if (!r) r = hipMemcpy(*out, S.data(), S.size() * 8, hipMemcpyHostToDevice);
     return r;
   };
-  if (!e) e = sine_matrix(Nx, &s->Sfx);
-  if (!e) {
+  if (!e && !s->Sfx) e = sine_matrix(Nx, &s->Sfx);
+  if (!e && !s->Sfy) {
     if (Ny == Nx) s->Sfy = s->Sfx;
     else e = sine_matrix(Ny, &s->Sfy);
   }
   if (e) return fail(IBPM_E_HIP, hipGetErrorString(e));
   s->box_on = true;
-  // the trial psi box: gather_kernel reads faces [ix - 3, ix + 3] x [iy - 3, iy + 3] of
-  // each panel, whose curl reads psi on vertices [ix - 4, ix + 3] x [iy - 4, iy + 3], i.e.
+  // the psi box: gather_kernel reads faces [ix - 3, ix + 3] x [iy - 3, iy + 3] of each
+  // panel, whose curl reads psi on vertices [ix - 4, ix + 3] x [iy - 4, iy + 3], i.e.
   // [ilo, ihi] x [jlo, jhi]; with those and one more vertex inside the level (and so no
   // face on its first row or column) no boundary value, hence no coarser level, is read.
   const int nx = s->g.nx, ny = s->g.ny;
@@ -673,18 +695,51 @@ int setup_box_update(ibpm_solver* s, const ibpm_problem* p) {
     s->pbw = ib - ia + 1;
     s->pj0 = ja - 1;
     s->pbh = jb - ja + 1;
-    // E S_y[:, J] and S_x[I, :] (.) against the inverse pass pairs over the trial's
-    // planes (levels 1 and 2)
+    // E S_y[:, J] and S_x[I, :] (.) against the inverse pass pairs over levels 1 and 2
     const double pcost = (double)Nx * Ny * s->pbh + (double)s->pbw * s->pbh * Nx;
     if (s->box_opt == 2 || pcost <= 0.5 * std::min(2, L) * (double)Nx * Ny * (double)(Nx + Ny) / 4.0) {
-      e = dalloc(&s->Pa, (size_t)kPsiSlicesA * Nx * s->pbh);
-      if (!e) e = dalloc(&s->T2, (size_t)Nx * s->pbh);
-      if (!e) e = dalloc(&s->Pb, (size_t)kPsiSlicesB * s->pbw * s->pbh);
+      e = ensure_buffer(&s->Pa, &s->pa_cap, (size_t)kPsiSlicesA * Nx * s->pbh);
+      if (!e) e = ensure_buffer(&s->T2, &s->t2_cap, (size_t)Nx * s->pbh);
+      if (!e) e = ensure_buffer(&s->Pb, &s->pb_cap, (size_t)kPsiSlicesB * s->pbw * s->pbh);
       if (e) return fail(IBPM_E_HIP, hipGetErrorString(e));
       s->pbox_on = true;
     }
   }
   return IBPM_OK;
+}
+
+// Static bodies: the boxes once, from the panel positions (the cells as body_update_kernel
+// computes them). Moving bodies: box_dyn, boxes every step (update_moving_boxes).
+int setup_box_update(ibpm_solver* s, const ibpm_problem* p) {
+  s->box_on = s->pbox_on = s->box_dyn = false;
+  if (box_sweeps(s) == 0) return IBPM_OK;
+  if (s->moving) {
+    HIPCHK(dalloc(&s->bbox_d, 4));
+    s->box_dyn = true;
+    return IBPM_OK;
+  }
+  int ilo = 1 << 30, ihi = -(1 << 30), jlo = 1 << 30, jhi = -(1 << 30);
+  for (int k = 0; k < s->n; ++k) {
+    const int ix = (int)rint((p->xb[k] - s->x0) / s->h + 1.0 - 0.25);
+    const int iy = (int)rint((p->xb[s->n + k] - s->y0) / s->h + 1.0 - 0.25);
+    ilo = std::min(ilo, ix - 4);
+    ihi = std::max(ihi, ix + 3);
+    jlo = std::min(jlo, iy - 4);
+    jhi = std::max(jhi, iy + 3);
+  }
+  return compute_boxes(s, ilo, ihi, jlo, jhi);
+}
+
+// Moving bodies: the boxes of this step from the range of the panels' cells (after
+// body_update_kernel); one small device-to-host read.
+int update_moving_boxes(ibpm_solver* s) {
+  if (!s->box_dyn) return IBPM_OK;
+  HIPCHK(launch_body_bbox(s->coords, s->n, s->bbox_d, s->st));
+  HIPCHK(hipMemcpyAsync(s->h_scal + 12, s->bbox_d, 4 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  const int ixmin = (int)s->h_scal[12], ixmax = (int)s->h_scal[13];
+  const int iymin = (int)s->h_scal[14], iymax = (int)s->h_scal[15];
+  return compute_boxes(s, ixmin - 4, ixmax + 3, iymin - 4, iymax + 3);
 }
 
 // Trial psi of level 1 on the box [pi0, pi0 + pbw) x [pj0, pj0 + pbh) from the level's
@@ -723,7 +778,10 @@ int psi_box_product(ibpm_solver* s, const double* E, double* psi) {
 }
 
 // B operator (coupling.jl:150-177) on ncol columns: x_c = E vort2flux(Ainv1 rot(E^T z_c))[1]
+int apply_B_box(ibpm_solver* s, int ncol, const double* const* z, double* const* x);
+
 int apply_B(ibpm_solver* s, int ncol, const double* const* z, double* const* x) {
+  if (s->box_on && s->pbox_on) return apply_B_box(s, ncol, z, x);
   const size_t P = s->plane();
   const int L = s->L;
   for (int c0 = 0; c0 < ncol; c0 += 64) {
@@ -788,6 +846,101 @@ int apply_B(ibpm_solver* s, int ncol, const double* const* z, double* const* x) 
   return IBPM_OK;
 }
 
+// B by the box updates: Ainv1 rot(E^T z) vanishes outside the level-1 box and coarsify!
+// carries it to the coarser boxes (the rest of every level is zero), so rot and the
+// Gauss-Seidel solve run on the box, each level's forward transform is a box product, the
+// ring chain runs as always, and E reads psi_1 on the psi box only.
+int apply_B_box(ibpm_solver* s, int ncol, const double* const* z, double* const* x) {
+  const size_t P = s->plane();
+  const int L = s->L;
+  const int r[4] = {s->bi0[0], s->bw[0], s->bj0[0], s->bh[0]};
+  if (r[0] != s->gtmp_box[0] || r[1] != s->gtmp_box[1] || r[2] != s->gtmp_box[2] ||
+      r[3] != s->gtmp_box[3] || ncol > s->gtmp_cols) {
+    // level 1 of the columns is written on the box only: clear what an earlier box left
+    for (int c = 0; c < s->bcols; ++c) HIPCHK(launch_fill(s->Gtmp + (size_t)c * L * P, P, 0.0, s->st));
+    for (int k = 0; k < 4; ++k) s->gtmp_box[k] = r[k];
+    s->gtmp_cols = s->bcols;
+  }
+  for (int c0 = 0; c0 < ncol; c0 += 64) {
+    const int nc = std::min(64, ncol - c0);
+    ScatterArgs sa;
+    memset(&sa, 0, sizeof sa);
+    sa.g = s->g;
+    sa.npanel = s->n;
+    sa.nrhs = nc;
+    sa.coords = s->coords;
+    sa.w = s->w;
+    sa.perm = s->perm;
+    sa.xstart = s->xstart;
+    sa.ywin = s->ywin;
+    sa.ri0 = r[0], sa.rw = r[1], sa.rj0 = r[2], sa.rh = r[3];
+    for (int c = 0; c < nc; ++c) {
+      sa.f[c] = z[c0 + c];
+      sa.out[c] = s->Sc + (size_t)(c0 + c) * P;
+    }
+    HIPCHK(launch_scatter_rot(sa, s->st));
+  }
+  std::vector<SolveJob> jobs(ncol);
+  for (int c = 0; c < ncol; ++c) {
+    SolveJob& j = jobs[c];
+    j.b = s->Sc + (size_t)c * P;
+    j.da = 1.0;
+    j.dnum = s->dt;
+    j.dden = 2.0 * s->Re * (s->hc[0] * s->hc[0]);
+    j.x = s->Gtmp + (size_t)c * L * P;
+    j.epi = EPI_STORE;
+    j.base = nullptr;
+    j.a = 1.0;
+    j.c = cA(s, 0);
+    j.ri0 = r[0], j.rw = r[1], j.rj0 = r[2], j.rh = r[3];
+  }
+  int rc = solve(s, jobs.data(), ncol);
+  if (rc) return rc;
+  for (int c = 0; c < ncol; ++c)
+    for (int l = 1; l < L; ++l)
+      HIPCHK(launch_coarsify(s->g, s->Gtmp + (size_t)c * L * P + l * P,
+                             s->Gtmp + (size_t)c * L * P + (l - 1) * P, nullptr, s->st));
+  const int gmax = std::max(1, kMaxPlanes / L);
+  for (int c0 = 0; c0 < ncol; c0 += gmax) {
+    const int ng = std::min(gmax, ncol - c0);
+    const double* sin[kMaxPlanes] = {};
+    double* out[kMaxPlanes];
+    const void* in[kMaxPlanes];
+    for (int c = 0; c < ng; ++c) {
+      // forward transforms of the column's levels into U: box products from zero
+      rc = box_update(s, s->Gtmp + (size_t)(c0 + c) * L * P, nullptr, s->U + (size_t)c * L * P, false);
+      if (rc) return rc;
+      for (int l = 0; l < L; ++l) {
+        sin[c * L + l] = s->U + (size_t)(c * L + l) * P;
+        in[c * L + l] = nullptr;
+        out[c * L + l] = s->psitmp + (size_t)(c0 + c) * L * P + l * P;
+      }
+    }
+    rc = v2f_group(s, ng, in, false, out, EPI_STORE, 1, sin, nullptr, true);
+    if (rc) return rc;
+  }
+  for (int c0 = 0; c0 < ncol; c0 += 64) {
+    const int nc = std::min(64, ncol - c0);
+    GatherArgs ga;
+    memset(&ga, 0, sizeof ga);
+    ga.g = s->g;
+    ga.npanel = s->n;
+    ga.nrhs = nc;
+    ga.coords = s->coords;
+    ga.w = s->w;
+    for (int c = 0; c < nc; ++c) {
+      ga.psi1[c] = s->psitmp + (size_t)(c0 + c) * L * P;
+      ga.psi2[c] = nullptr;  // the psi box keeps every read inside level 1
+      ga.out[c] = x[c0 + c];
+    }
+    memset(&ga.bf, 0, sizeof ga.bf);  // B has no base flux
+    ga.ub = nullptr;
+    ga.h = s->h;
+    HIPCHK(launch_gather(ga, s->st));
+  }
+  return IBPM_OK;
+}
+
 int alloc_bscratch(ibpm_solver* s, int ncol) {
   const size_t P = s->plane();
   if (s->Gtmp) (void)hipFree(s->Gtmp);
@@ -801,6 +954,8 @@ int alloc_bscratch(ibpm_solver* s, int ncol) {
   HIPCHK(dalloc(&s->Sc, (size_t)ncol * P));
   // coarse levels of Gtmp stay zero outside the centre region (Gamma_tmp .= 0)
   HIPCHK(hipMemsetAsync(s->Gtmp, 0, (size_t)ncol * s->L * P * sizeof(double), s->st));
+  s->gtmp_box[0] = s->gtmp_box[1] = s->gtmp_box[2] = s->gtmp_box[3] = -1;
+  s->gtmp_cols = 0;
   return IBPM_OK;
 }
 
@@ -1080,6 +1235,8 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
     BodyDev b = body_dev(s);
     HIPCHK(launch_body_update(b, 1, s->st));
     HIPCHK(launch_body_sort(b, s->st));
+    int rc = update_moving_boxes(s);
+    if (rc) return rc;
   }
   // get_trial_state (fluid.jl:467-506): nonlinear term and rhs of every level, then the
   // implicit solves of every level in one batch (levels only read the old state)
@@ -1572,15 +1729,15 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   }
   // B^-1 (static) or CG buffers (moving): coupling.jl:179-202
   if (n > 0) {
+    int rc = setup_box_update(s, p);  // the boxes (static bodies) also serve B's assembly
+    if (rc) return bail(rc);
     if (!s->moving) {
-      int rc = assemble_B_inverse(s, s->Binv);
+      rc = assemble_B_inverse(s, s->Binv);
       if (rc) return bail(rc);
       rc = alloc_bscratch(s, 0);  // static bodies never apply B again
       if (rc) return bail(rc);
-      rc = setup_box_update(s, p);
-      if (rc) return bail(rc);
     } else {
-      int rc = alloc_bscratch(s, 1);
+      rc = alloc_bscratch(s, 1);
       if (rc) return bail(rc);
       CK(dalloc(&s->cg_r, 2 * (size_t)n));
       CK(dalloc(&s->cg_u, 2 * (size_t)n));

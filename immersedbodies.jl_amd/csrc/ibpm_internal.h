@@ -260,6 +260,8 @@ struct ScatterArgs {
                          // rw = 0: the whole plane
 };
 hipError_t launch_scatter_rot(const ScatterArgs& a, hipStream_t st);
+// (min ix, max ix, min iy, max iy) of the panels' flux cells into out[0..3]
+hipError_t launch_body_bbox(const int* coords, int n, double* out, hipStream_t st);
 
 // x = M b for a dense column-major m x m matrix; part: gemv_partial_count(m) doubles.
 size_t gemv_partial_count(int m);

@@ -57,6 +57,13 @@ struct ibpm_solver {
   bool pbox_on = false;
   int pi0 = 0, pbw = 0, pj0 = 0, pbh = 0;
   double *Pa = nullptr, *Pb = nullptr, *T2 = nullptr;
+  // moving bodies: the boxes follow the panels, recomputed every step from the range of
+  // their cells (bbox_d on the device); buffer capacities grow as needed
+  bool box_dyn = false;
+  double* bbox_d = nullptr;
+  size_t tbox_cap = 0, pa_cap = 0, pb_cap = 0, t2_cap = 0;
+  int gtmp_box[4] = {-1, -1, -1, -1};  // level-1 box B last wrote into Gtmp (-1: none)
+  int gtmp_cols = 0;
   void *BfX = nullptr, *BfY = nullptr;                                  // bf16 tables
   // multi-level ring chain: sine rows/columns, per-plane factors, per-chain scratch
   double *edge = nullptr, *pw = nullptr;

@@ -60,13 +60,17 @@ def test_no_body_uniform_flow_stays_irrotational():
     _compare_state(solver, orc, fields=["q", "Gamma", "psi"])
 
 
-def test_moving_cylinder_cg_matches_oracle():
+@pytest.mark.parametrize("box", ["auto", "force"])
+def test_moving_cylinder_cg_matches_oracle(box):
+    """box "force": B and the step by the box updates, the boxes following the body."""
     prob = cases.moving_cylinder()
-    print("case moving_cylinder", flush=True)
-    solver = ib.create_solver(prob)
+    print("case moving_cylinder", box, flush=True)
+    solver = ib.create_solver(prob, box_update=box)
     orc = cases.oracle_for(prob)
     nsteps = 4
     stats = solver.advance(nsteps)
+    if box == "force":
+        assert solver.box_update(1) is not None  # the last step's boxes
     f_ref = cases.oracle_advance(orc, prob, nsteps)
     assert cases.relerr(solver.forces(stats), f_ref) <= 1e-8
     iters = solver.cg_iters(stats)
@@ -162,12 +166,13 @@ def test_naca_airfoil_matches_oracle(box):
     _compare_state(solver, orc, fields=["Gamma", "psi", "q", "Ftilde", "fb"])
 
 
-def test_plunging_ellipses_matches_oracle():
+@pytest.mark.parametrize("box", ["auto", "force"])
+def test_plunging_ellipses_matches_oracle(box):
     """Three plunging ellipses (moving bodies, CG) on 256 x 128 cells of [-4,12] x [-4,4],
-    3 levels: fields and forces against the oracle."""
+    3 levels: fields and forces against the oracle, with and without the box updates."""
     prob = cases.plunging_ellipses(nx=256, ny=128, nlevel=3, h=1.0 / 16)
-    print("case plunging_ellipses 256x128 L3", flush=True)
-    solver = ib.create_solver(prob)
+    print("case plunging_ellipses 256x128 L3", box, flush=True)
+    solver = ib.create_solver(prob, box_update=box)
     orc = cases.oracle_for(prob)
     stats = solver.advance(5)
     f_ref = cases.oracle_advance(orc, prob, 5)
@@ -194,6 +199,15 @@ def test_plunging_ellipses_full_size_warm_start_invariance():
         s.close()
     print("cg iterations", iters, flush=True)
     assert cases.relerr(forces["extrapolate"], forces["previous"]) <= 1e-7
+    # the box updates (on here: the boxes follow the bodies) against the full transforms
+    s = ib.create_solver(prob, cg=dict(maxiter=500, reltol=1e-10), cg_tol_mode=1, box_update="off")
+    s.set_warm_start("previous")
+    st = s.advance(6)
+    assert all(x.cg_converged for x in st)
+    off = s.forces(st)
+    s.close()
+    print("box on/off force rel err", cases.relerr(forces["previous"], off), flush=True)
+    assert cases.relerr(forces["previous"], off) <= 1e-7
 
 
 def test_staged_body_motion_matches_motion_array():
