@@ -111,12 +111,14 @@ typedef struct ibpm_options {
                          the error below 2^-56 max|x| (Jacobi contraction
                          rho = 4k/(da+4k) <= 0.54; the sine transforms elsewhere);
                          1 always the sine transforms (the reference's method)     */
-  int box_update;     /* FP64 with static bodies whose level-1 Helmholtz solve is
-                         Gauss-Seidel: 0 (default) the final vort2flux of a step adds
-                         the forward transform of the change of Gamma since the trial
-                         one (zero outside a box around the bodies) to the trial's,
-                         instead of transforming Gamma again, when that is cheaper;
-                         1 off; 2 wherever valid, cheaper or not (tests)           */
+  int box_update;     /* FP64 with bodies whose level-1 Helmholtz solve is Gauss-
+                         Seidel: 0 (default) the final vort2flux of a step adds the
+                         forward transform of the change of Gamma since the trial one
+                         (zero outside a box around the bodies) to the trial's instead
+                         of transforming Gamma again, the trial forms level 1's psi
+                         only where the panels read it, and B runs on the same boxes,
+                         when that is cheaper (the boxes follow moving bodies every
+                         step); 1 off; 2 wherever valid, cheaper or not (tests)    */
 } ibpm_options;
 
 typedef struct ibpm_step_stats {
