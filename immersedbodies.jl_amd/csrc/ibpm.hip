@@ -404,6 +404,9 @@ int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double
       ra.part_a[c] = s->rpart + c * npart;
       ra.part_c[c] = s->rpart + c * npart + npa;
     }
+    // E is read by the inverse passes of the levels below nout, or for level 1 only by
+    // the psi box products
+    ra.write_e = psi_box ? l == 0 : l < nout;
     HIPCHK(launch_ring_sweep(ra, ng, l > 0, s->st));
     if (l > 0) HIPCHK(launch_ring_factor(ra, ng, s->st));
   }

@@ -106,6 +106,7 @@ int dst_effective_variant(int v, int N);
 constexpr int kMaxChains = 16;
 struct RingChainArgs {
   int nx, ny, Nx, Ny;
+  int write_e;                      // 0: the level's E is not read (projections only)
   const void* F[kMaxChains];        // spectral plane of the level, per chain
   void* E[kMaxChains];              // E = D . (F + R), written (same precision as F)
   int F_f32;

@@ -86,8 +86,10 @@ __global__ __launch_bounds__(256) void ring_sweep_kernel(const RingChainArgs a, 
       if (fac) f += fuB * yv[1][yy] + fuT * yv[2][yy] + e1 * yv[3][yy] + eN * yv[4][yy];
       const double lam = -2.0 * ((cxk + yv[0][yy]) - 2.0);
       e = a.scale / lam * f;
-      if (a.F_f32) ((float*)E)[idx] = (float)e;
-      else ((double*)E)[idx] = e;
+      if (a.write_e) {
+        if (a.F_f32) ((float*)E)[idx] = (float)e;
+        else ((double*)E)[idx] = e;
+      }
       ab += e * yv[5][yy];
       at += e * yv[6][yy];
     }

@@ -54,10 +54,11 @@ class HipCnabSolver(AbstractSolver):
                  in FP64 where 64 half-sweeps bring the error below 2^-56 max|x| (Jacobi
                  contraction 4k/(1+4k) <= 0.54, k = dt / (2 Re h^2)), by the sine
                  transforms elsewhere; "sine" always by the sine transforms
-      box_update: "auto" (FP64, static bodies, Gauss-Seidel on level 1): the final
-                  vort2flux of a step transforms only the change of Gamma since the
-                  trial one, which is zero outside a box around the bodies; "off";
-                  "force": wherever valid, cheaper or not (tests)
+      box_update: "auto" (FP64, Gauss-Seidel on level 1): the final vort2flux of a
+                  step transforms only the change of Gamma since the trial one, which is
+                  zero outside a box around the bodies, the trial forms psi_1 only where
+                  the panels read it, and B runs on the same boxes (which follow moving
+                  bodies); "off"; "force": wherever valid, cheaper or not (tests)
     """
 
     def __init__(self, state: State, *, precision="fp64", refine_sweeps=2, cg=None,
