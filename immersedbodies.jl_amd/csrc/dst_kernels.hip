@@ -289,40 +289,36 @@ __global__ __launch_bounds__(256, (TB == 32 && PF == 1) ? 3 : 1) void dst_pass_f
       acc1[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
     }
 
+  // Loads are unconditional: indices clamped into range (always valid addresses), the
+  // values of out-of-range (i, b) and of the partner-less k = H zeroed when staged, so
+  // a chunk's loads issue back to back with no wait between them.
+  // element offsets of the four inputs x_k of fold index i in a line (grp-uniform)
+  const int o0m = grp == 0 ? 2 : 1, o1c = grp == 0 ? M - 2 : M - 2, o1m = grp == 0 ? 2 : 1;
   double r0[PF][RA], r1[PF][RA], r2[PF][RA], r3[PF][RA], rsa[PF][RB], rsb[PF][RB];
   auto fetch = [&](const int st, int kc) {
 #pragma unroll
     for (int r = 0; r < RA; ++r) {
       const int e = tid + 256 * r;
-      const int i = kc + e % KC;  // KC lanes read KC consecutive i
-      const int b = b0 + e / KC;
-      double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
-      if (i < Q && b < Nb) {
-        const InT* x = xin + (size_t)b * Nk;  // x_k at x[k-1]
-        if (grp == 0) {
-          v0 = (double)x[2 * i];               // x_{2i+1}
-          v1 = (double)x[M - 2 * i - 2];       // x_{M-2i-1}
-          v2 = (double)x[2 * i + 1];           // x_{2i+2}
-          if (2 * i + 2 < H) v3 = (double)x[M - 2 * i - 3];  // x_{M-2i-2}; k = H has no partner
-        } else {
-          v0 = (double)x[i];                   // x_{i+1}
-          v1 = (double)x[M - i - 2];           // x_{M-i-1}
-          v2 = (double)x[H - i - 2];           // x_{H-i-1}
-          v3 = (double)x[H + i];               // x_{H+i+1}
-        }
-      }
-      r0[st][r] = v0;
-      r1[st][r] = v1;
-      r2[st][r] = v2;
-      r3[st][r] = v3;
+      const int i = min(kc + e % KC, Q - 1);  // KC lanes read KC consecutive i
+      const int b = min(b0 + e / KC, Nb - 1);
+      const InT* x = xin + (size_t)b * Nk;  // x_k at x[k-1]
+      // grp 0: x_{2i+1}, x_{M-2i-1}, x_{2i+2}, x_{M-2i-2}
+      // grp 1: x_{i+1},  x_{M-i-1},  x_{H-i-1}, x_{H+i+1}
+      const int k0 = o0m * i, k1 = o1c - o1m * i;
+      const int k2 = grp == 0 ? 2 * i + 1 : H - i - 2;
+      const int k3 = grp == 0 ? M - 2 * i - 3 : H + i;
+      r0[st][r] = (double)x[k0];
+      r1[st][r] = (double)x[k1];
+      r2[st][r] = (double)x[k2];
+      r3[st][r] = (double)x[k3];
     }
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
       const int e = tid + 256 * r;
       const int i = kc + e / TM;
       const int m = m0 + e % TM;
-      rsa[st][r] = (i < Q && m < Q) ? SA[(size_t)i * Q + m] : 0.0;
-      rsb[st][r] = (i < nb_lim && m < nb_lim) ? SB[(size_t)i * nb_lim + m] : 0.0;
+      rsa[st][r] = SA[(size_t)min(i, Q - 1) * Q + min(m, Q - 1)];
+      rsb[st][r] = SB[(size_t)min(i, nb_lim - 1) * nb_lim + min(m, nb_lim - 1)];
     }
   };
   const int col = lane & 15;
@@ -332,25 +328,30 @@ __global__ __launch_bounds__(256, (TB == 32 && PF == 1) ? 3 : 1) void dst_pass_f
     for (int r = 0; r < RA; ++r) {
       const int e = tid + 256 * r;
       const int ii = e % KC;
+      const int i = kc + ii;
+      const bool ok = i < Q && b0 + e / KC < Nb;
       double a, b;
       if (grp == 0) {
+        const double v3 = 2 * i + 2 < H ? r3[st][r] : 0.0;  // k = H has no partner
         a = r0[st][r] + r1[st][r];  // P_{2i+1}
-        b = r2[st][r] + r3[st][r];  // P_{2i+2}
+        b = r2[st][r] + v3;         // P_{2i+2}
       } else {
         const double qk = r0[st][r] - r1[st][r];  // Q_{i+1}
         const double qh = r2[st][r] - r3[st][r];  // Q_{H-i-1}
-        const bool last = kc + ii == Q - 1;       // k = Q is its own partner
+        const bool last = i == Q - 1;             // k = Q is its own partner
         a = last ? qk : qk + qh;
         b = last ? 0.0 : qk - qh;
       }
-      X0s[ii][e / KC] = a;
-      X1s[ii][e / KC] = b;
+      X0s[ii][e / KC] = ok ? a : 0.0;
+      X1s[ii][e / KC] = ok ? b : 0.0;
     }
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
       const int e = tid + 256 * r;
-      Sa_s[e / TM][e % TM] = rsa[st][r];
-      Sb_s[e / TM][e % TM] = rsb[st][r];
+      const int i = kc + e / TM;
+      const int m = m0 + e % TM;
+      Sa_s[e / TM][e % TM] = (i < Q && m < Q) ? rsa[st][r] : 0.0;
+      Sb_s[e / TM][e % TM] = (i < nb_lim && m < nb_lim) ? rsb[st][r] : 0.0;
     }
   };
   auto multiply = [&]() {

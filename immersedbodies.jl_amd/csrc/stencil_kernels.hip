@@ -195,12 +195,20 @@ __global__ __launch_bounds__(256) void curl_all_kernel(GridDims g, int nlevel, d
   const double* p = psi + (size_t)lev * g.nG;
   const double* pc = lev + 1 < nlevel ? psi + (size_t)(lev + 1) * g.nG : nullptr;
   double* ql = q + (size_t)lev * g.nq;
+  // faces whose two vertices are interior read the plane directly (curl_u / curl_v's
+  // interior branch); the others go through the boundary rules
   if (u) {
     const int j = r + 1;
-    ql[(size_t)(i - 1) + (size_t)(j - 1) * (nx + 1)] = curl_u(p, pc, nx, ny, i, j);
+    const bool in = i >= 2 && i <= nx && j >= 2 && j <= ny - 1;
+    const double v = in ? p[(size_t)(i - 2) + (size_t)(j - 1) * g.Nx] - p[(size_t)(i - 2) + (size_t)(j - 2) * g.Nx]
+                        : curl_u(p, pc, nx, ny, i, j);
+    ql[(size_t)(i - 1) + (size_t)(j - 1) * (nx + 1)] = v;
   } else {
     const int j = r - ny + 1;
-    ql[(size_t)g.nu + (size_t)(i - 1) + (size_t)(j - 1) * nx] = curl_v(p, pc, nx, ny, i, j);
+    const bool in = i >= 2 && i <= nx - 1 && j >= 2 && j <= ny;
+    const double v = in ? p[(size_t)(i - 2) + (size_t)(j - 2) * g.Nx] - p[(size_t)(i - 1) + (size_t)(j - 2) * g.Nx]
+                        : curl_v(p, pc, nx, ny, i, j);
+    ql[(size_t)g.nu + (size_t)(i - 1) + (size_t)(j - 1) * nx] = v;
   }
 }
 
