@@ -68,12 +68,21 @@ __global__ __launch_bounds__(256) void ring_sweep_kernel(const RingChainArgs a, 
     yv[6][tid] = ok ? a.pw[Ny + ky - 1] : 0.0;
   }
   __syncthreads();
-  const double cxk = okx ? a.cosx[kx - 1] : 0.0;
-  const double fuB = okx && fac ? fac[kx - 1] : 0.0, fuT = okx && fac ? fac[Nx + kx - 1] : 0.0;
-  const double e1 = okx && fac ? a.edge[kx - 1] : 0.0, eN = okx && fac ? a.edge[Nx + kx - 1] : 0.0;
-  const double sxl = okx ? a.pw[2 * Ny + kx - 1] : 0.0, sxr = okx ? a.pw[2 * Ny + Nx + kx - 1] : 0.0;
+  // every load unconditional (index clamped into the plane), out-of-range values unused
+  const int kc = (okx ? kx : Nx) - 1;
+  const double cxk = okx ? a.cosx[kc] : 0.0;
+  const double fuB = fac ? fac[kc] : 0.0, fuT = fac ? fac[Nx + kc] : 0.0;
+  const double e1 = fac ? a.edge[kc] : 0.0, eN = fac ? a.edge[Nx + kc] : 0.0;
+  const double sxl = okx ? a.pw[2 * Ny + kc] : 0.0, sxr = okx ? a.pw[2 * Ny + Nx + kc] : 0.0;
   const void* __restrict__ F = a.F[c];
   void* __restrict__ E = a.E[c];
+  double fv[RTY / 4];
+#pragma unroll
+  for (int m = 0; m < RTY / 4; ++m) {
+    const int ky = min(ky0 + w * (RTY / 4) + m, Ny);
+    const size_t idx = (size_t)kc + (size_t)(ky - 1) * Nx;
+    fv[m] = a.F_f32 ? (double)((const float*)F)[idx] : ((const double*)F)[idx];
+  }
   double ab = 0.0, at = 0.0;
 #pragma unroll
   for (int m = 0; m < RTY / 4; ++m) {
@@ -82,7 +91,7 @@ __global__ __launch_bounds__(256) void ring_sweep_kernel(const RingChainArgs a, 
     double e = 0.0;
     if (okx && ky <= Ny) {
       const size_t idx = (size_t)(kx - 1) + (size_t)(ky - 1) * Nx;
-      double f = a.F_f32 ? (double)((const float*)F)[idx] : ((const double*)F)[idx];
+      double f = fv[m];
       if (fac) f += fuB * yv[1][yy] + fuT * yv[2][yy] + e1 * yv[3][yy] + eN * yv[4][yy];
       const double lam = -2.0 * ((cxk + yv[0][yy]) - 2.0);
       e = a.scale / lam * f;

@@ -49,28 +49,50 @@ __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
   // Gamma with the 0.25-scaled coarse ring outside (ge) on columns I-1..I+1, rows
   // J0-1..J0+NL_RJ; rows past the top edge (j > ny) are never used
   double ge[3][NL_RJ + 2];
+  // interior values by unconditional loads (index clamped into the plane); the boundary
+  // ring (ext_value) only where the window leaves the interior
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
     for (int r = 0; r < NL_RJ + 2; ++r) {
-      const int j = J0 - 1 + r;
-      ge[c][r] = (r <= nrow + 1 && j <= ny) ? ext_value(G, Gc, nx, ny, I - 1 + c, j, 0.25) : 0.0;
+      const int i = I - 1 + c, j = J0 - 1 + r;
+      const int ic = min(max(i, 1), g.Nx), jc = min(max(j, 1), g.Ny);
+      ge[c][r] = G[(size_t)(ic - 1) + (size_t)(jc - 1) * g.Nx];
+    }
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < NL_RJ + 2; ++r) {
+      const int i = I - 1 + c, j = J0 - 1 + r;
+      const bool interior = i > 0 && i < nx && j > 0 && j < ny;
+      if (!interior) ge[c][r] = (r <= nrow + 1 && j <= ny) ? ext_value(G, Gc, nx, ny, i, j, 0.25) : 0.0;
     }
   // Qy(I+1, j), j = J0 .. J0+NL_RJ+1 (v-faces up to ny+1)
   double qyv[NL_RJ + 2];
+  double qa[NL_RJ + 2], qb[NL_RJ + 2];
 #pragma unroll
   for (int r = 0; r < NL_RJ + 2; ++r) {
-    const int j = J0 + r;
-    qyv[r] = r <= nrow + 1
-                 ? (((QY(I + 1, j) + QY(I, j)) + base_flux_v(a.bf, lev, I + 1, j)) + base_flux_v(a.bf, lev, I, j)) / 2
-                 : 0.0;
+    const int jc = min(J0 + r, ny + 1);  // rows past nrow + 1 are loaded but unused
+    qa[r] = QY(I + 1, jc);
+    qb[r] = QY(I, jc);
   }
   // Qx(I + c, J + 1) for the window's rows J (u-faces up to ny)
   double qxr[3][NL_RJ + 1];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int r = 0; r < NL_RJ + 1; ++r) qxr[c][r] = r <= nrow ? QX(I + c, J0 + r) : 0.0;
+    for (int r = 0; r < NL_RJ + 1; ++r) qxr[c][r] = QX(I + c, min(J0 + r, ny));
+#pragma unroll
+  for (int r = 0; r < NL_RJ + 2; ++r) {
+    const int j = J0 + r;
+    qyv[r] = r <= nrow + 1
+                 ? (((qa[r] + qb[r]) + base_flux_v(a.bf, lev, I + 1, j)) + base_flux_v(a.bf, lev, I, j)) / 2
+                 : 0.0;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < NL_RJ + 1; ++r) qxr[c][r] = r <= nrow ? qxr[c][r] : 0.0;
 #undef QX
 #undef QY
 
