@@ -127,27 +127,23 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
 
   // PF register stages: the loads of chunk c + PF are issued while chunk c is multiplied
   double ra[PF][RA], rm[PF][RA], rso[PF][RB], rse[PF][RB];
+  // loads unconditional (indices clamped into range), zeroed when staged
   auto fetch = [&](const int st, int kc) {
 #pragma unroll
     for (int r = 0; r < RA; ++r) {
       const int e = tid + 256 * r;
-      const int k1 = kc + e % KC + 1;  // 1-based k; KC lanes read KC consecutive k
-      const int b = b0 + e / KC;
-      double va = 0.0, vm = 0.0;
-      if (k1 <= H && b < Nb) {
-        va = load_in<InT>(d, k1 - 1, b);
-        if (k1 < H || modd) vm = load_in<InT>(d, Nk - k1, b);
-      }
-      ra[st][r] = va;
-      rm[st][r] = vm;
+      const int k1c = min(kc + e % KC + 1, H);  // 1-based k; KC lanes read KC consecutive k
+      const int bc = min(b0 + e / KC, Nb - 1);
+      ra[st][r] = load_in<InT>(d, k1c - 1, bc);
+      rm[st][r] = load_in<InT>(d, Nk - k1c, bc);
     }
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
       const int e = tid + 256 * r;
-      const int k1 = kc + e / TM + 1;
+      const int k0 = kc + e / TM;
       const int m = m0 + e % TM;
-      rso[st][r] = (k1 <= H && m < H) ? SoT[(size_t)(k1 - 1) * H + m] : 0.0;
-      rse[st][r] = (k1 <= E && m < E) ? SeT[(size_t)(k1 - 1) * E + m] : 0.0;
+      rso[st][r] = SoT[(size_t)min(k0, H - 1) * H + min(m, H - 1)];
+      rse[st][r] = E > 0 ? SeT[(size_t)min(k0, E - 1) * E + min(m, E - 1)] : 0.0;
     }
   };
   const int col = lane & 15;
@@ -159,15 +155,21 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
     for (int r = 0; r < RA; ++r) {
       const int e = tid + 256 * r;
       const int kk = e % KC;
-      const bool mid = !modd && kc + kk + 1 == H;
-      Ps[kk][e / KC] = ra[st][r] + rm[st][r];
-      Qs[kk][e / KC] = mid ? 0.0 : ra[st][r] - rm[st][r];
+      const int k1 = kc + kk + 1;
+      const bool ok = k1 <= H && b0 + e / KC < Nb;
+      const double va = ok ? ra[st][r] : 0.0;
+      const double vm = ok && (k1 < H || modd) ? rm[st][r] : 0.0;
+      const bool mid = !modd && k1 == H;
+      Ps[kk][e / KC] = va + vm;
+      Qs[kk][e / KC] = mid ? 0.0 : va - vm;
     }
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
       const int e = tid + 256 * r;
-      So_s[e / TM][e % TM] = rso[st][r];
-      Se_s[e / TM][e % TM] = rse[st][r];
+      const int k1 = kc + e / TM + 1;
+      const int m = m0 + e % TM;
+      So_s[e / TM][e % TM] = (k1 <= H && m < H) ? rso[st][r] : 0.0;
+      Se_s[e / TM][e % TM] = (k1 <= E && m < E) ? rse[st][r] : 0.0;
     }
   };
   auto multiply = [&]() {
@@ -480,22 +482,19 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
   // loader mapping for the sine rows: 16-byte piece (row sm, k8) of table st (4 tables)
   const int sm = tid >> 2, sk = (tid & 3) * 8;
   bf8 rs[4];
+  // input loads unconditional (k and b clamped into range), zeroed when staged, so the
+  // chunk's loads issue back to back
   auto fetch = [&](int kc) {
 #pragma unroll
     for (int r = 0; r < RA; ++r) {
       const int e = tid + 256 * r;
       const int kp = e & 15;
-      const int b = b0 + (e >> 4);
+      const int bc = min(b0 + (e >> 4), Nb - 1);
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        const int k1 = kc + 2 * kp + j + 1;
-        double va = 0.0, vm = 0.0;
-        if (k1 <= H && b < Nb) {
-          va = load_in<InT>(d, k1 - 1, b);
-          if (k1 < H || modd) vm = load_in<InT>(d, Nk - k1, b);
-        }
-        ra[r][j] = va;
-        rm[r][j] = vm;
+        const int k1c = min(kc + 2 * kp + j + 1, H);
+        ra[r][j] = load_in<InT>(d, k1c - 1, bc);
+        rm[r][j] = load_in<InT>(d, Nk - k1c, bc);
       }
     }
     const size_t off = (size_t)(m0 + sm) * Hp + kc + sk;  // m0 + sm < Mp by padding
@@ -518,9 +517,13 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
       __bf16 ph[2], pl[2], qh[2], ql[2];
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        const bool mid = !modd && kc + 2 * kp + j + 1 == H;  // even M: k = H has no partner
-        split_bf16((float)(ra[r][j] + rm[r][j]), ph[j], pl[j]);
-        split_bf16(mid ? 0.f : (float)(ra[r][j] - rm[r][j]), qh[j], ql[j]);
+        const int k1 = kc + 2 * kp + j + 1;
+        const bool ok = k1 <= H && b0 + bb < Nb;
+        const double va = ok ? ra[r][j] : 0.0;
+        const double vm = ok && (k1 < H || modd) ? rm[r][j] : 0.0;
+        const bool mid = !modd && k1 == H;  // even M: k = H has no partner
+        split_bf16((float)(va + vm), ph[j], pl[j]);
+        split_bf16(mid ? 0.f : (float)(va - vm), qh[j], ql[j]);
       }
       typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
       *(bf2*)&Ah[0][bb][2 * kp] = (bf2){ph[0], ph[1]};
