@@ -81,3 +81,22 @@ def test_batch_against_oracle():
     fo = cases.oracle_advance(orc, p, 8)
     assert cases.relerr(f, fo) < 1e-9
     assert cases.relerr(s.read("Gamma"), orc.field("Gamma")) < 1e-10
+
+
+def test_batch_with_box_updates_matches_alone():
+    """Static problems with the box updates forced (the trial psi on the panels' box, the
+    final vort2flux by box products, project_circ on the box) advanced in the batch's
+    graphs end bitwise equal to the same solvers advanced alone."""
+    probs = _problems()[:2] + _problems()[3:]
+    alone = [ib.create_solver(p, box_update="force") for p in probs]
+    batched = [ib.create_solver(p, box_update="force") for p in probs]
+    assert all(s.box_update(1) is not None for s in alone)
+    for s in alone:
+        s.advance(6)
+    batch = ib.Batch(batched)
+    batch.advance(6)
+    batch.close()
+    for k, (a, b) in enumerate(zip(alone, batched)):
+        for name in FIELDS:
+            x, y = a.read(name), b.read(name)
+            assert np.array_equal(x, y), (k, name)
