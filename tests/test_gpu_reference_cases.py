@@ -12,13 +12,16 @@ import immersedbodies as ib
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("box", ["auto", "force"])
 @pytest.mark.parametrize("motion", ["static grid", "moving grid"])
 @pytest.mark.parametrize("bodies", ["single cyl", "double line", "moving cyl"])
-def test_reference_solver_cases(motion, bodies):
+def test_reference_solver_cases(motion, bodies, box):
+    """box "force": the box updates (of the step, and of B for the moving cylinder) on
+    these small grids too, the boxes following the moving cylinder."""
     prob = cases.reference_solver_problem(motion, bodies)
-    print("case reference solver", motion, bodies, flush=True)
+    print("case reference solver", motion, bodies, box, flush=True)
     state = ib.State(prob)
-    solver = ib.create_solver(state)
+    solver = ib.create_solver(state, box_update=box)
     dt = ib.timestep(prob)
     forces = ib.solve_(None, solver, t=3 * dt)
     assert abs(state.t - 3 * dt) < 1e-12  # @test state.t ≈ 3 * timestep(prob)
@@ -30,12 +33,13 @@ def test_reference_solver_cases(motion, bodies):
         assert cases.relerr(solver.read(name), orc.field(name)) <= (1e-9 if moving else 1e-10), name
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_odd_grid_matches_oracle(precision):
-    """61 x 53 cells (odd lengths), 3 levels, a cylinder."""
+@pytest.mark.parametrize("precision,box", [("fp64", "auto"), ("fp64", "force"), ("mixed", "auto")])
+def test_odd_grid_matches_oracle(precision, box):
+    """61 x 53 cells (odd lengths), 3 levels, a cylinder; FP64 also with the box updates
+    forced (odd coarsify ranges)."""
     prob = cases.cylinder(nx=61, ny=53, nlevel=3, h=0.05, x0=(-1.5, -1.3), npts=40)
-    print("case odd grid", precision, flush=True)
-    solver = ib.create_solver(prob, precision=precision)
+    print("case odd grid", precision, box, flush=True)
+    solver = ib.create_solver(prob, precision=precision, box_update=box)
     orc = cases.oracle_for(prob)
     stats = solver.advance(4)
     f_ref = cases.oracle_advance(orc, prob, 4)
