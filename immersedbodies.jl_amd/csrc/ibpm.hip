@@ -93,6 +93,7 @@ void free_all(ibpm_solver* s) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->h_scal) (void)hipHostFree(s->h_scal);
+  if (s->h_force) (void)hipHostFree(s->h_force);
   if (s->st) (void)hipStreamDestroy(s->st);
 }
 
@@ -1667,6 +1668,7 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   CK(dalloc(&s->w, 98 * (size_t)n));
   CK(dalloc(&s->cg_scal, 8));
   CK(hipHostMalloc((void**)&s->h_scal, 16 * sizeof(double), hipHostMallocDefault));
+  CK(hipHostMalloc((void**)&s->h_force, 2 * kForceChunk * sizeof(double), hipHostMallocDefault));
   // allocations hold max(count, 1) elements (dalloc)
   auto zero_bytes = [](size_t count, size_t elem) { return std::max<size_t>(count, 1) * elem; };
   CK(hipMemsetAsync(s->Ft, 0, zero_bytes(2 * (size_t)n, 8), s->st));
@@ -1785,14 +1787,28 @@ int ibpm_advance(ibpm_solver* s, int nsteps, const double* freestream, const dou
     motion = s->staged_motion.data();
   }
   HIPCHK(hipSetDevice(s->device));
+  // Each step's total force goes to the pinned h_force (an asynchronous copy: a copy into
+  // the caller's pageable stats would make the host wait for the stream every step), and
+  // into stats after a synchronisation every kForceChunk steps.
   for (int k = 0; k < nsteps; ++k) {
     ibpm_step_stats* sk = stats ? stats + k : nullptr;
     if (sk) memset(sk, 0, sizeof *sk);
     int rc = step(s, freestream ? freestream + (s->grid_moving ? 8 : 2) * k : nullptr,
                   motion ? motion + (size_t)6 * s->nbody * k : nullptr, sk);
     if (rc) return rc;
-    if (sk && s->n > 0) HIPCHK(hipMemcpyAsync(sk->force, s->total, 16, hipMemcpyDeviceToHost, s->st));
-    if (sk && s->n == 0) sk->force[0] = sk->force[1] = 0.0;
+    if (!sk) continue;
+    if (s->n == 0) {
+      sk->force[0] = sk->force[1] = 0.0;
+      continue;
+    }
+    HIPCHK(hipMemcpyAsync(s->h_force + 2 * (k % kForceChunk), s->total, 16, hipMemcpyDeviceToHost, s->st));
+    if ((k + 1) % kForceChunk == 0 || k + 1 == nsteps) {
+      HIPCHK(hipStreamSynchronize(s->st));
+      for (int j = k - k % kForceChunk; j <= k; ++j) {
+        stats[j].force[0] = s->h_force[2 * (j % kForceChunk)];
+        stats[j].force[1] = s->h_force[2 * (j % kForceChunk) + 1];
+      }
+    }
   }
   std::fill(s->staged_set.begin(), s->staged_set.end(), 0);
   HIPCHK(hipStreamSynchronize(s->st));

@@ -9,6 +9,8 @@
 #include "../../include/ibpm.h"
 #include "ibpm_internal.h"
 
+constexpr int kForceChunk = 1024;  // steps of total force held in pinned memory (h_force)
+
 struct ibpm_solver {
   ibpm::GridDims g{};
   int L = 0;
@@ -83,6 +85,7 @@ struct ibpm_solver {
   // CG
   double *cg_r = nullptr, *cg_u = nullptr, *cg_c = nullptr, *cg_scal = nullptr;
   double* h_scal = nullptr;  // pinned: [0..5] CG scalars, [8..10] warm-start scalars
+  double* h_force = nullptr;  // pinned: total force of the last kForceChunk steps (ibpm_advance)
   // CG warm start: extrapolation history F~b of the two steps before the previous one,
   // the extrapolated guess x_e, B x_e, and (rms(r_e), ||r_e||, ||b||)
   int ws_mode = IBPM_WS_PREVIOUS, ws_record = 0, nhist = 0;
