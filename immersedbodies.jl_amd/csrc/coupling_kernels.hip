@@ -383,7 +383,7 @@ __global__ void gemv_sum_kernel(int m, const double* __restrict__ part, double* 
 __global__ __launch_bounds__(1024) void forces_kernel(int n, const double* __restrict__ Ft,
                                                       double* __restrict__ fb,
                                                       double* __restrict__ total, double h,
-                                                      double dt) {
+                                                      double dt, double* __restrict__ host_total) {
   __shared__ double sx[1024], sy[1024];
   double ax = 0.0, ay = 0.0;
   for (int p = threadIdx.x; p < n; p += 1024) {
@@ -406,6 +406,10 @@ __global__ __launch_bounds__(1024) void forces_kernel(int n, const double* __res
   if (threadIdx.x == 0) {
     total[0] = sx[0];
     total[1] = sy[0];
+    if (host_total) {
+      host_total[0] = sx[0];
+      host_total[1] = sy[0];
+    }
   }
 }
 
@@ -420,8 +424,8 @@ hipError_t launch_gemv(int m, const double* M, const double* b, double* x, doubl
 }
 
 hipError_t launch_forces(int n, const double* Ft, double* fb, double* total, double h, double dt,
-                         hipStream_t st) {
-  hipLaunchKernelGGL(forces_kernel, dim3(1), dim3(1024), 0, st, n, Ft, fb, total, h, dt);
+                         double* host_total, hipStream_t st) {
+  hipLaunchKernelGGL(forces_kernel, dim3(1), dim3(1024), 0, st, n, Ft, fb, total, h, dt, host_total);
   return hipGetLastError();
 }
 

@@ -1299,7 +1299,7 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
   if (s->n > 0) {
     rc = couple_surface(s, stats);
     if (rc) return rc;
-    HIPCHK(launch_forces(s->n, s->Ft, s->fb, s->total, s->h, s->dt, s->st));
+    HIPCHK(launch_forces(s->n, s->Ft, s->fb, s->total, s->h, s->dt, s->force_slot, s->st));
     ScatterArgs sa;
     memset(&sa, 0, sizeof sa);
     sa.g = s->g;
@@ -1668,7 +1668,8 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   CK(dalloc(&s->w, 98 * (size_t)n));
   CK(dalloc(&s->cg_scal, 8));
   CK(hipHostMalloc((void**)&s->h_scal, 16 * sizeof(double), hipHostMallocDefault));
-  CK(hipHostMalloc((void**)&s->h_force, 2 * kForceChunk * sizeof(double), hipHostMallocDefault));
+  CK(hipHostMalloc((void**)&s->h_force, 2 * kForceChunk * sizeof(double), hipHostMallocMapped));
+  CK(hipHostGetDevicePointer((void**)&s->d_force, s->h_force, 0));
   // allocations hold max(count, 1) elements (dalloc)
   auto zero_bytes = [](size_t count, size_t elem) { return std::max<size_t>(count, 1) * elem; };
   CK(hipMemsetAsync(s->Ft, 0, zero_bytes(2 * (size_t)n, 8), s->st));
@@ -1787,21 +1788,22 @@ int ibpm_advance(ibpm_solver* s, int nsteps, const double* freestream, const dou
     motion = s->staged_motion.data();
   }
   HIPCHK(hipSetDevice(s->device));
-  // Each step's total force goes to the pinned h_force (an asynchronous copy: a copy into
-  // the caller's pageable stats would make the host wait for the stream every step), and
-  // into stats after a synchronisation every kForceChunk steps.
+  // The forces kernel writes each step's total force into the mapped pinned h_force
+  // (a copy into the caller's pageable stats would make the host wait for the stream
+  // every step); stats take them after a synchronisation every kForceChunk steps.
   for (int k = 0; k < nsteps; ++k) {
     ibpm_step_stats* sk = stats ? stats + k : nullptr;
     if (sk) memset(sk, 0, sizeof *sk);
+    s->force_slot = sk && s->n > 0 ? s->d_force + 2 * (k % kForceChunk) : nullptr;
     int rc = step(s, freestream ? freestream + (s->grid_moving ? 8 : 2) * k : nullptr,
                   motion ? motion + (size_t)6 * s->nbody * k : nullptr, sk);
+    s->force_slot = nullptr;
     if (rc) return rc;
     if (!sk) continue;
     if (s->n == 0) {
       sk->force[0] = sk->force[1] = 0.0;
       continue;
     }
-    HIPCHK(hipMemcpyAsync(s->h_force + 2 * (k % kForceChunk), s->total, 16, hipMemcpyDeviceToHost, s->st));
     if ((k + 1) % kForceChunk == 0 || k + 1 == nsteps) {
       HIPCHK(hipStreamSynchronize(s->st));
       for (int j = k - k % kForceChunk; j <= k; ++j) {

@@ -268,9 +268,10 @@ hipError_t launch_body_bbox(const int* coords, int n, double* out, hipStream_t s
 size_t gemv_partial_count(int m);
 hipError_t launch_gemv(int m, const double* M, const double* b, double* x, double* part,
                        hipStream_t st);
-// fb = Ft*h/dt and the total force (column sums), FP64.
+// fb = Ft*h/dt and the total force (column sums), FP64; the total also to host_total
+// (device address of mapped pinned host memory) when that is not null.
 hipError_t launch_forces(int n, const double* Ft, double* fb, double* total, double h, double dt,
-                         hipStream_t st);
+                         double* host_total, hipStream_t st);
 
 // Conjugate gradient helpers (single work-group kernels, FP64 reductions).
 struct CgDev {

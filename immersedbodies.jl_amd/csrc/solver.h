@@ -85,7 +85,9 @@ struct ibpm_solver {
   // CG
   double *cg_r = nullptr, *cg_u = nullptr, *cg_c = nullptr, *cg_scal = nullptr;
   double* h_scal = nullptr;  // pinned: [0..5] CG scalars, [8..10] warm-start scalars
-  double* h_force = nullptr;  // pinned: total force of the last kForceChunk steps (ibpm_advance)
+  double* h_force = nullptr;  // mapped pinned: total force of the last kForceChunk steps
+  double* d_force = nullptr;  // h_force's device address
+  double* force_slot = nullptr;  // where this step's forces kernel also writes the total (or null)
   // CG warm start: extrapolation history F~b of the two steps before the previous one,
   // the extrapolated guess x_e, B x_e, and (rms(r_e), ||r_e||, ||b||)
   int ws_mode = IBPM_WS_PREVIOUS, ws_record = 0, nhist = 0;
