@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 #include <limits.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "ibpm_device.h"
 #include "ibpm_internal.h"
 
@@ -332,14 +335,105 @@ __global__ __launch_bounds__(256) void scatter_rot_kernel(const ScatterArgs a) {
     ((double*)a.out[rhs])[idx] = v;
 }
 
+// The same sums from the vertex lists of static panels: a vertex reads its entry range,
+// then its entries (panel, 4 weights), then the panels' forces, and adds the terms in the
+// panel order of scatter_rot_kernel (a term a panel does not reach has weight 0, which
+// leaves the sum unchanged), so both kernels give the same result.
+__global__ __launch_bounds__(256) void scatter_rot_lists_kernel(const ScatterArgs a) {
+  const GridDims g = a.g;
+  const int rhs = blockIdx.z;
+  const int I = a.ri0 + blockIdx.x * 64 + (threadIdx.x & 63) + 1;
+  const int J = a.rj0 + blockIdx.y * 4 + (threadIdx.x >> 6) + 1;
+  if (I > a.ri0 + a.rw || J > a.rj0 + a.rh) return;
+  const int n = a.npanel;
+  const size_t idx = (size_t)(I - 1) + (size_t)(J - 1) * g.Nx;
+  const int e0 = a.rowptr[idx], e1 = a.rowptr[idx + 1];
+  const double* f = a.f[rhs];
+  const double2* ew = (const double2*)a.ew;
+  double ua = 0.0, ub = 0.0, va = 0.0, vb = 0.0;
+  constexpr int EB = 8;  // entries per batch, their loads issued together
+  for (int eb = e0; eb < e1; eb += EB) {
+    int pp[EB];
+    double2 wu[EB], wv[EB];
+#pragma unroll
+    for (int q = 0; q < EB; ++q) {
+      const int e = min(eb + q, e1 - 1);
+      pp[q] = a.ep[e];
+      wu[q] = ew[2 * (size_t)e];
+      wv[q] = ew[2 * (size_t)e + 1];
+    }
+    double fx[EB], fy[EB];
+#pragma unroll
+    for (int q = 0; q < EB; ++q) {
+      fx[q] = f[pp[q]];
+      fy[q] = f[n + pp[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < EB; ++q) {
+      if (eb + q < e1) {
+        ua += wu[q].x * fx[q];
+        ub += wu[q].y * fx[q];
+        va += wv[q].x * fy[q];
+        vb += wv[q].y * fy[q];
+      }
+    }
+  }
+  const double v = (ua - ub) + (va - vb);
+  if (a.out_f32)
+    ((float*)a.out[rhs])[idx] = (float)v;
+  else
+    ((double*)a.out[rhs])[idx] = v;
+}
+
 hipError_t launch_scatter_rot(const ScatterArgs& a0, hipStream_t st) {
   ScatterArgs a = a0;
   if (a.rw <= 0) a.ri0 = 0, a.rj0 = 0, a.rw = a.g.Nx, a.rh = a.g.Ny;
   if (a.ri0 < 0 || a.rj0 < 0 || a.ri0 + a.rw > a.g.Nx || a.rj0 + a.rh > a.g.Ny || a.rh <= 0)
     return hipErrorInvalidValue;
   dim3 grid((a.rw + 63) / 64, (a.rh + 3) / 4, a.nrhs);
-  hipLaunchKernelGGL(scatter_rot_kernel, grid, dim3(256), 0, st, a);
+  if (a.rowptr && a.ep && a.ew)
+    hipLaunchKernelGGL(scatter_rot_lists_kernel, grid, dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(scatter_rot_kernel, grid, dim3(256), 0, st, a);
   return hipGetLastError();
+}
+
+void build_scatter_lists(int n, int Nx, int Ny, const int* coords, const int* perm, const double* w,
+                         std::vector<int>& rowptr, std::vector<int>& ep, std::vector<double>& ew) {
+  // the terms scatter_rot_kernel adds at vertex (I, J) from the panel at cell (ix, iy):
+  // I in [ix - 4, ix + 3], J in [iy - 4, iy + 3], with its index conditions
+  const auto visit = [&](auto&& emit) {
+    for (int k = 0; k < n; ++k) {
+      const int p = perm[k], ix = coords[2 * p], iy = coords[2 * p + 1];
+      const double* wp = w + (size_t)p * 98;
+      for (int J = std::max(iy - 4, 1); J <= std::min(iy + 3, Ny); ++J)
+        for (int I = std::max(ix - 4, 1); I <= std::min(ix + 3, Nx); ++I) {
+          const int db = J - iy + 3, au = I + 1 - ix + 3, av = I - ix + 3;
+          const bool cu = au >= 0 && au <= 6;
+          const bool tua = cu && db >= 0, tub = cu && db + 1 <= 6;
+          const bool tvb = av >= 0 && av <= 6 && db + 1 <= 6;
+          if (!tua && !tub && !tvb) continue;
+          emit((size_t)(I - 1) + (size_t)(J - 1) * Nx, p, tua ? wp[db * 7 + au] : 0.0,
+               tub ? wp[(db + 1) * 7 + au] : 0.0, tub ? wp[49 + (db + 1) * 7 + au] : 0.0,
+               tvb ? wp[49 + (db + 1) * 7 + av] : 0.0);
+        }
+    }
+  };
+  const size_t nv = (size_t)Nx * Ny;
+  rowptr.assign(nv + 1, 0);
+  visit([&](size_t v, int, double, double, double, double) { ++rowptr[v + 1]; });
+  for (size_t v = 0; v < nv; ++v) rowptr[v + 1] += rowptr[v];
+  ep.assign(rowptr[nv], 0);
+  ew.assign(4 * (size_t)rowptr[nv], 0.0);
+  std::vector<int> fill(rowptr.begin(), rowptr.end() - 1);
+  visit([&](size_t v, int p, double a, double b, double c, double d) {
+    const int e = fill[v]++;
+    ep[e] = p;
+    ew[4 * (size_t)e] = a;
+    ew[4 * (size_t)e + 1] = b;
+    ew[4 * (size_t)e + 2] = c;
+    ew[4 * (size_t)e + 3] = d;
+  });
 }
 
 // ---------------------------------------------------------------------------------

@@ -83,6 +83,7 @@ void free_all(ibpm_solver* s) {
                   s->cosx, s->cosy, s->SoX, s->SeX, s->SoY, s->SeY, s->SqX, s->SqY, s->BfX, s->BfY,
                   s->xb, s->ub, s->xb_ref, s->Ft, s->fb, s->Ftmp, s->total, s->motion,
                   s->body_of, s->moving_d, s->coords, s->perm, s->xstart, s->err, s->w,
+                  s->sc_rowptr, s->sc_ep, s->sc_ew,
                   s->ywin, s->gemv_part, s->pc_P, s->pc_theta0, s->pc_t, s->pc_z,
                   s->Binv, s->Gtmp, s->psitmp, s->Sc, s->cg_r, s->cg_u, s->cg_c, s->cg_scal,
                   s->F1, s->G1, s->Sfx, s->Sfy != s->Sfx ? s->Sfy : nullptr, s->Ftr, s->Tbox, s->Pa, s->Pb, s->T2, s->bbox_d,
@@ -800,6 +801,9 @@ int apply_B(ibpm_solver* s, int ncol, const double* const* z, double* const* x) 
     sa.perm = s->perm;
     sa.xstart = s->xstart;
     sa.ywin = s->ywin;
+    sa.rowptr = s->sc_rowptr;
+    sa.ep = s->sc_ep;
+    sa.ew = s->sc_ew;
     for (int c = 0; c < nc; ++c) {
       sa.f[c] = z[c0 + c];
       sa.out[c] = s->Sc + (size_t)(c0 + c) * P;
@@ -877,6 +881,9 @@ int apply_B_box(ibpm_solver* s, int ncol, const double* const* z, double* const*
     sa.perm = s->perm;
     sa.xstart = s->xstart;
     sa.ywin = s->ywin;
+    sa.rowptr = s->sc_rowptr;
+    sa.ep = s->sc_ep;
+    sa.ew = s->sc_ew;
     sa.ri0 = r[0], sa.rw = r[1], sa.rj0 = r[2], sa.rh = r[3];
     for (int c = 0; c < nc; ++c) {
       sa.f[c] = z[c0 + c];
@@ -1310,6 +1317,9 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
     sa.perm = s->perm;
     sa.xstart = s->xstart;
     sa.ywin = s->ywin;
+    sa.rowptr = s->sc_rowptr;
+    sa.ep = s->sc_ep;
+    sa.ew = s->sc_ew;
     sa.f[0] = s->Ft;
     sa.out[0] = s->rhs;
     // with the box update, Ainv1 rot(E^T Ft) from zero vanishes outside the level-1 box
@@ -1732,6 +1742,24 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
       }
     }
     CK(launch_body_sort(bd, s->st));
+    if (!s->moving) {
+      // static panels: rot(E^T f) from per-vertex term lists (build_scatter_lists)
+      std::vector<int> hc(2 * (size_t)n), hp(n), rowptr, ep;
+      std::vector<double> hw(98 * (size_t)n), ew;
+      CK(hipMemcpyAsync(hc.data(), s->coords, hc.size() * sizeof(int), hipMemcpyDeviceToHost, s->st));
+      CK(hipMemcpyAsync(hp.data(), s->perm, hp.size() * sizeof(int), hipMemcpyDeviceToHost, s->st));
+      CK(hipMemcpyAsync(hw.data(), s->w, hw.size() * sizeof(double), hipMemcpyDeviceToHost, s->st));
+      CK(hipStreamSynchronize(s->st));
+      build_scatter_lists(n, s->g.Nx, s->g.Ny, hc.data(), hp.data(), hw.data(), rowptr, ep, ew);
+      CK(dalloc(&s->sc_rowptr, rowptr.size()));
+      CK(dalloc(&s->sc_ep, ep.size()));
+      CK(dalloc(&s->sc_ew, ew.size()));
+      CK(hipMemcpy(s->sc_rowptr, rowptr.data(), rowptr.size() * sizeof(int), hipMemcpyHostToDevice));
+      if (!ep.empty()) {
+        CK(hipMemcpy(s->sc_ep, ep.data(), ep.size() * sizeof(int), hipMemcpyHostToDevice));
+        CK(hipMemcpy(s->sc_ew, ew.data(), ew.size() * sizeof(double), hipMemcpyHostToDevice));
+      }
+    }
   }
   // B^-1 (static) or CG buffers (moving): coupling.jl:179-202
   if (n > 0) {

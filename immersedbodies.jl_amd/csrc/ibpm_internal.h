@@ -11,6 +11,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <vector>
+
 struct ibpm_net;  // warm-start network (include/ibpm.h)
 
 namespace ibpm {
@@ -259,7 +261,19 @@ struct ScatterArgs {
   int out_f32;
   int ri0, rj0, rw, rh;  // vertices written: [ri0, ri0 + rw) x [rj0, rj0 + rh) (0-based);
                          // rw = 0: the whole plane
+  // static panels: each vertex's terms listed once (build_scatter_lists), vertex v =
+  // (I-1) + (J-1) Nx owning entries rowptr[v] .. rowptr[v+1]-1, each a panel (ep) and its
+  // weights (ew: w_u of rows J and J+1 at column I+1, w_v of row J+1 at columns I+1 and
+  // I; zero for a term the panel does not reach), in the panel order of perm. Null: the
+  // panels are searched by column (moving bodies).
+  const int* rowptr;
+  const int* ep;
+  const double* ew;
 };
+// the vertex lists of ScatterArgs from the panels' cells, weights and sort order (host
+// copies); rowptr: Nx Ny + 1 entries
+void build_scatter_lists(int n, int Nx, int Ny, const int* coords, const int* perm, const double* w,
+                         std::vector<int>& rowptr, std::vector<int>& ep, std::vector<double>& ew);
 hipError_t launch_scatter_rot(const ScatterArgs& a, hipStream_t st);
 // (min ix, max ix, min iy, max iy) of the panels' flux cells into out[0..3]
 hipError_t launch_body_bbox(const int* coords, int n, double* out, hipStream_t st);

@@ -75,6 +75,9 @@ struct ibpm_solver {
   double *xb = nullptr, *ub = nullptr, *xb_ref = nullptr, *Ft = nullptr, *fb = nullptr;
   double *Ftmp = nullptr, *total = nullptr, *motion = nullptr;
   int *body_of = nullptr, *moving_d = nullptr, *coords = nullptr, *perm = nullptr;
+  // static panels: each vertex's rot(E^T f) terms (ScatterArgs rowptr, ep, ew)
+  int *sc_rowptr = nullptr, *sc_ep = nullptr;
+  double* sc_ew = nullptr;
   int *xstart = nullptr, *err = nullptr, *ywin = nullptr;
   double* gemv_part = nullptr;
   double* w = nullptr;
