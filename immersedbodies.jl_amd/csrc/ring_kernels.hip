@@ -56,33 +56,57 @@ __global__ __launch_bounds__(256) void ring_sweep_kernel(const RingChainArgs a, 
   const int ky0 = by * RTY + 1;
   const bool okx = kx <= Nx;
   const double* __restrict__ fac = a.fac_in[c];
-  if (tid < RTY) {
-    const int ky = ky0 + tid;
-    const bool ok = ky <= Ny;
-    yv[0][tid] = ok ? a.cosy[ky - 1] : 0.0;
-    yv[1][tid] = ok && fac ? a.edge[2 * Nx + ky - 1] : 0.0;
-    yv[2][tid] = ok && fac ? a.edge[2 * Nx + Ny + ky - 1] : 0.0;
-    yv[3][tid] = ok && fac ? fac[2 * Nx + ky - 1] : 0.0;
-    yv[4][tid] = ok && fac ? fac[2 * Nx + Ny + ky - 1] : 0.0;
-    yv[5][tid] = ok ? a.pw[ky - 1] : 0.0;
-    yv[6][tid] = ok ? a.pw[Ny + ky - 1] : 0.0;
+  // Loads in the order their values are needed, all issued before the first wait: the
+  // row tables of row tid % RTY (stored by the first RTY threads), then the plane and
+  // column values. Every index is clamped into range (out-of-range values unused).
+  const int yr = tid & (RTY - 1);
+  const int kyr = min(ky0 + yr, Ny) - 1;
+  const double y0 = a.cosy[kyr], y5 = a.pw[kyr], y6 = a.pw[Ny + kyr];
+  double y1 = 0.0, y2 = 0.0, y3 = 0.0, y4 = 0.0;
+  if (fac) {
+    y1 = a.edge[2 * Nx + kyr];
+    y2 = a.edge[2 * Nx + Ny + kyr];
+    y3 = fac[2 * Nx + kyr];
+    y4 = fac[2 * Nx + Ny + kyr];
   }
-  __syncthreads();
-  // every load unconditional (index clamped into the plane), out-of-range values unused
   const int kc = (okx ? kx : Nx) - 1;
-  const double cxk = okx ? a.cosx[kc] : 0.0;
-  const double fuB = fac ? fac[kc] : 0.0, fuT = fac ? fac[Nx + kc] : 0.0;
-  const double e1 = fac ? a.edge[kc] : 0.0, eN = fac ? a.edge[Nx + kc] : 0.0;
-  const double sxl = okx ? a.pw[2 * Ny + kc] : 0.0, sxr = okx ? a.pw[2 * Ny + Nx + kc] : 0.0;
   const void* __restrict__ F = a.F[c];
   void* __restrict__ E = a.E[c];
   double fv[RTY / 4];
+  float fs[RTY / 4];
+  if (a.F_f32) {  // uniform: each precision's loads issue back to back
 #pragma unroll
-  for (int m = 0; m < RTY / 4; ++m) {
-    const int ky = min(ky0 + w * (RTY / 4) + m, Ny);
-    const size_t idx = (size_t)kc + (size_t)(ky - 1) * Nx;
-    fv[m] = a.F_f32 ? (double)((const float*)F)[idx] : ((const double*)F)[idx];
+    for (int m = 0; m < RTY / 4; ++m)
+      fs[m] = ((const float*)F)[(size_t)kc + (size_t)(min(ky0 + w * (RTY / 4) + m, Ny) - 1) * Nx];
+  } else {
+#pragma unroll
+    for (int m = 0; m < RTY / 4; ++m)
+      fv[m] = ((const double*)F)[(size_t)kc + (size_t)(min(ky0 + w * (RTY / 4) + m, Ny) - 1) * Nx];
   }
+  const double cx_k = a.cosx[kc], sxl_k = a.pw[2 * Ny + kc], sxr_k = a.pw[2 * Ny + Nx + kc];
+  double fuB = 0.0, fuT = 0.0, e1 = 0.0, eN = 0.0;
+  if (fac) {
+    fuB = fac[kc];
+    fuT = fac[Nx + kc];
+    e1 = a.edge[kc];
+    eN = a.edge[Nx + kc];
+  }
+  if (tid < RTY) {
+    const bool ok = ky0 + tid <= Ny;
+    yv[0][tid] = ok ? y0 : 0.0;
+    yv[1][tid] = ok ? y1 : 0.0;
+    yv[2][tid] = ok ? y2 : 0.0;
+    yv[3][tid] = ok ? y3 : 0.0;
+    yv[4][tid] = ok ? y4 : 0.0;
+    yv[5][tid] = ok ? y5 : 0.0;
+    yv[6][tid] = ok ? y6 : 0.0;
+  }
+  if (a.F_f32) {
+#pragma unroll
+    for (int m = 0; m < RTY / 4; ++m) fv[m] = (double)fs[m];
+  }
+  __syncthreads();
+  const double cxk = okx ? cx_k : 0.0, sxl = okx ? sxl_k : 0.0, sxr = okx ? sxr_k : 0.0;
   double ab = 0.0, at = 0.0;
 #pragma unroll
   for (int m = 0; m < RTY / 4; ++m) {
