@@ -181,9 +181,10 @@ __global__ __launch_bounds__(256) void ring_reduce_kernel(const RingChainArgs a,
   for (int b = q0; b < q1; b += 16) {
     double v[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = ok && b + q < q1 ? src[(size_t)(b + q) * stride] : 0.0;
+    for (int q = 0; q < 16; ++q) v[q] = src[(size_t)min(b + q, q1 - 1) * stride];  // clamped: unconditional
 #pragma unroll
-    for (int q = 0; q < 16; ++q) sum += v[q];
+    for (int q = 0; q < 16; ++q)
+      if (b + q < q1) sum += v[q];
   }
   // quarters 0..3 of the entry sit in consecutive lanes: ((s0 + s1) + s2) + s3
   const double s1 = __shfl_down(sum, 1, 64), s2 = __shfl_down(sum, 2, 64), s3 = __shfl_down(sum, 3, 64);
