@@ -517,6 +517,41 @@ hipError_t launch_gemv(int m, const double* M, const double* b, double* x, doubl
   return hipGetLastError();
 }
 
+// x = M b with M row-major (m x m): one wave per row, lane l summing columns l, l + 64,
+// ... in order (all its loads of a 1024-column chunk in flight together), the lane sums
+// added by a fixed shuffle tree: one launch, a fixed summation order.
+__global__ __launch_bounds__(256) void gemv_rows_kernel(int m, const double* __restrict__ M,
+                                                        const double* __restrict__ b,
+                                                        double* __restrict__ x) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= m) return;  // whole waves
+  const double* row = M + (size_t)i * m;
+  double s = 0.0;
+  for (int j0 = 0; j0 < m; j0 += 1024) {
+    double mv[16], bv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int j = j0 + lane + 64 * q;
+      const int jc = j < m ? j : 0;  // clamped: unconditional
+      mv[q] = row[jc];
+      bv[q] = b[jc];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (j0 + lane + 64 * q < m) s += mv[q] * bv[q];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) x[i] = s;
+}
+
+hipError_t launch_gemv_rows(int m, const double* M, const double* b, double* x, hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gemv_rows_kernel, dim3((m + 3) / 4), dim3(256), 0, st, m, M, b, x);
+  return hipGetLastError();
+}
+
 hipError_t launch_forces(int n, const double* Ft, double* fb, double* total, double h, double dt,
                          double* host_total, hipStream_t st) {
   hipLaunchKernelGGL(forces_kernel, dim3(1), dim3(1024), 0, st, n, Ft, fb, total, h, dt, host_total);

@@ -1087,7 +1087,7 @@ int couple_surface(ibpm_solver* s, ibpm_step_stats* stats) {
   ga.h = s->h;
   HIPCHK(launch_gather(ga, s->st));
   if (!s->moving) {
-    HIPCHK(launch_gemv(2 * s->n, s->Binv, s->Ftmp, s->Ft, s->gemv_part, s->st));
+    HIPCHK(launch_gemv_rows(2 * s->n, s->Binv, s->Ftmp, s->Ft, s->st));
     if (stats) {
       stats->cg_iters = 0;
       stats->cg_converged = 1;
@@ -1768,6 +1768,15 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
     if (!s->moving) {
       rc = assemble_B_inverse(s, s->Binv);
       if (rc) return bail(rc);
+      {  // stored row-major for the per-step product (launch_gemv_rows)
+        const size_t m = 2 * (size_t)n;
+        std::vector<double> cm(m * m), rm(m * m);
+        CK(hipStreamSynchronize(s->st));  // the assembly runs on the solver's stream
+        CK(hipMemcpy(cm.data(), s->Binv, m * m * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t j = 0; j < m; ++j)
+          for (size_t i = 0; i < m; ++i) rm[i * m + j] = cm[i + j * m];
+        CK(hipMemcpy(s->Binv, rm.data(), m * m * sizeof(double), hipMemcpyHostToDevice));
+      }
       rc = alloc_bscratch(s, 0);  // static bodies never apply B again
       if (rc) return bail(rc);
     } else {

@@ -282,6 +282,8 @@ hipError_t launch_body_bbox(const int* coords, int n, double* out, hipStream_t s
 size_t gemv_partial_count(int m);
 hipError_t launch_gemv(int m, const double* M, const double* b, double* x, double* part,
                        hipStream_t st);
+// x = M b, M row-major m x m (one launch)
+hipError_t launch_gemv_rows(int m, const double* M, const double* b, double* x, hipStream_t st);
 // fb = Ft*h/dt and the total force (column sums), FP64; the total also to host_total
 // (device address of mapped pinned host memory) when that is not null.
 hipError_t launch_forces(int n, const double* Ft, double* fb, double* total, double h, double dt,

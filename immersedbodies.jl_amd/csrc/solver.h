@@ -81,7 +81,7 @@ struct ibpm_solver {
   int *xstart = nullptr, *err = nullptr, *ywin = nullptr;
   double* gemv_part = nullptr;
   double* w = nullptr;
-  double* Binv = nullptr;
+  double* Binv = nullptr;  // static bodies: B^-1, row-major once assembled
   // B operator scratch (ncol columns)
   int bcols = 0;
   double *Gtmp = nullptr, *psitmp = nullptr, *Sc = nullptr;
