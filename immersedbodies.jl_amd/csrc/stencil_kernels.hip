@@ -289,9 +289,15 @@ __global__ void fill_kernel(double* p, size_t n, double v) {
   size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i < n) p[i] = v;
 }
+// 4 elements per thread, 256 apart (coalesced), all four loads issued before the stores
 __global__ void copy_kernel(double* __restrict__ d, const double* __restrict__ s, size_t n) {
-  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) d[i] = s[i];
+  const size_t i0 = (size_t)blockIdx.x * 1024 + threadIdx.x;
+  double v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = s[i0 + 256 * k < n ? i0 + 256 * k : n - 1];  // clamped
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (i0 + 256 * k < n) d[i0 + 256 * k] = v[k];
 }
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t st) {
   if (!n) return hipSuccess;
@@ -312,7 +318,7 @@ hipError_t launch_sub(double* d, const double* a, const double* b, size_t n, hip
 
 hipError_t launch_copy(double* d, const double* s, size_t n, hipStream_t st) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, s, n);
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((n + 1023) / 1024)), dim3(256), 0, st, d, s, n);
   return hipGetLastError();
 }
 
