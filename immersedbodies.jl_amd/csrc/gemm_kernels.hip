@@ -20,6 +20,9 @@ constexpr int GT = 64;  // C tile edge
 constexpr int GK = 32;  // K chunk (the next chunk's loads fly during this one's products)
 constexpr int GR = GT * GK / 256;  // elements of A (and of B) each thread stages per chunk
 
+// DA, DB: some job of the launch has a difference term A2 (B2); without, the operand is
+// loaded once.
+template <bool DA, bool DB>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
   // the job's fields in registers (uniform), read once; a missing difference term reads
   // the matrix itself with weight 0 (a - 0 * a = a exactly), so no load is conditional
@@ -63,12 +66,12 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
     _Pragma("unroll") for (int r = 0; r < GR; ++r) {                              \
       const size_t o = am + (size_t)min((k0) + tq + 4 * r, K - 1) * lda;          \
       ra[r] = A[o];                                                               \
-      ra2[r] = A2[o];                                                             \
+      if (DA) ra2[r] = A2[o];                                                     \
     }                                                                             \
     _Pragma("unroll") for (int r = 0; r < GR; ++r) {                              \
       const size_t o = bk_ + (size_t)min(n0 + tnb + NS * r, N - 1) * ldb;         \
       rb[r] = B[o];                                                               \
-      rb2[r] = B2[o];                                                             \
+      if (DB) rb2[r] = B2[o];                                                     \
     }                                                                             \
   }
   // C entries of this thread (accumulate): loaded now, in flight during the products
@@ -89,7 +92,8 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
 #pragma unroll
     for (int r = 0; r < GR; ++r) {
       const bool aok = mok && k0 + tq + 4 * r < K, bok = kbok && n0 + tnb + NS * r < N;
-      const double a = fma(-wa, ra2[r], ra[r]), b = fma(-wb, rb2[r], rb[r]);
+      const double a = DA ? fma(-wa, ra2[r], ra[r]) : ra[r];
+      const double b = DB ? fma(-wb, rb2[r], rb[r]) : rb[r];
       As[tq + 4 * r][tm] = aok ? a : 0.0;
       Bs[tkb][tnb + NS * r] = bok ? b : 0.0;
     }
@@ -156,14 +160,25 @@ hipError_t launch_sum_slices(const double* P, size_t stride, int nslice, int ldp
 
 hipError_t launch_gemm_f64(const GemmBatch& gb, int njobs, hipStream_t st) {
   int tiles = 0;
+  bool da = false, db = false;
   for (int i = 0; i < njobs; ++i) {
     const GemmJob& j = gb.j[i];
     if (j.M <= 0 || j.N <= 0 || j.K <= 0) return hipErrorInvalidValue;
     const int t = ((j.M + GT - 1) / GT) * ((j.N + GT - 1) / GT);
     tiles = t > tiles ? t : tiles;
+    da = da || j.A2;
+    db = db || j.B2;
   }
   if (njobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(gemm_f64_kernel, dim3(tiles, njobs), dim3(256), 0, st, gb);
+  const dim3 grid(tiles, njobs);
+  if (da && db)
+    hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, dim3(256), 0, st, gb);
+  else if (da)
+    hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, dim3(256), 0, st, gb);
+  else if (db)
+    hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, dim3(256), 0, st, gb);
+  else
+    hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, dim3(256), 0, st, gb);
   return hipGetLastError();
 }
 
