@@ -74,18 +74,6 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
       if (DB) rb2[r] = B2[o];                                                     \
     }                                                                             \
   }
-  // C entries of this thread (accumulate): loaded now, in flight during the products
-  double cin[2][2][4];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = min(m0 + wm * 32 + j * 16 + col, M - 1);
-        const int n = min(n0 + wn * 32 + i * 16 + kq + 4 * r, N - 1);
-        cin[i][j][r] = accumulate ? C[(size_t)m + (size_t)n * ldc] : 0.0;
-      }
   GEMM_FETCH(0)
   for (int k0 = 0; k0 < K; k0 += GK) {
     const bool kbok = k0 + tkb < K;
@@ -116,6 +104,19 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
     __syncthreads();
   }
 #undef GEMM_FETCH
+  // C entries of this thread (accumulate), loaded after the products: not held in
+  // registers through them (other work-groups on the CU cover the wait)
+  double cin[2][2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = min(m0 + wm * 32 + j * 16 + col, M - 1);
+        const int n = min(n0 + wn * 32 + i * 16 + kq + 4 * r, N - 1);
+        cin[i][j][r] = accumulate ? C[(size_t)m + (size_t)n * ldc] : 0.0;
+      }
   // acc[i][j][r] at lane: C(m = m0 + wm*32 + j*16 + col, n = n0 + wn*32 + i*16 + kq + 4r)
 #pragma unroll
   for (int i = 0; i < 2; ++i)
