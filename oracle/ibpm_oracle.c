@@ -22,6 +22,23 @@
 #define REG_SUPPORT 3
 #define NSUPP 7
 
+/* Planted mistakes. The oracle's math checks (tests/test_oracle_pins.py) build this file
+ * again with -DIBO_MUTATION=k to show that each check fails on the slip it guards against.
+ * The default build (IBO_MUTATION 0) contains none of them.
+ *   1  get_trial_state: boundary-source factor 0.5 dt/(Re hc^2) instead of 0.25 (fluid.jl:475)
+ *   2  get_trial_state: coarse ring scaled by 0.5 instead of 0.25 (fluid.jl:480)
+ *   3  vort2flux: boundary values from the level being solved instead of level+1 (fluid.jl:195)
+ *   4  vort2flux: apply_bc with fac -1 instead of 1 (fluid.jl:196)
+ *   5  project_circ: Gamma_1 += instead of -= (coupling.jl:227)
+ *   6  couple_surface_rigid: + ub h instead of - ub h (coupling.jl:245)
+ *   7  get_trial_state: boundary-source factor with the step of level+1 (fluid.jl:475)
+ *   8  vort2flux: coarsify levels from coarse to fine, from stale finer planes (fluid.jl:182-184)
+ */
+#ifndef IBO_MUTATION
+#define IBO_MUTATION 0
+#endif
+#define MUT(k) (IBO_MUTATION == (k))
+
 /* ------------------------------------------------------------------------------------
  * Complex FFT (mixed radix, recursive decimation in time) used for the DST-I.
  * The reference calls FFTW's RODFT00 (fluid.jl:113-115); that library is not present,
@@ -450,8 +467,13 @@ static void vort2flux_diag(const ibo_grid* g, const double* dpois, double* psi, 
                            double* G) {
   int L = g->nlevel;
   size_t nG = n_gamma(g), nq = n_q(g), nbc = n_bc(g);
-  for (int lev = 2; lev <= L; ++lev)
-    ibo_coarsify(g, G + (size_t)(lev - 1) * nG, G + (size_t)(lev - 2) * nG);
+  if (MUT(8)) {
+    for (int lev = L; lev >= 2; --lev)
+      ibo_coarsify(g, G + (size_t)(lev - 1) * nG, G + (size_t)(lev - 2) * nG);
+  } else {
+    for (int lev = 2; lev <= L; ++lev)
+      ibo_coarsify(g, G + (size_t)(lev - 1) * nG, G + (size_t)(lev - 2) * nG);
+  }
   memset(psi, 0, sizeof(double) * nG * L);
   double* bc = (double*)calloc(nbc, sizeof(double));
   double* Gt = (double*)malloc(sizeof(double) * nG);
@@ -459,8 +481,8 @@ static void vort2flux_diag(const ibo_grid* g, const double* dpois, double* psi, 
   ibo_curl(g, q + (size_t)(L - 1) * nq, psi + (size_t)(L - 1) * nG, bc);
   for (int lev = L - 1; lev >= 1; --lev) {
     memcpy(Gt, G + (size_t)(lev - 1) * nG, sizeof(double) * nG);
-    ibo_get_bc(g, bc, psi + (size_t)lev * nG);
-    ibo_apply_bc(g, Gt, bc, 1.0);
+    ibo_get_bc(g, bc, psi + (size_t)(MUT(3) ? lev - 1 : lev) * nG);
+    ibo_apply_bc(g, Gt, bc, MUT(4) ? -1.0 : 1.0);
     ibo_lap_inv(g, dpois, psi + (size_t)(lev - 1) * nG, Gt);
     ibo_curl(g, q + (size_t)(lev - 1) * nq, psi + (size_t)(lev - 1) * nG, bc);
   }
@@ -896,10 +918,11 @@ static void get_trial_state(ibo_solver* s) {
     memset(s->rhsbc, 0, sizeof(double) * nG);
     double hc = gridstep(g, lev);
     if (lev < L) {
+      double hf = MUT(7) ? gridstep(g, lev + 1) : hc;
       ibo_get_bc(g, s->bc, s->G + (size_t)lev * nG);
-      ibo_apply_bc(g, s->rhsbc, s->bc, 0.25 * s->dt / (s->Re * (hc * hc)));
+      ibo_apply_bc(g, s->rhsbc, s->bc, (MUT(1) ? 0.5 : 0.25) * s->dt / (s->Re * (hf * hf)));
     }
-    for (size_t k = 0; k < s->nbc; ++k) s->bc[k] *= 0.25;
+    for (size_t k = 0; k < s->nbc; ++k) s->bc[k] *= MUT(2) ? 0.5 : 0.25;
     ibo_nonlinear(g, s->nonlin[0] + (size_t)(lev - 1) * nG, s->q + (size_t)(lev - 1) * nq,
                   s->q0 + (size_t)(lev - 1) * nq, s->G + (size_t)(lev - 1) * nG, s->bc, lev);
     lap_inv_level(s, s->dA + (size_t)(lev - 1) * nG, s->rhs, s->G + (size_t)(lev - 1) * nG);
@@ -992,7 +1015,7 @@ int ibo_advance(ibo_solver* s, const double fs_vel[2], const double* motion, cha
   if (n > 0) {
     for (size_t k = 0; k < nq; ++k) s->q1[k] = s->qs[k] + s->q0[k];
     ibo_regT(g, n, s->coords, s->w, s->Ftmp, s->q1);
-    for (int k = 0; k < 2 * n; ++k) s->Ftmp[k] -= s->ub[k] * h;
+    for (int k = 0; k < 2 * n; ++k) s->Ftmp[k] -= (MUT(6) ? -1.0 : 1.0) * (s->ub[k] * h);
     if (!s->moving) {
       int m = 2 * n;
       for (int i = 0; i < m; ++i) {
@@ -1011,7 +1034,10 @@ int ibo_advance(ibo_solver* s, const double fs_vel[2], const double* motion, cha
     ibo_reg(g, n, s->coords, s->w, s->q1, s->Ft);
     ibo_rot(g, s->rhsbc, s->q1);
     lap_inv_level(s, s->dAinv, s->Gs, s->rhsbc);
-    for (size_t k = 0; k < nG; ++k) s->G[k] -= s->Gs[k];
+    if (MUT(5))
+      for (size_t k = 0; k < nG; ++k) s->G[k] += s->Gs[k];
+    else
+      for (size_t k = 0; k < nG; ++k) s->G[k] -= s->Gs[k];
   }
   /* final vort2flux, solver.jl:184 */
   vort2flux_diag(g, s->dpois, s->psi, s->q, s->G);

@@ -9,6 +9,7 @@ The oracle restates the reference's FP64 algorithm (``src/fluid.jl``,
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -30,6 +31,29 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
+def build_variant(mutation: int, outdir: str) -> str:
+    """Build the oracle with planted mistake ``mutation`` (the IBO_MUTATION list at the top
+    of ibpm_oracle.c) into ``outdir``; returns the library path. Used only by the tests
+    that show each math check of the oracle fails on the slip it guards against."""
+    out = os.path.join(outdir, f"libibpm_oracle_m{int(mutation)}.so")
+    subprocess.run(["make", "-s", "-C", _HERE, f"MUTATION={int(mutation)}", f"OUT={out}",
+                    "variant"], check=True, stdout=subprocess.DEVNULL)
+    return out
+
+
+@contextlib.contextmanager
+def variant(path: str):
+    """Within the block, every function of this module (and every OracleSolver created in
+    it) uses the oracle library at ``path``."""
+    global _lib
+    saved = _lib
+    _lib = _bind(C.CDLL(path))
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
 class _Grid(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nlevel", C.c_int),
                 ("x0", C.c_double), ("y0", C.c_double), ("h", C.c_double)]
@@ -48,50 +72,53 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = C.CDLL(_LIB_PATH)
-        g = C.POINTER(_Grid)
-        L.ibo_dst2d.argtypes = [g, _dp]
-        L.ibo_dst1d_batch.argtypes = [C.c_int, C.c_int, _dp]
-        L.ibo_lap_eigs.argtypes = [g, _dp]
-        L.ibo_lap_inv.argtypes = [g, _dp, _dp, _dp]
-        L.ibo_diag_poisson.argtypes = [g, _dp]
-        L.ibo_diag_A.argtypes = [g, C.c_double, C.c_double, C.c_int, C.c_int, _dp]
-        L.ibo_rot.argtypes = [g, _dp, _dp]
-        L.ibo_curl.argtypes = [g, _dp, _dp, _dp]
-        L.ibo_coarsify.argtypes = [g, _dp, _dp]
-        L.ibo_get_bc.argtypes = [g, _dp, _dp]
-        L.ibo_apply_bc.argtypes = [g, _dp, _dp, C.c_double]
-        L.ibo_nonlinear.argtypes = [g, _dp, _dp, _dp, _dp, _dp, C.c_int]
-        L.ibo_vort2flux.argtypes = [g, _dp, _dp, _dp]
-        L.ibo_delta_h.argtypes = [C.c_double, C.c_double, C.c_double]
-        L.ibo_delta_h.restype = C.c_double
-        L.ibo_reg_update.argtypes = [g, C.c_int, _dp, _ip, _dp]
-        L.ibo_reg_update.restype = C.c_int
-        L.ibo_regT.argtypes = [g, C.c_int, _ip, _dp, _dp, _dp]
-        L.ibo_reg.argtypes = [g, C.c_int, _ip, _dp, _dp, _dp]
-        L.ibo_create.argtypes = [C.POINTER(_Config), C.c_char_p, C.c_size_t]
-        L.ibo_create.restype = C.c_void_p
-        L.ibo_destroy.argtypes = [C.c_void_p]
-        L.ibo_advance.argtypes = [C.c_void_p, _dp, _dp, C.c_char_p, C.c_size_t]
-        L.ibo_advance.restype = C.c_int
-        L.ibo_field.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_size_t)]
-        L.ibo_field.restype = _dp
-        L.ibo_index.argtypes = [C.c_void_p]
-        L.ibo_index.restype = C.c_int
-        L.ibo_time.argtypes = [C.c_void_p]
-        L.ibo_time.restype = C.c_double
-        L.ibo_set_index.argtypes = [C.c_void_p, C.c_int]
-        L.ibo_last_cg_iters.argtypes = [C.c_void_p]
-        L.ibo_last_cg_iters.restype = C.c_int
-        L.ibo_last_cg_residual.argtypes = [C.c_void_p]
-        L.ibo_last_cg_residual.restype = C.c_double
-        L.ibo_apply_B.argtypes = [C.c_void_p, _dp, _dp]
-        L.ibo_Binv.argtypes = [C.c_void_p]
-        L.ibo_Binv.restype = _dp
-        L.ibo_set_threads.argtypes = [C.c_int]
-        L.ibo_set_grid_motion.argtypes = [C.c_void_p, _dp]
-        _lib = L
+        _lib = _bind(C.CDLL(_LIB_PATH))
     return _lib
+
+
+def _bind(L):
+    g = C.POINTER(_Grid)
+    L.ibo_dst2d.argtypes = [g, _dp]
+    L.ibo_dst1d_batch.argtypes = [C.c_int, C.c_int, _dp]
+    L.ibo_lap_eigs.argtypes = [g, _dp]
+    L.ibo_lap_inv.argtypes = [g, _dp, _dp, _dp]
+    L.ibo_diag_poisson.argtypes = [g, _dp]
+    L.ibo_diag_A.argtypes = [g, C.c_double, C.c_double, C.c_int, C.c_int, _dp]
+    L.ibo_rot.argtypes = [g, _dp, _dp]
+    L.ibo_curl.argtypes = [g, _dp, _dp, _dp]
+    L.ibo_coarsify.argtypes = [g, _dp, _dp]
+    L.ibo_get_bc.argtypes = [g, _dp, _dp]
+    L.ibo_apply_bc.argtypes = [g, _dp, _dp, C.c_double]
+    L.ibo_nonlinear.argtypes = [g, _dp, _dp, _dp, _dp, _dp, C.c_int]
+    L.ibo_vort2flux.argtypes = [g, _dp, _dp, _dp]
+    L.ibo_delta_h.argtypes = [C.c_double, C.c_double, C.c_double]
+    L.ibo_delta_h.restype = C.c_double
+    L.ibo_reg_update.argtypes = [g, C.c_int, _dp, _ip, _dp]
+    L.ibo_reg_update.restype = C.c_int
+    L.ibo_regT.argtypes = [g, C.c_int, _ip, _dp, _dp, _dp]
+    L.ibo_reg.argtypes = [g, C.c_int, _ip, _dp, _dp, _dp]
+    L.ibo_create.argtypes = [C.POINTER(_Config), C.c_char_p, C.c_size_t]
+    L.ibo_create.restype = C.c_void_p
+    L.ibo_destroy.argtypes = [C.c_void_p]
+    L.ibo_advance.argtypes = [C.c_void_p, _dp, _dp, C.c_char_p, C.c_size_t]
+    L.ibo_advance.restype = C.c_int
+    L.ibo_field.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_size_t)]
+    L.ibo_field.restype = _dp
+    L.ibo_index.argtypes = [C.c_void_p]
+    L.ibo_index.restype = C.c_int
+    L.ibo_time.argtypes = [C.c_void_p]
+    L.ibo_time.restype = C.c_double
+    L.ibo_set_index.argtypes = [C.c_void_p, C.c_int]
+    L.ibo_last_cg_iters.argtypes = [C.c_void_p]
+    L.ibo_last_cg_iters.restype = C.c_int
+    L.ibo_last_cg_residual.argtypes = [C.c_void_p]
+    L.ibo_last_cg_residual.restype = C.c_double
+    L.ibo_apply_B.argtypes = [C.c_void_p, _dp, _dp]
+    L.ibo_Binv.argtypes = [C.c_void_p]
+    L.ibo_Binv.restype = _dp
+    L.ibo_set_threads.argtypes = [C.c_int]
+    L.ibo_set_grid_motion.argtypes = [C.c_void_p, _dp]
+    return L
 
 
 def _p(a: np.ndarray):
@@ -291,25 +318,26 @@ class OracleSolver:
                       _pi(self._bs), _pi(self._bc), _pi(self._bm), cg_maxiter, cg_reltol,
                       cg_abstol, cg_tol_mode, int(timing_only), int(grid_moving))
         err = C.create_string_buffer(512)
-        self._h = lib().ibo_create(C.byref(cfg), err, 512)
+        self._L = lib()
+        self._h = self._L.ibo_create(C.byref(cfg), err, 512)
         if not self._h:
             raise RuntimeError(err.value.decode())
         self.Re, self.dt, self.t0 = Re, dt, t0
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().ibo_destroy(self._h)
+            self._L.ibo_destroy(self._h)
             self._h = None
 
     __del__ = close
 
     @property
     def index(self):
-        return lib().ibo_index(self._h)
+        return self._L.ibo_index(self._h)
 
     @property
     def t(self):
-        return lib().ibo_time(self._h)
+        return self._L.ibo_time(self._h)
 
     def next_time(self):
         return self.t0 + self.dt * self.index
@@ -319,33 +347,43 @@ class OracleSolver:
         y, velocity x, y, angle, angular_vel)."""
         if grid_motion is not None:
             gm = np.ascontiguousarray(np.asarray(grid_motion, dtype=np.float64).reshape(6))
-            lib().ibo_set_grid_motion(self._h, _p(gm))
+            self._L.ibo_set_grid_motion(self._h, _p(gm))
         fs = np.ascontiguousarray(np.asarray(freestream, dtype=np.float64)[:2])
         m = np.zeros(6 * max(self.nbody, 1)) if motion is None else \
             np.ascontiguousarray(np.asarray(motion, dtype=np.float64).reshape(-1))
         err = C.create_string_buffer(512)
-        rc = lib().ibo_advance(self._h, _p(fs), _p(m), err, 512)
+        rc = self._L.ibo_advance(self._h, _p(fs), _p(m), err, 512)
         if rc:
             raise RuntimeError(err.value.decode())
 
     def field(self, name):
         n = C.c_size_t()
-        ptr = lib().ibo_field(self._h, FIELDS[name], C.byref(n))
+        ptr = self._L.ibo_field(self._h, FIELDS[name], C.byref(n))
         return np.ctypeslib.as_array(ptr, shape=(n.value,)).copy()
 
+    def set_field(self, name, values):
+        """Overwrite a state array (q, q0, Gamma, psi, nonlin1, nonlin2, ...) in place, in
+        the layout ``field`` returns."""
+        assert name != "total_force"
+        n = C.c_size_t()
+        ptr = self._L.ibo_field(self._h, FIELDS[name], C.byref(n))
+        v = np.asarray(values, dtype=np.float64).reshape(-1)
+        assert v.size == n.value
+        np.ctypeslib.as_array(ptr, shape=(n.value,))[:] = v
+
     def cg_iters(self):
-        return lib().ibo_last_cg_iters(self._h)
+        return self._L.ibo_last_cg_iters(self._h)
 
     def cg_residual(self):
-        return lib().ibo_last_cg_residual(self._h)
+        return self._L.ibo_last_cg_residual(self._h)
 
     def apply_B(self, z):
         x = np.zeros(2 * self.n)
-        lib().ibo_apply_B(self._h, _p(x), _p(np.ascontiguousarray(z, dtype=np.float64)))
+        self._L.ibo_apply_B(self._h, _p(x), _p(np.ascontiguousarray(z, dtype=np.float64)))
         return x
 
     def Binv(self):
-        p = lib().ibo_Binv(self._h)
+        p = self._L.ibo_Binv(self._h)
         if not p:
             return None
         m = 2 * self.n
