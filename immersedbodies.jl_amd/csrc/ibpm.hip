@@ -12,8 +12,10 @@
 #include <string.h>
 
 #include <algorithm>
-#include <string>
+#include <atomic>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/ibpm.h"
@@ -1411,6 +1413,65 @@ int solver_step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_st
 }
 }  // namespace ibpm
 
+// ---- Operator access that allocates nothing (ibpm_apply_noalloc) ----
+//
+// Its intermediate planes live in this file-scope device array: one copy per GPU, at its
+// own address, held until the process ends. On the host the name `scratch` only stands
+// for the symbol; each device's address is looked up once (hipGetSymbolAddress, at the
+// first ibpm_create on that device) and every launch takes that address.
+static __device__ __align__(256) double scratch[IBPM_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+std::atomic<double*> g_noalloc[kNoallocMaxDevices];
+hipError_t g_noalloc_err[kNoallocMaxDevices];
+
+// With `device` current: that device's address of `scratch`, once per device.
+void noalloc_setup(int device) {
+  if (device < 0 || device >= kNoallocMaxDevices) return;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (g_noalloc[device].load()) return;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_noalloc_err[device] = e;
+  if (e == hipSuccess && p) g_noalloc[device].store((double*)p);
+  else (void)hipGetLastError();
+}
+
+constexpr size_t round256(size_t doubles) { return (doubles + 31) & ~(size_t)31; }
+// carve-outs end 64 KiB before the array's end, so no clamped or vector load of the last
+// plane can reach past it
+constexpr size_t kNoallocUsable = IBPM_NOALLOC_SCRATCH_BYTES - 65536;
+
+// Points the solver's scratch (sine-pass planes T, U, R and the ring chain's factors and
+// partial projections) at a carve-out of `scratch` for one call, and back afterwards.
+struct ScratchSwap {
+  ibpm_solver* s;
+  double *T, *U, *R, *fac, *rvec, *rpart;
+  ScratchSwap(ibpm_solver* s_, double* T_, double* U_, double* R_, double* fac_,
+              double* rvec_, double* rpart_)
+      : s(s_), T(s_->T), U(s_->U), R(s_->R), fac(s_->fac), rvec(s_->rvec), rpart(s_->rpart) {
+    s->T = T_;
+    s->U = U_;
+    s->R = R_;
+    s->fac = fac_;
+    s->rvec = rvec_;
+    s->rpart = rpart_;
+  }
+  ~ScratchSwap() {
+    s->T = T;
+    s->U = U;
+    s->R = R;
+    s->fac = fac;
+    s->rvec = rvec;
+    s->rpart = rpart;
+  }
+};
+
+}  // namespace
+
 extern "C" {
 
 void ibpm_default_options(ibpm_options* o) {
@@ -1484,6 +1545,7 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= opt.device)
     return fail(IBPM_E_NO_DEVICE, "no HIP device");
   HIPCHK(hipSetDevice(opt.device));
+  noalloc_setup(opt.device);
 
   ibpm_solver* s = new ibpm_solver();
   s->g = make_dims(p->nx, p->ny);
@@ -2054,6 +2116,75 @@ int ibpm_apply(ibpm_solver* s, int op, int level, const double* in, double* out,
   HIPCHK(hipStreamSynchronize(s->st));
   HIPCHK(hipMemcpy(out, s->rhs + P, need * 8, hipMemcpyDeviceToHost));
   return IBPM_OK;
+}
+
+int ibpm_apply_noalloc(ibpm_solver* s, int op, int level, const double* in, double* out,
+                       size_t n) {
+  if (!s || !in || !out) return fail(IBPM_E_ARG, "null argument");
+  const size_t P = s->plane();
+  const int L = s->L;
+  if (op < IBPM_OP_POISSON || op > IBPM_OP_VORT2FLUX) return fail(IBPM_E_ARG, "unknown op");
+  const bool v2f = op == IBPM_OP_VORT2FLUX;
+  const size_t need = v2f ? P * L : P;
+  if (n != need) return fail(IBPM_E_ARG, "size mismatch: expected " + std::to_string(need));
+  if (op == IBPM_OP_HELMHOLTZ && (level < 1 || level > L)) return fail(IBPM_E_ARG, "bad level");
+  const uintptr_t a0 = (uintptr_t)in, b0 = (uintptr_t)out, len = need * sizeof(double);
+  if (a0 < b0 + len && b0 < a0 + len) return fail(IBPM_E_ARG, "in and out must not overlap");
+  if (s->device < 0 || s->device >= kNoallocMaxDevices)
+    return fail(IBPM_E_ARG, "device ordinal beyond the scratch table");
+  HIPCHK(hipSetDevice(s->device));
+  double* base = g_noalloc[s->device].load();
+  if (!base)
+    return fail(IBPM_E_HIP, std::string("no scratch address on this device: ") +
+                                hipGetErrorString(g_noalloc_err[s->device]));
+  // carve-outs, each starting on a 256-byte boundary: vort2flux stages its input (it
+  // coarsifies in place) and uses L planes of T, U (and R in mixed precision), the ring
+  // factors and the partial projections; a single solve one plane of each
+  const bool mixed = s->precision == IBPM_MIXED;
+  const size_t np = v2f ? (size_t)L : 1;
+  const size_t nf = 2 * (size_t)s->g.Nx + 2 * (size_t)s->g.Ny;
+  size_t off = 0;
+  auto take = [&](size_t k) { double* p = base + off; off += round256(k); return p; };
+  double* stage = v2f ? take(P * L) : nullptr;
+  double* T = take(P * np);
+  double* U = take(P * np);
+  double* R = mixed ? take(P * np) : nullptr;
+  double* fac = v2f ? take((size_t)L * nf) : nullptr;
+  double* rvec = v2f ? take(ring_vector_count(s->g.Nx, s->g.Ny)) : nullptr;
+  double* rpart = v2f ? take(ring_partial_count(s->g.Nx, s->g.Ny)) : nullptr;
+  if (off * sizeof(double) > kNoallocUsable)
+    return fail(IBPM_E_ARG, "operator needs " + std::to_string(off * sizeof(double)) +
+                                " bytes of scratch; ibpm_apply_noalloc has " +
+                                std::to_string(kNoallocUsable));
+  if (!v2f && L < 2) return fail(IBPM_E_ARG, "operator access needs nlevel >= 2");
+  ScratchSwap swap(s, T, U, R, fac, rvec, rpart);
+  if (v2f) {
+    HIPCHK(hipMemcpyAsync(stage, in, need * sizeof(double), hipMemcpyDeviceToDevice, s->st));
+    double* Gc[1] = {stage};
+    double* pc[1] = {out};
+    return vort2flux(s, 1, Gc, pc, nullptr, nullptr);
+  }
+  SolveJob j;
+  j.b = in;
+  j.x = out;
+  j.epi = EPI_STORE;
+  j.base = nullptr;
+  if (op == IBPM_OP_POISSON) {
+    j.da = 0.0;
+    j.dnum = 1.0;
+    j.dden = 1.0;
+    j.a = 0.0;
+    j.c = 1.0;
+  } else {
+    const double hc = s->hc[level - 1];
+    j.da = 1.0;
+    j.dnum = s->dt;
+    j.dden = 2.0 * s->Re * (hc * hc);
+    j.a = 1.0;
+    j.c = cA(s, level - 1);
+  }
+  // one job: solve()'s routing without its host-side job lists
+  return use_gs(s, j) ? solve_gs(s, &j, 1) : solve_chunk(s, &j, 1);
 }
 
 int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, double* flops,

@@ -34,7 +34,7 @@ EXPORTS = ("ibpm_default_options", "ibpm_create", "ibpm_advance", "ibpm_read", "
            "ibpm_net_train", "ibpm_net_param_count", "ibpm_net_get_params",
            "ibpm_net_set_params", "ibpm_net_forward", "ibpm_net_gradients", "ibpm_net_step",
            "ibpm_batch_create", "ibpm_batch_advance", "ibpm_batch_destroy", "ibpm_dst_tiles", "ibpm_set_body_motion",
-           "ibpm_box_update")
+           "ibpm_box_update", "ibpm_apply_noalloc")
 
 IBPM_WS_PREVIOUS = 0
 IBPM_WS_EXTRAPOLATE = 1
@@ -45,6 +45,7 @@ NET_OUTPUTS = 2
 IBPM_OP_POISSON = 0
 IBPM_OP_HELMHOLTZ = 1
 IBPM_OP_VORT2FLUX = 2
+NOALLOC_SCRATCH_BYTES = 32 << 20  # IBPM_NOALLOC_SCRATCH_BYTES
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -126,6 +127,10 @@ def load():
     L.ibpm_time_kernel.restype = C.c_int
     L.ibpm_apply.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_size_t]
     L.ibpm_apply.restype = C.c_int
+    # device pointers
+    L.ibpm_apply_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                     C.c_size_t]
+    L.ibpm_apply_noalloc.restype = C.c_int
     _fp = C.POINTER(C.c_float)
     L.ibpm_set_warm_start.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
     L.ibpm_set_warm_start.restype = C.c_int

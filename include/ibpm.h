@@ -179,6 +179,22 @@ int ibpm_synchronize(ibpm_solver* s);
 enum ibpm_op { IBPM_OP_POISSON = 0, IBPM_OP_HELMHOLTZ = 1, IBPM_OP_VORT2FLUX = 2 };
 int ibpm_apply(ibpm_solver* s, int op, int level, const double* in, double* out, size_t n);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * operators of ibpm_apply, same kernels and results, on DEVICE buffers `in` and `out`
+ * (non-overlapping), queued on the solver's stream without waiting (ibpm_synchronize
+ * waits). It calls no hipMalloc, hipFree or other allocating routine: its intermediate
+ * planes live in one static device buffer of the library, IBPM_NOALLOC_SCRATCH_BYTES per
+ * GPU (less a 64 KiB tail it leaves unused), held until the process ends. An operator that
+ * needs more fails with IBPM_E_ARG and the size in the message: about (2 + mixed) planes
+ * of (nx-1)(ny-1) doubles for the two solves, (3 + mixed) nlevel planes for vort2flux.
+ * Every call on one GPU shares that buffer: calls that overlap in time on one GPU (from
+ * different streams or threads) race, and must be serialised by the caller. Nothing in the
+ * buffer needs to start at zero; it keeps the last call's contents. */
+#define IBPM_NOALLOC_SCRATCH_BYTES (32u << 20)
+int ibpm_apply_noalloc(ibpm_solver* s, int op, int level, const double* in, double* out,
+                       size_t n);
+
 /* Instrumentation, no counterpart in the reference: time `reps` launches of one kernel of
  * the step with HIP events on the solver's stream, on the solver's current fields.
  *   kernel 0: sine-contraction pass of the batched implicit solve (all levels at once)
