@@ -262,6 +262,30 @@ def test_cylinder_1024_matches_oracle_fixture():
         assert err <= 1e-10 * np.linalg.norm(ref[f"{name}/norms"]), (name, err)
 
 
+def test_cylinder_1024_mixed_matches_oracle_fixture():
+    """The mixed path (3xBF16 sine contractions with FP32 accumulation, two FP64 refinement
+    sweeps) on the full-size 1024 x 1024, 5-level cylinder against the FP64 oracle frozen in
+    tests/golden/oracle_cylinder1024.npz: the force within 1e-6 at every step (the mixed
+    path's target), F~b within 1e-6, per-level norms within 1e-8."""
+    import os
+
+    path = os.path.join(os.path.dirname(__file__), "golden", "oracle_cylinder1024.npz")
+    ref = np.load(path)
+    prob = cases.cylinder_1024()
+    print("case cylinder_1024 fixture, mixed", flush=True)
+    solver = ib.create_solver(prob, precision="mixed", refine_sweeps=2)
+    f = solver.forces(solver.advance(len(ref["forces"])))
+    # per step, relative to the force vector (the lift of this symmetric flow is round-off)
+    e = np.linalg.norm(f - ref["forces"], axis=1) / np.linalg.norm(ref["forces"], axis=1)
+    print("force rel err per step", e, flush=True)
+    assert (e <= 1e-6).all(), e
+    assert cases.relerr(solver.read("Ftilde"), ref["Ftilde"]) <= 1e-6
+    L = prob.fluid.grid.nlevel
+    for name in ("Gamma", "psi", "q"):
+        norms = np.linalg.norm(solver.read(name).reshape(L, -1), axis=1)
+        assert np.allclose(norms, ref[f"{name}/norms"], rtol=1e-8, atol=0), name
+
+
 @pytest.mark.parametrize("nlevel", [4, 1])
 def test_box_update_matches_full_transform(nlevel):
     """The final vort2flux by the box update (the trial's spectral planes plus the transform

@@ -167,3 +167,42 @@ def test_learned_warm_start_gain():
     for mode in ("extrapolate", "learned"):
         d = np.abs(forces[mode] - forces["previous"]).max() / np.abs(forces["previous"]).max()
         assert d < 1e-6, (mode, d)
+
+
+def test_learned_warm_start_gain_plate_512():
+    """The pitching-plate workload at its full grid: 512 x 512 cells on [-2,6] x [-4,4]
+    (h = 1/64), 4 levels, Re 200, dt 0.002, the plate at the reference's panel spacing 2h
+    (33 points), CG to 1e-8 of ||b||. Trained on three reduced frequencies, tested on a
+    fourth: learned < extrapolated < previous-step CG iterations, and forces within the CG
+    tolerance of the reference's guess."""
+    net = WarmStartNet(channels=128, layers=4, kernel=5, max_points=64, capacity=4000,
+                       max_batch=64, seed=11)
+
+    def solver(k):
+        return ib.create_solver(cases.pitching_plate_512(k=k), cg=CG, cg_tol_mode=1)
+
+    print("case pitching plate 512^2 x 4, record", flush=True)
+    for k in (0.3, 0.6, 0.9):
+        s = solver(k)
+        s.set_warm_start("extrapolate", net=net, record=True)
+        s.advance(400)
+        s.close()
+    loss = net.train(3000, batch=64, lr=1e-3)
+    print("training loss first/last", loss[:50].mean(), loss[-50:].mean(), flush=True)
+    res, forces = {}, {}
+    for mode in ("previous", "extrapolate", "learned"):
+        s = solver(0.75)
+        s.set_warm_start("extrapolate")
+        s.advance(100)
+        s.set_warm_start(mode, net=net if mode == "learned" else None)
+        st = s.advance(200)
+        assert all(x.cg_converged for x in st), mode
+        res[mode] = (s.cg_iters(st)[3:].mean(), s.b_applies(st)[3:].mean())
+        forces[mode] = s.forces(st)
+        s.close()
+    print("mean CG iterations / B applications:", res, flush=True)
+    assert res["learned"][0] < res["extrapolate"][0] < res["previous"][0]
+    assert res["learned"][1] < 0.5 * res["previous"][1]
+    for mode in ("extrapolate", "learned"):
+        d = np.abs(forces[mode] - forces["previous"]).max() / np.abs(forces["previous"]).max()
+        assert d < 1e-6, (mode, d)
