@@ -117,6 +117,7 @@ __global__ __launch_bounds__(256) void body_update_kernel(const BodyDev b, int o
 }
 
 hipError_t launch_body_update(const BodyDev& b, int only_moving, hipStream_t st) {
+  if (g_skip) return no_skip();
   if (b.npanel == 0) return hipSuccess;
   hipLaunchKernelGGL(body_update_kernel, dim3((b.npanel + 255) / 256), dim3(256), 0, st, b,
                      only_moving);
@@ -209,12 +210,14 @@ __global__ __launch_bounds__(256) void body_bbox_kernel(const int* __restrict__ 
 }
 
 hipError_t launch_body_bbox(const int* coords, int n, double* out, hipStream_t st) {
+  if (g_skip) return no_skip();
   if (n <= 0) return hipErrorInvalidValue;
   hipLaunchKernelGGL(body_bbox_kernel, dim3(1), dim3(256), 0, st, coords, n, out);
   return hipGetLastError();
 }
 
 hipError_t launch_body_sort(const BodyDev& b, hipStream_t st) {
+  if (g_skip) return no_skip();
   if (b.npanel == 0) return hipSuccess;
   hipLaunchKernelGGL(body_sort_kernel, dim3(1), dim3(1024), 0, st, b);
   return hipGetLastError();
@@ -227,6 +230,7 @@ hipError_t launch_body_sort(const BodyDev& b, hipStream_t st) {
 // 49 products are summed by a fixed shuffle tree (deterministic).
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void gather_kernel(const GatherArgs a) {
+  if (skipped(a.skip)) return;
   const int lane = threadIdx.x & 63;
   const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int n = a.npanel;
@@ -257,8 +261,10 @@ __global__ __launch_bounds__(256) void gather_kernel(const GatherArgs a) {
   }
 }
 
-hipError_t launch_gather(const GatherArgs& a, hipStream_t st) {
-  if (a.npanel == 0) return hipSuccess;
+hipError_t launch_gather(const GatherArgs& a0, hipStream_t st) {
+  if (a0.npanel == 0) return hipSuccess;
+  GatherArgs a = a0;
+  a.skip = g_skip;
   hipLaunchKernelGGL(gather_kernel, dim3((2 * a.npanel + 3) / 4, a.nrhs), dim3(256), 0, st, a);
   return hipGetLastError();
 }
@@ -270,6 +276,7 @@ hipError_t launch_gather(const GatherArgs& a, hipStream_t st) {
 // Every vertex writes its own value (zero away from the bodies): no atomics.
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void scatter_rot_kernel(const ScatterArgs a) {
+  if (skipped(a.skip)) return;
   const GridDims g = a.g;
   const int rhs = blockIdx.z;
   const int I = a.ri0 + blockIdx.x * 64 + (threadIdx.x & 63) + 1;
@@ -340,6 +347,7 @@ __global__ __launch_bounds__(256) void scatter_rot_kernel(const ScatterArgs a) {
 // panel order of scatter_rot_kernel (a term a panel does not reach has weight 0, which
 // leaves the sum unchanged), so both kernels give the same result.
 __global__ __launch_bounds__(256) void scatter_rot_lists_kernel(const ScatterArgs a) {
+  if (skipped(a.skip)) return;
   const GridDims g = a.g;
   const int rhs = blockIdx.z;
   const int I = a.ri0 + blockIdx.x * 64 + (threadIdx.x & 63) + 1;
@@ -387,6 +395,7 @@ __global__ __launch_bounds__(256) void scatter_rot_lists_kernel(const ScatterArg
 
 hipError_t launch_scatter_rot(const ScatterArgs& a0, hipStream_t st) {
   ScatterArgs a = a0;
+  a.skip = g_skip;
   if (a.rw <= 0) a.ri0 = 0, a.rj0 = 0, a.rw = a.g.Nx, a.rh = a.g.Ny;
   if (a.ri0 < 0 || a.rj0 < 0 || a.ri0 + a.rw > a.g.Nx || a.rj0 + a.rh > a.g.Ny || a.rh <= 0)
     return hipErrorInvalidValue;
@@ -447,8 +456,10 @@ constexpr int GEMV_SPLIT = 16;
 
 __global__ __launch_bounds__(256) void gemv_kernel(int m, const double* __restrict__ M,
                                                    const double* __restrict__ b,
-                                                   double* __restrict__ part) {
+                                                   double* __restrict__ part,
+                                                   const double* skip) {
   __shared__ double red[4][64];
+  if (skipped(skip)) return;
   const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
   const int i = blockIdx.x * 64 + r;
   const int per = (m + GEMV_SPLIT - 1) / GEMV_SPLIT;
@@ -462,7 +473,9 @@ __global__ __launch_bounds__(256) void gemv_kernel(int m, const double* __restri
     part[(size_t)blockIdx.y * m + i] = ((red[0][r] + red[1][r]) + red[2][r]) + red[3][r];
 }
 
-__global__ void gemv_sum_kernel(int m, const double* __restrict__ part, double* __restrict__ x) {
+__global__ void gemv_sum_kernel(int m, const double* __restrict__ part, double* __restrict__ x,
+                                const double* skip) {
+  if (skipped(skip)) return;
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= m) return;
   double v[GEMV_SPLIT];
@@ -512,8 +525,8 @@ size_t gemv_partial_count(int m) { return (size_t)GEMV_SPLIT * (size_t)(m > 0 ? 
 hipError_t launch_gemv(int m, const double* M, const double* b, double* x, double* part,
                        hipStream_t st) {
   if (m <= 0) return hipSuccess;
-  hipLaunchKernelGGL(gemv_kernel, dim3((m + 63) / 64, GEMV_SPLIT), dim3(256), 0, st, m, M, b, part);
-  hipLaunchKernelGGL(gemv_sum_kernel, dim3((m + 255) / 256), dim3(256), 0, st, m, part, x);
+  hipLaunchKernelGGL(gemv_kernel, dim3((m + 63) / 64, GEMV_SPLIT), dim3(256), 0, st, m, M, b, part, g_skip);
+  hipLaunchKernelGGL(gemv_sum_kernel, dim3((m + 255) / 256), dim3(256), 0, st, m, part, x, g_skip);
   return hipGetLastError();
 }
 
@@ -547,6 +560,7 @@ __global__ __launch_bounds__(256) void gemv_rows_kernel(int m, const double* __r
 }
 
 hipError_t launch_gemv_rows(int m, const double* M, const double* b, double* x, hipStream_t st) {
+  if (g_skip) return no_skip();
   if (m <= 0) return hipSuccess;
   hipLaunchKernelGGL(gemv_rows_kernel, dim3((m + 3) / 4), dim3(256), 0, st, m, M, b, x);
   return hipGetLastError();
@@ -554,6 +568,7 @@ hipError_t launch_gemv_rows(int m, const double* M, const double* b, double* x, 
 
 hipError_t launch_forces(int n, const double* Ft, double* fb, double* total, double h, double dt,
                          double* host_total, hipStream_t st) {
+  if (g_skip) return no_skip();
   hipLaunchKernelGGL(forces_kernel, dim3(1), dim3(1024), 0, st, n, Ft, fb, total, h, dt, host_total);
   return hipGetLastError();
 }
@@ -578,6 +593,7 @@ __device__ double block_sum_1024(double v, double* sh) {
 
 __global__ __launch_bounds__(1024) void cg_init_kernel(const CgDev c) {
   __shared__ double sh[1024];
+  if (skipped(c.skip)) return;
   double rr = 0.0, bb = 0.0;
   for (int i = threadIdx.x; i < c.n; i += 1024) {
     const double r = c.b[i] - c.c[i];
@@ -649,6 +665,7 @@ __device__ __forceinline__ void pc_rotation(const PcDev& p, int i, double& cs, d
 
 // t = R^T r (a point's pair is (i, n + i))
 __global__ __launch_bounds__(256) void pcg_rot_in_kernel(const PcDev p, const double* r, double* t) {
+  if (skipped(p.skip)) return;
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= p.n) return;
   double cs, sn;
@@ -713,17 +730,22 @@ __global__ __launch_bounds__(1024) void pcg_step_kernel(const CgDev c) {
 
 // the search direction starts at zero with rho = 1 (so the first beta multiplies zero)
 __global__ void pcg_start_kernel(const CgDev c) {
+  if (skipped(c.skip)) return;
   for (int i = threadIdx.x; i < c.n; i += blockDim.x) c.u[i] = 0.0;
   if (threadIdx.x == 0) c.scal[6] = 1.0;
 }
 
-hipError_t launch_pcg_start(const CgDev& c, hipStream_t st) {
+hipError_t launch_pcg_start(const CgDev& c0, hipStream_t st) {
+  CgDev c = c0;
+  c.skip = g_skip;
   hipLaunchKernelGGL(pcg_start_kernel, dim3(1), dim3(1024), 0, st, c);
   return hipGetLastError();
 }
 
-hipError_t launch_pcg_precond_dir(const CgDev& c, const PcDev& p, double* t, double* zr,
+hipError_t launch_pcg_precond_dir(const CgDev& c, const PcDev& p0, double* t, double* zr,
                                   double* part, hipStream_t st) {
+  PcDev p = p0;
+  p.skip = g_skip;
   hipLaunchKernelGGL(pcg_rot_in_kernel, dim3((p.n + 255) / 256), dim3(256), 0, st, p,
                      (const double*)c.r, t);
   hipError_t e = launch_gemv(2 * p.n, p.P, t, zr, part, st);
@@ -819,6 +841,7 @@ __global__ void gj_identity_kernel(int m, double* X, int* singular) {
 
 hipError_t launch_gauss_jordan_inverse(int m, double* A, double* X, double* rowk, int* ipiv,
                                        double* pval, int* singular, hipStream_t st) {
+  if (g_skip) return no_skip();
   hipLaunchKernelGGL(gj_identity_kernel, dim3((unsigned)(((size_t)m * m + 255) / 256)), dim3(256), 0,
                      st, m, X, singular);
   for (int k = 0; k < m; ++k) {
@@ -833,7 +856,22 @@ hipError_t launch_gauss_jordan_inverse(int m, double* A, double* X, double* rowk
   return hipGetLastError();
 }
 
-hipError_t launch_cg_init(const CgDev& c, hipStream_t st) {
+__global__ void cg_note_kernel(double* scal, const double* ws_scal) {
+  if (threadIdx.x == 0) {
+    scal[8] = scal[0];
+    scal[7] = ws_scal && scal[0] <= ws_scal[1] ? 1.0 : 0.0;
+  }
+}
+
+hipError_t launch_cg_note(double* scal, const double* ws_scal, hipStream_t st) {
+  if (g_skip) return no_skip();
+  hipLaunchKernelGGL(cg_note_kernel, dim3(1), dim3(64), 0, st, scal, ws_scal);
+  return hipGetLastError();
+}
+
+hipError_t launch_cg_init(const CgDev& c0, hipStream_t st) {
+  CgDev c = c0;
+  c.skip = g_skip;
   hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(1024), 0, st, c);
   return hipGetLastError();
 }

@@ -25,6 +25,7 @@
 #include <map>
 #include <mutex>
 
+#include "ibpm_device.h"
 #include "ibpm_internal.h"
 
 namespace ibpm {
@@ -69,6 +70,7 @@ __device__ __forceinline__ void store_out(const PassPlane& d, int b, int jc, dou
 // logical tile (w % 8) * ceil(T/8) + w / 8: each XCD walks a contiguous range and reads
 // its rows of input from its own L2.
 __device__ __forceinline__ bool xcd_tile(const PassBatch& b, int& z, int& bt, int& mt) {
+  if (skipped(b.skip)) return false;
   const int T = b.nplanes * b.nbt * b.nmt;
   const int per = (T + 7) / 8;
   const int w = blockIdx.x;
@@ -640,6 +642,7 @@ template <template <typename, typename, int, int, int> class K, typename InT, ty
 static void launch_epi(const PassBatch& b_in, int nplanes, int maxNb, int maxH, int epi,
                        hipStream_t st) {
   PassBatch b = b_in;
+  b.skip = g_skip;
   // double fold: TM quarter rows of each of the two blocks (odd, even) per tile
   constexpr bool quarter = is_quarter<K<InT, OutT, EPI_STORE, TB, TM>>::value;
   dim3 g = tile_grid(b, nplanes, maxNb, quarter ? maxH / 2 : maxH, TB, TM);

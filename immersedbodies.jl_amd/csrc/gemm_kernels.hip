@@ -10,6 +10,7 @@
 // takes B^T as its first operand so that a lane's accumulator entries run along the
 // column of C (16 consecutive rows per store). Each C entry sums its K terms in one fixed
 // order, so results do not depend on scheduling.
+#include "ibpm_device.h"
 #include "ibpm_internal.h"
 
 namespace ibpm {
@@ -24,6 +25,7 @@ constexpr int GR = GT * GK / 256;  // elements of A (and of B) each thread stage
 // loaded once.
 template <bool DA, bool DB>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
+  if (skipped(gb.skip)) return;
   // the job's fields in registers (uniform), read once; a missing difference term reads
   // the matrix itself with weight 0 (a - 0 * a = a exactly), so no load is conditional
   const GemmJob& jb = gb.j[blockIdx.y];
@@ -134,7 +136,9 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
 // out[m + n ldo] = sum over s of P[s stride + m + n ldp], the slices in order
 __global__ __launch_bounds__(256) void sum_slices_kernel(const double* __restrict__ P, size_t stride,
                                                          int nslice, int ldp, int M, int N,
-                                                         double* __restrict__ out, int ldo) {
+                                                         double* __restrict__ out, int ldo,
+                                                         const double* skip) {
+  if (skipped(skip)) return;
   const int m = blockIdx.x * 64 + (threadIdx.x & 63);
   const int n = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (m >= M || n >= N) return;
@@ -155,11 +159,13 @@ hipError_t launch_sum_slices(const double* P, size_t stride, int nslice, int ldp
                              double* out, int ldo, hipStream_t st) {
   if (nslice < 1 || nslice > 16 || M <= 0 || N <= 0) return hipErrorInvalidValue;
   hipLaunchKernelGGL(sum_slices_kernel, dim3((M + 63) / 64, (N + 3) / 4), dim3(256), 0, st, P, stride,
-                     nslice, ldp, M, N, out, ldo);
+                     nslice, ldp, M, N, out, ldo, g_skip);
   return hipGetLastError();
 }
 
-hipError_t launch_gemm_f64(const GemmBatch& gb, int njobs, hipStream_t st) {
+hipError_t launch_gemm_f64(const GemmBatch& gb0, int njobs, hipStream_t st) {
+  GemmBatch gb = gb0;
+  gb.skip = g_skip;
   int tiles = 0;
   bool da = false, db = false;
   for (int i = 0; i < njobs; ++i) {

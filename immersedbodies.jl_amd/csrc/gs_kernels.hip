@@ -78,6 +78,7 @@ __global__ __launch_bounds__(kWaves * 64) void gs_helmholtz_kernel(const GsBatch
   const GsPlane& pl = bt.p[blockIdx.y];
   const int Nx = bt.Nx, Ny = bt.Ny, ld = bt.ld, H = pl.H, out_w = kTile - 2 * H, out_h = kTileH - 2 * H;
   if ((int)blockIdx.x >= pl.ntx * pl.nty) return;  // whole work-group: barriers stay uniform
+  if (skipped(bt.skip)) return;
   const int tx = blockIdx.x % pl.ntx, ty = blockIdx.x / pl.ntx;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -201,6 +202,7 @@ hipError_t launch_gs_helmholtz(GsBatch& bt, int nplanes, hipStream_t st) {
     tiles = q.ntx * q.nty > tiles ? q.ntx * q.nty : tiles;
   }
   bt.ntx = bt.nty = 0;
+  bt.skip = g_skip;
   if (rows == 8)
     hipLaunchKernelGGL(gs_helmholtz_kernel<8>, dim3(tiles, nplanes), dim3(kWaves * 64), 0, st, bt);
   else

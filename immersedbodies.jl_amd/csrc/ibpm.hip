@@ -37,6 +37,8 @@ int fail(int code, const std::string& msg) {
 }  // namespace
 
 namespace ibpm {
+thread_local const double* g_skip = nullptr;
+
 int set_error(int code, const char* msg) {
   g_err = msg;
   return code;
@@ -87,7 +89,7 @@ void free_all(ibpm_solver* s) {
                   s->body_of, s->moving_d, s->coords, s->perm, s->xstart, s->err, s->w,
                   s->sc_rowptr, s->sc_ep, s->sc_ew,
                   s->ywin, s->gemv_part, s->pc_P, s->pc_theta0, s->pc_t, s->pc_z,
-                  s->Binv, s->Gtmp, s->psitmp, s->Sc, s->cg_r, s->cg_u, s->cg_c, s->cg_scal,
+                  s->Binv, s->Gtmp, s->psitmp, s->Sc, s->cg_r, s->cg_u, s->cg_c, s->cg_scal, s->dyn_d,
                   s->F1, s->G1, s->Sfx, s->Sfy != s->Sfx ? s->Sfy : nullptr, s->Ftr, s->Tbox, s->Pa, s->Pb, s->T2, s->bbox_d,
                   s->edge, s->pw, s->fac, s->rvec, s->rpart, s->Mx,
                   s->My != s->Mx ? s->My : nullptr,
@@ -1101,6 +1103,7 @@ int couple_surface(ibpm_solver* s, ibpm_step_stats* stats) {
   // the warm-start modes only change that initial guess (ibpm_set_warm_start).
   const int n2 = 2 * s->n;
   CgDev c;
+  memset(&c, 0, sizeof c);
   c.n = n2;
   c.x = s->Ft;
   c.r = s->cg_r;
@@ -1164,6 +1167,37 @@ int couple_surface(ibpm_solver* s, ibpm_step_stats* stats) {
   if (rc) return rc;
   ++bapp;
   HIPCHK(launch_cg_init(c, s->st));
+  if (s->graph_mode) {
+    // A step captured into a batch graph: the same iterations, decided on the device.
+    // The learned guess is kept when it beats x_e (cg_scal[7] = 1), else CG restarts from
+    // x_e with the B x_e already known; then cg_maxiter iterations are queued, each of
+    // whose launches returns at once after the stopping test has been met (cg_scal[4]),
+    // so the iterates are those of the host-driven loop below, maxiter included.
+    HIPCHK(launch_cg_note(s->cg_scal, s->ws_mode == IBPM_WS_LEARNED ? s->ws_scal : nullptr, s->st));
+    if (s->ws_mode == IBPM_WS_LEARNED) {
+      SkipScope keep(s->cg_scal + 7);
+      HIPCHK(launch_copy(s->Ft, s->ws_xe, n2, s->st));
+      HIPCHK(launch_copy(s->cg_c, s->ws_Bxe, n2, s->st));
+      HIPCHK(launch_cg_init(c, s->st));
+    }
+    SkipScope converged(s->cg_scal + 4);
+    PcDev pd;
+    memset(&pd, 0, sizeof pd);
+    pd.n = s->n;
+    pd.P = s->pc_P;
+    pd.body_of = s->body_of;
+    pd.motion = s->motion;
+    pd.theta0 = s->pc_theta0;
+    if (s->cg_precond) HIPCHK(launch_pcg_start(c, s->st));
+    const double* zu[1] = {s->cg_u};
+    for (int it = 0; it < s->cg_maxiter; ++it) {
+      if (s->cg_precond) HIPCHK(launch_pcg_precond_dir(c, pd, s->pc_t, s->pc_z, s->gemv_part, s->st));
+      rc = apply_B(s, 1, zu, xc);
+      if (rc) return rc;
+      HIPCHK(s->cg_precond ? launch_pcg_step(c, s->st) : launch_cg_step(c, s->st));
+    }
+    return IBPM_OK;
+  }
   HIPCHK(hipMemcpyAsync(s->h_scal, s->cg_scal, 6 * sizeof(double), hipMemcpyDeviceToHost, s->st));
   if (need_res)
     HIPCHK(hipMemcpyAsync(s->h_scal + 8, s->ws_scal, 3 * sizeof(double), hipMemcpyDeviceToHost, s->st));
@@ -1243,8 +1277,13 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
   s->q0_shadow_valid = false;
   // _update_body_motion! (bodies.jl:43-51)
   if (s->moving) {
-    for (int b = 0; b < s->nbody; ++b) s->cur_theta[b] = motion[6 * b + 4];
-    HIPCHK(hipMemcpyAsync(s->motion, motion, 6 * s->nbody * sizeof(double), hipMemcpyHostToDevice, s->st));
+    // a step captured into a batch graph finds this step's motion on the device already
+    if (motion) {
+      for (int b = 0; b < s->nbody; ++b) s->cur_theta[b] = motion[6 * b + 4];
+      HIPCHK(hipMemcpyAsync(s->motion, motion, 6 * s->nbody * sizeof(double), hipMemcpyHostToDevice, s->st));
+    } else if (!s->graph_mode) {
+      return fail(IBPM_E_ARG, "moving bodies need this step's motion");
+    }
     BodyDev b = body_dev(s);
     HIPCHK(launch_body_update(b, 1, s->st));
     HIPCHK(launch_body_sort(b, s->st));
@@ -1381,6 +1420,7 @@ BaseFlux base_flux(const ibpm_solver* s) {
   BaseFlux b;
   memset(&b, 0, sizeof b);
   b.moving = s->grid_moving;
+  b.dyn = s->graph_mode ? s->dyn_d : nullptr;
   for (int l = 0; l < s->L && l < 16; ++l) {
     const double hc = s->hc[l];
     b.q0u[l] = s->fs[0] * hc;
@@ -1409,7 +1449,18 @@ BaseFlux base_flux(const ibpm_solver* s) {
 }
 
 int solver_step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stats* stats) {
+  if (s->graph_mode && s->box_dyn)
+    return fail(IBPM_E_ARG, "a captured step cannot follow moving bodies with box updates");
   return step(s, fsv, motion, stats);
+}
+
+void base_flux_dyn(const ibpm_solver* s, const double fs8[8], double out[14]) {
+  // the scalars base_flux() derives from the freestream and GridVelocity, same expressions
+  const double c = cos(fs8[6]), sn = sin(fs8[6]), om = fs8[7];
+  const double v[14] = {fs8[0], fs8[1], fs8[2], fs8[3], fs8[4], fs8[5], c, sn, -sn, c,
+                        om * -sn, om * -c, om * c, om * -sn};
+  (void)s;
+  for (int k = 0; k < 14; ++k) out[k] = v[k];
 }
 }  // namespace ibpm
 
@@ -1738,7 +1789,8 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   CK(dalloc(&s->gemv_part, gemv_partial_count(2 * n)));
   CK(dalloc(&s->err, 1));
   CK(dalloc(&s->w, 98 * (size_t)n));
-  CK(dalloc(&s->cg_scal, 8));
+  CK(dalloc(&s->cg_scal, 16));
+  CK(dalloc(&s->dyn_d, 16));
   CK(hipHostMalloc((void**)&s->h_scal, 16 * sizeof(double), hipHostMallocDefault));
   CK(hipHostMalloc((void**)&s->h_force, 2 * kForceChunk * sizeof(double), hipHostMallocMapped));
   CK(hipHostGetDevicePointer((void**)&s->d_force, s->h_force, 0));

@@ -7,6 +7,10 @@
 
 namespace ibpm {
 
+// The launch context's flag (g_skip of the launching thread): a kernel launched while it
+// was set returns at once when the flag is nonzero.
+__device__ __forceinline__ bool skipped(const double* flag) { return flag && *flag != 0.0; }
+
 // ---------------------------------------------------------------------------------
 // get_bc! entry (fluid.jl:278-306): value on `side` (0 left, 1 right, 2 bottom, 3 top)
 // at 1-based position k of the boundary vector, interpolated from the coarse plane r.
@@ -83,12 +87,21 @@ __host__ __device__ __forceinline__ double curl_v(const double* __restrict__ p,
 // Base flux on u-face (i, j) and v-face (i, j) of level l (0-based level, 1-based faces),
 // in the order of base_flux! (fluid.jl:34-67): u-faces sit at (x0 + h (i-1),
 // y0 + h/2 + h (j-1)), v-faces at (x0 + h/2 + h (i-1), y0 + h (j-1)) (problem.jl:318-334).
+// With b.dyn set, the step's freestream and grid motion come from device memory (the
+// same values base_flux() puts in the arguments, so results are the same bits).
 __host__ __device__ inline double base_flux_u(const BaseFlux& b, int l, int i, int j) {
 #pragma clang fp contract(off)
-  if (!b.moving) return b.q0u[l];
+  const double* d = b.dyn;
+  if (!b.moving) return d ? d[0] * b.hc[l] : b.q0u[l];
   const double hc = b.hc[l];
   const double x = b.x0[l] + hc * (double)(i - 1);
   const double y = (b.y0[l] + hc / 2) + hc * (double)(j - 1);
+  if (d) {
+    const double dx = x - d[2], dy = y - d[3];
+    const double wx = (d[0] - d[4]) - (d[10] * dx + d[11] * dy);
+    const double wy = (d[1] - d[5]) - (d[12] * dx + d[13] * dy);
+    return hc * (d[6] * wx + d[7] * wy);
+  }
   const double dx = x - b.cx, dy = y - b.cy;
   const double wx = (b.ux - b.vx) - (b.rv11 * dx + b.rv12 * dy);
   const double wy = (b.uy - b.vy) - (b.rv21 * dx + b.rv22 * dy);
@@ -97,10 +110,17 @@ __host__ __device__ inline double base_flux_u(const BaseFlux& b, int l, int i, i
 
 __host__ __device__ inline double base_flux_v(const BaseFlux& b, int l, int i, int j) {
 #pragma clang fp contract(off)
-  if (!b.moving) return b.q0v[l];
+  const double* d = b.dyn;
+  if (!b.moving) return d ? d[1] * b.hc[l] : b.q0v[l];
   const double hc = b.hc[l];
   const double x = (b.x0[l] + hc / 2) + hc * (double)(i - 1);
   const double y = b.y0[l] + hc * (double)(j - 1);
+  if (d) {
+    const double dx = x - d[2], dy = y - d[3];
+    const double wx = (d[0] - d[4]) - (d[10] * dx + d[11] * dy);
+    const double wy = (d[1] - d[5]) - (d[12] * dx + d[13] * dy);
+    return hc * (d[8] * wx + d[9] * wy);
+  }
   const double dx = x - b.cx, dy = y - b.cy;
   const double wx = (b.ux - b.vx) - (b.rv11 * dx + b.rv12 * dy);
   const double wy = (b.uy - b.vy) - (b.rv21 * dx + b.rv22 * dy);

@@ -20,6 +20,19 @@ namespace ibpm {
 // Sets the thread-local message returned by ibpm_last_error() and returns `code`.
 int set_error(int code, const char* msg);
 
+// Launch context of the calling thread. While `g_skip` points at a device flag, every
+// kernel launched by the launchers below returns at once when that flag is nonzero; the
+// CG iterations of a captured step run under the CG's converged flag, so a replay does
+// the work of the iterations CG needs and exits the rest. Launchers of kernels that do not
+// take the flag refuse to launch while it is set (hipErrorNotSupported).
+extern thread_local const double* g_skip;
+struct SkipScope {
+  const double* saved;
+  explicit SkipScope(const double* flag) : saved(g_skip) { g_skip = flag; }
+  ~SkipScope() { g_skip = saved; }
+};
+inline hipError_t no_skip() { return g_skip ? hipErrorNotSupported : hipSuccess; }
+
 // Geometry of one multi-domain grid (same for all its levels).
 struct GridDims {
   int nx, ny;   // cells per level
@@ -73,6 +86,7 @@ constexpr int kMaxPlanes = 16;
 struct PassBatch {
   PassPlane p[kMaxPlanes];
   int nbt, nmt, nplanes;  // tile grid (filled by the launcher)
+  const double* skip;     // launch context flag (filled by the launcher)
 };
 
 // Host launchers (dst_kernels.hip).
@@ -124,6 +138,7 @@ struct RingChainArgs {
   const double* pw;                 // S_y[:, ny/4], S_y[:, 3ny/4], S_x[nx/4, :], S_x[3nx/4, :]
   const double* Mx;                 // Nx x Nx ring operator (row-major)
   const double* My;                 // Ny x Ny
+  const double* skip;               // launch context flag (filled by the launchers)
 };
 // E = D . (F + R) of every chain and, with do_proj, the four projections (vec): a sweep
 // and a reduction launch.
@@ -158,6 +173,9 @@ size_t sine_table_bf16_even_count(int N);
 // (evaluated by base_flux_u / base_flux_v in ibpm_device.h).
 struct BaseFlux {
   int moving;
+  // when set (batched steps): this step's values on the device, in the order ux, uy, cx,
+  // cy, vx, vy, rx11, rx21, rx12, rx22, rv11, rv12, rv21, rv22; q0u = ux hc, q0v = uy hc
+  const double* dyn;
   double q0u[16], q0v[16];         // static grid
   double hc[16], x0[16], y0[16];   // moving grid: step and lower-left corner per level
   double ux, uy;                   // freestream velocity (global frame)
@@ -204,6 +222,7 @@ struct ResidualArgs {
   const double* ring[16];  // vort2flux boundary values (added to b), may be null
   const double* xbase[16]; // when set, x stands for xbase - x (solution stored as base - x)
   int r_f32;
+  const double* skip;      // launch context flag (filled by the launcher)
 };
 hipError_t launch_residual(const ResidualArgs& a, hipStream_t st);
 
@@ -243,6 +262,7 @@ struct GatherArgs {
   BaseFlux bf;              // base flux (level 1 used; moving = 0 and zeros for none)
   const double* ub;         // subtract ub*h if not null
   double h;
+  const double* skip;       // launch context flag (filled by the launcher)
 };
 hipError_t launch_gather(const GatherArgs& a, hipStream_t st);
 
@@ -269,6 +289,7 @@ struct ScatterArgs {
   const int* rowptr;
   const int* ep;
   const double* ew;
+  const double* skip;  // launch context flag (filled by the launcher)
 };
 // the vertex lists of ScatterArgs from the panels' cells, weights and sort order (host
 // copies); rowptr: Nx Ny + 1 entries
@@ -302,8 +323,13 @@ struct CgDev {
   int maxiter;
   double reltol, abstol;
   int tol_mode;
+  const double* skip;  // launch context flag (filled by the launchers)
 };
 hipError_t launch_cg_init(const CgDev& c, hipStream_t st);
+// After cg_init of a captured step: scal[8] = scal[0] (the initial residual) and, with
+// ws_scal, scal[7] = 1 when that residual is at most ws_scal[1] = ||r_e|| (the learned
+// guess beats the extrapolated one), else 0
+hipError_t launch_cg_note(double* scal, const double* ws_scal, hipStream_t st);
 hipError_t launch_cg_step(const CgDev& c, hipStream_t st);
 // Preconditioned CG: the dense inverse P (2n x 2n, column-major) of B at an earlier body
 // configuration, applied in each body's frame (angle change motion[6b+4] - theta0[b]).
@@ -313,6 +339,7 @@ struct PcDev {
   const int* body_of;
   const double* motion;  // 6 per body: pos, vel, angle, angular velocity (this step)
   const double* theta0;  // per body: angle when P was assembled
+  const double* skip;    // launch context flag (filled by the launcher)
 };
 // X = A^-1 by Gauss-Jordan with partial pivoting (A is destroyed); column-major m x m.
 // rowk: 2m doubles; ipiv, pval, singular: one element each (singular = 1 on a zero pivot).
@@ -345,6 +372,7 @@ struct GsBatch {
   GsPlane p[kMaxPlanes];
   int Nx, Ny, ntx, nty;  // region solved (x = 0 outside it); ntx, nty unused
   int ld;                // row pitch of every plane pointer (0: Nx)
+  const double* skip;    // launch context flag (filled by the launcher)
 };
 hipError_t launch_gs_helmholtz(GsBatch& bt, int nplanes, hipStream_t st);
 
@@ -377,6 +405,7 @@ struct GemmJob {
 };
 struct GemmBatch {
   GemmJob j[kMaxPlanes];
+  const double* skip;  // launch context flag (filled by the launcher)
 };
 hipError_t launch_gemm_f64(const GemmBatch& gb, int njobs, hipStream_t st);
 // out[m + n ldo] = sum over s < nslice (<= 16) of P[s stride + m + n ldp], in order of s

@@ -428,6 +428,7 @@ __global__ void ws_record_kernel(const WsArgs a, int stride, float* ds_feat, flo
 }  // namespace
 
 hipError_t launch_conv(const ConvArgs& a, int epi, hipStream_t st) {
+  if (g_skip) return no_skip();
   if (a.Cin % 32 || a.Cout % 32 || a.Rpad % 64 || a.K < 1 || a.K > 7 || !(a.K & 1))
     return hipErrorInvalidValue;
   const dim3 grid(a.Rpad / 64, a.Cout / 32);
@@ -500,18 +501,21 @@ hipError_t launch_sumsq(const float* x, size_t n, double* partial, int nblocks, 
 
 hipError_t launch_ws_extrap(int n2, const double* F1, const double* F2, const double* F3, int nhist,
                             double* xe, hipStream_t st) {
+  if (g_skip) return no_skip();
   hipLaunchKernelGGL(ws_extrap_kernel, dim3((n2 + 255) / 256), dim3(256), 0, st, n2, F1, F2, F3,
                      nhist, xe);
   return hipGetLastError();
 }
 
 hipError_t launch_ws_features(const WsArgs& a, hipStream_t st) {
+  if (g_skip) return no_skip();
   hipLaunchKernelGGL(ws_features_kernel, dim3(1), dim3(1024), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_ws_apply(int n, const double* xe, const float* y, int Cp, const double* scal,
                            float out_scale, double* x0, hipStream_t st) {
+  if (g_skip) return no_skip();
   hipLaunchKernelGGL(ws_apply_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, xe, y, Cp, scal,
                      out_scale, x0);
   return hipGetLastError();
@@ -519,6 +523,7 @@ hipError_t launch_ws_apply(int n, const double* xe, const float* y, int Cp, cons
 
 hipError_t launch_ws_record(const WsArgs& a, int stride, float* ds_feat, float* ds_tgt, int* ds_body,
                             hipStream_t st) {
+  if (g_skip) return no_skip();
   hipLaunchKernelGGL(ws_record_kernel, dim3((stride + 255) / 256), dim3(256), 0, st, a, stride,
                      ds_feat, ds_tgt, ds_body);
   return hipGetLastError();

@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "ibpm_device.h"
 #include "ibpm_internal.h"
 
 namespace ibpm {
@@ -48,6 +49,7 @@ __global__ __launch_bounds__(256) void ring_sweep_kernel(const RingChainArgs a, 
   __shared__ double rs[2][RTY][RT + 1];  // S_x-weighted E of the tile, per row, for the row sums
   __shared__ double pa[2][4][RT];        // per-wave partial a_b, a_t
   __shared__ double part[2][8][RTY];     // row sums over eighths of the tile's kx
+  if (skipped(a.skip)) return;
   const int c = blockIdx.z;
   const int Nx = a.Nx, Ny = a.Ny;
   const int tid = threadIdx.x, tx = tid & 63, w = tid >> 6;
@@ -157,6 +159,7 @@ __global__ __launch_bounds__(256) void ring_sweep_kernel(const RingChainArgs a, 
 // each summing a quarter of the tiles in order (all its loads in flight together), and
 // the quarters are added in order: a fixed summation order.
 __global__ __launch_bounds__(256) void ring_reduce_kernel(const RingChainArgs a, int ntx, int nty) {
+  if (skipped(a.skip)) return;
   const int c = blockIdx.y;
   const int Nx = a.Nx, Ny = a.Ny;
   const int t = blockIdx.x * 256 + threadIdx.x;
@@ -195,6 +198,7 @@ __global__ __launch_bounds__(256) void ring_reduce_kernel(const RingChainArgs a,
 // One wave per row j of M (row-major, j contiguous over the input index); 2 right-hand
 // sides per wave.
 __global__ __launch_bounds__(256) void ring_factor_kernel(const RingChainArgs a) {
+  if (skipped(a.skip)) return;
   const int lane = threadIdx.x & 63;
   const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
   const bool ydir = blockIdx.y == 1;
@@ -236,7 +240,9 @@ __global__ __launch_bounds__(256) void ring_factor_kernel(const RingChainArgs a)
   }
 }
 
-hipError_t launch_ring_sweep(const RingChainArgs& a, int nchain, bool do_proj, hipStream_t st) {
+hipError_t launch_ring_sweep(const RingChainArgs& a0, int nchain, bool do_proj, hipStream_t st) {
+  RingChainArgs a = a0;
+  a.skip = g_skip;
   const int ntx = (a.Nx + RT - 1) / RT, nty = (a.Ny + RTY - 1) / RTY;
   hipLaunchKernelGGL(ring_sweep_kernel, dim3(ntx, nty, nchain), dim3(256), 0, st, a, do_proj ? 1 : 0);
   hipError_t e = hipGetLastError();
@@ -253,7 +259,9 @@ size_t ring_partial_count(int Nx, int Ny) {
   return ring_part_a_count(Nx, Ny) + (size_t)2 * ntx * Ny;
 }
 
-hipError_t launch_ring_factor(const RingChainArgs& a, int nchain, hipStream_t st) {
+hipError_t launch_ring_factor(const RingChainArgs& a0, int nchain, hipStream_t st) {
+  RingChainArgs a = a0;
+  a.skip = g_skip;
   const int rows = std::max(a.Nx, a.Ny);
   hipLaunchKernelGGL(ring_factor_kernel, dim3((rows + 3) / 4, 2, nchain), dim3(256), 0, st, a);
   return hipGetLastError();

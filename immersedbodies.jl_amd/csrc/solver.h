@@ -91,6 +91,13 @@ struct ibpm_solver {
   double* h_force = nullptr;  // mapped pinned: total force of the last kForceChunk steps
   double* d_force = nullptr;  // h_force's device address
   double* force_slot = nullptr;  // where this step's forces kernel also writes the total (or null)
+  // a step captured into a batch's graph: no host reads or decisions; the CG runs
+  // cg_maxiter iterations whose launches exit once it has converged, the learned guess
+  // is kept or dropped on the device, and the step's freestream / grid motion come from
+  // dyn_d (14 values, BaseFlux::dyn) and its body motion from `motion`, both filled on the
+  // device before the step
+  bool graph_mode = false;
+  double* dyn_d = nullptr;
   // CG warm start: extrapolation history F~b of the two steps before the previous one,
   // the extrapolated guess x_e, B x_e, and (rms(r_e), ||r_e||, ||b||)
   int ws_mode = IBPM_WS_PREVIOUS, ws_record = 0, nhist = 0;
@@ -115,4 +122,8 @@ BaseFlux base_flux(const ibpm_solver* s);
 // One advance! of a solver (solver.jl:155-190) queued on s->st: no host synchronisation
 // for static bodies. fsv: freestream at the new time; motion: 6 x nbody or null.
 int solver_step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stats* stats);
+// BaseFlux::dyn values of a step from its freestream and GridVelocity fs8 = (ux, uy,
+// centre x, y, velocity x, y, angle, angular_vel): the numbers base_flux() puts in the
+// kernel arguments, bit for bit.
+void base_flux_dyn(const ibpm_solver* s, const double fs8[8], double out[14]);
 }  // namespace ibpm

@@ -141,6 +141,7 @@ __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
 }
 
 hipError_t launch_nonlin_rhs(const NonlinArgs& a, hipStream_t st) {
+  if (g_skip) return no_skip();
   dim3 grid((a.g.Nx + 63) / 64, (a.g.Ny + 4 * NL_RJ - 1) / (4 * NL_RJ), a.nlevel);
   hipLaunchKernelGGL(nonlin_rhs_kernel, grid, dim3(256), 0, st, a);
   return hipGetLastError();
@@ -158,7 +159,9 @@ hipError_t launch_nonlin_rhs(const NonlinArgs& a, hipStream_t st) {
 
 __global__ __launch_bounds__(256) void coarsify_kernel(GridDims g, double* __restrict__ coarse,
                                                        const double* __restrict__ fine,
-                                                       const double* __restrict__ outer) {
+                                                       const double* __restrict__ outer,
+                                                       const double* skip) {
+  if (skipped(skip)) return;
   const CoarsifyRange cr = coarsify_range(g.nx, g.ny);
   int Ic, Jc;
   if (outer) {
@@ -193,7 +196,7 @@ hipError_t launch_coarsify(GridDims g, double* coarse, const double* fine, const
     if (cr.ni == 0 || cr.nj == 0) return hipSuccess;
     grid = dim3((cr.ni + 63) / 64, (cr.nj + 3) / 4, 1);
   }
-  hipLaunchKernelGGL(coarsify_kernel, grid, dim3(256), 0, st, g, coarse, fine, outer);
+  hipLaunchKernelGGL(coarsify_kernel, grid, dim3(256), 0, st, g, coarse, fine, outer, g_skip);
   return hipGetLastError();
 }
 
@@ -235,6 +238,7 @@ __global__ __launch_bounds__(256) void curl_all_kernel(GridDims g, int nlevel, d
 }
 
 hipError_t launch_curl_all(GridDims g, int nlevel, double* q, const double* psi, hipStream_t st) {
+  if (g_skip) return no_skip();
   dim3 grid((g.nx + 1 + 255) / 256, 2 * g.ny + 1, nlevel);
   hipLaunchKernelGGL(curl_all_kernel, grid, dim3(256), 0, st, g, nlevel, q, psi);
   return hipGetLastError();
@@ -246,6 +250,7 @@ hipError_t launch_curl_all(GridDims g, int nlevel, double* q, const double* psi,
 // when a ring plane is given. Feeds the mixed-precision refinement sweeps.
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void residual_kernel(const ResidualArgs a) {
+  if (skipped(a.skip)) return;
   const GridDims g = a.g;
   const int pl = blockIdx.z;
   const int I = blockIdx.x * 64 + (threadIdx.x & 63) + 1;
@@ -276,7 +281,9 @@ __global__ __launch_bounds__(256) void residual_kernel(const ResidualArgs a) {
     ((double*)a.r[pl])[idx] = r;
 }
 
-hipError_t launch_residual(const ResidualArgs& a, hipStream_t st) {
+hipError_t launch_residual(const ResidualArgs& a0, hipStream_t st) {
+  ResidualArgs a = a0;
+  a.skip = g_skip;
   dim3 grid((a.g.Nx + 63) / 64, (a.g.Ny + 3) / 4, a.nplanes);
   hipLaunchKernelGGL(residual_kernel, grid, dim3(256), 0, st, a);
   return hipGetLastError();
@@ -285,12 +292,15 @@ hipError_t launch_residual(const ResidualArgs& a, hipStream_t st) {
 // ---------------------------------------------------------------------------------
 // small utilities
 // ---------------------------------------------------------------------------------
-__global__ void fill_kernel(double* p, size_t n, double v) {
+__global__ void fill_kernel(double* p, size_t n, double v, const double* skip) {
+  if (skipped(skip)) return;
   size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i < n) p[i] = v;
 }
 // 4 elements per thread, 256 apart (coalesced), all four loads issued before the stores
-__global__ void copy_kernel(double* __restrict__ d, const double* __restrict__ s, size_t n) {
+__global__ void copy_kernel(double* __restrict__ d, const double* __restrict__ s, size_t n,
+                            const double* skip) {
+  if (skipped(skip)) return;
   const size_t i0 = (size_t)blockIdx.x * 1024 + threadIdx.x;
   double v[4];
 #pragma unroll
@@ -301,7 +311,7 @@ __global__ void copy_kernel(double* __restrict__ d, const double* __restrict__ s
 }
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t st) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+  hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v, g_skip);
   return hipGetLastError();
 }
 __global__ void sub_kernel(double* __restrict__ d, const double* __restrict__ a,
@@ -311,6 +321,7 @@ __global__ void sub_kernel(double* __restrict__ d, const double* __restrict__ a,
 }
 
 hipError_t launch_sub(double* d, const double* a, const double* b, size_t n, hipStream_t st) {
+  if (g_skip) return no_skip();
   if (!n) return hipSuccess;
   hipLaunchKernelGGL(sub_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, a, b, n);
   return hipGetLastError();
@@ -318,7 +329,7 @@ hipError_t launch_sub(double* d, const double* a, const double* b, size_t n, hip
 
 hipError_t launch_copy(double* d, const double* s, size_t n, hipStream_t st) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((n + 1023) / 1024)), dim3(256), 0, st, d, s, n);
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((n + 1023) / 1024)), dim3(256), 0, st, d, s, n, g_skip);
   return hipGetLastError();
 }
 

@@ -160,7 +160,7 @@ def load():
     L.ibpm_net_step.restype = C.c_int
     L.ibpm_batch_create.argtypes = [C.c_int, C.POINTER(C.c_void_p), _dp, C.POINTER(C.c_void_p)]
     L.ibpm_batch_create.restype = C.c_int
-    L.ibpm_batch_advance.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.ibpm_batch_advance.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.POINTER(StepStats)]
     L.ibpm_batch_advance.restype = C.c_int
     L.ibpm_batch_destroy.argtypes = [C.c_void_p]
     L.ibpm_batch_destroy.restype = None

@@ -227,21 +227,39 @@ void ibpm_destroy(ibpm_solver* s);
  *
  * No counterpart in the reference, where each CnabSolver advances alone
  * (solver.jl:155-190). A batch advances many independent solvers together: any grid
- * sizes, level counts, body-point counts and Reynolds numbers, static bodies or none
- * (a moving body's CG synchronises with the host every iteration). One step of every
- * solver is captured into a hipGraph (two, one per parity of the nonlinear-term swap)
- * spread over concurrent streams; each batch step is one graph replay. Every solver
- * ends in the same state as when advanced alone.
+ * sizes, level counts, body-point counts and Reynolds numbers, static or moving bodies,
+ * static or moving grids. One step of every solver is captured into a hipGraph (two, one
+ * per parity of the nonlinear-term swap) spread over concurrent streams; each batch step
+ * is one graph replay, with no host synchronisation. Every solver ends in the same state,
+ * bit for bit, as when advanced alone with the same inputs.
  *
- * freestream: 2 x nprob, the velocity of each problem, fixed for the batch (NULL: 0).
- * The batch must be destroyed before its solvers. Between batch steps the solvers may be
- * read, written or advanced alone, as long as all of them advance by the same parity of
- * steps (otherwise ibpm_batch_advance fails). */
+ * Each step's freestream, grid motion and body motion are read by the graph from device
+ * tables that ibpm_batch_advance fills from its arguments. A moving body's CG is captured
+ * as cg_maxiter iterations (at most 64) whose launches return at once after the stopping
+ * test is met: the same iterates, and the same maxiter, as the solver alone. Moving-body
+ * solvers in a batch need: box_update = 1 (off); no warm-start recording; three steps of
+ * history for the extrapolated or learned guess; an assembled preconditioner (one step)
+ * and cg_precond_refresh = 0 when preconditioned. ibpm_batch_create says which is missing.
+ *
+ * freestream: 2 x nprob, the velocity held for steps that ibpm_batch_advance gives none
+ * (NULL: each solver's last). The batch must be destroyed before its solvers. Between
+ * batch steps the solvers may be read, written or advanced alone, as long as all of them
+ * advance by the same parity of steps (otherwise ibpm_batch_advance fails). */
 typedef struct ibpm_batch ibpm_batch;
 int ibpm_batch_create(int nprob, ibpm_solver* const* solvers, const double* freestream,
                       ibpm_batch** out);
-/* forces: nsteps x nprob x 2 (total force of every problem after every step) or NULL. */
-int ibpm_batch_advance(ibpm_batch* b, int nsteps, double* forces);
+/* freestream: nsteps x nprob x 8 (at each new time: ux, uy, then the grid's GridVelocity:
+ *             centre x, y, velocity x, y, angle, angular_vel; the last six are ignored for
+ *             static grids), or NULL (the held freestream, grids at rest);
+ * motion:     nsteps x (6 x nbody of problem 0, 6 x nbody of problem 1, ...), as in
+ *             ibpm_advance (entries of static bodies ignored); NULL only without moving
+ *             bodies;
+ * forces:     nsteps x nprob x 2 (total force of every problem after every step) or NULL;
+ * stats:      nsteps x nprob entries or NULL (pc_assembled is always 0).
+ * A panel that leaves the allowed region fails the call with IBPM_E_OUTSIDE_REGION after
+ * the steps of the call have run (the message names the problem and the step). */
+int ibpm_batch_advance(ibpm_batch* b, int nsteps, const double* freestream, const double* motion,
+                       double* forces, ibpm_step_stats* stats);
 void ibpm_batch_destroy(ibpm_batch* b);
 
 /* ---- Warm start of the CG surface-force solve of moving bodies ----

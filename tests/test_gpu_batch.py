@@ -64,12 +64,6 @@ def test_batch_then_alone_and_parity_check():
     batch.close()
 
 
-def test_batch_rejects_moving_bodies():
-    s = ib.create_solver(cases.moving_cylinder())
-    with pytest.raises(ib._lib.IbpmError, match="moving"):
-        ib.Batch([s])
-
-
 def test_batch_against_oracle():
     """One batched problem against the FP64 oracle (the batch adds nothing to the step)."""
     p = cases.ellipse_body(nx=96, ny=64, nlevel=2, h=0.05, x0=(-1.2, -1.6), Re=250.0)
