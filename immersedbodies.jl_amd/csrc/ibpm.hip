@@ -2102,10 +2102,12 @@ int ibpm_set_warm_start(ibpm_solver* s, int mode, ibpm_net* net, int record) {
     return fail(IBPM_E_ARG, "the network lives on another device than the solver");
   if (net && s->n > net_max_points(net))
     return fail(IBPM_E_ARG, "the solver has more body points than the network's max_points");
+  // the F~b history is kept when both the old and the new setting extend it every step
+  const bool had = s->ws_mode != IBPM_WS_PREVIOUS || s->ws_record;
   s->ws_mode = mode;
   s->ws_record = record ? 1 : 0;
   s->net = net;
-  s->nhist = 0;
+  if (!(had && (mode != IBPM_WS_PREVIOUS || record))) s->nhist = 0;
   return IBPM_OK;
 }
 

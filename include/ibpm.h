@@ -314,6 +314,8 @@ void ibpm_net_destroy(ibpm_net* net);
  * and must outlive its use by the solver. record != 0 appends one training sample
  * (features, (F̃b - x_e) / rms(r_e)) per step with moving bodies to the net's training
  * set; it computes r_e (one extra application of B) in every mode. */
+/* Switching between the extrapolated and learned modes (or recording) keeps the F~b history
+ * of the last steps; switching from IBPM_WS_PREVIOUS starts it anew. */
 int ibpm_set_warm_start(ibpm_solver* s, int mode, ibpm_net* net, int record);
 
 int ibpm_net_dataset_size(const ibpm_net* net);  /* samples held */

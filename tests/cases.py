@@ -149,6 +149,46 @@ def plunging_ellipses(nx=2048, ny=1024, nlevel=6, h=1.0 / 128, x0=(-4.0, -4.0), 
                       scheme=ib.CNAB(dt=dt or ib.default_scheme(grid, Umax=1.0, cfl=0.2).dt))
 
 
+def multibody_ensemble(nprob=32, seed=2025, nx=2048, ny=1024, nlevel=6, h=1.0 / 128,
+                       x0=(-4.0, -4.0), amp=0.2):
+    """Combined workload: ``nprob`` independent problems of 2-4 plunging ellipses each.
+    Per problem (numpy default_rng(seed)): Re uniform in [100, 1000]; 2-4 ellipses with
+    semi-axes uniform in [0.1, 0.5], centres uniform in [0, 4] x [-1.5, 1.5], redrawn until
+    no two bounding circles (plus the plunge travel 2 amp and 0.1) overlap; each plunges
+    y = cy + amp sin(2 pi f t) with f uniform in [0.1, 0.5]. Full size: 2048 x 1024 cells
+    on [-4,12] x [-4,4] (h = 1/128), 6 levels; panels spaced 2h; dt from CFL 0.2 at the
+    freestream speed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nprob):
+        grid = ib.MultiDomainGrid(ib.CartesianGrid(tuple(x0), (nx, ny), h), nlevel)
+        Re = float(rng.uniform(100.0, 1000.0))
+        fluid = ib.Fluid(grid=grid, Re=Re, freestream_vel=lambda t: (1.0, 0.0))
+        nb = int(rng.integers(2, 5))
+        bodies = {}
+        placed = []
+        while len(placed) < nb:
+            a, b = float(rng.uniform(0.1, 0.5)), float(rng.uniform(0.1, 0.5))
+            cx, cy = float(rng.uniform(0.0, 4.0)), float(rng.uniform(-1.5, 1.5))
+            r = max(a, b)
+            if all(math.hypot(cx - px, cy - py) >= r + pr + 2 * amp + 0.1
+                   for (px, py, pr) in placed):
+                placed.append((cx, cy, r))
+                f = float(rng.uniform(0.1, 0.5))
+                pan = curves.panels(grid, curves.ellipse(a, b))
+
+                def motion(t, cx=cx, cy=cy, f=f):
+                    w = 2 * math.pi * f
+                    return ib.RigidBodyTransform(pos=(cx, cy + amp * math.sin(w * t)),
+                                                 vel=(0.0, w * amp * math.cos(w * t)),
+                                                 angle=0.0, angular_vel=0.0)
+
+                bodies[f"ellipse{len(placed)}"] = ib.MovingRigidBody(motion, pan)
+        out.append(ib.Problem(fluid=fluid, bodies=ib.Bodies(**bodies),
+                              scheme=ib.default_scheme(grid, Umax=1.0, cfl=0.2)))
+    return out
+
+
 def reference_solver_problem(grid_motion="static grid", bodies="single cyl"):
     """The problems of the reference's solver tests (test/solver.jl:1-63): a 3-level grid
     on [-1,2] x [-1.5,1.5] with the Re 100 default step (h = 0.02: 150 x 150 cells, not

@@ -163,3 +163,60 @@ def test_batch_reports_panels_leaving_the_region():
     with pytest.raises(ib._lib.IbpmError, match="problem 1.*outside allowed region"):
         batch.advance(20)
     batch.close()
+
+
+def test_multibody_ensemble_mixed_learned_batched():
+    """The combined workload at reduced size: 4 problems of 2-4 plunging ellipses
+    (cases.multibody_ensemble on 512 x 256 cells, 4 levels), mixed-precision sine
+    contractions and the learned CG warm start, batched. Each problem ends bitwise equal
+    to itself advanced alone, and its forces stay within 1e-6 of the FP64 path with the
+    reference's plain CG from the previous step's solution."""
+    probs = cases.multibody_ensemble(4, nx=512, ny=256, nlevel=4, h=1.0 / 32)
+    cg = dict(maxiter=64, reltol=1e-10)
+    net = WarmStartNet(channels=64, layers=4, kernel=5, max_points=256, capacity=2000,
+                       max_batch=16, seed=3)
+    print("case multibody ensemble 4 x 512x256: record", flush=True)
+    for p in probs[:2]:
+        s = ib.create_solver(p, box_update="off", cg=cg, cg_tol_mode=1)
+        s.set_warm_start("extrapolate", net=net, record=True)
+        s.advance(100)
+        s.close()
+    net.train(400, batch=16, lr=1e-3)
+
+    def make(p):
+        s = ib.create_solver(p, precision="mixed", box_update="off", cg=cg, cg_tol_mode=1)
+        s.set_warm_start("extrapolate")
+        return s
+
+    alone = [make(p) for p in probs]
+    batched = [make(p) for p in probs]
+    warm, steps = 3, 12
+    f_warm = [s.forces(s.advance(warm)) for s in alone]
+    for s in batched:
+        s.advance(warm)
+    for s in alone + batched:
+        s.set_warm_start("learned", net=net)
+    f_alone, it_alone = [], []
+    for s in alone:
+        st = s.advance(steps)
+        f_alone.append(s.forces(st))
+        it_alone.append(s.cg_iters(st))
+    print("case multibody ensemble: batched", flush=True)
+    batch = ib.Batch(batched)
+    f_b = batch.advance(steps)
+    it_b = batch.cg_iters()
+    batch.close()
+    print("CG iterations per step (learned):", np.mean(it_alone, 1), flush=True)
+    assert np.array_equal(f_b, np.stack(f_alone, 1))
+    assert np.array_equal(it_b, np.stack(it_alone, 1))
+    for a, b in zip(alone, batched):
+        for name in ("Gamma", "psi", "Ftilde", "xb"):
+            assert np.array_equal(a.read(name), b.read(name)), name
+    for k, p in enumerate(probs):
+        ref = ib.create_solver(p, cg=dict(maxiter=5000, reltol=1e-10), cg_tol_mode=1)
+        fr = ref.forces(ref.advance(warm + steps))
+        ref.close()
+        got = np.concatenate([f_warm[k], f_alone[k]])
+        e = np.linalg.norm(got - fr, axis=1) / np.linalg.norm(fr, axis=1)
+        print(f"problem {k}: force rel err vs FP64 plain CG, max {e.max():.2e}", flush=True)
+        assert e.max() <= 1e-6, (k, e)
