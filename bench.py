@@ -82,6 +82,78 @@ def kernel_roofline(solver, kernel, reps, peak_tflops):
             "avg_launch_ms": round(ms.value, 5), "bytes_per_launch": by.value}
 
 
+def step_roofline(solver, ms_per_step, pass_flops):
+    """Whole-step roofline of the FP64 step as it executes (folded sine passes, box-update
+    products, Gauss-Seidel Helmholtz solves, spectral ring chains): per kernel class, the algorithmic matrix-core flops and minimum HBM bytes
+    (each input read once, each output written once), its floor max(flops / 78.6 TFLOP/s,
+    bytes / 8 TB/s), and the step's floor as the sum of those (the classes run one after
+    another). pass_flops: matrix-core flops of one 5-plane sine pass (ibpm_time_kernel)."""
+    g = solver.state.prob.fluid.grid
+    nx, ny = g.base.n
+    L = g.nlevel
+    Nx, Ny = nx - 1, ny - 1
+    P = Nx * Ny                       # plane values
+    nq = (nx + 1) * ny + nx * (ny + 1)
+    n = solver.state.prob.bodies.npanel
+    B8 = 8.0
+    comp = []
+
+    def add(name, launches, flops, nbytes):
+        comp.append({"kernel": name, "launches": launches, "flops": float(flops),
+                     "bytes": float(nbytes)})
+
+    boxes = [solver.box_update(l) for l in range(1, L + 1)]
+    pbox = solver.box_update(0)
+    on = boxes[0] is not None
+    add("nonlin_rhs (avg_flux, direct_product, rot, CNAB rhs; all levels)", 1, 0,
+        L * (nq + 4 * P) * B8)
+    add("Helmholtz solves, Gauss-Seidel (all levels)", 1, 0, 2 * L * P * B8)
+    # two vort2flux per step; coarsify reads the fine plane, writes the coarse centre
+    # (trial) or the whole coarse plane with the outer copy fused (final)
+    add("coarsify (trial)", L - 1, 0, (L - 1) * (P + P / 4) * B8)
+    add("coarsify + state copy (final)", L - 1, 0, (L - 1) * 3 * P * B8)
+    # sine passes: forward pair of the trial, inverse pair of the final (box updates),
+    # 5 planes in and out per launch
+    npass = 4 if on else 8
+    add("sine passes (double fold, FP64 MFMA)", npass, npass * pass_flops, npass * 2 * L * P * B8)
+    # ring chains: every level's spectral plane read, E written where psi is formed, the
+    # ring operators M_x, M_y read per level step
+    m_bytes = (Nx * Nx + Ny * Ny) * B8
+    add("ring chain (sweep, reduce, factor) x 2", 2 * (3 * L - 2), 0,
+        2 * L * P * B8 + (1 + L) * P * B8 + 2 * (L - 1) * m_bytes)
+    if on:
+        fl = by = 0.0
+        for b in boxes:
+            if b is None or b[1] == 0:
+                continue
+            bw, bh = b[1], b[3]
+            fl += min(2.0 * Nx * bw * bh + 2.0 * Nx * bh * Ny, 2.0 * bw * bh * Ny + 2.0 * Nx * bw * Ny)
+            by += 2 * P * B8 + (bw * bh + Nx * bw + Ny * bh) * B8
+        add("box update products (FP64 MFMA)", 2, fl, by)
+    if pbox is not None:
+        bw, bh = pbox[1], pbox[3]
+        add("trial psi on the panels' box (FP64 MFMA, K-split)", 4,
+            2.0 * Nx * Ny * bh + 2.0 * bw * Nx * bh, (P + Ny * bh + bw * Nx + bw * bh) * B8)
+    add("couple_surface (gather, dense B^-1 product, forces)", 3, 2.0 * (2 * n) ** 2,
+        ((2 * n) ** 2 + 98 * 2 * n) * B8)
+    add("project_circ (copy, rot E^T on the box, Gauss-Seidel on the box)", 3, 0, 2 * P * B8)
+    add("curl (all levels)", 1, 0, L * (P + nq) * B8)
+    floor = 0.0
+    for c in comp:
+        tf = c["flops"] / (PEAK["fp64"] * 1e12)
+        tb = c["bytes"] / (HBM_PEAK * 1e9)
+        c["bound"] = "mfma" if tf > tb else "hbm"
+        c["floor_us"] = round(1e6 * max(tf, tb), 2)
+        floor += max(tf, tb)
+    return {"bound": "sum over kernel classes of max(flops / MFMA peak, bytes / HBM peak)",
+            "floor_ms": round(1e3 * floor, 4), "measured_ms": round(ms_per_step, 4),
+            "frac": round(1e3 * floor / ms_per_step, 4),
+            "mfma_flops_per_step": sum(c["flops"] for c in comp),
+            "hbm_bytes_per_step": sum(c["bytes"] for c in comp),
+            "peaks": {"fp64_mfma_tflops": PEAK["fp64"], "hbm_GBps": HBM_PEAK},
+            "components": comp}
+
+
 def measured_traffic(precision, tiles, levels):
     """HBM bytes per launch of the sine pass timed by kernel_roofline (the 5-plane launch of
     the 1024^2 workload, tile variant ``tiles``), from the rocprofv3 FETCH_SIZE / WRITE_SIZE
@@ -309,6 +381,234 @@ def moving_bench(args):
     return out
 
 
+def multibody_bench(args):
+    """Combined workload: problems of 2-4 plunging ellipses each (cases.multibody_ensemble:
+    2048 x 1024 cells on [-4,12] x [-4,4], 6 levels, Re U[100,1000]), all advanced together
+    by one graph replay per step. CG to a 1e-8 ||b||-relative residual (at most 64
+    iterations), preconditioned by B^-1 of the first step, starting from the learned guess
+    (a network trained on steps the solver recorded on two of the problems). Forces of one
+    problem are checked against the FP64 path with the reference's plain CG."""
+    import immersedbodies as ib
+    from immersedbodies.warmstart import WarmStartNet
+    import cases
+
+    n_max = max(args.mb_problems, args.mb_mixed_problems)
+    probs = cases.multibody_ensemble(n_max)
+    cg = dict(maxiter=64, reltol=1e-8)
+    warm = 3
+    npts = max(p.bodies.npanel for p in probs)
+    net = WarmStartNet(channels=128, layers=4, kernel=5, max_points=1024,
+                       capacity=2 * args.mb_record, max_batch=16, seed=2)
+    t0 = time.perf_counter()
+    for p in probs[:2]:
+        s = ib.create_solver(p, box_update="off", cg=cg, cg_tol_mode=1, cg_precond=True)
+        s.set_warm_start("extrapolate", net=net, record=True)
+        s.advance(args.mb_record)
+        s.close()
+    loss = net.train(args.mb_train_iters, batch=16, lr=1e-3)
+    t_net = time.perf_counter() - t0
+    log(f"multibody: network recorded and trained in {t_net:.1f} s")
+    out = {"workload": "problems of 2-4 ellipses (semi-axes U[0.1,0.5], centres U[0,4]x"
+                       "[-1.5,1.5], no overlap) plunging y = 0.2 sin(2 pi f t), f U[0.1,0.5], "
+                       "2048x1024 cells on [-4,12]x[-4,4], 6 levels, Re U[100,1000], dt 0.001, "
+                       "panels spaced 2h, numpy default_rng(2025)",
+           "cg": {"reltol": 1e-8, "tol_mode": "||b||-relative", "maxiter": 64,
+                  "preconditioner": "B^-1 of the first step, rotated with each body",
+                  "initial_guess": "learned (network trained on "
+                                   f"{2 * args.mb_record} recorded steps)"},
+           "net_record_train_s": round(t_net, 2),
+           "train_loss_first_last": [float(loss[:20].mean()), float(loss[-20:].mean())],
+           "panels_max": npts, "runs": {}}
+    f0 = None
+    for prec, nprob in (("fp64", args.mb_problems), ("mixed", args.mb_mixed_problems)):
+        if nprob <= 0:
+            continue
+        t0 = time.perf_counter()
+        sol = []
+        for p in probs[:nprob]:
+            s = ib.create_solver(p, precision=prec, refine_sweeps=args.refine, box_update="off",
+                                 cg=cg, cg_tol_mode=1, cg_precond=True)
+            s.set_warm_start("extrapolate")
+            s.advance(warm)  # the preconditioner's assembly and the guess's history
+            s.set_warm_start("learned", net=net)
+            sol.append(s)
+        setup = time.perf_counter() - t0
+        batch = ib.Batch(sol)
+        fw = batch.advance(1)
+        t0 = time.perf_counter()
+        f = batch.advance(args.mb_steps)
+        dt = time.perf_counter() - t0
+        st = batch.last_stats
+        it = batch.cg_iters()
+        kept = np.mean([[x.ws_used for x in row] for row in st])
+        conv = all(x.cg_converged for row in st for x in row)
+        batch.close()
+        for s in sol:
+            s.close()
+        run = {"problems": nprob, "precision": prec, "steps": args.mb_steps,
+               "setup_s": round(setup, 2),
+               "problem_steps_per_s": round(nprob * args.mb_steps / dt, 3),
+               "ensemble_steps_per_s": round(args.mb_steps / dt, 4),
+               "ms_per_problem_step": round(1e3 * dt / (nprob * args.mb_steps), 3),
+               "cg_iters_per_step": round(float(it.mean()), 3),
+               "learned_guess_kept": round(float(kept), 3), "all_converged": bool(conv),
+               "bodies_per_problem": [len(p.bodies.sections) for p in probs[:nprob]]}
+        out["runs"][prec] = run
+        if prec == args.mb_check_precision or f0 is None:
+            f0 = (prec, np.concatenate([fw[:, 0], f[:, 0]]))
+        log(f"multibody {prec}: {run['problem_steps_per_s']} problem-steps/s, "
+            f"{run['cg_iters_per_step']} CG iterations/step")
+    # problem 0 with the FP64 path and the reference's plain CG from the previous solution
+    ref = ib.create_solver(probs[0], cg=dict(maxiter=5000, reltol=1e-10), cg_tol_mode=1)
+    fr = ref.forces(ref.advance(warm + 1 + args.mb_steps))[warm:]
+    ref.close()
+    prec, fb = f0
+    e = np.linalg.norm(fb - fr, axis=1) / np.linalg.norm(fr, axis=1)
+    out["force_check"] = {"problem": 0, "path": prec, "against": "FP64, plain CG to 1e-10",
+                          "rel_err_max": float(e.max()), "meets_1e-6": bool(e.max() <= 1e-6)}
+    return out
+
+
+# ---- measurement helpers for the rocprofv3 summaries under profiles/ ----
+# (bench.py --kernel-workload K: the program to profile; bench.py --summarize MODE PATH:
+# the tables the summaries quote)
+
+import sqlite3  # noqa: E402
+from collections import defaultdict  # noqa: E402
+
+
+def step_breakdown(db_path: str, steps: int = 20, marker: str = "nonlin_rhs_kernel"):
+    db = sqlite3.connect(db_path)
+    rows = list(db.execute("select name, start, end from kernels order by start"))
+    # a step's marker is followed by the step's first Helmholtz-solve kernel, a sine
+    # contraction pass or a Gauss-Seidel launch (isolated marker launches, such as a
+    # kernel-timing loop, are not step boundaries)
+    marks = [i for i, r in enumerate(rows)
+             if marker in r[0] and i + 1 < len(rows)
+             and ("dst_pass" in rows[i + 1][0] or "gs_helmholtz" in rows[i + 1][0])]
+    if len(marks) < 2:
+        raise ValueError("fewer than two marker dispatches")
+    sel = marks[-(steps + 1):]
+    n = len(sel) - 1
+    per = defaultdict(lambda: [0, 0.0])
+    busy = 0.0
+    for i in range(sel[0], sel[-1]):
+        name, s, e = rows[i]
+        d = (e - s) * 1e-3  # ns -> us
+        short = name.replace("(anonymous namespace)::", "").split("(")[0]
+        short = short.replace("void ", "").replace("ibpm::", "")
+        per[short][0] += 1
+        per[short][1] += d
+        busy += d
+    wall = (rows[sel[-1]][1] - rows[sel[0]][1]) * 1e-3
+    table = sorted(((k, c / n, t / n) for k, (c, t) in per.items()), key=lambda x: -x[2])
+    return n, wall / n, busy / n, table
+
+
+def workload(kernel: int, precision: str, reps: int, copy_mib: int = 2048, levels: int = 5,
+             tiles: int = -1):
+    """Launch one kernel of the 1024^2 cylinder step ``reps`` times (ibpm_time_kernel), then
+    one plain device-to-device copy of ``copy_mib`` MiB whose byte count is known: the
+    reference for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+
+    import cases  # workload builders (synthetic inputs)
+    import immersedbodies as ib
+    from immersedbodies import _lib
+
+    s = ib.create_solver(cases.cylinder_1024(nlevel=levels), precision=precision, dst_tiles=tiles)
+    s.advance(2)
+    ms, fl, by = C.c_double(), C.c_double(), C.c_double()
+    _lib.check(_lib.load().ibpm_time_kernel(s._h, kernel, reps, C.byref(ms), C.byref(fl),
+                                            C.byref(by)))
+    print(f"kernel {kernel} {precision}: avg {ms.value:.5f} ms, flops {fl.value:.6g}, "
+          f"min bytes {by.value:.6g}", flush=True)
+    s.close()
+    # reference copy: one device-to-device hipMemcpy (the runtime's copy kernel) of a source
+    # written before a 1 GiB flush buffer, so that neither L2 nor the 256 MB last-level
+    # cache still holds it
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = C.c_size_t(copy_mib * (1 << 20))
+    fbytes = C.c_size_t(1 << 30)
+    a, b, f = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    for ptr, nb in ((a, nbytes), (b, nbytes), (f, fbytes)):
+        if hip.hipMalloc(C.byref(ptr), nb) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, nbytes)
+    hip.hipMemset(f, 2, fbytes)
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(b, a, nbytes, 3) != 0:  # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    for ptr in (a, b, f):
+        hip.hipFree(ptr)
+    print(f"copy: {copy_mib} MiB read, {copy_mib} MiB written", flush=True)
+
+
+def pmc_summary(db_path: str, match: str, last: int = 11, copy_match: str = "copyBuffer"):
+    """From a rocprofv3 --pmc database: the mean counter value (KiB) over the last ``last``
+    dispatches of the kernel whose name contains ``match`` (those with the grid of the
+    final dispatch: the timing loop of ``workload``), and the largest value on a dispatch
+    of the reference copy kernel."""
+    db = sqlite3.connect(db_path)
+    rows = list(db.execute("select dispatch_id, kernel_name, grid_size, counter_name, value "
+                           "from counters_collection order by dispatch_id"))
+    per = defaultdict(lambda: defaultdict(float))  # counter -> dispatch -> value
+    meta = {}
+    for d, name, grid, counter, value in rows:
+        per[counter][d] += value
+        meta[d] = (name, grid)
+    out = {}
+    for counter, disp in per.items():
+        ks = [d for d in sorted(disp) if match in meta[d][0]]
+        if ks:
+            g = meta[ks[-1]][1]
+            ks = [d for d in ks if meta[d][1] == g][-last:]
+        cv = [disp[d] for d in disp if copy_match in meta[d][0]]
+        out[counter] = (sum(disp[d] for d in ks) / len(ks) if ks else None, len(ks),
+                        meta[ks[-1]][0] if ks else None, max(cv) if cv else None)
+    return out
+
+
+def trace_tail(csv_path: str, match: str, last: int = 200):
+    """From a rocprofv3 --kernel-trace CSV: mean and spread of the durations (us) of the last
+    ``last`` dispatches of the kernel whose name contains ``match`` that share the grid of
+    its final dispatch (the timing loop that follows the steps in bench.py)."""
+    import csv
+
+    rows = []
+    with open(csv_path) as fh:
+        for r in csv.DictReader(fh):
+            if match in r["Kernel_Name"]:
+                rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], int(r["Grid_Size_X"]),
+                             (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3))
+    rows.sort()
+    g, name = rows[-1][2], rows[-1][1]
+    d = [r[3] for r in rows if r[2] == g and r[1] == name][-last:]
+    mean = sum(d) / len(d)
+    return name, g, len(d), mean, min(d), max(d)
+
+
+def summarize(mode, path, match, last, steps):
+    if mode == "trace":
+        name, g, n, mean, lo, hi = trace_tail(path, match, last)
+        print(f"`{name}` grid {g}: last {n} dispatches mean {mean:.2f} us "
+              f"(min {lo:.2f}, max {hi:.2f})")
+    elif mode == "pmc":
+        for counter, (mean, n, kname, copyv) in pmc_summary(path, match, last).items():
+            scale = 2048 * 1024.0 / copyv if copyv else None
+            print(f"{counter}: `{kname}`: {mean:.1f} KiB per dispatch (mean of {n}); copy of "
+                  f"2048 MiB: {copyv:.1f} KiB, scale {scale:.4f}; scaled "
+                  f"{mean * scale / 1024:.2f} MiB per dispatch")
+    else:
+        n, wall, busy, table = step_breakdown(path, steps)
+        print(f"steps aggregated: {n}; wall per step {wall:.1f} us; kernel time per step "
+              f"{busy:.1f} us\n")
+        print("| kernel | calls/step | us/step | % of kernel time |\n|---|---|---|---|")
+        for k, c, t in table:
+            print(f"| `{k}` | {c:.1f} | {t:.1f} | {100 * t / busy:.1f} |")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -334,7 +634,32 @@ def main():
     ap.add_argument("--no-moving", action="store_true", help="skip the plunging-ellipse section")
     ap.add_argument("--moving-warmup", type=int, default=5)
     ap.add_argument("--moving-steps", type=int, default=20)
+    ap.add_argument("--no-multibody", action="store_true",
+                    help="skip the batched multi-body plunging section")
+    ap.add_argument("--mb-problems", type=int, default=32)
+    ap.add_argument("--mb-mixed-problems", type=int, default=8)
+    ap.add_argument("--mb-steps", type=int, default=5)
+    ap.add_argument("--mb-record", type=int, default=150)
+    ap.add_argument("--mb-train-iters", type=int, default=1500)
+    ap.add_argument("--mb-check-precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--kernel-workload", type=int, default=None,
+                    help="only launch kernel K of the 1024^2 step --reps times (ibpm_time_kernel) "
+                         "and a 2048 MiB reference copy: the program rocprofv3 profiles")
+    ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--tiles", type=int, default=-1)
+    ap.add_argument("--summarize", nargs=2, metavar=("MODE", "PATH"), default=None,
+                    help="MODE breakdown (rocprofv3 results db), pmc (--pmc results db) or "
+                         "trace (kernel-trace csv): print the table a profile summary quotes")
+    ap.add_argument("--match", default="dst_pass")
+    ap.add_argument("--last", type=int, default=200)
     args = ap.parse_args()
+    if args.summarize:
+        summarize(args.summarize[0], args.summarize[1], args.match, args.last, args.steps)
+        return
+    if args.kernel_workload is not None:
+        workload(args.kernel_workload, "mixed" if args.precision == "mixed" else "fp64",
+                 args.reps, levels=args.levels, tiles=args.tiles)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(every problem fits one MI355X); run with --gpus 1", file=sys.stderr)
@@ -355,8 +680,11 @@ def main():
         roof["frac_of_sustained"] = round(roof["achieved"] / ceil, 4) if ceil else None
         roof["traffic"] = measured_traffic(prec, roof["tiles_TB_TM_PF_fold"], args.levels)
         roof_nl = kernel_roofline(sv, 1, args.roofline_reps, None)
+        step_roof = step_roofline(sv, 1e3 * dt / steps, roof["flops_per_launch"]) \
+            if prec == "fp64" else None
         sv.close()
-        runs[prec] = dict(dt=dt, setup=setup, f=f, it=it, roof=roof, roof_nl=roof_nl)
+        runs[prec] = dict(dt=dt, setup=setup, f=f, it=it, roof=roof, roof_nl=roof_nl,
+                          step_roof=step_roof)
         log(f"{prec}: {steps / dt:.2f} steps/s")
     tol = 1e-6
     err = None
@@ -390,6 +718,7 @@ def main():
         "final_force": [float(h["f"][-1, 0]), float(h["f"][-1, 1])],
         "roofline": h["roof"],
         "roofline_nonlinear_kernel": h["roof_nl"],
+        "roofline_step": h["step_roof"],
         "paths": {p: {"steps_per_s": round(steps / r["dt"], 3),
                       "ms_per_step": round(1e3 * r["dt"] / steps, 4),
                       "roofline_sine_pass": r["roof"]} for p, r in runs.items()},
@@ -410,6 +739,8 @@ def main():
             res["batch"][prec] = batch_bench(args)
     if not args.no_moving:
         res["moving_bodies"] = moving_bench(args)
+    if not args.no_multibody:
+        res["multibody_batch"] = multibody_bench(args)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(prob, args.cpu_steps)
     print(json.dumps(res), flush=True)

@@ -229,6 +229,7 @@ hipError_t launch_body_sort(const BodyDev& b, hipStream_t st) {
 // l < 49 forms the product of weight (a, b) = (l % 7, l / 7) with the flux there, and the
 // 49 products are summed by a fixed shuffle tree (deterministic).
 // ---------------------------------------------------------------------------------
+template <bool DYN>
 __global__ __launch_bounds__(256) void gather_kernel(const GatherArgs a) {
   if (skipped(a.skip)) return;
   const int lane = threadIdx.x & 63;
@@ -248,9 +249,9 @@ __global__ __launch_bounds__(256) void gather_kernel(const GatherArgs a) {
     const int j = iy + bb - 3, i = ix + aa - 3;
     double qv;
     if (!comp)  // u(i,j) = psi_e(i-1, j) - psi_e(i-1, j-1)
-      qv = curl_u(p1, p2, nx, ny, i, j) + base_flux_u(a.bf, 0, i, j);
+      qv = curl_u(p1, p2, nx, ny, i, j) + base_flux_u<DYN>(a.bf, 0, i, j);
     else  // v(i,j) = psi_e(i-1, j-1) - psi_e(i, j-1)
-      qv = curl_v(p1, p2, nx, ny, i, j) + base_flux_v(a.bf, 0, i, j);
+      qv = curl_v(p1, p2, nx, ny, i, j) + base_flux_v<DYN>(a.bf, 0, i, j);
     s = qv * a.w[(size_t)p * 98 + comp * 49 + lane];
   }
 #pragma unroll
@@ -265,7 +266,11 @@ hipError_t launch_gather(const GatherArgs& a0, hipStream_t st) {
   if (a0.npanel == 0) return hipSuccess;
   GatherArgs a = a0;
   a.skip = g_skip;
-  hipLaunchKernelGGL(gather_kernel, dim3((2 * a.npanel + 3) / 4, a.nrhs), dim3(256), 0, st, a);
+  const dim3 grid((2 * a.npanel + 3) / 4, a.nrhs);
+  if (a.bf.dyn)
+    hipLaunchKernelGGL(gather_kernel<true>, grid, dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(gather_kernel<false>, grid, dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

@@ -1464,6 +1464,18 @@ void base_flux_dyn(const ibpm_solver* s, const double fs8[8], double out[14]) {
 }
 }  // namespace ibpm
 
+// Reads n doubles (grid-stride) so their lines replace whatever the caches held, without
+// leaving dirty lines behind; the sum is stored only in the impossible case that it is
+// exactly `never`, which keeps the loads.
+__global__ void flush_read_kernel(const double* __restrict__ p, size_t n, double never,
+                                  double* __restrict__ sink) {
+  double s = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    s += p[i];
+  if (s == never) sink[0] = s;
+}
+
 // ---- Operator access that allocates nothing (ibpm_apply_noalloc) ----
 //
 // Its intermediate planes live in this file-scope device array: one copy per GPU, at its
@@ -2297,17 +2309,44 @@ int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, doubl
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  HIPCHK(hipEventRecord(e0, s->st));
-  for (int r = 0; r < reps; ++r) {
-    rc = launch();
-    if (rc) return rc;
-  }
-  HIPCHK(hipEventRecord(e1, s->st));
-  HIPCHK(hipEventSynchronize(e1));
+  // kernels 0 and 1: each launch timed alone after a 512 MiB read has evicted its inputs
+  // from the L2s and the 256 MB last-level cache (inside a step they come from HBM);
+  // kernel 2: back-to-back launches
+  const size_t flush_n = kernel == 2 ? 0 : ((size_t)512 << 20) / sizeof(double);
+  double* flush = nullptr;
+  hipError_t e = flush_n ? hipMalloc((void**)&flush, flush_n * sizeof(double)) : hipSuccess;
   float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  if (e == hipSuccess && !flush_n) e = hipEventRecord(e0, s->st);
+  for (int r = 0; r < reps && e == hipSuccess && rc == IBPM_OK; ++r) {
+    if (flush_n) {
+      if (r == 0) e = launch_fill(flush, flush_n, 0.0, s->st);
+      hipLaunchKernelGGL(flush_read_kernel, dim3(4096), dim3(256), 0, s->st, (const double*)flush,
+                         flush_n, -1.0, flush);
+      if (e == hipSuccess) e = hipGetLastError();
+      if (e == hipSuccess) e = hipEventRecord(e0, s->st);
+    }
+    if (e == hipSuccess) rc = launch();
+    if (e == hipSuccess && rc == IBPM_OK && flush_n) {
+      e = hipEventRecord(e1, s->st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float one = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&one, e0, e1);
+      ms += one;
+    }
+  }
+  if (e == hipSuccess && rc == IBPM_OK && !flush_n) {
+    e = hipEventRecord(e1, s->st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  }
+  if (flush) {
+    (void)hipStreamSynchronize(s->st);
+    (void)hipFree(flush);
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  HIPCHK(e);
   *avg_ms = ms / reps;
   if (kernel == 2) {
     *flops = probe_flops;

@@ -87,11 +87,13 @@ __host__ __device__ __forceinline__ double curl_v(const double* __restrict__ p,
 // Base flux on u-face (i, j) and v-face (i, j) of level l (0-based level, 1-based faces),
 // in the order of base_flux! (fluid.jl:34-67): u-faces sit at (x0 + h (i-1),
 // y0 + h/2 + h (j-1)), v-faces at (x0 + h/2 + h (i-1), y0 + h (j-1)) (problem.jl:318-334).
-// With b.dyn set, the step's freestream and grid motion come from device memory (the
-// same values base_flux() puts in the arguments, so results are the same bits).
+// With DYN (b.dyn set), the step's freestream and grid motion come from device memory
+// (the same values base_flux() puts in the arguments, so results are the same bits); the
+// kernels are instantiated for both, so a solver's own steps read only their arguments.
+template <bool DYN = true>
 __host__ __device__ inline double base_flux_u(const BaseFlux& b, int l, int i, int j) {
 #pragma clang fp contract(off)
-  const double* d = b.dyn;
+  const double* d = DYN ? b.dyn : nullptr;
   if (!b.moving) return d ? d[0] * b.hc[l] : b.q0u[l];
   const double hc = b.hc[l];
   const double x = b.x0[l] + hc * (double)(i - 1);
@@ -108,9 +110,10 @@ __host__ __device__ inline double base_flux_u(const BaseFlux& b, int l, int i, i
   return hc * (b.rx11 * wx + b.rx21 * wy);
 }
 
+template <bool DYN = true>
 __host__ __device__ inline double base_flux_v(const BaseFlux& b, int l, int i, int j) {
 #pragma clang fp contract(off)
-  const double* d = b.dyn;
+  const double* d = DYN ? b.dyn : nullptr;
   if (!b.moving) return d ? d[1] * b.hc[l] : b.q0v[l];
   const double hc = b.hc[l];
   const double x = (b.x0[l] + hc / 2) + hc * (double)(i - 1);

@@ -30,6 +30,7 @@ namespace ibpm {
 // ---------------------------------------------------------------------------------
 constexpr int NL_RJ = 4;  // rows J per thread: a sliding window over the column
 
+template <bool DYN>
 __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
   const GridDims g = a.g;
   const int lev = blockIdx.z;
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
   for (int r = 0; r < NL_RJ + 2; ++r) {
     const int j = J0 + r;
     qyv[r] = r <= nrow + 1
-                 ? (((qa[r] + qb[r]) + base_flux_v(a.bf, lev, I + 1, j)) + base_flux_v(a.bf, lev, I, j)) / 2
+                 ? (((qa[r] + qb[r]) + base_flux_v<DYN>(a.bf, lev, I + 1, j)) + base_flux_v<DYN>(a.bf, lev, I, j)) / 2
                  : 0.0;
   }
 #pragma unroll
@@ -102,7 +103,7 @@ __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
     if (r >= nrow) break;
     const int J = J0 + r;
     auto Qx = [&](int c) {
-      return (((qxr[c][r + 1] + qxr[c][r]) + base_flux_u(a.bf, lev, I + c, J + 1)) + base_flux_u(a.bf, lev, I + c, J)) / 2;
+      return (((qxr[c][r + 1] + qxr[c][r]) + base_flux_u<DYN>(a.bf, lev, I + c, J + 1)) + base_flux_u<DYN>(a.bf, lev, I + c, J)) / 2;
     };
     const double g00 = ge[1][r + 1];
     const double gm0 = ge[0][r + 1];
@@ -143,7 +144,10 @@ __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
 hipError_t launch_nonlin_rhs(const NonlinArgs& a, hipStream_t st) {
   if (g_skip) return no_skip();
   dim3 grid((a.g.Nx + 63) / 64, (a.g.Ny + 4 * NL_RJ - 1) / (4 * NL_RJ), a.nlevel);
-  hipLaunchKernelGGL(nonlin_rhs_kernel, grid, dim3(256), 0, st, a);
+  if (a.bf.dyn)
+    hipLaunchKernelGGL(nonlin_rhs_kernel<true>, grid, dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(nonlin_rhs_kernel<false>, grid, dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

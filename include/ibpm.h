@@ -201,6 +201,9 @@ int ibpm_apply_noalloc(ibpm_solver* s, int op, int level, const double* in, doub
  *   kernel 1: fused nonlinear term + CNAB right-hand side (all levels)
  *   kernel 2: register-only loop of the MFMA instruction of the solver's precision
  *             (f64 16x16x4 or bf16 16x16x32): the matrix-core rate the device sustains
+ * Kernels 0 and 1 are timed one launch at a time, each after a 512 MiB read that evicts
+ * their inputs from the caches (as inside a step, where they come from HBM); kernel 2
+ * back to back.
  * avg_ms: mean launch duration; flops: MFMA flops executed per launch (0 for kernel 1);
  * bytes: minimum HBM bytes per launch (each input read once, each output written once). */
 int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, double* flops,
