@@ -384,8 +384,8 @@ def moving_bench(args):
 def multibody_bench(args):
     """Combined workload: problems of 2-4 plunging ellipses each (cases.multibody_ensemble:
     2048 x 1024 cells on [-4,12] x [-4,4], 6 levels, Re U[100,1000]), all advanced together
-    by one graph replay per step. CG to a 1e-8 ||b||-relative residual (at most 64
-    iterations), preconditioned by B^-1 of the first step, starting from the learned guess
+    by two graph replays per step (up to the CG, and the rest). CG to a 1e-8 ||b||-relative
+    residual, preconditioned by B^-1 of the first step, starting from the learned guess
     (a network trained on steps the solver recorded on two of the problems). Forces of one
     problem are checked against the FP64 path with the reference's plain CG."""
     import immersedbodies as ib
@@ -394,7 +394,7 @@ def multibody_bench(args):
 
     n_max = max(args.mb_problems, args.mb_mixed_problems)
     probs = cases.multibody_ensemble(n_max)
-    cg = dict(maxiter=64, reltol=1e-8)
+    cg = dict(maxiter=5000, reltol=1e-8)
     warm = 3
     npts = max(p.bodies.npanel for p in probs)
     net = WarmStartNet(channels=128, layers=4, kernel=5, max_points=1024,
@@ -412,7 +412,7 @@ def multibody_bench(args):
                        "[-1.5,1.5], no overlap) plunging y = 0.2 sin(2 pi f t), f U[0.1,0.5], "
                        "2048x1024 cells on [-4,12]x[-4,4], 6 levels, Re U[100,1000], dt 0.001, "
                        "panels spaced 2h, numpy default_rng(2025)",
-           "cg": {"reltol": 1e-8, "tol_mode": "||b||-relative", "maxiter": 64,
+           "cg": {"reltol": 1e-8, "tol_mode": "||b||-relative", "maxiter": 5000,
                   "preconditioner": "B^-1 of the first step, rotated with each body",
                   "initial_guess": "learned (network trained on "
                                    f"{2 * args.mb_record} recorded steps)"},

@@ -12,10 +12,15 @@
 // step, and replayed alternately. A step of the batch is then one hipGraphLaunch,
 // whatever the number of problems or kernels.
 //
-// Moving bodies: their step's CG (cg!, coupling.jl:194-202) is captured as cg_maxiter
-// iterations whose launches return at once after the stopping test is met (the solver's
-// graph mode, ibpm.hip), so the graph does what a host-driven loop with that maxiter does
-// and never synchronises with the host.
+// Moving bodies: a step becomes two graphs. Graph A runs each solver's step up to and
+// including its CG (cg!, coupling.jl:194-202), captured as a budget of iterations whose
+// launches return at once after the stopping test is met (the solver's graph mode,
+// ibpm.hip); graph B runs the rest (forces, project_circ, the final vort2flux). Between
+// them the host reads each moving problem's CG state (one small copy): a problem whose CG
+// has not met its stopping test within the budget continues on its own stream, with the
+// solver's host-driven loop, up to its own cg_maxiter. The iterates are those of the
+// solver advanced alone. The budget follows the iterations the problems need (grown and
+// the graphs captured again when a step needed more).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
@@ -37,7 +42,8 @@ constexpr int kBranches = 4;      // concurrent streams of the graph (hardware q
 constexpr int kHistSteps = 1024;  // device tables and history ring, in steps
 constexpr int kDyn = 14;          // BaseFlux::dyn values per problem and step
 constexpr int kRec = 10;          // history values per problem and step (see record kernel)
-constexpr int kMaxBatchCg = 64;   // largest cg_maxiter of a batched moving-body solver
+constexpr int kCgBudget0 = 8;     // CG iterations captured per step at first
+constexpr int kCgBudgetMax = 32;  // most CG iterations captured per step
 
 // This step's inputs of one problem: dyn row -> the solver's dyn buffer, motion row
 // (6 per body) -> its motion buffer.
@@ -73,6 +79,14 @@ __global__ void batch_record_kernel(const double* total, const double* scal, con
   r[9] = err ? (double)err[0] : 0.0;
 }
 
+// After graph A: a moving problem's CG iterations and converged flag.
+__global__ void batch_cgstate_kernel(const double* scal, double* cgst, int slot) {
+  if (threadIdx.x == 0) {
+    cgst[2 * slot] = scal[3];
+    cgst[2 * slot + 1] = scal[4];
+  }
+}
+
 __global__ void batch_tick_kernel(int* counter) {
   if (threadIdx.x == 0) *counter += 1;
 }
@@ -97,7 +111,11 @@ struct ibpm_batch {
   hipStream_t br[kBranches] = {};
   hipEvent_t fork = nullptr;
   hipEvent_t join[kBranches] = {};
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};  // per parity: the step (A with moving bodies)
+  hipGraphExec_t execB = nullptr;               // moving bodies: the rest of the step
+  bool split = false;                           // moving bodies: graphs A and B
+  int budget = kCgBudget0;                      // CG iterations captured per step
+  double* cgst = nullptr;                       // nprob x 2: CG iterations, converged
   std::vector<double*> nl1_at[2];  // each solver's nl1 when graph g starts
   std::vector<double*> wsh_at[2];  // and its warm-start history head
   double* hist = nullptr;
@@ -118,7 +136,9 @@ namespace {
 void batch_free(ibpm_batch* b) {
   for (hipGraphExec_t& e : b->exec)
     if (e) (void)hipGraphExecDestroy(e);
-  for (void* p : {(void*)b->hist, (void*)b->counter, (void*)b->dyn_tab, (void*)b->mot_tab})
+  if (b->execB) (void)hipGraphExecDestroy(b->execB);
+  for (void* p : {(void*)b->hist, (void*)b->counter, (void*)b->dyn_tab, (void*)b->mot_tab,
+                  (void*)b->cgst})
     if (p) (void)hipFree(p);
   if (b->fork) (void)hipEventDestroy(b->fork);
   for (int i = 0; i < kBranches; ++i) {
@@ -130,10 +150,11 @@ void batch_free(ibpm_batch* b) {
 
 bool uses_ws(const ibpm_solver* s) { return s->ws_mode != IBPM_WS_PREVIOUS || s->ws_record; }
 
-// Capture one step of every solver into exec[g]. Host-side effects of the step on the
-// solvers (time index, freestream, the buffer swaps) happen once per capture; the caller
-// restores the index.
-int capture(ibpm_batch* b, int g) {
+// Capture into *out one step of every solver (phase 0), or with moving bodies its first
+// part (phase 1, graph A) or the rest (phase 2, graph B). Host-side effects of the step on
+// the solvers (time index, freestream, the buffer swaps) happen once per capture of a
+// step's first part; the caller restores the index.
+int capture(ibpm_batch* b, int phase, hipGraphExec_t* out) {
   BCHK(hipStreamBeginCapture(b->origin, hipStreamCaptureModeRelaxed));
   int rc = IBPM_OK;
   hipError_t e = hipEventRecord(b->fork, b->origin);
@@ -143,16 +164,25 @@ int capture(ibpm_batch* b, int g) {
     hipStream_t own = s->st;
     s->st = b->br[b->br_of[p]];
     const bool mov = s->moving && s->n > 0;
-    hipLaunchKernelGGL(batch_inputs_kernel, dim3(1), dim3(64), 0, s->st, (const double*)b->dyn_tab,
-                       (const double*)b->mot_tab, (const int*)b->counter, b->nprob, p, b->mrow,
-                       b->moff[p], mov ? 6 * s->nbody : 0, s->dyn_d, mov ? s->motion : nullptr);
-    e = hipGetLastError();
+    if (phase != 2) {
+      hipLaunchKernelGGL(batch_inputs_kernel, dim3(1), dim3(64), 0, s->st, (const double*)b->dyn_tab,
+                         (const double*)b->mot_tab, (const int*)b->counter, b->nprob, p, b->mrow,
+                         b->moff[p], mov ? 6 * s->nbody : 0, s->dyn_d, mov ? s->motion : nullptr);
+      e = hipGetLastError();
+    }
     if (e == hipSuccess) {
       s->graph_mode = true;
-      rc = solver_step(s, &b->fs[8 * p], nullptr, nullptr);
+      s->graph_cg_budget = b->split ? b->budget : 0;
+      rc = solver_step(s, &b->fs[8 * p], nullptr, nullptr, phase);
       s->graph_mode = false;
+      s->graph_cg_budget = 0;
     }
-    if (e == hipSuccess && rc == IBPM_OK) {
+    if (e == hipSuccess && rc == IBPM_OK && phase == 1 && mov) {
+      hipLaunchKernelGGL(batch_cgstate_kernel, dim3(1), dim3(64), 0, s->st,
+                         (const double*)s->cg_scal, b->cgst, p);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess && rc == IBPM_OK && phase != 1) {
       const bool need_res = s->ws_mode == IBPM_WS_LEARNED;
       hipLaunchKernelGGL(batch_record_kernel, dim3(1), dim3(64), 0, s->st,
                          (const double*)(s->n > 0 ? s->total : nullptr),
@@ -168,7 +198,7 @@ int capture(ibpm_batch* b, int g) {
     e = hipEventRecord(b->join[i], b->br[i]);
     if (e == hipSuccess) e = hipStreamWaitEvent(b->origin, b->join[i], 0);
   }
-  if (e == hipSuccess) {
+  if (e == hipSuccess && phase != 1) {
     hipLaunchKernelGGL(batch_tick_kernel, dim3(1), dim3(64), 0, b->origin, b->counter);
     e = hipGetLastError();
   }
@@ -183,11 +213,50 @@ int capture(ibpm_batch* b, int g) {
     return set_error(IBPM_E_HIP, (std::string("batch graph capture: ") +
                                   hipGetErrorString(e != hipSuccess ? e : ec)).c_str());
   }
-  e = hipGraphInstantiate(&b->exec[g], graph, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess)
     return set_error(IBPM_E_HIP, (std::string("hipGraphInstantiate: ") + hipGetErrorString(e)).c_str());
   return IBPM_OK;
+}
+
+// (Re)capture the batch's graphs from the solvers' current state: exec[0] the step of the
+// current parity, exec[1] of the next (and execB with moving bodies); the solvers' host
+// state is left as it was.
+int capture_all(ibpm_batch* b) {
+  const int np = b->nprob;
+  for (hipGraphExec_t& e : b->exec)
+    if (e) (void)hipGraphExecDestroy(e), e = nullptr;
+  if (b->execB) (void)hipGraphExecDestroy(b->execB), b->execB = nullptr;
+  std::vector<int> index(np);
+  std::vector<double> fs0(2 * (size_t)np), gm0(6 * (size_t)np);
+  for (int p = 0; p < np; ++p) {
+    index[p] = b->s[p]->index;
+    for (int k = 0; k < 2; ++k) fs0[2 * p + k] = b->s[p]->fs[k];
+    for (int k = 0; k < 6; ++k) gm0[6 * p + k] = b->s[p]->gm[k];
+  }
+  int rc = IBPM_OK;
+  for (int g = 0; g < 2 && rc == IBPM_OK; ++g) {
+    b->nl1_at[g].resize(np);
+    b->wsh_at[g].resize(np);
+    for (int p = 0; p < np; ++p) {
+      b->nl1_at[g][p] = b->s[p]->nl1;
+      b->wsh_at[g][p] = b->s[p]->ws_h0;
+    }
+    rc = capture(b, b->split ? 1 : 0, &b->exec[g]);
+    if (rc == IBPM_OK && g == 0 && b->split) rc = capture(b, 2, &b->execB);
+  }
+  for (int p = 0; p < np; ++p) {
+    ibpm_solver* s = b->s[p];
+    s->index = index[p];
+    for (int k = 0; k < 2; ++k) s->fs[k] = fs0[2 * p + k];
+    for (int k = 0; k < 6; ++k) s->gm[k] = gm0[6 * p + k];
+    // two captured steps swap the buffers back; a failed capture may not have
+    if (b->nl1_at[0].size() == (size_t)np && s->nl1 != b->nl1_at[0][p]) std::swap(s->nl1, s->nl2);
+    if (b->wsh_at[0].size() == (size_t)np && uses_ws(s) && s->ws_h0 != b->wsh_at[0][p])
+      std::swap(s->ws_h0, s->ws_h1);
+  }
+  return rc;
 }
 
 // Why solver s cannot be captured, or null.
@@ -195,8 +264,6 @@ const char* batch_refusal(const ibpm_solver* s) {
   if (!s->moving) return nullptr;
   if (s->box_dyn)
     return "moving bodies in a batch need box_update = 1 (their boxes follow the bodies on the host)";
-  if (s->cg_maxiter > kMaxBatchCg)
-    return "moving bodies in a batch need cg_maxiter <= 64 (every iteration is captured)";
   if (s->ws_record) return "recording warm-start samples is not available in a batch";
   if (s->ws_mode != IBPM_WS_PREVIOUS && s->nhist < 3)
     return "the extrapolated or learned warm start needs three steps of history before batching";
@@ -271,39 +338,15 @@ int ibpm_batch_create(int nprob, ibpm_solver* const* solvers, const double* free
   if (e == hipSuccess) e = hipMalloc((void**)&b->dyn_tab, (size_t)kHistSteps * nprob * kDyn * sizeof(double));
   if (e == hipSuccess)
     e = hipMalloc((void**)&b->mot_tab, (size_t)kHistSteps * std::max(b->mrow, 1) * sizeof(double));
+  b->split = b->any_moving;
+  if (e == hipSuccess) e = hipMalloc((void**)&b->cgst, 2 * (size_t)nprob * sizeof(double));
   for (int p = 0; p < nprob && e == hipSuccess; ++p) e = hipStreamSynchronize(b->s[p]->st);
   if (e != hipSuccess) {
     batch_free(b);
     delete b;
     return set_error(IBPM_E_HIP, (std::string("ibpm_batch_create: ") + hipGetErrorString(e)).c_str());
   }
-  std::vector<int> index(nprob);
-  std::vector<double> fs0(2 * (size_t)nprob), gm0(6 * (size_t)nprob);
-  for (int p = 0; p < nprob; ++p) {
-    index[p] = b->s[p]->index;
-    for (int k = 0; k < 2; ++k) fs0[2 * p + k] = b->s[p]->fs[k];
-    for (int k = 0; k < 6; ++k) gm0[6 * p + k] = b->s[p]->gm[k];
-  }
-  int rc = IBPM_OK;
-  for (int g = 0; g < 2 && rc == IBPM_OK; ++g) {
-    b->nl1_at[g].resize(nprob);
-    b->wsh_at[g].resize(nprob);
-    for (int p = 0; p < nprob; ++p) {
-      b->nl1_at[g][p] = b->s[p]->nl1;
-      b->wsh_at[g][p] = b->s[p]->ws_h0;
-    }
-    rc = capture(b, g);
-  }
-  for (int p = 0; p < nprob; ++p) {
-    ibpm_solver* s = b->s[p];
-    s->index = index[p];
-    for (int k = 0; k < 2; ++k) s->fs[k] = fs0[2 * p + k];
-    for (int k = 0; k < 6; ++k) s->gm[k] = gm0[6 * p + k];
-    // two captured steps swap the buffers back; a failed capture may not have
-    if (b->nl1_at[0].size() == (size_t)nprob && s->nl1 != b->nl1_at[0][p]) std::swap(s->nl1, s->nl2);
-    if (b->wsh_at[0].size() == (size_t)nprob && uses_ws(s) && s->ws_h0 != b->wsh_at[0][p])
-      std::swap(s->ws_h0, s->ws_h1);
-  }
+  const int rc = capture_all(b);
   if (rc != IBPM_OK) {
     batch_free(b);
     delete b;
@@ -334,6 +377,7 @@ int ibpm_batch_advance(ibpm_batch* b, int nsteps, const double* freestream, cons
   const int np = b->nprob;
   std::vector<double> dyn((size_t)std::min(nsteps, kHistSteps) * np * kDyn);
   std::vector<double> rec((size_t)std::min(nsteps, kHistSteps) * np * kRec);
+  std::vector<double> cgst(2 * (size_t)np);
   const double* last_fs = b->fs.data();  // 8 per problem: the values of the last step
   int done = 0;
   int bad_step = -1, bad_prob = -1, bad_panel = 0;
@@ -354,6 +398,46 @@ int ibpm_batch_advance(ibpm_batch* b, int nsteps, const double* freestream, cons
     for (int i = 0; i < k; ++i) {
       BCHK(hipGraphLaunch(b->exec[g], b->origin));
       g ^= 1;
+      if (!b->split) continue;
+      // moving bodies: continue the CG of problems the budget left short of the stopping
+      // test, then run the rest of the step
+      BCHK(hipStreamSynchronize(b->origin));
+      BCHK(hipMemcpy(cgst.data(), b->cgst, cgst.size() * sizeof(double), hipMemcpyDeviceToHost));
+      int needed = 0;
+      for (int p = 0; p < np; ++p) {
+        ibpm_solver* s = b->s[p];
+        if (!(s->moving && s->n > 0) || cgst[2 * p + 1] != 0.0) continue;
+        const int rc = solver_cg_continue(s);
+        if (rc) return rc;
+        BCHK(hipStreamSynchronize(s->st));
+        needed = std::max(needed, (int)s->h_scal[3]);
+      }
+      BCHK(hipGraphLaunch(b->execB, b->origin));
+      if (needed > b->budget && b->budget < kCgBudgetMax) {
+        // capture more iterations from the next step on (from the state after this one)
+        BCHK(hipStreamSynchronize(b->origin));
+        b->budget = std::min(kCgBudgetMax, needed + 2);
+        // the solvers' host pointers are those after the step just run only once the
+        // swaps it made are applied; apply them, capture, and undo
+        const int steps_run = done + i + 1;
+        for (int p = 0; p < np; ++p) {
+          ibpm_solver* s = b->s[p];
+          if (steps_run & 1) {
+            std::swap(s->nl1, s->nl2);
+            if (uses_ws(s)) std::swap(s->ws_h0, s->ws_h1);
+          }
+        }
+        const int rc = capture_all(b);
+        for (int p = 0; p < np; ++p) {
+          ibpm_solver* s = b->s[p];
+          if (steps_run & 1) {
+            std::swap(s->nl1, s->nl2);
+            if (uses_ws(s)) std::swap(s->ws_h0, s->ws_h1);
+          }
+        }
+        if (rc) return rc;
+        g = 0;
+      }
     }
     BCHK(hipMemcpyAsync(rec.data(), b->hist, (size_t)k * np * kRec * sizeof(double),
                         hipMemcpyDeviceToHost, b->origin));

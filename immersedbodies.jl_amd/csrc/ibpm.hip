@@ -1073,6 +1073,62 @@ BodyDev body_dev(ibpm_solver* s) {
   return b;
 }
 
+CgDev cg_dev(ibpm_solver* s) {
+  CgDev c;
+  memset(&c, 0, sizeof c);
+  c.n = 2 * s->n;
+  c.x = s->Ft;
+  c.r = s->cg_r;
+  c.u = s->cg_u;
+  c.c = s->cg_c;
+  c.b = s->Ftmp;
+  c.scal = s->cg_scal;
+  c.maxiter = s->cg_maxiter;
+  c.reltol = s->cg_reltol;
+  c.abstol = s->cg_abstol;
+  c.tol_mode = s->cg_tol_mode;
+  return c;
+}
+
+PcDev pc_dev(ibpm_solver* s) {
+  PcDev pd;
+  memset(&pd, 0, sizeof pd);
+  pd.n = s->n;
+  pd.P = s->pc_P;
+  pd.body_of = s->body_of;
+  pd.motion = s->motion;
+  pd.theta0 = s->pc_theta0;
+  return pd;
+}
+
+// Host-driven CG iterations (cg!, IterativeSolvers' CGIterable / PCGIterable order) from
+// the state in cg_* with its scalars in h_scal, until the stopping test (or maxiter);
+// `start` begins the preconditioned search direction first.
+int cg_iterate(ibpm_solver* s, bool start) {
+  const CgDev c = cg_dev(s);
+  double* xc[1] = {s->cg_c};
+  const double* zu[1] = {s->cg_u};
+  const PcDev pd = pc_dev(s);
+  if (s->cg_precond && start && s->h_scal[4] == 0.0) HIPCHK(launch_pcg_start(c, s->st));
+  while (s->h_scal[4] == 0.0) {
+    if (s->cg_precond) HIPCHK(launch_pcg_precond_dir(c, pd, s->pc_t, s->pc_z, s->gemv_part, s->st));
+    int rc = apply_B(s, 1, zu, xc);
+    if (rc) return rc;
+    HIPCHK(s->cg_precond ? launch_pcg_step(c, s->st) : launch_cg_step(c, s->st));
+    HIPCHK(hipMemcpyAsync(s->h_scal, s->cg_scal, 6 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+  }
+  return IBPM_OK;
+}
+
+// Continue a CG whose first iterations a captured step ran (a batch's CG budget) until its
+// stopping test: the iterates the uncaptured step would have reached.
+int cg_continue(ibpm_solver* s) {
+  HIPCHK(hipMemcpyAsync(s->h_scal, s->cg_scal, 6 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return cg_iterate(s, false);
+}
+
 // couple_surface_rigid (coupling.jl:233-251) incl. the CG of moving bodies
 int couple_surface(ibpm_solver* s, ibpm_step_stats* stats) {
   const size_t P = s->plane();
@@ -1102,19 +1158,7 @@ int couple_surface(ibpm_solver* s, ibpm_step_stats* stats) {
   // cg! (coupling.jl:194-202, solver.jl:137). The reference starts from the previous F~b;
   // the warm-start modes only change that initial guess (ibpm_set_warm_start).
   const int n2 = 2 * s->n;
-  CgDev c;
-  memset(&c, 0, sizeof c);
-  c.n = n2;
-  c.x = s->Ft;
-  c.r = s->cg_r;
-  c.u = s->cg_u;
-  c.c = s->cg_c;
-  c.b = s->Ftmp;
-  c.scal = s->cg_scal;
-  c.maxiter = s->cg_maxiter;
-  c.reltol = s->cg_reltol;
-  c.abstol = s->cg_abstol;
-  c.tol_mode = s->cg_tol_mode;
+  const CgDev c = cg_dev(s);
   double* xc[1] = {s->cg_c};
   int rc = IBPM_OK;
   int bapp = 0;
@@ -1181,16 +1225,12 @@ int couple_surface(ibpm_solver* s, ibpm_step_stats* stats) {
       HIPCHK(launch_cg_init(c, s->st));
     }
     SkipScope converged(s->cg_scal + 4);
-    PcDev pd;
-    memset(&pd, 0, sizeof pd);
-    pd.n = s->n;
-    pd.P = s->pc_P;
-    pd.body_of = s->body_of;
-    pd.motion = s->motion;
-    pd.theta0 = s->pc_theta0;
+    const PcDev pd = pc_dev(s);
     if (s->cg_precond) HIPCHK(launch_pcg_start(c, s->st));
     const double* zu[1] = {s->cg_u};
-    for (int it = 0; it < s->cg_maxiter; ++it) {
+    const int K = s->graph_cg_budget > 0 ? std::min(s->graph_cg_budget, s->cg_maxiter)
+                                         : s->cg_maxiter;
+    for (int it = 0; it < K; ++it) {
       if (s->cg_precond) HIPCHK(launch_pcg_precond_dir(c, pd, s->pc_t, s->pc_z, s->gemv_part, s->st));
       rc = apply_B(s, 1, zu, xc);
       if (rc) return rc;
@@ -1220,34 +1260,10 @@ int couple_surface(ibpm_solver* s, ibpm_step_stats* stats) {
   }
   const double bnorm = s->h_scal[5];
   const double res0 = s->h_scal[0];
-  if (s->cg_precond) {
-    PcDev pd;
-    pd.n = s->n;
-    pd.P = s->pc_P;
-    pd.body_of = s->body_of;
-    pd.motion = s->motion;
-    pd.theta0 = s->pc_theta0;
-    if (s->h_scal[4] == 0.0) HIPCHK(launch_pcg_start(c, s->st));
-    while (s->h_scal[4] == 0.0) {
-      HIPCHK(launch_pcg_precond_dir(c, pd, s->pc_t, s->pc_z, s->gemv_part, s->st));
-      const double* zu[1] = {s->cg_u};
-      rc = apply_B(s, 1, zu, xc);
-      if (rc) return rc;
-      HIPCHK(launch_pcg_step(c, s->st));
-      HIPCHK(hipMemcpyAsync(s->h_scal, s->cg_scal, 6 * sizeof(double), hipMemcpyDeviceToHost, s->st));
-      HIPCHK(hipStreamSynchronize(s->st));
-    }
-    if (s->cg_precond_refresh > 0 && s->h_scal[3] > s->cg_precond_refresh) s->pc_due = true;
-  } else {
-    while (s->h_scal[4] == 0.0) {
-      const double* zu[1] = {s->cg_u};
-      rc = apply_B(s, 1, zu, xc);
-      if (rc) return rc;
-      HIPCHK(launch_cg_step(c, s->st));
-      HIPCHK(hipMemcpyAsync(s->h_scal, s->cg_scal, 6 * sizeof(double), hipMemcpyDeviceToHost, s->st));
-      HIPCHK(hipStreamSynchronize(s->st));
-    }
-  }
+  rc = cg_iterate(s, true);
+  if (rc) return rc;
+  if (s->cg_precond && s->cg_precond_refresh > 0 && s->h_scal[3] > s->cg_precond_refresh)
+    s->pc_due = true;
   if (s->ws_record) {
     rc = net_record(s->net, wa, s->st);
     if (rc) return rc;
@@ -1265,7 +1281,24 @@ int couple_surface(ibpm_solver* s, ibpm_step_stats* stats) {
   return IBPM_OK;
 }
 
-int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stats* stats) {
+// Level 1's spectral plane S Gamma_s S is kept when the sine transforms solve level 1
+// (FP64); both vort2flux calls of the step then skip its forward passes.
+bool keeps_spec1(const ibpm_solver* s) {
+  SolveJob j;
+  j.b = nullptr;
+  j.da = 1.0;
+  j.dnum = s->dt;
+  j.dden = 2.0 * s->Re * (s->hc[0] * s->hc[0]);
+  j.x = nullptr;
+  j.epi = EPI_STORE;
+  j.base = nullptr;
+  j.a = 1.0;
+  j.c = cA(s, 0);
+  return s->F1 && !use_gs(s, j);
+}
+
+// advance! up to and including couple_surface (its CG); step_tail does the rest.
+int step_head(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stats* stats) {
   const size_t P = s->plane();
   const int L = s->L;
   // inc_time_index!, freestream (solver.jl:164-167)
@@ -1325,9 +1358,7 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
     j.a = 1.0;
     j.c = cA(s, l);
   }
-  // level 1's spectral plane S Gamma_s S, kept when the sine transforms solve level 1
-  // (FP64); both vort2flux calls of the step then skip its forward passes
-  const bool spec1 = s->F1 && !use_gs(s, jobs[0]);
+  const bool spec1 = keeps_spec1(s);
   jobs[0].spec = spec1 ? s->F1 : nullptr;
   int rc = solve(s, jobs.data(), L);
   if (rc) return rc;
@@ -1342,11 +1373,19 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
                    s->box_on && !spec1 ? s->Ftr : nullptr, false, s->box_on && !spec1);
     if (rc) return rc;
   }
-  // couple_surface!, then project_circ! (coupling.jl:217-231): Gamma = Gamma_s,
-  // Gamma[:,1] -= Ainv1 rot(E^T Ft)
+  // couple_surface! (coupling.jl:233-251)
+  if (s->n > 0) return couple_surface(s, stats);
+  return IBPM_OK;
+}
+
+// advance! after couple_surface: the forces, project_circ! (coupling.jl:217-231):
+// Gamma = Gamma_s, Gamma[:,1] -= Ainv1 rot(E^T Ft), and the final vort2flux.
+int step_tail(ibpm_solver* s, ibpm_step_stats* stats) {
+  const size_t P = s->plane();
+  const int L = s->L;
+  const bool spec1 = keeps_spec1(s);
+  int rc = IBPM_OK;
   if (s->n > 0) {
-    rc = couple_surface(s, stats);
-    if (rc) return rc;
     HIPCHK(launch_forces(s->n, s->Ft, s->fb, s->total, s->h, s->dt, s->force_slot, s->st));
     ScatterArgs sa;
     memset(&sa, 0, sizeof sa);
@@ -1413,6 +1452,11 @@ int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stat
   return IBPM_OK;
 }
 
+int step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stats* stats) {
+  const int rc = step_head(s, fsv, motion, stats);
+  return rc ? rc : step_tail(s, stats);
+}
+
 }  // namespace
 
 namespace ibpm {
@@ -1448,11 +1492,16 @@ BaseFlux base_flux(const ibpm_solver* s) {
   return b;
 }
 
-int solver_step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stats* stats) {
+int solver_step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stats* stats,
+                int phase) {
   if (s->graph_mode && s->box_dyn)
     return fail(IBPM_E_ARG, "a captured step cannot follow moving bodies with box updates");
+  if (phase == 1) return step_head(s, fsv, motion, stats);
+  if (phase == 2) return step_tail(s, stats);
   return step(s, fsv, motion, stats);
 }
+
+int solver_cg_continue(ibpm_solver* s) { return cg_continue(s); }
 
 void base_flux_dyn(const ibpm_solver* s, const double fs8[8], double out[14]) {
   // the scalars base_flux() derives from the freestream and GridVelocity, same expressions

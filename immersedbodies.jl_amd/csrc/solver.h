@@ -98,6 +98,7 @@ struct ibpm_solver {
   // device before the step
   bool graph_mode = false;
   double* dyn_d = nullptr;
+  int graph_cg_budget = 0;  // CG iterations captured per step (0: cg_maxiter)
   // CG warm start: extrapolation history F~b of the two steps before the previous one,
   // the extrapolated guess x_e, B x_e, and (rms(r_e), ||r_e||, ||b||)
   int ws_mode = IBPM_WS_PREVIOUS, ws_record = 0, nhist = 0;
@@ -121,7 +122,13 @@ namespace ibpm {
 BaseFlux base_flux(const ibpm_solver* s);
 // One advance! of a solver (solver.jl:155-190) queued on s->st: no host synchronisation
 // for static bodies. fsv: freestream at the new time; motion: 6 x nbody or null.
-int solver_step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stats* stats);
+// phase 0: the whole step; 1: up to and including couple_surface (with a captured step's
+// CG budget); 2: the rest (forces, project_circ, final vort2flux).
+int solver_step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step_stats* stats,
+                int phase = 0);
+// Host-driven CG iterations from where a captured step's budget left them, to the stopping
+// test, on the solver's stream.
+int solver_cg_continue(ibpm_solver* s);
 // BaseFlux::dyn values of a step from its freestream and GridVelocity fs8 = (ux, uy,
 // centre x, y, velocity x, y, angle, angular_vel): the numbers base_flux() puts in the
 // kernel arguments, bit for bit.

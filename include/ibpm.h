@@ -237,12 +237,15 @@ void ibpm_destroy(ibpm_solver* s);
  * bit for bit, as when advanced alone with the same inputs.
  *
  * Each step's freestream, grid motion and body motion are read by the graph from device
- * tables that ibpm_batch_advance fills from its arguments. A moving body's CG is captured
- * as cg_maxiter iterations (at most 64) whose launches return at once after the stopping
- * test is met: the same iterates, and the same maxiter, as the solver alone. Moving-body
- * solvers in a batch need: box_update = 1 (off); no warm-start recording; three steps of
- * history for the extrapolated or learned guess; an assembled preconditioner (one step)
- * and cg_precond_refresh = 0 when preconditioned. ibpm_batch_create says which is missing.
+ * tables that ibpm_batch_advance fills from its arguments. With moving bodies a step is
+ * two graph replays: the first runs up to and including the CG, captured as a budget of
+ * iterations whose launches return at once after the stopping test is met; a CG that has
+ * not met it then continues on the host's side, up to cg_maxiter; the second replay runs
+ * the rest. The budget grows to the iterations the problems need (at most 32; the graphs
+ * are captured again). The iterates are those of the solver alone. Moving-body solvers in
+ * a batch need: box_update = 1 (off); no warm-start recording; three steps of history for
+ * the extrapolated or learned guess; an assembled preconditioner (one step) and
+ * cg_precond_refresh = 0 when preconditioned. ibpm_batch_create says which is missing.
  *
  * freestream: 2 x nprob, the velocity held for steps that ibpm_batch_advance gives none
  * (NULL: each solver's last). The batch must be destroyed before its solvers. Between

@@ -1,7 +1,8 @@
 """Batched moving bodies, moving grids and time-varying freestreams (one graph per step).
 
-A moving body's CG (cg!, coupling.jl:194-202) runs inside the captured step as cg_maxiter
-iterations whose launches return at once after the stopping test is met, and each step's
+A moving body's CG (cg!, coupling.jl:194-202) runs inside the captured step as a budget
+of iterations whose launches return at once after the stopping test is met, continued on
+the host's side up to cg_maxiter when the budget falls short, and each step's
 body motion (bodies.jl:43-51), freestream (solver.jl:165) and grid motion (fluid.jl:34-67)
 come from device tables the batch fills per call. Every solver advanced in a batch must
 end bitwise equal to the same solver advanced alone with the same inputs: fields, forces
@@ -21,7 +22,7 @@ from immersedbodies.warmstart import WarmStartNet
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("q", "Gamma", "psi", "nonlin1", "nonlin2", "Ftilde", "fb", "xb", "ub")
-CG = dict(maxiter=64, reltol=1e-10)
+CG = dict(maxiter=200, reltol=1e-10)
 
 
 def gusty_cylinder():
@@ -136,9 +137,6 @@ def test_batch_refusals():
     s = ib.create_solver(mc, box_update="force", cg=CG)
     with pytest.raises(ib._lib.IbpmError, match="box_update"):
         ib.Batch([s])
-    s = ib.create_solver(cases.moving_cylinder(), box_update="off")  # maxiter 5000
-    with pytest.raises(ib._lib.IbpmError, match="cg_maxiter"):
-        ib.Batch([s])
     s = ib.create_solver(cases.moving_cylinder(), box_update="off", cg=CG)
     s.set_warm_start("extrapolate")
     with pytest.raises(ib._lib.IbpmError, match="history"):
@@ -172,7 +170,7 @@ def test_multibody_ensemble_mixed_learned_batched():
     to itself advanced alone, and its forces stay within 1e-6 of the FP64 path with the
     reference's plain CG from the previous step's solution."""
     probs = cases.multibody_ensemble(4, nx=512, ny=256, nlevel=4, h=1.0 / 32)
-    cg = dict(maxiter=64, reltol=1e-10)
+    cg = dict(maxiter=500, reltol=1e-10)
     net = WarmStartNet(channels=64, layers=4, kernel=5, max_points=256, capacity=2000,
                        max_batch=16, seed=3)
     print("case multibody ensemble 4 x 512x256: record", flush=True)
