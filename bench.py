@@ -507,9 +507,10 @@ def step_breakdown(db_path: str, steps: int = 20, marker: str = "nonlin_rhs_kern
 
 def workload(kernel: int, precision: str, reps: int, copy_mib: int = 2048, levels: int = 5,
              tiles: int = -1):
-    """Launch one kernel of the 1024^2 cylinder step ``reps`` times (ibpm_time_kernel), then
-    one plain device-to-device copy of ``copy_mib`` MiB whose byte count is known: the
-    reference for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    """Time one kernel class of the 1024^2 cylinder step (ibpm_time_kernel: kernels 0 and 1
+    inside ``reps`` real steps, kernel 2 in a loop), then one plain device-to-device copy of
+    ``copy_mib`` MiB whose byte count is known: the reference for scaling the FETCH_SIZE /
+    WRITE_SIZE counters."""
     import ctypes as C
 
     import cases  # workload builders (synthetic inputs)
@@ -643,8 +644,8 @@ def main():
     ap.add_argument("--mb-train-iters", type=int, default=1500)
     ap.add_argument("--mb-check-precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--kernel-workload", type=int, default=None,
-                    help="only launch kernel K of the 1024^2 step --reps times (ibpm_time_kernel) "
-                         "and a 2048 MiB reference copy: the program rocprofv3 profiles")
+                    help="only time kernel K of the 1024^2 step (ibpm_time_kernel; K = 0, 1 inside "
+                         "--reps steps) and a 2048 MiB reference copy: the program rocprofv3 profiles")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--tiles", type=int, default=-1)
     ap.add_argument("--summarize", nargs=2, metavar=("MODE", "PATH"), default=None,

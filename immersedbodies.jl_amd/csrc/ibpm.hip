@@ -115,6 +115,18 @@ const void* tab_even(const ibpm_solver* s, bool xdir) {
   return xdir ? s->SeX : s->SeY;
 }
 
+// Instrumentation: events around one launch of the probed kernel class (the pool holds
+// the pairs of one step; ibpm_time_kernel reads them after each step).
+void probe_begin(ibpm_solver* s) {
+  if (2 * s->probe_n + 1 < (int)s->probe_ev.size()) (void)hipEventRecord(s->probe_ev[2 * s->probe_n], s->st);
+}
+void probe_end(ibpm_solver* s) {
+  if (2 * s->probe_n + 1 < (int)s->probe_ev.size()) {
+    (void)hipEventRecord(s->probe_ev[2 * s->probe_n + 1], s->st);
+    ++s->probe_n;
+  }
+}
+
 // One pass over up to kMaxPlanes planes.
 int run_pass(ibpm_solver* s, PassBatch& pb, int np, bool xdir, int epi, bool in_f32,
              bool out_f32) {
@@ -131,10 +143,13 @@ int run_pass(ibpm_solver* s, PassBatch& pb, int np, bool xdir, int epi, bool in_
     pb.p[i].dscale = s->scale;
   }
   const int H = (Nk + 1) / 2;
+  const bool probed = s->probe == 0 && np == s->L;
+  if (probed) probe_begin(s);
   if (s->precision == IBPM_MIXED)
     HIPCHK(launch_dst_pass_mixed(pb, np, Nb, H, epi, in_f32, out_f32, s->st, s->dst_tiles));
   else
     HIPCHK(launch_dst_pass_f64(pb, np, Nb, H, epi, s->st, s->dst_tiles));
+  if (probed) probe_end(s);
   return IBPM_OK;
 }
 
@@ -1344,7 +1359,9 @@ int step_head(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step
   na.dt = s->dt;
   na.beta1 = 1.5;
   na.beta2 = -0.5;
+  if (s->probe == 1) probe_begin(s);
   HIPCHK(launch_nonlin_rhs(na, s->st));
+  if (s->probe == 1) probe_end(s);
   std::vector<SolveJob> jobs(L);
   for (int l = 0; l < L; ++l) {
     SolveJob& j = jobs[l];
@@ -1512,18 +1529,6 @@ void base_flux_dyn(const ibpm_solver* s, const double fs8[8], double out[14]) {
   for (int k = 0; k < 14; ++k) out[k] = v[k];
 }
 }  // namespace ibpm
-
-// Reads n doubles (grid-stride) so their lines replace whatever the caches held, without
-// leaving dirty lines behind; the sum is stored only in the impossible case that it is
-// exactly `never`, which keeps the loads.
-__global__ void flush_read_kernel(const double* __restrict__ p, size_t n, double never,
-                                  double* __restrict__ sink) {
-  double s = 0.0;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    s += p[i];
-  if (s == never) sink[0] = s;
-}
 
 // ---- Operator access that allocates nothing (ibpm_apply_noalloc) ----
 //
@@ -2305,104 +2310,71 @@ int ibpm_apply_noalloc(ibpm_solver* s, int op, int level, const double* in, doub
 int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, double* flops,
                      double* bytes) {
   if (!s || reps < 1 || !avg_ms || !flops || !bytes) return fail(IBPM_E_ARG, "bad arguments");
+  if (kernel < 0 || kernel > 2) return fail(IBPM_E_ARG, "unknown kernel");
   HIPCHK(hipSetDevice(s->device));
   const size_t P = s->plane();
-  const int L = std::min(s->L, kMaxPlanes);
   const bool mixed = s->precision == IBPM_MIXED;
-  NonlinArgs na;
-  PassBatch pb;
-  double probe_flops = 0.0;
-  const int probe_blocks = 8 * 256;  // needs probe_blocks x 256 doubles of scratch
-  auto launch = [&]() -> int {
-    if (kernel == 2) {
-      if ((size_t)probe_blocks * 256 > kMaxPlanes * P) return fail(IBPM_E_ARG, "grid too small for the probe");
-      HIPCHK(launch_mfma_probe(mixed, 1000, probe_blocks, s->T, s->st, &probe_flops));
-      return IBPM_OK;
-    }
-    if (kernel == 0) {
-      memset(&pb, 0, sizeof pb);
-      for (int i = 0; i < L; ++i) {
-        pb.p[i].in = s->rhs + i * P;
-        pb.p[i].out = s->T + i * P;
-      }
-      return run_pass(s, pb, L, true, EPI_STORE, false, mixed);
-    }
-    HIPCHK(launch_nonlin_rhs(na, s->st));
-    return IBPM_OK;
-  };
-  if (kernel == 1) {
-    memset(&na, 0, sizeof na);
-    na.g = s->g;
-    na.nlevel = s->L;
-    na.q = s->q;
-    na.G = s->G;
-    na.nl_new = s->T;  // scratch outputs (kMaxPlanes planes), the state is untouched
-    na.nl_old = s->nl2;
-    na.rhs = s->rhs;
-    na.bf = base_flux(s);
-    for (int l = 0; l < s->L; ++l) {
-      const double hc = s->hc[l];
-      na.inv_hc2[l] = 1.0 / (hc * hc);
-      na.cA[l] = cA(s, l);
-      na.fac_bc[l] = 0.25 * s->dt / (s->Re * (hc * hc));
-    }
-    na.dt = s->dt;
-    na.beta1 = 1.5;
-    na.beta2 = -0.5;
-    if (s->L > kMaxPlanes) return fail(IBPM_E_ARG, "too many levels for the timing scratch");
-  } else if (kernel != 0 && kernel != 2) {
-    return fail(IBPM_E_ARG, "unknown kernel");
-  }
-  int rc = launch();  // warm-up
-  if (rc) return rc;
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  // kernels 0 and 1: each launch timed alone after a 512 MiB read has evicted its inputs
-  // from the L2s and the 256 MB last-level cache (inside a step they come from HBM);
-  // kernel 2: back-to-back launches
-  const size_t flush_n = kernel == 2 ? 0 : ((size_t)512 << 20) / sizeof(double);
-  double* flush = nullptr;
-  hipError_t e = flush_n ? hipMalloc((void**)&flush, flush_n * sizeof(double)) : hipSuccess;
-  float ms = 0.f;
-  if (e == hipSuccess && !flush_n) e = hipEventRecord(e0, s->st);
-  for (int r = 0; r < reps && e == hipSuccess && rc == IBPM_OK; ++r) {
-    if (flush_n) {
-      if (r == 0) e = launch_fill(flush, flush_n, 0.0, s->st);
-      hipLaunchKernelGGL(flush_read_kernel, dim3(4096), dim3(256), 0, s->st, (const double*)flush,
-                         flush_n, -1.0, flush);
-      if (e == hipSuccess) e = hipGetLastError();
-      if (e == hipSuccess) e = hipEventRecord(e0, s->st);
-    }
-    if (e == hipSuccess) rc = launch();
-    if (e == hipSuccess && rc == IBPM_OK && flush_n) {
-      e = hipEventRecord(e1, s->st);
-      if (e == hipSuccess) e = hipEventSynchronize(e1);
-      float one = 0.f;
-      if (e == hipSuccess) e = hipEventElapsedTime(&one, e0, e1);
-      ms += one;
-    }
-  }
-  if (e == hipSuccess && rc == IBPM_OK && !flush_n) {
-    e = hipEventRecord(e1, s->st);
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-  }
-  if (flush) {
-    (void)hipStreamSynchronize(s->st);
-    (void)hipFree(flush);
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (rc) return rc;
-  HIPCHK(e);
-  *avg_ms = ms / reps;
   if (kernel == 2) {
+    // register-only MFMA loop, back-to-back launches
+    const int probe_blocks = 8 * 256;  // needs probe_blocks x 256 doubles of scratch
+    if ((size_t)probe_blocks * 256 > kMaxPlanes * P) return fail(IBPM_E_ARG, "grid too small for the probe");
+    double probe_flops = 0.0;
+    HIPCHK(launch_mfma_probe(mixed, 1000, probe_blocks, s->T, s->st, &probe_flops));  // warm-up
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    hipError_t e = hipEventRecord(e0, s->st);
+    for (int r = 0; r < reps && e == hipSuccess; ++r)
+      e = launch_mfma_probe(mixed, 1000, probe_blocks, s->T, s->st, &probe_flops);
+    if (e == hipSuccess) e = hipEventRecord(e1, s->st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(e);
+    *avg_ms = ms / reps;
     *flops = probe_flops;
     *bytes = 0.0;
-  } else if (kernel == 0) {
-    // matrix-core flops of the contraction the launch runs: the single fold multiplies
-    // H x H and (H-1) x (H-1) tables, the double fold three Q x Q and one (Q-1) x (Q-1)
+    return IBPM_OK;
+  }
+  // kernels 0 and 1: `reps` real steps (static bodies or none; the solver advances) with
+  // events around each launch of the kernel class
+  if (s->moving) return fail(IBPM_E_ARG, "kernel timing runs real steps: static bodies only");
+  HIPCHK(hipStreamSynchronize(s->st));
+  s->probe_ev.assign(64, nullptr);
+  hipError_t e = hipSuccess;
+  for (hipEvent_t& ev : s->probe_ev)
+    if (e == hipSuccess) e = hipEventCreate(&ev);
+  double fs8[8] = {s->fs[0], s->fs[1], s->gm[0], s->gm[1], s->gm[2], s->gm[3], s->gm[4], s->gm[5]};
+  double total = 0.0;
+  long count = 0;
+  int rc = IBPM_OK;
+  s->probe = kernel;
+  for (int r = 0; r < reps && e == hipSuccess && rc == IBPM_OK; ++r) {
+    s->probe_n = 0;
+    rc = step(s, fs8, nullptr, nullptr);
+    if (rc == IBPM_OK) e = hipStreamSynchronize(s->st);
+    for (int k = 0; k < s->probe_n && e == hipSuccess; ++k) {
+      float one = 0.f;
+      e = hipEventElapsedTime(&one, s->probe_ev[2 * k], s->probe_ev[2 * k + 1]);
+      total += one;
+      ++count;
+    }
+  }
+  s->probe = -1;
+  for (hipEvent_t ev : s->probe_ev)
+    if (ev) (void)hipEventDestroy(ev);
+  s->probe_ev.clear();
+  if (rc) return rc;
+  HIPCHK(e);
+  if (!count) return fail(IBPM_E_ARG, "the step launched no kernel of that class");
+  *avg_ms = total / count;
+  if (kernel == 0) {
+    // matrix-core flops of the contraction one launch (all levels) runs: the single fold
+    // multiplies H x H and (H-1) x (H-1) tables, the double fold three Q x Q and one
+    // (Q-1) x (Q-1)
+    const int L = s->L;
     const double Nk = s->g.Nx, Nb = s->g.Ny;
     const int Hi = (s->g.Nx + 1) / 2;
     const double H = Hi, Q = (s->g.Nx + 1) / 4, E = ((s->g.Nx + 1) & 1) ? Hi : Hi - 1;
@@ -2413,7 +2385,7 @@ int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, doubl
     const double per_product = fold == 4 ? 2.0 * Nb * (3.0 * Q * Q + (Q - 1) * (Q - 1))
                                          : 2.0 * Nb * (H * H + E * E);
     *flops = L * per_product * (mixed ? 3.0 : 1.0);
-    *bytes = L * Nk * Nb * (8.0 + (mixed ? 4.0 : 8.0));  // fp64 plane in, T plane out
+    *bytes = L * Nk * Nb * (8.0 + (mixed ? 4.0 : 8.0));  // fp64 plane in, plane out
   } else {
     *flops = 0.0;
     // q (nq) + Gamma + nl_old in, nl_new + rhs out, per level

@@ -99,6 +99,11 @@ struct ibpm_solver {
   bool graph_mode = false;
   double* dyn_d = nullptr;
   int graph_cg_budget = 0;  // CG iterations captured per step (0: cg_maxiter)
+  // instrumentation (ibpm_time_kernel): HIP events around each launch of kernel class
+  // `probe` (0: sine passes over all levels, 1: the nonlinear kernel) inside real steps
+  int probe = -1;
+  std::vector<hipEvent_t> probe_ev;  // pairs
+  int probe_n = 0;
   // CG warm start: extrapolation history F~b of the two steps before the previous one,
   // the extrapolated guess x_e, B x_e, and (rms(r_e), ||r_e||, ||b||)
   int ws_mode = IBPM_WS_PREVIOUS, ws_record = 0, nhist = 0;

@@ -195,15 +195,15 @@ int ibpm_apply(ibpm_solver* s, int op, int level, const double* in, double* out,
 int ibpm_apply_noalloc(ibpm_solver* s, int op, int level, const double* in, double* out,
                        size_t n);
 
-/* Instrumentation, no counterpart in the reference: time `reps` launches of one kernel of
- * the step with HIP events on the solver's stream, on the solver's current fields.
- *   kernel 0: sine-contraction pass of the batched implicit solve (all levels at once)
- *   kernel 1: fused nonlinear term + CNAB right-hand side (all levels)
- *   kernel 2: register-only loop of the MFMA instruction of the solver's precision
- *             (f64 16x16x4 or bf16 16x16x32): the matrix-core rate the device sustains
- * Kernels 0 and 1 are timed one launch at a time, each after a 512 MiB read that evicts
- * their inputs from the caches (as inside a step, where they come from HBM); kernel 2
- * back to back.
+/* Instrumentation, no counterpart in the reference: the mean duration of one kernel of the
+ * step, measured with HIP events on the solver's stream.
+ *   kernel 0: the sine-contraction passes over all levels, inside `reps` real steps
+ *   kernel 1: the fused nonlinear term + CNAB right-hand side, inside `reps` real steps
+ *             (kernels 0 and 1 advance the solver by reps steps at its current freestream;
+ *             static bodies or none)
+ *   kernel 2: `reps` launches of a register-only loop of the MFMA instruction of the
+ *             solver's precision (f64 16x16x4 or bf16 16x16x32): the matrix-core rate the
+ *             device sustains
  * avg_ms: mean launch duration; flops: MFMA flops executed per launch (0 for kernel 1);
  * bytes: minimum HBM bytes per launch (each input read once, each output written once). */
 int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, double* flops,
