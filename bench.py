@@ -546,11 +546,11 @@ def workload(kernel: int, precision: str, reps: int, copy_mib: int = 2048, level
     print(f"copy: {copy_mib} MiB read, {copy_mib} MiB written", flush=True)
 
 
-def pmc_summary(db_path: str, match: str, last: int = 11, copy_match: str = "copyBuffer"):
-    """From a rocprofv3 --pmc database: the mean counter value (KiB) over the last ``last``
-    dispatches of the kernel whose name contains ``match`` (those with the grid of the
-    final dispatch: the timing loop of ``workload``), and the largest value on a dispatch
-    of the reference copy kernel."""
+def pmc_summary(db_path: str, match: str, last: int = 40, copy_match: str = "copyBuffer"):
+    """From a rocprofv3 --pmc database: per kernel whose name contains ``match``, the mean
+    counter value (KiB) over its last ``last`` dispatches with the grid of its final
+    dispatch (the launches inside the timed steps, not the setup's), and the largest value
+    on a dispatch of the reference copy kernel."""
     db = sqlite3.connect(db_path)
     rows = list(db.execute("select dispatch_id, kernel_name, grid_size, counter_name, value "
                            "from counters_collection order by dispatch_id"))
@@ -559,15 +559,16 @@ def pmc_summary(db_path: str, match: str, last: int = 11, copy_match: str = "cop
     for d, name, grid, counter, value in rows:
         per[counter][d] += value
         meta[d] = (name, grid)
-    out = {}
+    out = []
     for counter, disp in per.items():
-        ks = [d for d in sorted(disp) if match in meta[d][0]]
-        if ks:
+        cv = [disp[d] for d in disp if copy_match in meta[d][0]]
+        names = sorted({meta[d][0] for d in disp if match in meta[d][0]})
+        for name in names:
+            ks = [d for d in sorted(disp) if meta[d][0] == name]
             g = meta[ks[-1]][1]
             ks = [d for d in ks if meta[d][1] == g][-last:]
-        cv = [disp[d] for d in disp if copy_match in meta[d][0]]
-        out[counter] = (sum(disp[d] for d in ks) / len(ks) if ks else None, len(ks),
-                        meta[ks[-1]][0] if ks else None, max(cv) if cv else None)
+            out.append((counter, name, sum(disp[d] for d in ks) / len(ks), len(ks),
+                        max(cv) if cv else None))
     return out
 
 
@@ -596,7 +597,7 @@ def summarize(mode, path, match, last, steps):
         print(f"`{name}` grid {g}: last {n} dispatches mean {mean:.2f} us "
               f"(min {lo:.2f}, max {hi:.2f})")
     elif mode == "pmc":
-        for counter, (mean, n, kname, copyv) in pmc_summary(path, match, last).items():
+        for counter, kname, mean, n, copyv in pmc_summary(path, match, last):
             scale = 2048 * 1024.0 / copyv if copyv else None
             print(f"{counter}: `{kname}`: {mean:.1f} KiB per dispatch (mean of {n}); copy of "
                   f"2048 MiB: {copyv:.1f} KiB, scale {scale:.4f}; scaled "
@@ -674,9 +675,9 @@ def main():
     runs = {}
     for prec in paths:
         sv, dt, setup, f, it = time_solver(prob, prec, args.refine, steps, warm)
+        ceil = kernel_roofline(sv, 2, 5, None)  # sustained MFMA rate, register-only loop
         roof = kernel_roofline(sv, 0, args.roofline_reps, PEAK[prec])
         roof["tiles_TB_TM_PF_fold"] = list(sv.dst_tiles(args.levels))
-        ceil = kernel_roofline(sv, 2, 5, None)  # sustained MFMA rate, register-only loop
         roof["mfma_sustained_tflops"] = ceil
         roof["frac_of_sustained"] = round(roof["achieved"] / ceil, 4) if ceil else None
         roof["traffic"] = measured_traffic(prec, roof["tiles_TB_TM_PF_fold"], args.levels)
