@@ -436,7 +436,10 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
   lo = (__bf16)(x - (float)hi);
 }
 
-template <typename InT, typename OutT, int EPI, int TB, int TM>
+// PF: register stages of prefetch (the loads of the next PF chunks in flight while one
+// runs on the matrix cores). Inputs are held in their own type until folded: the FP32 sum
+// or difference of two FP32 values is the same correctly rounded number however formed.
+template <typename InT, typename OutT, int EPI, int TB, int TM, int PF>
 __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch batch) {
   constexpr int BS = TB / 32;  // 16-wide b subtiles per wave
   constexpr int MS = TM / 16;  // 16-wide m subtiles per wave
@@ -480,13 +483,14 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
     for (int j = 0; j < MS; ++j) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
 
   // loader mapping for the input: element pair (2kp, 2kp+1) of row bb, 16 lanes per row
-  double ra[RA][2], rm[RA][2];
+  const InT* xin = (const InT*)d.in;
+  InT ra[PF][RA][2], rm[PF][RA][2];
   // loader mapping for the sine rows: 16-byte piece (row sm, k8) of table st (4 tables)
   const int sm = tid >> 2, sk = (tid & 3) * 8;
-  bf8 rs[4];
+  bf8 rs[PF][4];
   // input loads unconditional (k and b clamped into range), zeroed when staged, so the
   // chunk's loads issue back to back
-  auto fetch = [&](int kc) {
+  auto fetch = [&](const int st, int kc) {
 #pragma unroll
     for (int r = 0; r < RA; ++r) {
       const int e = tid + 256 * r;
@@ -495,23 +499,22 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         const int k1c = min(kc + 2 * kp + j + 1, H);
-        ra[r][j] = load_in<InT>(d, k1c - 1, bc);
-        rm[r][j] = load_in<InT>(d, Nk - k1c, bc);
+        ra[st][r][j] = xin[(size_t)(k1c - 1) + (size_t)bc * Nk];
+        rm[st][r][j] = xin[(size_t)(Nk - k1c) + (size_t)bc * Nk];
       }
     }
     const size_t off = (size_t)(m0 + sm) * Hp + kc + sk;  // m0 + sm < Mp by padding
     if (sm < TM && m0 + sm < Mp) {
-      rs[0] = *(const bf8*)(So + off);
-      rs[1] = *(const bf8*)(So + tab + off);
-      rs[2] = *(const bf8*)(Se + off);
-      rs[3] = *(const bf8*)(Se + tab + off);
+      rs[st][0] = *(const bf8*)(So + off);
+      rs[st][1] = *(const bf8*)(So + tab + off);
+      rs[st][2] = *(const bf8*)(Se + off);
+      rs[st][3] = *(const bf8*)(Se + tab + off);
     } else {
       for (int t = 0; t < 4; ++t)
-        for (int i = 0; i < 8; ++i) rs[t][i] = (__bf16)0.f;
+        for (int i = 0; i < 8; ++i) rs[st][t][i] = (__bf16)0.f;
     }
   };
-  fetch(0);
-  for (int kc = 0; kc < H; kc += MX_KC) {
+  auto stage = [&](const int st, int kc) {
 #pragma unroll
     for (int r = 0; r < RA; ++r) {
       const int e = tid + 256 * r;
@@ -521,8 +524,8 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
       for (int j = 0; j < 2; ++j) {
         const int k1 = kc + 2 * kp + j + 1;
         const bool ok = k1 <= H && b0 + bb < Nb;
-        const double va = ok ? ra[r][j] : 0.0;
-        const double vm = ok && (k1 < H || modd) ? rm[r][j] : 0.0;
+        const InT va = ok ? ra[st][r][j] : (InT)0;
+        const InT vm = ok && (k1 < H || modd) ? rm[st][r][j] : (InT)0;
         const bool mid = !modd && k1 == H;  // even M: k = H has no partner
         split_bf16((float)(va + vm), ph[j], pl[j]);
         split_bf16(mid ? 0.f : (float)(va - vm), qh[j], ql[j]);
@@ -534,13 +537,13 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
       *(bf2*)&Al[1][bb][2 * kp] = (bf2){ql[0], ql[1]};
     }
     if (sm < TM) {
-      *(bf8*)&Bh[0][sm][sk] = rs[0];
-      *(bf8*)&Bl[0][sm][sk] = rs[1];
-      *(bf8*)&Bh[1][sm][sk] = rs[2];
-      *(bf8*)&Bl[1][sm][sk] = rs[3];
+      *(bf8*)&Bh[0][sm][sk] = rs[st][0];
+      *(bf8*)&Bl[0][sm][sk] = rs[st][1];
+      *(bf8*)&Bh[1][sm][sk] = rs[st][2];
+      *(bf8*)&Bl[1][sm][sk] = rs[st][3];
     }
-    __syncthreads();
-    if (kc + MX_KC < H) fetch(kc + MX_KC);
+  };
+  auto multiply = [&]() {
     bf8 ah[BS], al[BS];
 #pragma unroll
     for (int bs = 0; bs < BS; ++bs) {
@@ -559,7 +562,22 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
         acc[bs][ms] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bs], bhv, acc[bs][ms], 0, 0, 0);
       }
     }
-    __syncthreads();
+  };
+#pragma unroll
+  for (int st = 0; st < PF; ++st)
+    if (st * MX_KC < H) fetch(st, st * MX_KC);
+  for (int kc = 0; kc < H; kc += PF * MX_KC) {
+#pragma unroll
+    for (int st = 0; st < PF; ++st) {
+      const int k = kc + st * MX_KC;  // the same for every thread: barriers stay uniform
+      if (k < H) {
+        stage(st, k);
+        __syncthreads();
+        if (k + PF * MX_KC < H) fetch(st, k + PF * MX_KC);
+        multiply();
+        __syncthreads();
+      }
+    }
   }
 
   // the fp32 16x16 accumulator of the bf16 MFMA holds D[row = 4*(lane/16) + r][col = lane%16]
@@ -585,9 +603,9 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
 // ---------------------------------------------------------------------------------
 // Tile variants (fold, TB, TM, prefetch depth); the autotuner times them on a launch
 // shape and keeps the best. FP64: 0..7 single fold, 8..15 double fold (TM counts quarter
-// rows there). The mixed pass has one prefetch stage and uses variants 0..3.
+// rows there). The mixed pass uses variants 0..7 (one fold; 4..7 two prefetch stages).
 constexpr int kVariants = kDstVariants;
-constexpr int kMixedVariants = 4;
+constexpr int kMixedVariants = 8;
 constexpr int kVarTB[kVariants] = {64, 64, 32, 32, 64, 64, 32, 32, 64, 64, 32, 32, 64, 64, 32, 32};
 constexpr int kVarTM[kVariants] = {64, 32, 64, 32, 64, 32, 64, 32, 64, 32, 64, 32, 64, 32, 64, 32};
 constexpr int kVarPF[kVariants] = {1, 1, 1, 1, 2, 2, 2, 2, 1, 1, 1, 1, 2, 2, 2, 2};
@@ -610,11 +628,14 @@ struct F64KC {
     }
   };
 };
-template <typename InT, typename OutT, int EPI, int TB, int TM>
-struct MixK {
-  static void go(dim3 g, hipStream_t st, const PassBatch& b) {
-    hipLaunchKernelGGL((dst_pass_mixed_kernel<InT, OutT, EPI, TB, TM>), g, dim3(256), 0, st, b);
-  }
+template <int PF>
+struct MixKC {
+  template <typename InT, typename OutT, int EPI, int TB, int TM>
+  struct K {
+    static void go(dim3 g, hipStream_t st, const PassBatch& b) {
+      hipLaunchKernelGGL((dst_pass_mixed_kernel<InT, OutT, EPI, TB, TM, PF>), g, dim3(256), 0, st, b);
+    }
+  };
 };
 
 template <int KC, int PF>
@@ -677,12 +698,19 @@ static void launch_f64_variant(int v, const PassBatch& b, int nplanes, int maxNb
   else launch_variant<F64KC<16, 1>::K, double, double>(v, b, nplanes, maxNb, maxH, epi, st);
 }
 
+template <template <typename, typename, int, int, int> class K>
+static void launch_mixed_types(int v, const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
+                               bool in_f32, bool out_f32, hipStream_t st) {
+  if (!in_f32 && out_f32) launch_variant<K, double, float>(v, b, nplanes, maxNb, maxH, epi, st);
+  else if (in_f32 && out_f32) launch_variant<K, float, float>(v, b, nplanes, maxNb, maxH, epi, st);
+  else if (in_f32 && !out_f32) launch_variant<K, float, double>(v, b, nplanes, maxNb, maxH, epi, st);
+  else launch_variant<K, double, double>(v, b, nplanes, maxNb, maxH, epi, st);
+}
+
 static void launch_mixed_variant(int v, const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
                                  bool in_f32, bool out_f32, hipStream_t st) {
-  if (!in_f32 && out_f32) launch_variant<MixK, double, float>(v, b, nplanes, maxNb, maxH, epi, st);
-  else if (in_f32 && out_f32) launch_variant<MixK, float, float>(v, b, nplanes, maxNb, maxH, epi, st);
-  else if (in_f32 && !out_f32) launch_variant<MixK, float, double>(v, b, nplanes, maxNb, maxH, epi, st);
-  else launch_variant<MixK, double, double>(v, b, nplanes, maxNb, maxH, epi, st);
+  if (v >= 4) launch_mixed_types<MixKC<2>::K>(v - 4, b, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
+  else launch_mixed_types<MixKC<1>::K>(v, b, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
 }
 
 // ---- autotuner ----

@@ -97,8 +97,9 @@ typedef struct ibpm_options {
                          v % 4 picks (TB, TM) = (64,64), (64,32), (32,64), (32,32),
                          FP64 loads run 1 (v % 8 < 4) or 2 chunks ahead, v >= 8
                          folds the FP64 sine matrix twice instead of once; mixed uses
-                         v % 4. Variants of one fold give identical results (same
-                         summation order); the two folds agree to rounding.        */
+                         v % 8 (one fold; TB, TM and the prefetch depth as above).
+                         Variants of one fold give identical results (same summation
+                         order); the two folds agree to rounding.                   */
   int cg_precond;     /* moving bodies: 0 (default) the reference's unpreconditioned
                          cg!; 1 preconditions it with the inverse of B assembled at
                          an earlier body configuration, applied in each body's frame

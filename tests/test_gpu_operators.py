@@ -128,7 +128,7 @@ def test_tile_variants_give_identical_results(precision):
     """Every sine-pass tile variant of one fold sums in the same order: identical fields
     after steps. The single and double folds of the FP64 path agree to rounding."""
     prob = cases.cylinder(nx=96, ny=64, nlevel=3, h=0.05, x0=(-1.6, -1.6), Re=100.0)
-    variants = range(16) if precision == "fp64" else range(4)
+    variants = range(16) if precision == "fp64" else range(8)
     ref = {}
     for v in variants:
         s = ib.create_solver(prob, precision=precision, dst_tiles=v)
