@@ -2,7 +2,9 @@
 
 No counterpart in ImmersedBodies.jl, where each CnabSolver advances alone
 (solver.jl:155-190). ``Batch`` wraps ``ibpm_batch_*`` (include/ibpm.h): one step of every
-solver is captured into a hipGraph and each batch step is one replay. Problems may
+solver is captured into hipGraphs; a batch step is one replay, or two around a host check
+of the CG state when bodies move (a CG the captured budget of iterations leaves short of
+its stopping test continues up to its own maxiter in between). Problems may
 differ in grid size, levels, Reynolds number, time step and bodies (static or moving)
 and may have static or moving grids; every solver ends in the state it would reach
 advanced alone. Each step's freestream, grid motion and body motion are evaluated here
@@ -25,9 +27,9 @@ def _dp(a):
 
 class Batch:
     def __init__(self, solvers):
-        """solvers: HipCnabSolver list. Moving-body solvers need box_update="off", CG
-        maxiter <= 64, no warm-start recording, three steps of history for the
-        extrapolated or learned guess and, when preconditioned, one step taken (the
+        """solvers: HipCnabSolver list. Moving-body solvers need box_update="off", no
+        warm-start recording, three steps of history for the extrapolated or learned
+        guess and, when preconditioned, one step taken and no preconditioner refresh (the
         library names what is missing)."""
         self.solvers = list(solvers)
         n = len(self.solvers)

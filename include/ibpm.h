@@ -233,8 +233,9 @@ void ibpm_destroy(ibpm_solver* s);
  * (solver.jl:155-190). A batch advances many independent solvers together: any grid
  * sizes, level counts, body-point counts and Reynolds numbers, static or moving bodies,
  * static or moving grids. One step of every solver is captured into a hipGraph (two, one
- * per parity of the nonlinear-term swap) spread over concurrent streams; each batch step
- * is one graph replay, with no host synchronisation. Every solver ends in the same state,
+ * per parity of the nonlinear-term swap) spread over concurrent streams; without moving
+ * bodies each batch step is one graph replay, with no host synchronisation. Every solver
+ * ends in the same state,
  * bit for bit, as when advanced alone with the same inputs.
  *
  * Each step's freestream, grid motion and body motion are read by the graph from device
