@@ -162,7 +162,7 @@ def measured_traffic(precision, tiles, levels):
     path = os.path.join(ROOT, "profiles", f"dst_pass_{precision}_cylinder1024.md")
     if levels != 5 or not os.path.exists(path):
         return None
-    key = "traffic_bytes_per_launch[{},{},{},{}]:".format(*tiles)  # TB, TM, PF, fold
+    key = "traffic_bytes_per_launch[{},{},{},{},{}]:".format(*tiles)  # TB, TM, PF, fold, KC
     for line in open(path):
         if line.startswith(key):
             return float(line.split(":", 1)[1])
@@ -261,6 +261,21 @@ def warm_start_bench(args):
     ref = forces["previous"]
     err = {m: float(np.max(np.linalg.norm(forces[m] - ref, axis=1) / np.linalg.norm(ref, axis=1)))
            for m in forces if m != "previous"}
+    # the plate as first stated: 200 points (spacing 0.32 h), plain and preconditioned CG
+    stated = {}
+    for key, precond in (("previous", False), ("previous+precond", True)):
+        s = ib.create_solver(cases.pitching_plate_512(k_test, npts=200),
+                             precision=args.ws_precision, refine_sweeps=args.refine, cg=cg,
+                             cg_tol_mode=1, cg_precond=precond)
+        t0 = time.perf_counter()
+        st = s.advance(3)
+        dt = time.perf_counter() - t0
+        stated[key] = {"cg_iters": [int(x.cg_iters) for x in st],
+                       "converged": [bool(x.cg_converged) for x in st],
+                       "cg_residual": [float(x.cg_residual) for x in st],
+                       "s_per_step": round(dt / 3, 3)}
+        s.close()
+        log(f"warm start: 200-point plate {key} {stated[key]}")
     return {
         "workload": "pitching plate, 512x512 cells, 4 levels, Re 200, dt 0.002, 33 points "
                     "(spacing 2h), alpha = 20+10 sin(2 pi f t) deg; train k = "
@@ -278,6 +293,9 @@ def warm_start_bench(args):
         "b_apply_reduction_vs_previous": round(modes["previous"]["b_applies_per_step"] /
                                                modes["learned"]["b_applies_per_step"], 3),
         "force_rel_err_vs_previous_max": err,
+        "stated_200_points": {"workload": "the same plate with 200 points (spacing 0.32 h), "
+                                          "k = 0.6, first 3 steps, CG maxiter 5000",
+                              "modes": stated},
     }
 
 
@@ -677,10 +695,10 @@ def main():
         sv, dt, setup, f, it = time_solver(prob, prec, args.refine, steps, warm)
         ceil = kernel_roofline(sv, 2, 5, None)  # sustained MFMA rate, register-only loop
         roof = kernel_roofline(sv, 0, args.roofline_reps, PEAK[prec])
-        roof["tiles_TB_TM_PF_fold"] = list(sv.dst_tiles(args.levels))
+        roof["tiles_TB_TM_PF_fold_KC"] = list(sv.dst_tiles(args.levels))
         roof["mfma_sustained_tflops"] = ceil
         roof["frac_of_sustained"] = round(roof["achieved"] / ceil, 4) if ceil else None
-        roof["traffic"] = measured_traffic(prec, roof["tiles_TB_TM_PF_fold"], args.levels)
+        roof["traffic"] = measured_traffic(prec, roof["tiles_TB_TM_PF_fold_KC"], args.levels)
         roof_nl = kernel_roofline(sv, 1, args.roofline_reps, None)
         step_roof = step_roofline(sv, 1e3 * dt / steps, roof["flops_per_launch"]) \
             if prec == "fp64" else None

@@ -237,7 +237,9 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
 //   Y_{4m+2} = sum_{i<Q} T2[i][m] P'_{i+1},   P'_k = Q_k + Q_{H-k} (k < Q), P'_Q = Q_Q
 //   Y_{4m+4} = sum_{i<Q-1} T4[i][m] Q'_{i+1}, Q'_k = Q_k - Q_{H-k}
 // with T1[i][m] = 2 sin(pi (2m+1)(2i+1)/M), T2[i][m] = 2 sin(pi (2m+1)(2i+2)/M) (Q x Q)
-// and T4[i][m] = 2 sin(pi 4 (m+1)(i+1)/M) ((Q-1) x (Q-1)), stored [T1 | T2 | T4], i-major.
+// and T4[i][m] = 2 sin(pi 4 (m+1)(i+1)/M) ((Q-1) x (Q-1)), stored [T1 | T2 | T4], i-major,
+// each in a Qp x Qp block (Qp = Q rounded up to 64) whose rows and columns beyond the
+// table are zero.
 // Four Q-long contractions replace two H-long ones: half the matrix-core work of the
 // single fold. Work-group: one block (odd: A and B, even: the two quarter outputs), TB
 // values of b x TM quarter rows, K chunk KC, PF register stages; wave w computes the
@@ -276,9 +278,10 @@ __global__ __launch_bounds__(256, (TB == 32 && PF == 1) ? 3 : 1) void dst_pass_f
   const int wave = tid >> 6;
   const int bh = wave & 1;
   const int mh = wave >> 1;
+  const int Qp = quarter_pitch(Q);
   const double* T1 = (const double*)d.Sq;
-  const double* T2 = T1 + (size_t)Q * Q;
-  const double* T4 = T2 + (size_t)Q * Q;
+  const double* T2 = T1 + (size_t)Qp * Qp;
+  const double* T4 = T2 + (size_t)Qp * Qp;
   const double* SA = grp == 0 ? T1 : T2;
   const double* SB = grp == 0 ? T2 : T4;
   const int nb_lim = grp == 0 ? Q : Q - 1;  // rows and columns of SB
@@ -321,8 +324,8 @@ __global__ __launch_bounds__(256, (TB == 32 && PF == 1) ? 3 : 1) void dst_pass_f
       const int e = tid + 256 * r;
       const int i = kc + e / TM;
       const int m = m0 + e % TM;
-      rsa[st][r] = SA[(size_t)min(i, Q - 1) * Q + min(m, Q - 1)];
-      rsb[st][r] = SB[(size_t)min(i, nb_lim - 1) * nb_lim + min(m, nb_lim - 1)];
+      rsa[st][r] = SA[(size_t)min(i, Q - 1) * Qp + min(m, Q - 1)];
+      rsb[st][r] = SB[(size_t)min(i, nb_lim - 1) * Qp + min(m, nb_lim - 1)];
     }
   };
   const int col = lane & 15;
@@ -395,6 +398,207 @@ __global__ __launch_bounds__(256, (TB == 32 && PF == 1) ? 3 : 1) void dst_pass_f
         __syncthreads();
       }
     }
+  }
+
+  // epilogue: the f64 16x16 accumulator holds D[row = lane/16 + 4r][col = lane%16]
+  const int rq = lane >> 4;
+#pragma unroll
+  for (int bs = 0; bs < BS; ++bs)
+#pragma unroll
+    for (int ms = 0; ms < MS; ++ms) {
+      const int m = m0 + mh * (TM / 2) + ms * 16 + col;
+      if (m >= Q) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int b = b0 + bh * (TB / 2) + bs * 16 + rq + 4 * r;
+        if (b >= Nb) continue;
+        const double u = acc0[bs][ms][r], v = acc1[bs][ms][r];
+        if (grp == 0) {
+          store_out<OutT, EPI>(d, b, 2 * m, u + v);          // row 2m+1
+          store_out<OutT, EPI>(d, b, M - 2 - 2 * m, u - v);  // row M-1-2m
+        } else {
+          store_out<OutT, EPI>(d, b, 4 * m + 1, u);                  // row 4m+2
+          if (m < Q - 1) store_out<OutT, EPI>(d, b, 4 * m + 3, v);  // row 4m+4
+        }
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// FP64 double-fold pass with the sine tables copied straight into LDS. The same
+// contraction, work-group tile and wave layout as dst_pass_f64q_kernel (and the same sums
+// in the same order: bitwise the same result), for launches whose every plane has Q a
+// multiple of KC (no partial K chunk):
+//   - the table tiles (KC rows x TM columns of the Qp-pitched, zero-padded tables) go
+//     from global memory to LDS by global_load_lds_dwordx4 (16 B per lane), no registers,
+//     masks or clamps; a wave instruction fills 1 KB, 128 / TM rows;
+//   - each thread keeps 32-bit byte offsets of its four inputs per slot and steps them by
+//     a uniform amount per chunk (no 64-bit address arithmetic in the loop);
+//   - the folded input tile is XOR-swizzled (column b ^ k) instead of padded.
+// The smaller register and LDS footprint (TB = 32, TM = 64: 96 VGPRs, 24 KB) runs five
+// work-groups per CU: the 1280 tiles of a 5-plane 1023 x 1023 pass all run at once.
+// Loop per chunk: stage the inputs fetched one chunk ahead, wait for the table copy,
+// barrier, fetch the next chunk's inputs, multiply, barrier, start the next table copy.
+// ---------------------------------------------------------------------------------
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(1))) const void global_void;
+
+template <typename InT, typename OutT, int EPI, int TB, int TM, int KC, int MINW>
+__global__ __launch_bounds__(256, MINW) void dst_pass_f64d_kernel(const PassBatch batch) {
+  constexpr int BS = TB / 32, MS = TM / 32, RA = KC * TB / 256;
+  constexpr int RPI = 128 / TM;      // table rows per wave instruction (1 KB)
+  constexpr int NT = KC * TM / 128;  // wave instructions per table and chunk
+  constexpr int NI = (NT + 3) / 4;   // per wave (waves past NT / NI issue none)
+  static_assert(BS >= 1 && MS >= 1 && RA >= 1 && NT >= 1 && KC <= 16, "tile");
+  int z, bt, mt2;
+  if (!xcd_tile(batch, z, bt, mt2)) return;
+  const int grp = mt2 & 1;  // 0: odd rows, 1: even rows
+  const int mt = mt2 >> 1;
+  const PassPlane& d = batch.p[z];
+  const int Nk = d.Nk, Nb = d.Nb;
+  const int M = Nk + 1, H = M / 2, Q = M / 4, Qp = quarter_pitch(Q);
+  const int b0 = bt * TB;
+  const int m0 = mt * TM;
+  if (b0 >= Nb || m0 >= Q) return;
+
+  __shared__ double X0s[KC][TB];
+  __shared__ double X1s[KC][TB];
+  __shared__ __attribute__((aligned(16))) double Sa_s[KC][TM];
+  __shared__ __attribute__((aligned(16))) double Sb_s[KC][TM];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int bh = wave & 1;
+  const int mh = wave >> 1;
+  const size_t tbytes = (size_t)Qp * Qp * 8;
+  const char* T1 = (const char*)d.Sq;
+  const char* SA = grp == 0 ? T1 : T1 + tbytes;
+  const char* SB = grp == 0 ? T1 + tbytes : T1 + 2 * tbytes;
+  const char* xb = (const char*)d.in;
+
+  // input offsets: grp 0 reads x_{2i+1}, x_{M-2i-1}, x_{2i+2}, x_{M-2i-2}; grp 1 x_{i+1},
+  // x_{M-i-1}, x_{H-i-1}, x_{H+i+1} (x_k at element k-1 of the line); rows past Nb read
+  // the last row and stage zeros
+  unsigned o0[RA], o1[RA], o2[RA], o3[RA];
+  bool rowok[RA];
+#pragma unroll
+  for (int r = 0; r < RA; ++r) {
+    const int e = tid + 256 * r;
+    const int ii = e % KC;
+    const int b = b0 + e / KC;
+    rowok[r] = b < Nb;
+    const unsigned line = (unsigned)min(b, Nb - 1) * (unsigned)Nk;
+    const int k0 = grp == 0 ? 2 * ii : ii;
+    const int k1 = grp == 0 ? M - 2 - 2 * ii : M - 2 - ii;
+    const int k2 = grp == 0 ? 2 * ii + 1 : H - 2 - ii;
+    const int k3 = grp == 0 ? M - 3 - 2 * ii : H + ii;
+    o0[r] = (line + k0) * 8u;
+    o1[r] = (line + k1) * 8u;
+    o2[r] = (line + k2) * 8u;
+    o3[r] = (line + k3) * 8u;
+  }
+  const unsigned s0 = (unsigned)((grp == 0 ? 2 : 1) * KC * 8);
+  const unsigned s2 = (unsigned)((grp == 0 ? 2 : -1) * KC * 8);
+  // table copy: wave instruction j fills rows (wave NI + j) RPI + lane / (TM/2), columns
+  // m0 + 2 (lane % (TM/2)) and the next, of the chunk's KC = NT RPI rows
+  const int rr = lane / (TM / 2), cp = lane % (TM / 2);
+  unsigned ot = (unsigned)((wave * NI * RPI + rr) * Qp + m0 + 2 * cp) * 8u;
+  const unsigned ostep = (unsigned)(RPI * Qp * 8), tstep = (unsigned)(KC * Qp * 8);
+  auto copy_tables = [&]() {
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+      const int w = wave * NI + j;
+      if (NT % 4 == 0 || w < NT) {
+        const int r0 = w * RPI;
+        __builtin_amdgcn_global_load_lds((global_void*)(SA + ot + j * ostep), (lds_void*)&Sa_s[r0][0], 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((global_void*)(SB + ot + j * ostep), (lds_void*)&Sb_s[r0][0], 16, 0, 0);
+      }
+    }
+    ot += tstep;
+  };
+
+  d4 acc0[BS][MS], acc1[BS][MS];
+#pragma unroll
+  for (int i = 0; i < BS; ++i)
+#pragma unroll
+    for (int j = 0; j < MS; ++j) {
+      acc0[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+      acc1[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+    }
+  double r0[RA], r1[RA], r2[RA], r3[RA];
+  auto at = [](const char* base, unsigned off) { return (double)*(const InT*)(base + off); };
+  auto fetch = [&]() {
+#pragma unroll
+    for (int r = 0; r < RA; ++r) {
+      r0[r] = at(xb, o0[r]);
+      r1[r] = at(xb, o1[r]);
+      r2[r] = at(xb, o2[r]);
+      r3[r] = at(xb, o3[r]);
+      o0[r] += s0;
+      o1[r] -= s0;
+      o2[r] += s2;
+      o3[r] -= s2;
+    }
+  };
+  const int col = lane & 15;
+  const int kq = lane >> 4;
+  auto stage = [&](int kc) {
+#pragma unroll
+    for (int r = 0; r < RA; ++r) {
+      const int e = tid + 256 * r;
+      const int ii = e % KC;
+      const int i = kc + ii;
+      const int bl = e / KC;
+      double a, b;
+      if (grp == 0) {
+        const double v3 = 2 * i + 2 < H ? r3[r] : 0.0;  // k = H has no partner
+        a = r0[r] + r1[r];                               // P_{2i+1}
+        b = r2[r] + v3;                                  // P_{2i+2}
+      } else {
+        const double qk = r0[r] - r1[r];  // Q_{i+1}
+        const double qh = r2[r] - r3[r];  // Q_{H-i-1}
+        const bool last = i == Q - 1;     // k = Q is its own partner
+        a = last ? qk : qk + qh;
+        b = last ? 0.0 : qk - qh;
+      }
+      X0s[ii][bl ^ ii] = rowok[r] ? a : 0.0;
+      X1s[ii][bl ^ ii] = rowok[r] ? b : 0.0;
+    }
+  };
+  auto multiply = [&]() {
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ++ks) {
+      const int krow = ks * 4 + kq;
+      double a0[BS], a1[BS];
+#pragma unroll
+      for (int bs = 0; bs < BS; ++bs) {
+        const int c = (bh * (TB / 2) + bs * 16 + col) ^ krow;
+        a0[bs] = X0s[krow][c];
+        a1[bs] = X1s[krow][c];
+      }
+#pragma unroll
+      for (int ms = 0; ms < MS; ++ms) {
+        const double ba = Sa_s[krow][mh * (TM / 2) + ms * 16 + col];
+        const double bb = Sb_s[krow][mh * (TM / 2) + ms * 16 + col];
+#pragma unroll
+        for (int bs = 0; bs < BS; ++bs) {
+          acc0[bs][ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[bs], ba, acc0[bs][ms], 0, 0, 0);
+          acc1[bs][ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[bs], bb, acc1[bs][ms], 0, 0, 0);
+        }
+      }
+    }
+  };
+  fetch();
+  copy_tables();
+  for (int kc = 0; kc < Q; kc += KC) {  // the same for every thread: barriers stay uniform
+    stage(kc);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this chunk's table copy has landed
+    __syncthreads();
+    if (kc + KC < Q) fetch();
+    multiply();
+    __syncthreads();
+    if (kc + KC < Q) copy_tables();
   }
 
   // epilogue: the f64 16x16 accumulator holds D[row = lane/16 + 4r][col = lane%16]
@@ -601,15 +805,22 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
 // ---------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------
-// Tile variants (fold, TB, TM, prefetch depth); the autotuner times them on a launch
-// shape and keeps the best. FP64: 0..7 single fold, 8..15 double fold (TM counts quarter
-// rows there). The mixed pass uses variants 0..7 (one fold; 4..7 two prefetch stages).
+// Tile variants (fold, TB, TM, prefetch depth, K chunk); the autotuner times them on a
+// launch shape and keeps the best. FP64: 0..7 single fold, 8..15 double fold (TM counts
+// quarter rows there), 16..19 double fold with the tables copied straight into LDS
+// (dst_pass_f64d_kernel; where Q is not a multiple of their K chunk they run as variant 10
+// or 11). The mixed pass uses variants 0..7 (one fold; 4..7 two prefetch stages).
 constexpr int kVariants = kDstVariants;
 constexpr int kMixedVariants = 8;
-constexpr int kVarTB[kVariants] = {64, 64, 32, 32, 64, 64, 32, 32, 64, 64, 32, 32, 64, 64, 32, 32};
-constexpr int kVarTM[kVariants] = {64, 32, 64, 32, 64, 32, 64, 32, 64, 32, 64, 32, 64, 32, 64, 32};
-constexpr int kVarPF[kVariants] = {1, 1, 1, 1, 2, 2, 2, 2, 1, 1, 1, 1, 2, 2, 2, 2};
-constexpr int kVarFold[kVariants] = {2, 2, 2, 2, 2, 2, 2, 2, 4, 4, 4, 4, 4, 4, 4, 4};
+constexpr int kVarTB[kVariants] = {64, 64, 32, 32, 64, 64, 32, 32, 64, 64,
+                                   32, 32, 64, 64, 32, 32, 32, 32, 32, 32};
+constexpr int kVarTM[kVariants] = {64, 32, 64, 32, 64, 32, 64, 32, 64, 32,
+                                   64, 32, 64, 32, 64, 32, 64, 64, 32, 32};
+constexpr int kVarPF[kVariants] = {1, 1, 1, 1, 2, 2, 2, 2, 1, 1, 1, 1, 2, 2, 2, 2, 0, 0, 0, 0};
+constexpr int kVarFold[kVariants] = {2, 2, 2, 2, 2, 2, 2, 2, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4};
+constexpr int kVarKC[kVariants] = {16, 16, 16, 16, 16, 16, 16, 16, 16, 16,
+                                   16, 16, 16, 16, 16, 16, 16, 8, 16, 8};
+constexpr int kFirstDirect = 16;
 
 static dim3 tile_grid(PassBatch& b, int nplanes, int maxNb, int maxH, int TB, int TM) {
   b.nbt = (maxNb + TB - 1) / TB;
@@ -645,6 +856,19 @@ struct F64QKC {
     static constexpr bool quarter = true;
     static void go(dim3 g, hipStream_t st, const PassBatch& b) {
       hipLaunchKernelGGL((dst_pass_f64q_kernel<InT, OutT, EPI, TB, TM, KC, PF>), g, dim3(256), 0, st, b);
+    }
+  };
+};
+
+// Direct table copies: five work-groups per CU for 64-wide tiles, six for 32-wide ones.
+template <int KC>
+struct F64DKC {
+  template <typename InT, typename OutT, int EPI, int TB, int TM>
+  struct K {
+    static constexpr bool quarter = true;
+    static void go(dim3 g, hipStream_t st, const PassBatch& b) {
+      hipLaunchKernelGGL((dst_pass_f64d_kernel<InT, OutT, EPI, TB, TM, KC, TM == 64 ? 5 : 6>), g,
+                         dim3(256), 0, st, b);
     }
   };
 };
@@ -692,7 +916,16 @@ static void launch_variant(int v, const PassBatch& b, int nplanes, int maxNb, in
 
 static void launch_f64_variant(int v, const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
                                hipStream_t st) {
-  if (v >= 12) launch_variant<F64QKC<16, 2>::K, double, double>(v - 12, b, nplanes, maxNb, maxH, epi, st);
+  if (v >= kFirstDirect) {
+    const bool tm64 = kVarTM[v] == 64;  // TB is 32 for all of them
+    if (kVarKC[v] == 8) {
+      if (tm64) launch_epi<F64DKC<8>::K, double, double, 32, 64>(b, nplanes, maxNb, maxH, epi, st);
+      else launch_epi<F64DKC<8>::K, double, double, 32, 32>(b, nplanes, maxNb, maxH, epi, st);
+    } else {
+      if (tm64) launch_epi<F64DKC<16>::K, double, double, 32, 64>(b, nplanes, maxNb, maxH, epi, st);
+      else launch_epi<F64DKC<16>::K, double, double, 32, 32>(b, nplanes, maxNb, maxH, epi, st);
+    }
+  } else if (v >= 12) launch_variant<F64QKC<16, 2>::K, double, double>(v - 12, b, nplanes, maxNb, maxH, epi, st);
   else if (v >= 8) launch_variant<F64QKC<16, 1>::K, double, double>(v - 8, b, nplanes, maxNb, maxH, epi, st);
   else if (v >= 4) launch_variant<F64KC<16, 2>::K, double, double>(v - 4, b, nplanes, maxNb, maxH, epi, st);
   else launch_variant<F64KC<16, 1>::K, double, double>(v, b, nplanes, maxNb, maxH, epi, st);
@@ -747,11 +980,28 @@ static bool quarter_ok(const PassBatch& b, int nplanes) {
   return true;
 }
 
-int dst_effective_variant(int v, int N) { return v >= 8 && (N + 1) % 4 ? v - 8 : v; }
+// The direct-copy variants need Q = (N + 1) / 4 a multiple of their K chunk on every
+// plane, and planes small enough for 32-bit byte offsets; elsewhere they run as the
+// double-fold variant of the same tiles with register staging.
+static bool direct_ok(const PassBatch& b, int nplanes, int KC) {
+  for (int i = 0; i < nplanes; ++i) {
+    const int Q = (b.p[i].Nk + 1) / 4;
+    if (Q % KC || (size_t)b.p[i].Nk * b.p[i].Nb * 8 >= ((size_t)1 << 31)) return false;
+  }
+  return true;
+}
+static int direct_fallback(int v) { return v < kFirstDirect + 2 ? 10 : 11; }
+
+int dst_effective_variant(int v, int N) {
+  if (v >= kFirstDirect && ((N + 1) % 4 || ((N + 1) / 4) % kVarKC[v])) v = direct_fallback(v);
+  return v >= 8 && (N + 1) % 4 ? v - 8 : v;
+}
 
 hipError_t launch_dst_pass_f64(const PassBatch& batch, int nplanes, int maxNb, int maxH, int epi,
                                hipStream_t st, int variant) {
   int v = variant >= 0 && variant < kVariants ? variant : tuned_variant(0, nplanes, maxNb, maxH);
+  if (v >= kFirstDirect && !(quarter_ok(batch, nplanes) && direct_ok(batch, nplanes, kVarKC[v])))
+    v = direct_fallback(v);
   if (v >= 8 && !quarter_ok(batch, nplanes)) v -= 8;
   launch_f64_variant(v, batch, nplanes, maxNb, maxH, epi, st);
   return hipGetLastError();
@@ -795,6 +1045,7 @@ hipError_t tune_dst_pass(const PassBatch& batch, int nplanes, int maxNb, int max
   for (int v = 0; v < kVariants; ++v) t[v] = 1e30f;
   for (int round = 0; round < rounds && e == hipSuccess; ++round) {
     for (int v = v0; v < v1 && e == hipSuccess; ++v) {
+      if (v >= kFirstDirect && !direct_ok(batch, nplanes, kVarKC[v])) continue;
       auto run = [&]() {
         if (mixed) launch_mixed_variant(v, batch, nplanes, maxNb, maxH, EPI_STORE, in_f32, out_f32, st);
         else launch_f64_variant(v, batch, nplanes, maxNb, maxH, EPI_STORE, st);
@@ -835,13 +1086,14 @@ int dst_tuned_variant(int mixed, int nplanes, int maxNb, int maxH) {
   return it == g_tune.end() ? -1 : it->second;
 }
 
-int dst_variant_tiles(int v, bool mixed, int* TB, int* TM, int* PF, int* fold) {
+int dst_variant_tiles(int v, bool mixed, int* TB, int* TM, int* PF, int* fold, int* KC) {
   if (v < 0 || v >= kVariants) return 0;
   if (mixed) v %= kMixedVariants;
   *TB = kVarTB[v];
   *TM = kVarTM[v];
   *PF = kVarPF[v];
   if (fold) *fold = kVarFold[v];
+  if (KC) *KC = mixed ? 32 : kVarKC[v];
   return 1;
 }
 
@@ -921,22 +1173,23 @@ void build_sine_tables_f64(int N, double* So, double* Se) {
 }
 
 size_t sine_table_quarter_count(int N) {
-  const size_t Q = (size_t)(N + 1) / 4;
-  return 2 * Q * Q + (Q - 1) * (Q - 1);
+  const size_t Qp = quarter_pitch((N + 1) / 4);
+  return 3 * Qp * Qp;
 }
 
 void build_sine_tables_f64q(int N, double* Sq) {
-  const long long M = N + 1, Q = M / 4;
+  const long long M = N + 1, Q = M / 4, Qp = quarter_pitch((int)Q);
+  for (long long k = 0; k < 3 * Qp * Qp; ++k) Sq[k] = 0.0;
   double* T1 = Sq;
-  double* T2 = T1 + Q * Q;
-  double* T4 = T2 + Q * Q;
+  double* T2 = T1 + Qp * Qp;
+  double* T4 = T2 + Qp * Qp;
   for (long long i = 0; i < Q; ++i)
     for (long long m = 0; m < Q; ++m) {
-      T1[i * Q + m] = sine_entry((2 * m + 1) * (2 * i + 1), M);
-      T2[i * Q + m] = sine_entry((2 * m + 1) * (2 * i + 2), M);
+      T1[i * Qp + m] = sine_entry((2 * m + 1) * (2 * i + 1), M);
+      T2[i * Qp + m] = sine_entry((2 * m + 1) * (2 * i + 2), M);
     }
   for (long long i = 0; i < Q - 1; ++i)
-    for (long long m = 0; m < Q - 1; ++m) T4[i * (Q - 1) + m] = sine_entry(4 * (m + 1) * (i + 1), M);
+    for (long long m = 0; m < Q - 1; ++m) T4[i * Qp + m] = sine_entry(4 * (m + 1) * (i + 1), M);
 }
 
 static inline uint16_t bf16_bits_rne(float f) {

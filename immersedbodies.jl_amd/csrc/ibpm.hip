@@ -1630,7 +1630,7 @@ int ibpm_box_update(const ibpm_solver* s, int level, int box[4]) {
 }
 
 int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM, int* PF,
-                   int* fold) {
+                   int* fold, int* KC) {
   if (!s || !TB || !TM || !PF) return fail(IBPM_E_ARG, "null argument");
   int v = s->dst_tiles;
   if (v < 0) {
@@ -1639,9 +1639,10 @@ int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM
     if (v < 0) v = dst_default_variant(s->precision == IBPM_MIXED);
   }
   v = dst_effective_variant(v, xdir ? s->g.Nx : s->g.Ny);
-  int f = 2;
-  dst_variant_tiles(v, s->precision == IBPM_MIXED, TB, TM, PF, &f);
+  int f = 2, kc = 16;
+  dst_variant_tiles(v, s->precision == IBPM_MIXED, TB, TM, PF, &f, &kc);
   if (fold) *fold = f;
+  if (KC) *KC = kc;
   return IBPM_OK;
 }
 

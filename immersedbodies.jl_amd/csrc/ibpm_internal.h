@@ -104,8 +104,9 @@ hipError_t launch_dst_pass_mixed(const PassBatch& b, int nplanes, int maxNb, int
 // passes on the given planes (EPI_STORE); not to be called under graph capture.
 hipError_t tune_dst_pass(const PassBatch& b, int nplanes, int maxNb, int maxH, bool mixed,
                          bool in_f32, bool out_f32, hipStream_t st, int* chosen, float* best_ms);
-int dst_variant_tiles(int v, bool mixed, int* TB, int* TM, int* PF, int* fold);
-constexpr int kDstVariants = 16;
+// (TB, TM, PF, fold, KC) of variant v; PF 0: sine tables copied straight into LDS
+int dst_variant_tiles(int v, bool mixed, int* TB, int* TM, int* PF, int* fold, int* KC = nullptr);
+constexpr int kDstVariants = 20;
 // Register-only MFMA loop (f64 16x16x4 or bf16 16x16x32) on `blocks` x 256 threads; out
 // needs blocks x 256 doubles. flops: matrix flops of the launch.
 hipError_t launch_mfma_probe(bool bf16, int iters, int blocks, double* out, hipStream_t st,
@@ -155,7 +156,9 @@ void build_ring_operator_input(int n, double* B);
 
 // Sine tables for a DST-I of length N (N+1 even). Host builds, returns element counts.
 void build_sine_tables_f64(int N, double* So, double* Se);
-// double-fold tables of the length-N transform (N + 1 a multiple of 4)
+// double-fold tables of the length-N transform (N + 1 a multiple of 4): three Qp x Qp
+// blocks, Qp = Q = (N + 1) / 4 rounded up to a multiple of 64, zero beyond each table
+__host__ __device__ inline int quarter_pitch(int Q) { return (Q + 63) / 64 * 64; }
 size_t sine_table_quarter_count(int N);
 void build_sine_tables_f64q(int N, double* Sq);
 // Split bf16 hi/lo tables, laid out for the MFMA B operand (k contiguous per column).

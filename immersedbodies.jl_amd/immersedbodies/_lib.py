@@ -164,7 +164,7 @@ def load():
     L.ibpm_batch_advance.restype = C.c_int
     L.ibpm_batch_destroy.argtypes = [C.c_void_p]
     L.ibpm_batch_destroy.restype = None
-    L.ibpm_dst_tiles.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _ip, _ip]
+    L.ibpm_dst_tiles.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip]
     L.ibpm_dst_tiles.restype = C.c_int
     L.ibpm_set_body_motion.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp, _dp, C.c_double,
                                        C.c_double]

@@ -46,10 +46,11 @@ class HipCnabSolver(AbstractSolver):
                   still stops on the true residual. cg_precond_refresh > 0 reassembles
                   after a step that needed more iterations than that.
       device: HIP device ordinal
-      dst_tiles: sine-pass tile variant; -1 autotunes at creation, 0..15 fix it:
-                 v % 4 picks (TB, TM) = (64,64), (64,32), (32,64), (32,32), FP64 loads
-                 run 1 (v % 8 < 4) or 2 K chunks ahead, v >= 8 folds the FP64 sine
-                 matrix twice
+      dst_tiles: sine-pass tile variant; -1 autotunes at creation, 0..19 fix it:
+                 for v < 16, v % 4 picks (TB, TM) = (64,64), (64,32), (32,64), (32,32),
+                 FP64 loads run 1 (v % 8 < 4) or 2 K chunks ahead, v >= 8 folds the FP64
+                 sine matrix twice; 16..19 fold twice and copy the sine tables straight
+                 into LDS, (TB, TM, KC) = (32,64,16), (32,64,8), (32,32,16), (32,32,8)
       helmholtz: "auto" solves the implicit Helmholtz systems by red-black Gauss-Seidel
                  in FP64 where 64 half-sweeps bring the error below 2^-56 max|x| (Jacobi
                  contraction 4k/(1+4k) <= 0.54, k = dt / (2 Re h^2)), by the sine
@@ -197,13 +198,14 @@ class HipCnabSolver(AbstractSolver):
         return np.array([s.pc_assembled for s in stats])
 
     def dst_tiles(self, nplanes: int, xdir: bool = True):
-        """(TB, TM, PF, fold): tile shape, prefetch depth and fold (2 or 4) of the sine
-        passes over ``nplanes`` planes (autotuned)."""
-        tb, tm, pf, fold = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        """(TB, TM, PF, fold, KC): tile shape, prefetch depth (0: tables copied straight
+        into LDS), fold (2 or 4) and K chunk of the sine passes over ``nplanes`` planes
+        (autotuned)."""
+        tb, tm, pf, fold, kc = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
         _lib.check(_lib.load().ibpm_dst_tiles(self._h, int(nplanes), int(bool(xdir)),
                                               C.byref(tb), C.byref(tm), C.byref(pf),
-                                              C.byref(fold)))
-        return tb.value, tm.value, pf.value, fold.value
+                                              C.byref(fold), C.byref(kc)))
+        return tb.value, tm.value, pf.value, fold.value, kc.value
 
     def box_update(self, level: int = 1):
         """None when the steps transform Gamma again in the final vort2flux; else the box

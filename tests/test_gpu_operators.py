@@ -123,12 +123,14 @@ def test_vort2flux(precision, refine, tol, shape):
     s.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_tile_variants_give_identical_results(precision):
+@pytest.mark.parametrize("precision,nx", [("fp64", 96), ("fp64", 128), ("mixed", 96)])
+def test_tile_variants_give_identical_results(precision, nx):
     """Every sine-pass tile variant of one fold sums in the same order: identical fields
-    after steps. The single and double folds of the FP64 path agree to rounding."""
-    prob = cases.cylinder(nx=96, ny=64, nlevel=3, h=0.05, x0=(-1.6, -1.6), Re=100.0)
-    variants = range(16) if precision == "fp64" else range(8)
+    after steps. The single and double folds of the FP64 path agree to rounding. With
+    nx = 96 (Q = 24 and 16 along x and y) the direct-copy variants run with K chunk 8 and
+    fall back to register staging for K chunk 16; with nx = 128 (Q = 32, 16) all run."""
+    prob = cases.cylinder(nx=nx, ny=64, nlevel=3, h=0.05, x0=(-1.6, -1.6), Re=100.0)
+    variants = range(20) if precision == "fp64" else range(8)
     ref = {}
     for v in variants:
         s = ib.create_solver(prob, precision=precision, dst_tiles=v)
@@ -146,8 +148,14 @@ def test_tile_variants_give_identical_results(precision):
     s.advance(4)
     got = np.concatenate([s.read("Gamma"), s.read("psi"), s.read("Ftilde")])
     tiles = [s.dst_tiles(1), s.dst_tiles(3), s.dst_tiles(3, False)]
-    print("autotuned tiles (TB, TM, PF, fold)", tiles)
+    print("autotuned tiles (TB, TM, PF, fold, KC)", tiles)
     s.close()
     fold = 4 if precision == "fp64" else 2
     assert all(t[3] == fold for t in tiles)
     assert np.array_equal(got, ref[fold])
+    if precision == "fp64":
+        # the direct-copy variant with K chunk 16 runs along x only where Q = 32
+        s = ib.create_solver(prob, precision=precision, dst_tiles=16)
+        assert s.dst_tiles(3) == ((32, 64, 0, 4, 16) if nx == 128 else (32, 64, 1, 4, 16))
+        assert s.dst_tiles(3, False) == (32, 64, 0, 4, 16)
+        s.close()

@@ -206,3 +206,21 @@ def test_learned_warm_start_gain_plate_512():
     for mode in ("extrapolate", "learned"):
         d = np.abs(forces[mode] - forces["previous"]).max() / np.abs(forces["previous"]).max()
         assert d < 1e-6, (mode, d)
+
+
+def test_stated_200_point_plate():
+    """The plate as first stated, 200 points on the 512^2 grid (spacing 0.32 h): B is
+    numerically singular there, and plain CG from the reference's guess either fails its
+    1e-8 ||b||-relative test within 3000 iterations or needs far more iterations than the
+    33-point plate at the reference's 2h spacing."""
+    cg = dict(maxiter=3000, reltol=1e-8)
+    res = {}
+    for npts in (33, 200):
+        s = ib.create_solver(cases.pitching_plate_512(k=0.6, npts=npts), cg=cg, cg_tol_mode=1)
+        st = s.advance(2)
+        res[npts] = [(x.cg_iters, bool(x.cg_converged), x.cg_residual) for x in st]
+        s.close()
+    print("CG per step (iterations, converged, residual):", res, flush=True)
+    assert all(c for _, c, _ in res[33])
+    it33 = max(i for i, _, _ in res[33])
+    assert all((not c) or i > 10 * it33 for i, c, _ in res[200]), res
