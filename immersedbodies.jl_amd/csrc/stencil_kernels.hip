@@ -1,7 +1,8 @@
 // Grid-stencil kernels of the IBPM step: the fused nonlinear term + CNAB right-hand side,
 // coarsify!, curl! and the residual used by the mixed-precision refinement.
-// All are HBM-bound sweeps; one thread owns one output value and reads its neighbours
-// through L2 (neighbouring threads share every load).
+// All are HBM-bound sweeps. In the nonlinear kernel a lane takes its neighbours' columns
+// from the lanes beside it; in the others one thread owns one output value and reads its
+// neighbours through L2 (neighbouring threads share every load).
 #include <hip/hip_runtime.h>
 
 #include "ibpm_device.h"
@@ -28,15 +29,35 @@ namespace ibpm {
 // Dirichlet values, the operator the DST diagonalises) and rhsbc = apply_bc! of the
 // unscaled ring with fac = 0.25 dt / (Re hc^2).
 // ---------------------------------------------------------------------------------
-constexpr int NL_RJ = 4;  // rows J per thread: a sliding window over the column
+// Work layout: a wave owns 62 consecutive columns I and a window of NL_RJ rows; its 64
+// lanes hold columns bx*62 + lane, so every lane loads one value per row of each input
+// (Gamma on the vertices, Qy at v-face column I+1, Qx at u-face column I+1) and takes
+// its neighbours' columns from the lanes beside it (DPP wave shifts); lanes 0 and 63
+// only feed their neighbours. That is about 3 loads per row instead of the 8 a thread
+// reading all its neighbours issues; the values, and their sums, are the same bits.
+constexpr int NL_RJ = 2;   // rows J per wave window
+constexpr int NL_W = 62;   // output columns per wave
+
+// Value of lane L-1 (lane 0 keeps its own) and of lane L+1 (lane 63 keeps its own).
+__device__ __forceinline__ double lane_prev(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false),
+                          __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double lane_next(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false),
+                          __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false));
+}
 
 template <bool DYN>
 __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
   const GridDims g = a.g;
   const int lev = blockIdx.z;
-  const int I = blockIdx.x * 64 + (threadIdx.x & 63) + 1;
+  const int lane = threadIdx.x & 63;
+  const int I = blockIdx.x * NL_W + lane;  // lanes 0 and 63: neighbour columns only
   const int J0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * NL_RJ + 1;
-  if (I > g.Nx || J0 > g.Ny) return;
+  if (J0 > g.Ny) return;  // uniform per wave: the lane shifts below see every lane
   const int nx = g.nx, ny = g.ny;
   const int nrow = min(NL_RJ, g.Ny - J0 + 1);
   const double* q = a.q + (size_t)lev * g.nq;
@@ -45,79 +66,67 @@ __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
   const double* G = a.G + (size_t)lev * g.nG;
   const double* Gc = lev + 1 < a.nlevel ? a.G + (size_t)(lev + 1) * g.nG : nullptr;
 
-#define QX(i, j) qx[((i)-1) + (size_t)((j)-1) * (nx + 1)]
-#define QY(i, j) qy[((i)-1) + (size_t)((j)-1) * nx]
-  // Gamma with the 0.25-scaled coarse ring outside (ge) on columns I-1..I+1, rows
-  // J0-1..J0+NL_RJ; rows past the top edge (j > ny) are never used
-  double ge[3][NL_RJ + 2];
-  // interior values by unconditional loads (index clamped into the plane); the boundary
-  // ring (ext_value) only where the window leaves the interior
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int r = 0; r < NL_RJ + 2; ++r) {
-      const int i = I - 1 + c, j = J0 - 1 + r;
-      const int ic = min(max(i, 1), g.Nx), jc = min(max(j, 1), g.Ny);
-      ge[c][r] = G[(size_t)(ic - 1) + (size_t)(jc - 1) * g.Nx];
-    }
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int r = 0; r < NL_RJ + 2; ++r) {
-      const int i = I - 1 + c, j = J0 - 1 + r;
-      const bool interior = i > 0 && i < nx && j > 0 && j < ny;
-      if (!interior) ge[c][r] = (r <= nrow + 1 && j <= ny) ? ext_value(G, Gc, nx, ny, i, j, 0.25) : 0.0;
-    }
-  // Qy(I+1, j), j = J0 .. J0+NL_RJ+1 (v-faces up to ny+1)
-  double qyv[NL_RJ + 2];
-  double qa[NL_RJ + 2], qb[NL_RJ + 2];
+  // Gamma at column I, rows J0-1 .. J0+NL_RJ, with the 0.25-scaled coarse ring outside
+  // the interior: interior values by unconditional loads (index clamped into the plane),
+  // the ring (ext_value) only where the window leaves the interior; rows past the top
+  // edge (j > ny) and columns past nx are never used
+  const int Ic = min(max(I, 1), g.Nx);
+  double gv[NL_RJ + 2];
 #pragma unroll
   for (int r = 0; r < NL_RJ + 2; ++r) {
-    const int jc = min(J0 + r, ny + 1);  // rows past nrow + 1 are loaded but unused
-    qa[r] = QY(I + 1, jc);
-    qb[r] = QY(I, jc);
+    const int jc = min(max(J0 - 1 + r, 1), g.Ny);
+    gv[r] = G[(size_t)(Ic - 1) + (size_t)(jc - 1) * g.Nx];
   }
-  // Qx(I + c, J + 1) for the window's rows J (u-faces up to ny)
-  double qxr[3][NL_RJ + 1];
 #pragma unroll
-  for (int c = 0; c < 3; ++c)
+  for (int r = 0; r < NL_RJ + 2; ++r) {
+    const int j = J0 - 1 + r;
+    const bool interior = I > 0 && I < nx && j > 0 && j < ny;
+    if (!interior) gv[r] = (r <= nrow + 1 && j <= ny && I <= nx) ? ext_value(G, Gc, nx, ny, I, j, 0.25) : 0.0;
+  }
+  // Qy(I+1, j), j = J0 .. J0+NL_RJ+1 (v-faces up to ny+1); Qy(I, j) is the previous lane's
+  const int Iy = min(I + 1, nx);
+  double qa[NL_RJ + 2];
 #pragma unroll
-    for (int r = 0; r < NL_RJ + 1; ++r) qxr[c][r] = QX(I + c, min(J0 + r, ny));
+  for (int r = 0; r < NL_RJ + 2; ++r) qa[r] = qy[(Iy - 1) + (size_t)(min(J0 + r, ny + 1) - 1) * nx];
+  // Qx(I+1, j), j = J0 .. J0+NL_RJ (u-faces up to ny)
+  const int Ix = min(I + 1, nx + 1);
+  double qxr[NL_RJ + 1];
+#pragma unroll
+  for (int r = 0; r < NL_RJ + 1; ++r) qxr[r] = qx[(Ix - 1) + (size_t)(min(J0 + r, ny) - 1) * (nx + 1)];
+  double qyv[NL_RJ + 2];
 #pragma unroll
   for (int r = 0; r < NL_RJ + 2; ++r) {
     const int j = J0 + r;
+    const double qb = lane_prev(qa[r]);
     qyv[r] = r <= nrow + 1
-                 ? (((qa[r] + qb[r]) + base_flux_v<DYN>(a.bf, lev, I + 1, j)) + base_flux_v<DYN>(a.bf, lev, I, j)) / 2
+                 ? (((qa[r] + qb) + base_flux_v<DYN>(a.bf, lev, I + 1, j)) + base_flux_v<DYN>(a.bf, lev, I, j)) / 2
                  : 0.0;
   }
 #pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int r = 0; r < NL_RJ + 1; ++r) qxr[c][r] = r <= nrow ? qxr[c][r] : 0.0;
-#undef QX
-#undef QY
+  for (int r = 0; r < NL_RJ + 1; ++r) qxr[r] = r <= nrow ? qxr[r] : 0.0;
 
   const double fac = Gc ? a.fac_bc[lev] : 0.0;
+  const bool out_lane = lane > 0 && lane < 63 && I <= g.Nx;
 #pragma unroll
   for (int r = 0; r < NL_RJ; ++r) {
     if (r >= nrow) break;
     const int J = J0 + r;
-    auto Qx = [&](int c) {
-      return (((qxr[c][r + 1] + qxr[c][r]) + base_flux_u<DYN>(a.bf, lev, I + c, J + 1)) + base_flux_u<DYN>(a.bf, lev, I + c, J)) / 2;
-    };
-    const double g00 = ge[1][r + 1];
-    const double gm0 = ge[0][r + 1];
-    const double gp0 = ge[2][r + 1];
-    const double g0m = ge[1][r];
-    const double g0p = ge[1][r + 2];
+    // Qx at u-face column I+1 (this lane), I and I+2 (the lanes beside it)
+    const double qxb = (((qxr[r + 1] + qxr[r]) + base_flux_u<DYN>(a.bf, lev, I + 1, J + 1)) + base_flux_u<DYN>(a.bf, lev, I + 1, J)) / 2;
+    const double qxa = lane_prev(qxb), qxc = lane_next(qxb);
+    const double g00 = gv[r + 1];
+    const double gm0 = lane_prev(g00);
+    const double gp0 = lane_next(g00);
+    const double g0m = gv[r];
+    const double g0p = gv[r + 2];
     const double qy1 = qyv[r], qy2 = qyv[r + 1], qy3 = qyv[r + 2];
-    const double qxa = Qx(0), qxb = Qx(1), qxc = Qx(2);
 
     const double fqx_a = (qy2 * g00 + qy1 * g0m) / 2;   // fqx(I+1, J)
     const double fqx_b = (qy3 * g0p + qy2 * g00) / 2;   // fqx(I+1, J+1)
     const double fqy_a = -(qxc * gp0 + qxb * g00) / 2;  // fqy(I+1, J+1)
     const double fqy_b = -(qxb * g00 + qxa * gm0) / 2;  // fqy(I, J+1)
     const double nl = ((fqx_a - fqx_b) + (fqy_a - fqy_b)) * a.inv_hc2[lev];
+    if (!out_lane) continue;
 
     const size_t idx = (size_t)(I - 1) + (size_t)(J - 1) * g.Nx + (size_t)lev * g.nG;
     a.nl_new[idx] = nl;
@@ -143,7 +152,7 @@ __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
 
 hipError_t launch_nonlin_rhs(const NonlinArgs& a, hipStream_t st) {
   if (g_skip) return no_skip();
-  dim3 grid((a.g.Nx + 63) / 64, (a.g.Ny + 4 * NL_RJ - 1) / (4 * NL_RJ), a.nlevel);
+  dim3 grid((a.g.Nx + NL_W - 1) / NL_W, (a.g.Ny + 4 * NL_RJ - 1) / (4 * NL_RJ), a.nlevel);
   if (a.bf.dyn)
     hipLaunchKernelGGL(nonlin_rhs_kernel<true>, grid, dim3(256), 0, st, a);
   else
