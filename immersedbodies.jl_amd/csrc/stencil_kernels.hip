@@ -220,39 +220,54 @@ hipError_t launch_coarsify(GridDims g, double* coarse, const double* fine, const
 // boundary (zero on the coarsest level, fluid.jl:187-190), with the reference's choice of
 // side at the corners (curl_u / curl_v).
 // ---------------------------------------------------------------------------------
+// Work layout: a wave holds psi columns c = bx*63 + lane (0 and nx: the boundary) for a
+// window of CURL_R face rows; each lane loads its column once per row and writes the
+// u-face and v-face of column c + 1, the v-face's second vertex coming from the next lane;
+// lane 63 only feeds lane 62.
+constexpr int CURL_R = 4;
+
 __global__ __launch_bounds__(256) void curl_all_kernel(GridDims g, int nlevel, double* __restrict__ q,
                                                        const double* __restrict__ psi) {
-  // block row r: u-faces of row j = r + 1 (r < ny, nx + 1 faces) or v-faces of row
-  // j = r - ny + 1 (nx faces); no index division
   const int lev = blockIdx.z;
-  const int r = blockIdx.y;
+  const int lane = threadIdx.x & 63;
   const int nx = g.nx, ny = g.ny;
-  const int i = blockIdx.x * 256 + threadIdx.x + 1;
-  const bool u = r < ny;
-  if (i > (u ? nx + 1 : nx)) return;
+  const int c = blockIdx.x * 63 + lane;
+  const int J0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * CURL_R + 1;
+  if (J0 > ny + 1) return;  // uniform per wave: the lane shifts below see every lane
   const double* p = psi + (size_t)lev * g.nG;
   const double* pc = lev + 1 < nlevel ? psi + (size_t)(lev + 1) * g.nG : nullptr;
   double* ql = q + (size_t)lev * g.nq;
-  // faces whose two vertices are interior read the plane directly (curl_u / curl_v's
-  // interior branch); the others go through the boundary rules
-  if (u) {
-    const int j = r + 1;
-    const bool in = i >= 2 && i <= nx && j >= 2 && j <= ny - 1;
-    const double v = in ? p[(size_t)(i - 2) + (size_t)(j - 1) * g.Nx] - p[(size_t)(i - 2) + (size_t)(j - 2) * g.Nx]
-                        : curl_u(p, pc, nx, ny, i, j);
-    ql[(size_t)(i - 1) + (size_t)(j - 1) * (nx + 1)] = v;
-  } else {
-    const int j = r - ny + 1;
-    const bool in = i >= 2 && i <= nx - 1 && j >= 2 && j <= ny;
-    const double v = in ? p[(size_t)(i - 2) + (size_t)(j - 2) * g.Nx] - p[(size_t)(i - 1) + (size_t)(j - 2) * g.Nx]
-                        : curl_v(p, pc, nx, ny, i, j);
-    ql[(size_t)g.nu + (size_t)(i - 1) + (size_t)(j - 1) * nx] = v;
+  // psi(c, J0-1+r) by unconditional loads (index clamped into the plane); faces with a
+  // boundary vertex go through the boundary rules (curl_u / curl_v) instead
+  const int cc = min(max(c, 1), g.Nx);
+  double pv[CURL_R + 1];
+#pragma unroll
+  for (int r = 0; r < CURL_R + 1; ++r) {
+    const int jc = min(max(J0 - 1 + r, 1), g.Ny);
+    pv[r] = p[(size_t)(cc - 1) + (size_t)(jc - 1) * g.Nx];
+  }
+#pragma unroll
+  for (int r = 0; r < CURL_R; ++r) {
+    const int j = J0 + r;
+    const double pn = lane_next(pv[r]);  // psi(c + 1, j - 1)
+    if (lane == 63) continue;
+    const int i = c + 1;
+    if (j <= ny && i <= nx + 1) {
+      const bool in = i >= 2 && i <= nx && j >= 2 && j <= ny - 1;
+      const double v = in ? pv[r + 1] - pv[r] : curl_u(p, pc, nx, ny, i, j);
+      ql[(size_t)(i - 1) + (size_t)(j - 1) * (nx + 1)] = v;
+    }
+    if (j <= ny + 1 && i <= nx) {
+      const bool in = i >= 2 && i <= nx - 1 && j >= 2 && j <= ny;
+      const double v = in ? pv[r] - pn : curl_v(p, pc, nx, ny, i, j);
+      ql[(size_t)g.nu + (size_t)(i - 1) + (size_t)(j - 1) * nx] = v;
+    }
   }
 }
 
 hipError_t launch_curl_all(GridDims g, int nlevel, double* q, const double* psi, hipStream_t st) {
   if (g_skip) return no_skip();
-  dim3 grid((g.nx + 1 + 255) / 256, 2 * g.ny + 1, nlevel);
+  dim3 grid((g.nx + 1 + 62) / 63, (g.ny + 1 + 4 * CURL_R - 1) / (4 * CURL_R), nlevel);
   hipLaunchKernelGGL(curl_all_kernel, grid, dim3(256), 0, st, g, nlevel, q, psi);
   return hipGetLastError();
 }
