@@ -10,6 +10,10 @@
 // takes B^T as its first operand so that a lane's accumulator entries run along the
 // column of C (16 consecutive rows per store). Each C entry sums its K terms in one fixed
 // order, so results do not depend on scheduling.
+#include <string.h>
+
+#include <algorithm>
+
 #include "ibpm_device.h"
 #include "ibpm_internal.h"
 
@@ -133,34 +137,50 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
     }
 }
 
-// out[m + n ldo] = sum over s of P[s stride + m + n ldp], the slices in order
-__global__ __launch_bounds__(256) void sum_slices_kernel(const double* __restrict__ P, size_t stride,
-                                                         int nslice, int ldp, int M, int N,
-                                                         double* __restrict__ out, int ldo,
-                                                         const double* skip) {
-  if (skipped(skip)) return;
+// out[m + n ldo] = sum over s of P[s stride + m + n ldp], the slices in order; one job
+// per blockIdx.z
+__global__ __launch_bounds__(256) void sum_slices_kernel(const SliceSumBatch sb) {
+  if (skipped(sb.skip)) return;
+  const SliceSumJob& jb = sb.j[blockIdx.z];
   const int m = blockIdx.x * 64 + (threadIdx.x & 63);
   const int n = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (m >= M || n >= N) return;
-  const double* p = P + (size_t)m + (size_t)n * ldp;
+  if (m >= jb.M || n >= jb.N) return;
+  const double* p = jb.P + (size_t)m + (size_t)n * jb.ldp;
+  const int nslice = jb.nslice;
   double v[16];
 #pragma unroll
-  for (int s = 0; s < 16; ++s) v[s] = s < nslice ? p[(size_t)s * stride] : 0.0;
+  for (int s = 0; s < 16; ++s) v[s] = s < nslice ? p[(size_t)s * jb.stride] : 0.0;
   double sum = v[0];
 #pragma unroll
   for (int s = 1; s < 16; ++s)
     if (s < nslice) sum += v[s];
-  out[(size_t)m + (size_t)n * ldo] = sum;
+  jb.out[(size_t)m + (size_t)n * jb.ldo] = sum;
 }
 
 }  // namespace
 
+hipError_t launch_sum_slices_batch(const SliceSumBatch& sb0, int njobs, hipStream_t st) {
+  if (njobs <= 0) return hipSuccess;
+  if (njobs > kMaxPlanes) return hipErrorInvalidValue;
+  SliceSumBatch sb = sb0;
+  sb.skip = g_skip;
+  int mx = 0, nx = 0;
+  for (int i = 0; i < njobs; ++i) {
+    const SliceSumJob& j = sb.j[i];
+    if (j.nslice < 1 || j.nslice > 16 || j.M <= 0 || j.N <= 0) return hipErrorInvalidValue;
+    mx = std::max(mx, j.M);
+    nx = std::max(nx, j.N);
+  }
+  hipLaunchKernelGGL(sum_slices_kernel, dim3((mx + 63) / 64, (nx + 3) / 4, njobs), dim3(256), 0, st, sb);
+  return hipGetLastError();
+}
+
 hipError_t launch_sum_slices(const double* P, size_t stride, int nslice, int ldp, int M, int N,
                              double* out, int ldo, hipStream_t st) {
-  if (nslice < 1 || nslice > 16 || M <= 0 || N <= 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(sum_slices_kernel, dim3((M + 63) / 64, (N + 3) / 4), dim3(256), 0, st, P, stride,
-                     nslice, ldp, M, N, out, ldo, g_skip);
-  return hipGetLastError();
+  SliceSumBatch sb;
+  memset(&sb, 0, sizeof sb);
+  sb.j[0] = SliceSumJob{P, stride, nslice, ldp, M, N, out, ldo};
+  return launch_sum_slices_batch(sb, 1, st);
 }
 
 hipError_t launch_gemm_f64(const GemmBatch& gb0, int njobs, hipStream_t st) {

@@ -90,7 +90,7 @@ void free_all(ibpm_solver* s) {
                   s->sc_rowptr, s->sc_ep, s->sc_ew,
                   s->ywin, s->gemv_part, s->pc_P, s->pc_theta0, s->pc_t, s->pc_z,
                   s->Binv, s->Gtmp, s->psitmp, s->Sc, s->cg_r, s->cg_u, s->cg_c, s->cg_scal, s->dyn_d,
-                  s->F1, s->G1, s->Sfx, s->Sfy != s->Sfx ? s->Sfy : nullptr, s->Ftr, s->Tbox, s->Pa, s->Pb, s->T2, s->bbox_d,
+                  s->F1, s->G1, s->Sfx, s->Sfy != s->Sfx ? s->Sfy : nullptr, s->Ftr, s->Tbox, s->Tpart, s->Pa, s->Pb, s->T2, s->bbox_d,
                   s->edge, s->pw, s->fac, s->rvec, s->rpart, s->Mx,
                   s->My != s->Mx ? s->My : nullptr,
                   s->ws_h0, s->ws_h1, s->ws_xe,
@@ -271,6 +271,7 @@ int solve_chunk(ibpm_solver* s, const SolveJob* jobs, int nj) {
 // ---- Helmholtz solves by red-black Gauss-Seidel (gs_kernels.hip) ----
 constexpr int kGsMaxSweeps = 64;
 constexpr int kPsiSlicesA = 8, kPsiSlicesB = 16;  // K slices of the trial psi box products
+constexpr int kBoxSliceK = 128;  // K per slice of a long first box-update product
 
 // Half-sweeps m with rho^(m-1) <= 2^-56 for (da I + k (-L)) x = b, k = dnum / dden, where
 // rho = 4k / (da + 4k) bounds the error contraction per half-sweep in the max norm; 0 when
@@ -560,32 +561,44 @@ int box_update(ibpm_solver* s, const double* G, const double* Gs, double* F, boo
   const size_t P = s->plane();
   const int Nx = s->g.Nx, Ny = s->g.Ny;
   GemmBatch g1, g2;
+  SliceSumBatch sb;
   memset(&g1, 0, sizeof g1);
   memset(&g2, 0, sizeof g2);
-  int nj = 0;
+  memset(&sb, 0, sizeof sb);
+  int n1 = 0, n2 = 0, nsum = 0;
   for (int l = 0; l < s->L; ++l) {
     if (s->bw[l] <= 0 || s->bh[l] <= 0) continue;
     const size_t o = (size_t)s->bi0[l] + (size_t)s->bj0[l] * Nx;
     const double *Gl = G + l * P + o, *Gsl = Gs ? Gs + l * P + o : nullptr;
     double* T = s->Tbox + s->btoff[l];
-    GemmJob& a = g1.j[nj];
-    GemmJob& b = g2.j[nj];
-    ++nj;
-    if (s->bxfirst[l]) {
-      // T (Nx x bh) = S_x[:, I] D (bw x bh);  F += T S_y[J, :]
-      a.A = s->Sfx + (size_t)s->bi0[l] * Nx, a.lda = Nx;
-      a.B = Gl, a.B2 = Gsl, a.ldb = Nx;
-      a.C = T, a.ldc = Nx;
-      a.M = Nx, a.N = s->bh[l], a.K = s->bw[l];
+    const bool xf = s->bxfirst[l] != 0;
+    // T (Nx x bh) = S_x[:, I] D (bw x bh);  F += T S_y[J, :]   (x first), or
+    // T (bw x Ny) = D S_y[J, :];            F += S_x[:, I] T   (y first)
+    const int M1 = xf ? Nx : s->bw[l], N1 = xf ? s->bh[l] : Ny, K1 = xf ? s->bw[l] : s->bh[l];
+    const int ldt = xf ? Nx : s->bw[l];
+    const size_t tsize = (size_t)M1 * N1;
+    const int ns = std::max(1, s->bns[l]);
+    const int kk = ns > 1 ? ((K1 + ns - 1) / ns + 31) / 32 * 32 : K1;
+    int made = 0;
+    for (int k0 = 0; k0 < K1; k0 += kk, ++made) {
+      GemmJob& a = g1.j[n1++];
+      if (xf) {
+        a.A = s->Sfx + (size_t)(s->bi0[l] + k0) * Nx, a.lda = Nx;
+        a.B = Gl + k0, a.B2 = Gsl ? Gsl + k0 : nullptr, a.ldb = Nx;
+      } else {
+        a.A = Gl + (size_t)k0 * Nx, a.A2 = Gsl ? Gsl + (size_t)k0 * Nx : nullptr, a.lda = Nx;
+        a.B = s->Sfy + s->bj0[l] + k0, a.ldb = Ny;
+      }
+      a.C = ns > 1 ? s->Tpart + s->bpoff[l] + (size_t)made * tsize : T, a.ldc = ldt;
+      a.M = M1, a.N = N1, a.K = std::min(kk, K1 - k0);
+    }
+    if (ns > 1) sb.j[nsum++] = SliceSumJob{s->Tpart + s->bpoff[l], tsize, made, ldt, M1, N1, T, ldt};
+    GemmJob& b = g2.j[n2++];
+    if (xf) {
       b.A = T, b.lda = Nx;
       b.B = s->Sfy + s->bj0[l], b.ldb = Ny;
       b.K = s->bh[l];
     } else {
-      // T (bw x Ny) = D S_y[J, :];  F += S_x[:, I] T
-      a.A = Gl, a.A2 = Gsl, a.lda = Nx;
-      a.B = s->Sfy + s->bj0[l], a.ldb = Ny;
-      a.C = T, a.ldc = s->bw[l];
-      a.M = s->bw[l], a.N = Ny, a.K = s->bh[l];
       b.A = s->Sfx + (size_t)s->bi0[l] * Nx, b.lda = Nx;
       b.B = T, b.ldb = s->bw[l];
       b.K = s->bw[l];
@@ -594,8 +607,9 @@ int box_update(ibpm_solver* s, const double* G, const double* Gs, double* F, boo
     b.M = Nx, b.N = Ny;
     b.accumulate = accumulate ? 1 : 0;
   }
-  HIPCHK(launch_gemm_f64(g1, nj, s->st));
-  HIPCHK(launch_gemm_f64(g2, nj, s->st));
+  HIPCHK(launch_gemm_f64(g1, n1, s->st));
+  if (nsum) HIPCHK(launch_sum_slices_batch(sb, nsum, s->st));
+  HIPCHK(launch_gemm_f64(g2, n2, s->st));
   return IBPM_OK;
 }
 
@@ -684,6 +698,30 @@ int compute_boxes(ibpm_solver* s, int ilo, int ihi, int jlo, int jhi) {
     s->btoff[l] = toff;
     toff += s->bxfirst[l] ? (size_t)Nx * s->bh[l] : (size_t)s->bw[l] * Ny;
   }
+  // A first product whose contraction (bw, or bh) exceeds 2 kBoxSliceK runs as K-slices
+  // of about kBoxSliceK in separate work-groups, summed in slice order: a long K in few
+  // tiles (a wide box of moving bodies) is otherwise a chain of sequential chunks.
+  // At most kMaxPlanes slice jobs per launch: the longest levels give up slices first.
+  int ns_total = 0;
+  for (int l = 0; l < L; ++l) {
+    const int K1 = s->bxfirst[l] ? s->bw[l] : s->bh[l];
+    s->bns[l] = K1 > 2 * kBoxSliceK ? std::min(16, (K1 + kBoxSliceK - 1) / kBoxSliceK) : (K1 > 0 ? 1 : 0);
+    ns_total += s->bns[l];
+  }
+  while (ns_total > kMaxPlanes) {
+    int lm = 0;
+    for (int l = 1; l < L; ++l)
+      if (s->bns[l] > s->bns[lm]) lm = l;
+    if (s->bns[lm] <= 1) break;
+    --s->bns[lm];
+    --ns_total;
+  }
+  size_t poff = 0;
+  for (int l = 0; l < L; ++l) {
+    s->bpoff[l] = poff;
+    if (s->bns[l] > 1)
+      poff += (size_t)s->bns[l] * (s->bxfirst[l] ? (size_t)Nx * s->bh[l] : (size_t)s->bw[l] * Ny);
+  }
   // the forward pass pair it replaces: about Nx Ny (Nx + Ny) / 4 multiply-adds per plane
   // with the twice-folded sine matrices; take the box update when it needs at most half
   const double passes = (double)L * Nx * Ny * (double)(Nx + Ny) / 4.0;
@@ -691,6 +729,7 @@ int compute_boxes(ibpm_solver* s, int ilo, int ihi, int jlo, int jhi) {
   const size_t P = s->plane();
   hipError_t e = s->Ftr ? hipSuccess : dalloc(&s->Ftr, (size_t)L * P);
   if (!e) e = ensure_buffer(&s->Tbox, &s->tbox_cap, toff);
+  if (!e && poff) e = ensure_buffer(&s->Tpart, &s->tpart_cap, poff);
   auto sine_matrix = [](int Nn, double** out) -> hipError_t {
     const long long n = Nn + 1;
     std::vector<double> tab(2 * n), S((size_t)Nn * Nn);

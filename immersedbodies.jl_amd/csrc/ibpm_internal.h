@@ -414,6 +414,19 @@ hipError_t launch_gemm_f64(const GemmBatch& gb, int njobs, hipStream_t st);
 // out[m + n ldo] = sum over s < nslice (<= 16) of P[s stride + m + n ldp], in order of s
 hipError_t launch_sum_slices(const double* P, size_t stride, int nslice, int ldp, int M, int N,
                              double* out, int ldo, hipStream_t st);
+// the same for up to kMaxPlanes matrices in one launch
+struct SliceSumJob {
+  const double* P;
+  size_t stride;
+  int nslice, ldp, M, N;
+  double* out;
+  int ldo;
+};
+struct SliceSumBatch {
+  SliceSumJob j[kMaxPlanes];
+  const double* skip;  // launch context flag (filled by the launcher)
+};
+hipError_t launch_sum_slices_batch(const SliceSumBatch& sb, int njobs, hipStream_t st);
 
 // d = a - b elementwise
 hipError_t launch_sub(double* d, const double* a, const double* b, size_t n, hipStream_t st);

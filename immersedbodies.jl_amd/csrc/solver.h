@@ -50,10 +50,14 @@ struct ibpm_solver {
   int box_opt = 0;  // 0: when it pays, 1: off, 2: wherever valid
   bool box_on = false;
   int bi0[16] = {}, bw[16] = {}, bj0[16] = {}, bh[16] = {}, bxfirst[16] = {};
+  // K-slices of a level's first box product (1: one product) and their partials' offsets
+  int bns[16] = {};
+  size_t bpoff[16] = {};
   size_t btoff[16] = {};
   double *Sfx = nullptr, *Sfy = nullptr;  // full sine matrices 2 sin(pi k i / n)
   double *Ftr = nullptr;                   // L spectral planes of the trial vort2flux
   double *Tbox = nullptr;                  // per-level intermediate products
+  double* Tpart = nullptr;                 // K-slice partials of the long ones
   // trial psi on level 1 only where couple_surface's E reads it (a box around the panels
   // clear of the level's edge): S_x[I, :] E_1 S_y[:, J] by K-split products
   bool pbox_on = false;
@@ -63,7 +67,7 @@ struct ibpm_solver {
   // their cells (bbox_d on the device); buffer capacities grow as needed
   bool box_dyn = false;
   double* bbox_d = nullptr;
-  size_t tbox_cap = 0, pa_cap = 0, pb_cap = 0, t2_cap = 0;
+  size_t tbox_cap = 0, pa_cap = 0, pb_cap = 0, t2_cap = 0, tpart_cap = 0;
   int gtmp_box[4] = {-1, -1, -1, -1};  // level-1 box B last wrote into Gtmp (-1: none)
   int gtmp_cols = 0;
   void *BfX = nullptr, *BfY = nullptr;                                  // bf16 tables
