@@ -246,6 +246,23 @@ __global__ __launch_bounds__(256) void dst_pass_f64_kernel(const PassBatch batch
 // TB/2 x TM/2 block (b-half w&1, m-half w>>1) of both of the block's products, so the
 // odd block's butterfly A +- B stays in registers.
 // ---------------------------------------------------------------------------------
+// K-chunk order of the double-fold passes. The fold index i runs in blocks of BLK: the
+// even-row group (grp 1) takes them in order, the odd-row group (grp 0) as 0, nb-1, 1,
+// nb-2, ... Block b of the even group reads the input near elements BLK b, H-BLK b,
+// H+BLK b and M-BLK b of each line (x at i+1, H-i-1, H+i+1, M-i-1), block b of the odd
+// group near 2 BLK b and M-2 BLK b (x at 2i+1, 2i+2 and their mirrors), so blocks b and
+// nb-1-b of the odd group read, over the same two blocks' time, the stretches that blocks
+// 2b and 2b+1 of the even group read: the work-groups of a b-tile walk its lines together
+// and share them in L2, instead of each group fetching the whole lines at its own time. A
+// block is BLK / KC chunks taken in order, so every K chunk size sums in the same order
+// and all FP64 double-fold variants (BLK 16) give the same bits.
+template <int BLK = 16>
+__device__ __forceinline__ int fold_chunk(int grp, int t, int KC, int nb) {
+  const int per = BLK / KC, tb = t / per, h = t - tb * per;
+  const int blk = grp ? tb : ((tb & 1) ? nb - 1 - (tb >> 1) : (tb >> 1));
+  return BLK * blk + KC * h;
+}
+
 // The 32-row tiles with one register stage run at 3 waves per SIMD or more: the epilogues
 // that scale or combine (EPI_DIAG, EPI_SUB, EPI_ADD) would otherwise take enough
 // registers to drop to 2.
@@ -383,17 +400,19 @@ __global__ __launch_bounds__(256, (TB == 32 && PF == 1) ? 3 : 1) void dst_pass_f
       }
     }
   };
+  static_assert(KC == 16, "one chunk per block of fold_chunk");
+  const int nb = (Q + 15) / 16;  // chunks, taken in fold_chunk order
 #pragma unroll
   for (int st = 0; st < PF; ++st)
-    if (st * KC < Q) fetch(st, st * KC);
-  for (int kc = 0; kc < Q; kc += PF * KC) {
+    if (st < nb) fetch(st, fold_chunk(grp, st, KC, nb));
+  for (int t0 = 0; t0 < nb; t0 += PF) {
 #pragma unroll
     for (int st = 0; st < PF; ++st) {
-      const int k = kc + st * KC;  // the same for every thread: barriers stay uniform
-      if (k < Q) {
-        stage(st, k);
+      const int t = t0 + st;  // the same for every thread: barriers stay uniform
+      if (t < nb) {
+        stage(st, fold_chunk(grp, t, KC, nb));
         __syncthreads();
-        if (k + PF * KC < Q) fetch(st, k + PF * KC);
+        if (t + PF < nb) fetch(st, fold_chunk(grp, t + PF, KC, nb));
         multiply();
         __syncthreads();
       }
@@ -498,24 +517,26 @@ __global__ __launch_bounds__(256, MINW) void dst_pass_f64d_kernel(const PassBatc
     o2[r] = (line + k2) * 8u;
     o3[r] = (line + k3) * 8u;
   }
-  const unsigned s0 = (unsigned)((grp == 0 ? 2 : 1) * KC * 8);
-  const unsigned s2 = (unsigned)((grp == 0 ? 2 : -1) * KC * 8);
+  // per chunk starting at fold index kc: x_{2i+1}, x_{2i+2} (grp 0) or x_{i+1} (grp 1)
+  // move up by a = (2 or 1) 8 kc bytes and their mirrors down; x_{H-i-1}, x_{H+i+1}
+  // (grp 1) move down and up by 8 kc
+  const unsigned sa = grp == 0 ? 16u : 8u;
   // table copy: wave instruction j fills rows (wave NI + j) RPI + lane / (TM/2), columns
   // m0 + 2 (lane % (TM/2)) and the next, of the chunk's KC = NT RPI rows
   const int rr = lane / (TM / 2), cp = lane % (TM / 2);
-  unsigned ot = (unsigned)((wave * NI * RPI + rr) * Qp + m0 + 2 * cp) * 8u;
-  const unsigned ostep = (unsigned)(RPI * Qp * 8), tstep = (unsigned)(KC * Qp * 8);
-  auto copy_tables = [&]() {
+  const unsigned ot = (unsigned)((wave * NI * RPI + rr) * Qp + m0 + 2 * cp) * 8u;
+  const unsigned ostep = (unsigned)(RPI * Qp * 8), tstep = (unsigned)(Qp * 8);
+  auto copy_tables = [&](int kc) {
+    const unsigned o = ot + (unsigned)kc * tstep;
 #pragma unroll
     for (int j = 0; j < NI; ++j) {
       const int w = wave * NI + j;
       if (NT % 4 == 0 || w < NT) {
         const int r0 = w * RPI;
-        __builtin_amdgcn_global_load_lds((global_void*)(SA + ot + j * ostep), (lds_void*)&Sa_s[r0][0], 16, 0, 0);
-        __builtin_amdgcn_global_load_lds((global_void*)(SB + ot + j * ostep), (lds_void*)&Sb_s[r0][0], 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((global_void*)(SA + o + j * ostep), (lds_void*)&Sa_s[r0][0], 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((global_void*)(SB + o + j * ostep), (lds_void*)&Sb_s[r0][0], 16, 0, 0);
       }
     }
-    ot += tstep;
   };
 
   d4 acc0[BS][MS], acc1[BS][MS];
@@ -528,17 +549,14 @@ __global__ __launch_bounds__(256, MINW) void dst_pass_f64d_kernel(const PassBatc
     }
   double r0[RA], r1[RA], r2[RA], r3[RA];
   auto at = [](const char* base, unsigned off) { return (double)*(const InT*)(base + off); };
-  auto fetch = [&]() {
+  auto fetch = [&](int kc) {
+    const unsigned a = (unsigned)kc * sa, c = grp == 0 ? a : 0u - (unsigned)kc * 8u;
 #pragma unroll
     for (int r = 0; r < RA; ++r) {
-      r0[r] = at(xb, o0[r]);
-      r1[r] = at(xb, o1[r]);
-      r2[r] = at(xb, o2[r]);
-      r3[r] = at(xb, o3[r]);
-      o0[r] += s0;
-      o1[r] -= s0;
-      o2[r] += s2;
-      o3[r] -= s2;
+      r0[r] = at(xb, o0[r] + a);
+      r1[r] = at(xb, o1[r] - a);
+      r2[r] = at(xb, o2[r] + c);
+      r3[r] = at(xb, o3[r] - c);
     }
   };
   const int col = lane & 15;
@@ -589,16 +607,21 @@ __global__ __launch_bounds__(256, MINW) void dst_pass_f64d_kernel(const PassBatc
       }
     }
   };
-  fetch();
-  copy_tables();
-  for (int kc = 0; kc < Q; kc += KC) {  // the same for every thread: barriers stay uniform
+  // chunks in fold_chunk order (Q a multiple of 16: every chunk is whole)
+  const int nb = Q / 16, nsteps = Q / KC;
+  int kc = fold_chunk(grp, 0, KC, nb);
+  fetch(kc);
+  copy_tables(kc);
+  for (int t = 0; t < nsteps; ++t) {  // the same for every thread: barriers stay uniform
+    const int kn = t + 1 < nsteps ? fold_chunk(grp, t + 1, KC, nb) : 0;
     stage(kc);
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this chunk's table copy has landed
     __syncthreads();
-    if (kc + KC < Q) fetch();
+    if (t + 1 < nsteps) fetch(kn);
     multiply();
     __syncthreads();
-    if (kc + KC < Q) copy_tables();
+    if (t + 1 < nsteps) copy_tables(kn);
+    kc = kn;
   }
 
   // epilogue: the f64 16x16 accumulator holds D[row = lane/16 + 4r][col = lane%16]
@@ -803,15 +826,250 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
 }
 
 // ---------------------------------------------------------------------------------
+// Mixed pass with the second fold (M = N+1 = 4Q): the contraction of dst_pass_f64q_kernel
+// (odd rows Y_{2m+1}, Y_{M-1-2m} = A_m +- B_m from T1 . P_{2i+1} and T2 . P_{2i+2}; even
+// rows Y_{4m+2}, Y_{4m+4} from T2 . P'_{i+1} and T4 . Q'_{i+1}) with the folded inputs and
+// the tables split into bf16 hi + lo and three MFMA products per pair (hi*hi + hi*lo +
+// lo*hi), fp32 sums: half the products of the single fold, and half its LDS reads of
+// table fragments, which bound that pass. Folding runs in the input's type, rounded once
+// to fp32 before the split. Tables: [T1 | T2 | T4], each hi then lo, Qm x Qm bf16 with k
+// contiguous per row m (Qm = Q rounded up to 32; zero beyond each table). Work-group: one
+// block (grp 0 odd rows, grp 1 even rows), TB values of b x TM quarter rows, K chunk 32 in
+// fold_chunk<32> order, the loads of the next PF chunks in flight; wave w computes the
+// (b-half w&1, m-half w>>1) quarter of both of the block's products, so the odd block's
+// butterfly A +- B stays in registers. The 32-row tiles with one stage of fp32 inputs are
+// held to five work-groups per CU: the 1280 tiles of a 5-plane 1023 x 1023 pass then run
+// in one round (four per CU leave a second round a quarter full).
+// ---------------------------------------------------------------------------------
+template <typename InT, typename OutT, int EPI, int TB, int TM, int PF>
+__global__ __launch_bounds__(256, (TB == 32 && PF == 1 && sizeof(InT) == 4) ? 5 : 1) void dst_pass_mixedq_kernel(
+    const PassBatch batch) {
+  constexpr int BS = TB / 32;  // 16-wide b subtiles per wave
+  constexpr int MS = TM / 32;  // 16-wide m subtiles per wave
+  constexpr int RA = TB / 16;  // (b, i-pair) input slots per thread and chunk
+  static_assert(BS >= 1 && MS >= 1 && RA >= 1, "tile");
+  int z, bt, mt2;
+  if (!xcd_tile(batch, z, bt, mt2)) return;
+  const int grp = mt2 & 1;  // 0: odd rows, 1: even rows
+  const int mt = mt2 >> 1;
+  const PassPlane& d = batch.p[z];
+  const int Nk = d.Nk, Nb = d.Nb;
+  const int M = Nk + 1, H = M / 2, Q = M / 4, Qm = (Q + 31) & ~31;
+  const int b0 = bt * TB;
+  const int m0 = mt * TM;
+  if (b0 >= Nb || m0 >= Q) return;
+
+  __shared__ __attribute__((aligned(16))) __bf16 Xh[2][TB][MX_LDK];  // [X0|X1] hi
+  __shared__ __attribute__((aligned(16))) __bf16 Xl[2][TB][MX_LDK];  // [X0|X1] lo
+  __shared__ __attribute__((aligned(16))) __bf16 Th[2][TM][MX_LDK];  // [SA|SB] hi
+  __shared__ __attribute__((aligned(16))) __bf16 Tl[2][TM][MX_LDK];  // [SA|SB] lo
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int bh = wave & 1;
+  const int mh = wave >> 1;
+  const int col = lane & 15;
+  const int kq = lane >> 4;
+
+  const size_t tq = (size_t)Qm * Qm;  // one table half (hi or lo)
+  const __bf16* T1 = (const __bf16*)d.Sqm;
+  const __bf16* SA = grp == 0 ? T1 : T1 + 2 * tq;
+  const __bf16* SB = grp == 0 ? T1 + 2 * tq : T1 + 4 * tq;
+
+  f4 acc0[BS][MS], acc1[BS][MS];
+#pragma unroll
+  for (int i = 0; i < BS; ++i)
+#pragma unroll
+    for (int j = 0; j < MS; ++j) {
+      acc0[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+      acc1[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+    }
+
+  // input quad (line bl, fold indices i0..i0+3 with i0 = kc + 4 (tid % 8)): grp 0 reads the
+  // runs x_{2i0+1..2i0+8} and x_{M-2i0-8..M-2i0-1}; grp 1 x_{i0+1..i0+4}, x_{M-i0-4..M-i0-1},
+  // x_{H-i0-4..H-i0-1} and x_{H+i0+1..H+i0+4}: four contiguous runs, each read as 16-byte
+  // vector loads from 32-bit element offsets (dword-aligned; planes below 2^31 bytes). With
+  // Q a multiple of 4 a quad is wholly inside or outside the range; loads are
+  // unconditional (i0 and b clamped into range), values of out-of-range quads zeroed when
+  // staged.
+  constexpr int RQ = TB / 32;                // quads per thread and chunk
+  constexpr int VE = 16 / (int)sizeof(InT);  // elements per 16-byte load
+  typedef InT vec16 __attribute__((ext_vector_type(VE), aligned(sizeof(InT))));
+  const char* xb = (const char*)d.in;
+  auto run = [&](unsigned e, InT* out, int n) {  // n consecutive elements from element e
+#pragma unroll
+    for (int v = 0; v < n; v += VE) {
+      const vec16 w = *(const vec16*)(xb + (size_t)((e + v) * (unsigned)sizeof(InT)));
+#pragma unroll
+      for (int u = 0; u < VE; ++u) out[v + u] = w[u];
+    }
+  };
+  unsigned line[RQ];
+#pragma unroll
+  for (int r = 0; r < RQ; ++r) line[r] = (unsigned)min(b0 + (tid >> 3) + 32 * r, Nb - 1) * (unsigned)Nk;
+  const int q4 = 4 * (tid & 7);
+  InT rx[PF][RQ][16];
+  // table loader: 16-byte piece (row sm, k8) of SA hi, SA lo, SB hi, SB lo
+  const int sm = tid >> 2, sk = (tid & 3) * 8;
+  const char* SAc = (const char*)SA;
+  const char* SBc = (const char*)SB;
+  const unsigned toff = (unsigned)((m0 + sm) * Qm + sk) * 2u, tqb = (unsigned)tq * 2u;
+  bf8 rs[PF][4];
+  auto fetch = [&](const int st, int kc) {
+    const unsigned i0 = (unsigned)min(kc + q4, Q - 4);
+#pragma unroll
+    for (int r = 0; r < RQ; ++r) {
+      const unsigned l = line[r];
+      InT* o = rx[st][r];
+      if (grp == 0) {
+        run(l + 2 * i0, o, 8);                               // x_{2i0+1..2i0+8}
+        run(l + (unsigned)(M - 9) - 2 * i0, o + 8, 8);       // x_{M-2i0-8..M-2i0-1}
+      } else {
+        run(l + i0, o, 4);                                   // x_{i0+1..i0+4}
+        run(l + (unsigned)(M - 5) - i0, o + 4, 4);           // x_{M-i0-4..M-i0-1}
+        run(l + (unsigned)(H - 5) - i0, o + 8, 4);           // x_{H-i0-4..H-i0-1}
+        run(l + (unsigned)H + i0, o + 12, 4);                // x_{H+i0+1..H+i0+4}
+      }
+    }
+    const unsigned off = toff + (unsigned)kc * 2u;
+    if (sm < TM && m0 + sm < Qm) {
+      rs[st][0] = *(const bf8*)(SAc + off);
+      rs[st][1] = *(const bf8*)(SAc + tqb + off);
+      rs[st][2] = *(const bf8*)(SBc + off);
+      rs[st][3] = *(const bf8*)(SBc + tqb + off);
+    } else {
+      for (int t = 0; t < 4; ++t)
+        for (int i = 0; i < 8; ++i) rs[st][t][i] = (__bf16)0.f;
+    }
+  };
+  auto stage = [&](const int st, int kc) {
+    typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+    const int i0 = kc + q4;
+#pragma unroll
+    for (int r = 0; r < RQ; ++r) {
+      const int bl = (tid >> 3) + 32 * r;
+      const bool ok = i0 < Q && b0 + bl < Nb;
+      const InT* v = rx[st][r];
+      bf4 ah, al, ch, cl;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int i = i0 + j;
+        InT a, c;
+        if (grp == 0) {
+          a = v[2 * j] + v[15 - 2 * j];                              // P_{2i+1}
+          c = v[2 * j + 1] + (2 * i + 2 < H ? v[14 - 2 * j] : (InT)0);  // P_{2i+2} (k = H: no partner)
+        } else {
+          const InT qk = v[j] - v[7 - j];      // Q_{i+1}
+          const InT qh = v[11 - j] - v[12 + j];  // Q_{H-i-1}
+          const bool last = i == Q - 1;        // k = Q is its own partner
+          a = last ? qk : qk + qh;
+          c = last ? (InT)0 : qk - qh;
+        }
+        __bf16 h, lo;
+        split_bf16(ok ? (float)a : 0.f, h, lo);
+        ah[j] = h;
+        al[j] = lo;
+        split_bf16(ok ? (float)c : 0.f, h, lo);
+        ch[j] = h;
+        cl[j] = lo;
+      }
+      *(bf4*)&Xh[0][bl][q4] = ah;
+      *(bf4*)&Xl[0][bl][q4] = al;
+      *(bf4*)&Xh[1][bl][q4] = ch;
+      *(bf4*)&Xl[1][bl][q4] = cl;
+    }
+    if (sm < TM) {
+      *(bf8*)&Th[0][sm][sk] = rs[st][0];
+      *(bf8*)&Tl[0][sm][sk] = rs[st][1];
+      *(bf8*)&Th[1][sm][sk] = rs[st][2];
+      *(bf8*)&Tl[1][sm][sk] = rs[st][3];
+    }
+  };
+  auto multiply = [&]() {
+    bf8 x0h[BS], x0l[BS], x1h[BS], x1l[BS];
+#pragma unroll
+    for (int bs = 0; bs < BS; ++bs) {
+      const int row = bh * (TB / 2) + bs * 16 + col;
+      x0h[bs] = *(const bf8*)&Xh[0][row][kq * 8];
+      x0l[bs] = *(const bf8*)&Xl[0][row][kq * 8];
+      x1h[bs] = *(const bf8*)&Xh[1][row][kq * 8];
+      x1l[bs] = *(const bf8*)&Xl[1][row][kq * 8];
+    }
+#pragma unroll
+    for (int ms = 0; ms < MS; ++ms) {
+      const int row = mh * (TM / 2) + ms * 16 + col;
+      const bf8 sah = *(const bf8*)&Th[0][row][kq * 8];
+      const bf8 sal = *(const bf8*)&Tl[0][row][kq * 8];
+      const bf8 sbh = *(const bf8*)&Th[1][row][kq * 8];
+      const bf8 sbl = *(const bf8*)&Tl[1][row][kq * 8];
+#pragma unroll
+      for (int bs = 0; bs < BS; ++bs) {
+        acc0[bs][ms] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x0l[bs], sah, acc0[bs][ms], 0, 0, 0);
+        acc0[bs][ms] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x0h[bs], sal, acc0[bs][ms], 0, 0, 0);
+        acc0[bs][ms] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x0h[bs], sah, acc0[bs][ms], 0, 0, 0);
+        acc1[bs][ms] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x1l[bs], sbh, acc1[bs][ms], 0, 0, 0);
+        acc1[bs][ms] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x1h[bs], sbl, acc1[bs][ms], 0, 0, 0);
+        acc1[bs][ms] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x1h[bs], sbh, acc1[bs][ms], 0, 0, 0);
+      }
+    }
+  };
+  const int nb = (Q + MX_KC - 1) / MX_KC;  // chunks, taken in fold_chunk order
+#pragma unroll
+  for (int st = 0; st < PF; ++st)
+    if (st < nb) fetch(st, fold_chunk<MX_KC>(grp, st, MX_KC, nb));
+  for (int t0 = 0; t0 < nb; t0 += PF) {
+#pragma unroll
+    for (int st = 0; st < PF; ++st) {
+      const int t = t0 + st;  // the same for every thread: barriers stay uniform
+      if (t < nb) {
+        stage(st, fold_chunk<MX_KC>(grp, t, MX_KC, nb));
+        __syncthreads();
+        if (t + PF < nb) fetch(st, fold_chunk<MX_KC>(grp, t + PF, MX_KC, nb));
+        multiply();
+        __syncthreads();
+      }
+    }
+  }
+
+  // the fp32 16x16 accumulator of the bf16 MFMA holds D[row = 4*(lane/16) + r][col = lane%16]
+  const int rq = lane >> 4;
+#pragma unroll
+  for (int bs = 0; bs < BS; ++bs)
+#pragma unroll
+    for (int ms = 0; ms < MS; ++ms) {
+      const int m = m0 + mh * (TM / 2) + ms * 16 + col;
+      if (m >= Q) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int b = b0 + bh * (TB / 2) + bs * 16 + 4 * rq + r;
+        if (b >= Nb) continue;
+        const double u = acc0[bs][ms][r], v = acc1[bs][ms][r];
+        if (grp == 0) {
+          store_out<OutT, EPI>(d, b, 2 * m, u + v);          // row 2m+1
+          store_out<OutT, EPI>(d, b, M - 2 - 2 * m, u - v);  // row M-1-2m
+        } else {
+          store_out<OutT, EPI>(d, b, 4 * m + 1, u);                  // row 4m+2
+          if (m < Q - 1) store_out<OutT, EPI>(d, b, 4 * m + 3, v);  // row 4m+4
+        }
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------
 // Tile variants (fold, TB, TM, prefetch depth, K chunk); the autotuner times them on a
 // launch shape and keeps the best. FP64: 0..7 single fold, 8..15 double fold (TM counts
 // quarter rows there), 16..19 double fold with the tables copied straight into LDS
-// (dst_pass_f64d_kernel; where Q is not a multiple of their K chunk they run as variant 10
-// or 11). The mixed pass uses variants 0..7 (one fold; 4..7 two prefetch stages).
+// (dst_pass_f64d_kernel; where Q is not a multiple of 16 they run as variant 10 or 11).
+// The mixed pass has its own 20: 0..7 one fold (4..7 two prefetch stages), 8..19 the
+// double fold (dst_pass_mixedq_kernel) with 1, 2 or 3 prefetch stages; v % 4 picks the
+// tiles as for 0..15.
 constexpr int kVariants = kDstVariants;
-constexpr int kMixedVariants = 8;
+constexpr int kMixedVariants = kDstVariants;
+// the single-fold mixed variant of the same tiles (two prefetch stages for 12..19)
+static int mixed_single(int v) { return v < 8 ? v : (v < 12 ? v % 4 : 4 + v % 4); }
 constexpr int kVarTB[kVariants] = {64, 64, 32, 32, 64, 64, 32, 32, 64, 64,
                                    32, 32, 64, 64, 32, 32, 32, 32, 32, 32};
 constexpr int kVarTM[kVariants] = {64, 32, 64, 32, 64, 32, 64, 32, 64, 32,
@@ -845,6 +1103,17 @@ struct MixKC {
   struct K {
     static void go(dim3 g, hipStream_t st, const PassBatch& b) {
       hipLaunchKernelGGL((dst_pass_mixed_kernel<InT, OutT, EPI, TB, TM, PF>), g, dim3(256), 0, st, b);
+    }
+  };
+};
+
+template <int PF>
+struct MixQ {
+  template <typename InT, typename OutT, int EPI, int TB, int TM>
+  struct K {
+    static constexpr bool quarter = true;
+    static void go(dim3 g, hipStream_t st, const PassBatch& b) {
+      hipLaunchKernelGGL((dst_pass_mixedq_kernel<InT, OutT, EPI, TB, TM, PF>), g, dim3(256), 0, st, b);
     }
   };
 };
@@ -942,16 +1211,19 @@ static void launch_mixed_types(int v, const PassBatch& b, int nplanes, int maxNb
 
 static void launch_mixed_variant(int v, const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
                                  bool in_f32, bool out_f32, hipStream_t st) {
-  if (v >= 4) launch_mixed_types<MixKC<2>::K>(v - 4, b, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
+  if (v >= 16) launch_mixed_types<MixQ<3>::K>(v - 16, b, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
+  else if (v >= 12) launch_mixed_types<MixQ<2>::K>(v - 12, b, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
+  else if (v >= 8) launch_mixed_types<MixQ<1>::K>(v - 8, b, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
+  else if (v >= 4) launch_mixed_types<MixKC<2>::K>(v - 4, b, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
   else launch_mixed_types<MixKC<1>::K>(v, b, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
 }
 
 // ---- autotuner ----
 // Process-wide table: launch shape (precision, planes, Nb and H of the largest plane) ->
-// the fastest variant measured on it. The FP64 path is tuned over the double-fold
-// variants only, so every FP64 launch sums in the same order whichever tiles it gets;
-// shapes never tuned use the default variant of their precision.
-int dst_default_variant(bool mixed) { return mixed ? 0 : 10; }
+// the fastest variant measured on it. Both paths are tuned over the double-fold variants
+// where the length allows them (else the single fold), so every launch of a shape sums in
+// the same order whichever tiles it gets; shapes never tuned use double fold (32, 64).
+int dst_default_variant(bool mixed) { (void)mixed; return 10; }
 namespace {
 struct TuneKey {
   int mixed, nplanes, maxNb, maxH;
@@ -972,28 +1244,34 @@ int tuned_variant(int mixed, int nplanes, int maxNb, int maxH) {
 }
 }  // namespace
 
-// The double fold needs M = N + 1 to be a multiple of 4 on every plane of the launch;
-// elsewhere a double-fold variant runs as the single-fold variant of the same tiles.
-static bool quarter_ok(const PassBatch& b, int nplanes) {
-  for (int i = 0; i < nplanes; ++i)
-    if ((b.p[i].Nk + 1) % 4) return false;
+// The double fold needs M = N + 1 to be a multiple of 4 on every plane of the launch (in
+// mixed precision a multiple of 16, whole quads of fold indices, and the split double-fold
+// tables); elsewhere a double-fold variant runs as the single-fold variant of the same
+// tiles.
+static bool quarter_ok(const PassBatch& b, int nplanes, bool mixed = false) {
+  for (int i = 0; i < nplanes; ++i) {
+    if (mixed ? !(mixed_quarter_ok(b.p[i].Nk) && b.p[i].Sqm) : (b.p[i].Nk + 1) % 4 != 0) return false;
+  }
   return true;
 }
 
-// The direct-copy variants need Q = (N + 1) / 4 a multiple of their K chunk on every
-// plane, and planes small enough for 32-bit byte offsets; elsewhere they run as the
-// double-fold variant of the same tiles with register staging.
+// The direct-copy variants need Q = (N + 1) / 4 a multiple of 16 (whole blocks of
+// fold_chunk, whatever their K chunk) on every plane, and planes small enough for 32-bit
+// byte offsets; elsewhere they run as the double-fold variant of the same tiles with
+// register staging.
 static bool direct_ok(const PassBatch& b, int nplanes, int KC) {
+  (void)KC;
   for (int i = 0; i < nplanes; ++i) {
     const int Q = (b.p[i].Nk + 1) / 4;
-    if (Q % KC || (size_t)b.p[i].Nk * b.p[i].Nb * 8 >= ((size_t)1 << 31)) return false;
+    if (Q % 16 || (size_t)b.p[i].Nk * b.p[i].Nb * 8 >= ((size_t)1 << 31)) return false;
   }
   return true;
 }
 static int direct_fallback(int v) { return v < kFirstDirect + 2 ? 10 : 11; }
 
-int dst_effective_variant(int v, int N) {
-  if (v >= kFirstDirect && ((N + 1) % 4 || ((N + 1) / 4) % kVarKC[v])) v = direct_fallback(v);
+int dst_effective_variant(int v, int N, bool mixed) {
+  if (mixed) return v >= 8 && !mixed_quarter_ok(N) ? mixed_single(v) : v;
+  if (v >= kFirstDirect && ((N + 1) % 4 || ((N + 1) / 4) % 16)) v = direct_fallback(v);
   return v >= 8 && (N + 1) % 4 ? v - 8 : v;
 }
 
@@ -1009,8 +1287,8 @@ hipError_t launch_dst_pass_f64(const PassBatch& batch, int nplanes, int maxNb, i
 
 hipError_t launch_dst_pass_mixed(const PassBatch& batch, int nplanes, int maxNb, int maxH, int epi,
                                  bool in_f32, bool out_f32, hipStream_t st, int variant) {
-  const int v = (variant >= 0 && variant < kVariants ? variant : tuned_variant(1, nplanes, maxNb, maxH)) %
-                kMixedVariants;
+  int v = variant >= 0 && variant < kVariants ? variant : tuned_variant(1, nplanes, maxNb, maxH);
+  if (v >= 8 && !quarter_ok(batch, nplanes, true)) v = mixed_single(v);
   launch_mixed_variant(v, batch, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
   return hipGetLastError();
 }
@@ -1038,9 +1316,9 @@ hipError_t tune_dst_pass(const PassBatch& batch, int nplanes, int maxNb, int max
   // three rounds over the variants, best mean of 4 launches per round (interleaving the
   // rounds spreads clock and power-state drift over all variants)
   constexpr int reps = 4, rounds = 3;
-  // FP64: the double-fold variants where the length allows them, else the single fold
-  const bool q = !mixed && quarter_ok(batch, nplanes);
-  const int v0 = mixed || !q ? 0 : 8, v1 = mixed ? kMixedVariants : (q ? kVariants : 8);
+  // the double-fold variants where the length allows them, else the single fold
+  const bool q = quarter_ok(batch, nplanes, mixed);
+  const int v0 = q ? 8 : 0, v1 = q ? (mixed ? kMixedVariants : kVariants) : 8;
   float t[kVariants];
   for (int v = 0; v < kVariants; ++v) t[v] = 1e30f;
   for (int round = 0; round < rounds && e == hipSuccess; ++round) {
@@ -1088,12 +1366,19 @@ int dst_tuned_variant(int mixed, int nplanes, int maxNb, int maxH) {
 
 int dst_variant_tiles(int v, bool mixed, int* TB, int* TM, int* PF, int* fold, int* KC) {
   if (v < 0 || v >= kVariants) return 0;
-  if (mixed) v %= kMixedVariants;
+  if (mixed) {  // (TB, TM) by v % 4 as for 0..15; PF 1, 2 (one fold), 1, 2, 3 (double fold)
+    *TB = kVarTB[v % 4];
+    *TM = kVarTM[v % 4];
+    *PF = v < 8 ? 1 + v / 4 : 1 + (v - 8) / 4;
+    if (fold) *fold = v < 8 ? 2 : 4;
+    if (KC) *KC = MX_KC;
+    return 1;
+  }
   *TB = kVarTB[v];
   *TM = kVarTM[v];
   *PF = kVarPF[v];
   if (fold) *fold = kVarFold[v];
-  if (KC) *KC = mixed ? 32 : kVarKC[v];
+  if (KC) *KC = kVarKC[v];
   return 1;
 }
 
@@ -1212,6 +1497,29 @@ static size_t bf16_tab(int N) {
   const size_t Mp = (H + 31) / 32 * 32;
   return Mp * Hp;
 }
+size_t sine_table_bf16q_count(int N) {
+  const size_t Qm = ((size_t)(N + 1) / 4 + 31) & ~(size_t)31;
+  return 6 * Qm * Qm;
+}
+
+void build_sine_tables_bf16q(int N, uint16_t* T) {
+  const long long M = N + 1, Q = M / 4, Qm = (Q + 31) & ~31LL, tq = Qm * Qm;
+  memset(T, 0, 6 * tq * 2);
+  auto put = [&](int t, long long m, long long i, double v) {
+    const uint16_t hi = bf16_bits_rne((float)v);
+    const uint16_t lo = bf16_bits_rne((float)(v - (double)bf16_to_float(hi)));
+    T[(2 * t) * tq + m * Qm + i] = hi;
+    T[(2 * t + 1) * tq + m * Qm + i] = lo;
+  };
+  for (long long m = 0; m < Q; ++m)
+    for (long long i = 0; i < Q; ++i) {
+      put(0, m, i, sine_entry((2 * m + 1) * (2 * i + 1), M));  // T1
+      put(1, m, i, sine_entry((2 * m + 1) * (2 * i + 2), M));  // T2
+    }
+  for (long long m = 0; m < Q - 1; ++m)
+    for (long long i = 0; i < Q - 1; ++i) put(2, m, i, sine_entry(4 * (m + 1) * (i + 1), M));  // T4
+}
+
 size_t sine_table_bf16_odd_count(int N) { return 2 * bf16_tab(N); }
 size_t sine_table_bf16_even_count(int N) { return 2 * bf16_tab(N); }
 

@@ -84,7 +84,7 @@ namespace {
 
 void free_all(ibpm_solver* s) {
   void* ptrs[] = {s->q, s->G, s->psi, s->nl1, s->nl2, s->Gs, s->rhs, s->T, s->U, s->R,
-                  s->cosx, s->cosy, s->SoX, s->SeX, s->SoY, s->SeY, s->SqX, s->SqY, s->BfX, s->BfY,
+                  s->cosx, s->cosy, s->SoX, s->SeX, s->SoY, s->SeY, s->SqX, s->SqY, s->BfX, s->BfY, s->BqX, s->BqY,
                   s->xb, s->ub, s->xb_ref, s->Ft, s->fb, s->Ftmp, s->total, s->motion,
                   s->body_of, s->moving_d, s->coords, s->perm, s->xstart, s->err, s->w,
                   s->sc_rowptr, s->sc_ep, s->sc_ew,
@@ -138,6 +138,7 @@ int run_pass(ibpm_solver* s, PassBatch& pb, int np, bool xdir, int epi, bool in_
     pb.p[i].So = tab_odd(s, xdir);
     pb.p[i].Se = tab_even(s, xdir);
     pb.p[i].Sq = xdir ? s->SqX : s->SqY;
+    pb.p[i].Sqm = xdir ? s->BqX : s->BqY;
     pb.p[i].cosb = xdir ? s->cosy : s->cosx;
     pb.p[i].cosj = xdir ? s->cosx : s->cosy;
     pb.p[i].dscale = s->scale;
@@ -184,6 +185,7 @@ int autotune_passes(ibpm_solver* s) {
         pb.p[i].So = tab_odd(s, xdir);
         pb.p[i].Se = tab_even(s, xdir);
         pb.p[i].Sq = xdir ? s->SqX : s->SqY;
+        pb.p[i].Sqm = xdir ? s->BqX : s->BqY;
         pb.p[i].cosb = xdir ? s->cosy : s->cosx;
         pb.p[i].cosj = xdir ? s->cosx : s->cosy;
         pb.p[i].dscale = s->scale;
@@ -1677,7 +1679,7 @@ int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM
     v = dst_tuned_variant(s->precision == IBPM_MIXED, nplanes, Nb, (Nk + 1) / 2);
     if (v < 0) v = dst_default_variant(s->precision == IBPM_MIXED);
   }
-  v = dst_effective_variant(v, xdir ? s->g.Nx : s->g.Ny);
+  v = dst_effective_variant(v, xdir ? s->g.Nx : s->g.Ny, s->precision == IBPM_MIXED);
   int f = 2, kc = 16;
   dst_variant_tiles(v, s->precision == IBPM_MIXED, TB, TM, PF, &f, &kc);
   if (fold) *fold = f;
@@ -1779,7 +1781,7 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
   }
   // sine tables
   {
-    auto build = [&](int N, void** So, void** Se, void** Sq, void** Bf) -> hipError_t {
+    auto build = [&](int N, void** So, void** Se, void** Sq, void** Bf, void** Bq) -> hipError_t {
       std::vector<double> o(sine_table_odd_count(N)), e(std::max<size_t>(1, sine_table_even_count(N)));
       build_sine_tables_f64(N, o.data(), e.data());
       std::vector<double> q(sine_table_quarter_count(N));
@@ -1804,11 +1806,19 @@ int ibpm_create(const ibpm_problem* p, const ibpm_options* o, ibpm_solver** out)
         r = dalloc((uint16_t**)Bf, bf.size());
         if (r) return r;
         r = hipMemcpy(*Bf, bf.data(), bf.size() * 2, hipMemcpyHostToDevice);
+        if (r) return r;
+        if (mixed_quarter_ok(N)) {
+          std::vector<uint16_t> bq(sine_table_bf16q_count(N));
+          build_sine_tables_bf16q(N, bq.data());
+          r = dalloc((uint16_t**)Bq, bq.size());
+          if (r) return r;
+          r = hipMemcpy(*Bq, bq.data(), bq.size() * 2, hipMemcpyHostToDevice);
+        }
       }
       return r;
     };
-    CK(build(s->g.Nx, &s->SoX, &s->SeX, &s->SqX, &s->BfX));
-    CK(build(s->g.Ny, &s->SoY, &s->SeY, &s->SqY, &s->BfY));
+    CK(build(s->g.Nx, &s->SoX, &s->SeX, &s->SqX, &s->BfX, &s->BqX));
+    CK(build(s->g.Ny, &s->SoY, &s->SeY, &s->SqY, &s->BfY, &s->BqY));
     // ring chain tables: the first and last rows of S (apply_bc! edges) and the ring-line
     // projection weights, with exact argument reduction
     const int nx = s->g.nx, ny = s->g.ny, Nx = s->g.Nx, Ny = s->g.Ny;
@@ -2421,7 +2431,7 @@ int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, doubl
     int v = s->dst_tiles >= 0 ? s->dst_tiles : dst_tuned_variant(mixed, L, s->g.Ny, (s->g.Nx + 1) / 2);
     if (v < 0) v = dst_default_variant(mixed);
     int TB = 0, TM = 0, PF = 0, fold = 2;
-    dst_variant_tiles(dst_effective_variant(v, s->g.Nx), mixed, &TB, &TM, &PF, &fold);
+    dst_variant_tiles(dst_effective_variant(v, s->g.Nx, mixed), mixed, &TB, &TM, &PF, &fold);
     const double per_product = fold == 4 ? 2.0 * Nb * (3.0 * Q * Q + (Q - 1) * (Q - 1))
                                          : 2.0 * Nb * (H * H + E * E);
     *flops = L * per_product * (mixed ? 3.0 : 1.0);

@@ -66,6 +66,7 @@ struct PassPlane {
   const void* So;      // odd-row sine table of the length-Nk transform
   const void* Se;      // even-row sine table
   const void* Sq;      // FP64 double-fold tables [T1 | T2 | T4]
+  const void* Sqm;     // mixed double-fold tables, split bf16 (null: single fold only)
   double* spec;        // EPI_DIAG, FP64: also store val / (da + lambda dnum / dden) here
                        // (the plane a forward transform of the solve's result would give)
   double dscale, da, dnum, dden;
@@ -90,11 +91,12 @@ struct PassBatch {
 };
 
 // Host launchers (dst_kernels.hip).
-// variant: tile variant 0..15 (FP64): v % 4 picks (TB, TM) = (64,64), (64,32), (32,64),
-// (32,32), loads are issued 1 (v % 8 < 4) or 2 K chunks ahead, and v >= 8 folds the sine
-// matrix twice (four quarter-size contractions) instead of once; the mixed pass uses
-// v % 4. -1 takes the autotuned variant of the launch shape (dst_default_variant when the
-// shape was never tuned).
+// variant: tile variant 0..19: v % 4 picks (TB, TM) = (64,64), (64,32), (32,64), (32,32)
+// for v < 16, loads are issued 1 (v % 8 < 4) or 2 K chunks ahead, and v >= 8 folds the
+// sine matrix twice (four quarter-size contractions) instead of once; 16..19 (FP64) copy
+// the tables straight into LDS. The mixed pass has 0..7 (one fold) and 8..11 (double
+// fold, one prefetch stage). -1 takes the autotuned variant of the launch shape
+// (dst_default_variant when the shape was never tuned).
 hipError_t launch_dst_pass_f64(const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
                                hipStream_t st, int variant = -1);
 hipError_t launch_dst_pass_mixed(const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
@@ -113,11 +115,12 @@ hipError_t launch_mfma_probe(bool bf16, int iters, int blocks, double* out, hipS
                              double* flops);
 // Tuned variant of a launch shape, -1 when never tuned.
 int dst_tuned_variant(int mixed, int nplanes, int maxNb, int maxH);
-// Variant of the shapes never tuned: FP64 double fold (32 x 64), mixed 64 x 64.
+// Variant of the shapes never tuned: double fold (32 x 64) in both precisions.
 int dst_default_variant(bool mixed);
 // The variant a launch along a length-N transform runs: a double-fold variant becomes its
-// single-fold counterpart unless N + 1 is a multiple of 4.
-int dst_effective_variant(int v, int N);
+// single-fold counterpart unless N + 1 is a multiple of 4 (mixed: of 16, where the solver
+// builds the split double-fold tables).
+int dst_effective_variant(int v, int N, bool mixed = false);
 
 // ---- multi-level ring chain (ring_kernels.hip) ----
 constexpr int kMaxChains = 16;
@@ -168,6 +171,13 @@ size_t sine_table_odd_count(int N);
 size_t sine_table_even_count(int N);
 size_t sine_table_bf16_odd_count(int N);
 size_t sine_table_bf16_even_count(int N);
+// Split bf16 double-fold tables of the length-N transform (N + 1 a multiple of 4):
+// [T1 | T2 | T4], each hi then lo, Qm x Qm with k contiguous per output row m (Qm = Q
+// rounded up to 32, zero beyond each table); returns element counts.
+size_t sine_table_bf16q_count(int N);
+// the mixed double fold runs where M = N + 1 is a multiple of 16 (Q a multiple of 4)
+inline bool mixed_quarter_ok(int N) { return (N + 1) % 16 == 0; }
+void build_sine_tables_bf16q(int N, uint16_t* T);
 
 // base_flux! of one step (fluid.jl:22-67): the flux of the freestream seen in the grid's
 // frame. Static grid: uniform (ux, uy) h_l per level. Moving grid: per face,

@@ -43,6 +43,7 @@ struct ibpm_solver {
   double *cosx = nullptr, *cosy = nullptr;  // cos(pi k / n): spectral eigenvalues
   void *SoX = nullptr, *SeX = nullptr, *SoY = nullptr, *SeY = nullptr;  // FP64 tables
   void *SqX = nullptr, *SqY = nullptr;  // FP64 double-fold tables
+  void *BqX = nullptr, *BqY = nullptr;  // split bf16 double-fold tables (mixed, N + 1 = 4Q)
   double *F1 = nullptr, *G1 = nullptr;  // FP64: level-1 spectral planes of the step
   // box update of the final vort2flux (FP64, static bodies): the change of Gamma after the
   // trial vort2flux lies in box [bi0, bi0 + bw) x [bj0, bj0 + bh) of each level (0-based
