@@ -351,12 +351,10 @@ __global__ __launch_bounds__(256) void scatter_rot_kernel(const ScatterArgs a) {
 // then its entries (panel, 4 weights), then the panels' forces, and adds the terms in the
 // panel order of scatter_rot_kernel (a term a panel does not reach has weight 0, which
 // leaves the sum unchanged), so both kernels give the same result.
-__global__ __launch_bounds__(256) void scatter_rot_lists_kernel(const ScatterArgs a) {
-  if (skipped(a.skip)) return;
+__device__ __forceinline__ void scatter_rot_lists(const ScatterArgs& a, int bx, int by, int rhs) {
   const GridDims g = a.g;
-  const int rhs = blockIdx.z;
-  const int I = a.ri0 + blockIdx.x * 64 + (threadIdx.x & 63) + 1;
-  const int J = a.rj0 + blockIdx.y * 4 + (threadIdx.x >> 6) + 1;
+  const int I = a.ri0 + bx * 64 + (threadIdx.x & 63) + 1;
+  const int J = a.rj0 + by * 4 + (threadIdx.x >> 6) + 1;
   if (I > a.ri0 + a.rw || J > a.rj0 + a.rh) return;
   const int n = a.npanel;
   const size_t idx = (size_t)(I - 1) + (size_t)(J - 1) * g.Nx;
@@ -396,6 +394,11 @@ __global__ __launch_bounds__(256) void scatter_rot_lists_kernel(const ScatterArg
     ((float*)a.out[rhs])[idx] = (float)v;
   else
     ((double*)a.out[rhs])[idx] = v;
+}
+
+__global__ __launch_bounds__(256) void scatter_rot_lists_kernel(const ScatterArgs a) {
+  if (skipped(a.skip)) return;
+  scatter_rot_lists(a, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 hipError_t launch_scatter_rot(const ScatterArgs& a0, hipStream_t st) {
@@ -492,26 +495,32 @@ __global__ void gemv_sum_kernel(int m, const double* __restrict__ part, double* 
   x[i] = s;
 }
 
-__global__ __launch_bounds__(1024) void forces_kernel(int n, const double* __restrict__ Ft,
-                                                      double* __restrict__ fb,
-                                                      double* __restrict__ total, double h,
-                                                      double dt, double* __restrict__ host_total) {
+// The force sums of one work-group of NT threads (1024 or 256): NT threads play the 1024
+// slots of the reduction, so the sums are the same bits for either size.
+template <int NT>
+__device__ __forceinline__ void forces_sum(int n, const double* __restrict__ Ft,
+                                           double* __restrict__ fb, double* __restrict__ total,
+                                           double h, double dt, double* __restrict__ host_total) {
   __shared__ double sx[1024], sy[1024];
-  double ax = 0.0, ay = 0.0;
-  for (int p = threadIdx.x; p < n; p += 1024) {
-    const double fx = Ft[p] * h / dt, fy = Ft[n + p] * h / dt;
-    fb[p] = fx;
-    fb[n + p] = fy;
-    ax += fx;
-    ay += fy;
+#pragma unroll
+  for (int k = 0; k < 1024 / NT; ++k) {
+    const int t = threadIdx.x + NT * k;
+    double ax = 0.0, ay = 0.0;
+    for (int p = t; p < n; p += 1024) {
+      const double fx = Ft[p] * h / dt, fy = Ft[n + p] * h / dt;
+      fb[p] = fx;
+      fb[n + p] = fy;
+      ax += fx;
+      ay += fy;
+    }
+    sx[t] = ax;
+    sy[t] = ay;
   }
-  sx[threadIdx.x] = ax;
-  sy[threadIdx.x] = ay;
   __syncthreads();
   for (int s = 512; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      sx[threadIdx.x] += sx[threadIdx.x + s];
-      sy[threadIdx.x] += sy[threadIdx.x + s];
+    for (int t = threadIdx.x; t < s; t += NT) {
+      sx[t] += sx[t + s];
+      sy[t] += sy[t + s];
     }
     __syncthreads();
   }
@@ -523,6 +532,54 @@ __global__ __launch_bounds__(1024) void forces_kernel(int n, const double* __res
       host_total[1] = sy[0];
     }
   }
+}
+
+__global__ __launch_bounds__(1024) void forces_kernel(int n, const double* __restrict__ Ft,
+                                                      double* __restrict__ fb,
+                                                      double* __restrict__ total, double h,
+                                                      double dt, double* __restrict__ host_total) {
+  forces_sum<1024>(n, Ft, fb, total, h, dt, host_total);
+}
+
+// project_circ's first launch for static bodies (three independent jobs by block range):
+// rot(E^T Ft) on the box from the vertex lists, Gamma_1 = Gamma_s1 (the solve then
+// subtracts the correction on the box), and the forces of update_vars!.
+__global__ __launch_bounds__(256) void project_prologue_kernel(const ScatterArgs a, int gx, int nsc,
+                                                               double* __restrict__ G,
+                                                               const double* __restrict__ Gs,
+                                                               size_t np, double* __restrict__ fb,
+                                                               double* __restrict__ total, double h,
+                                                               double dt, double* __restrict__ host_total) {
+  const int b = blockIdx.x;
+  if (b < nsc) {
+    scatter_rot_lists(a, b % gx, b / gx, 0);
+  } else if (b == nsc) {
+    forces_sum<256>(a.npanel, a.f[0], fb, total, h, dt, host_total);
+  } else {  // 1024 values per block, all four loads issued before the stores
+    const size_t i0 = (size_t)(b - nsc - 1) * 1024 + threadIdx.x;
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = Gs[i0 + 256 * k < np ? i0 + 256 * k : np - 1];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (i0 + 256 * k < np) G[i0 + 256 * k] = v[k];
+  }
+}
+
+hipError_t launch_project_prologue(const ScatterArgs& a0, double* G, const double* Gs, size_t np,
+                                   double* fb, double* total, double h, double dt,
+                                   double* host_total, hipStream_t st) {
+  if (g_skip) return no_skip();
+  ScatterArgs a = a0;
+  a.skip = nullptr;
+  if (!a.rowptr || a.nrhs != 1 || a.rw <= 0 || a.rh <= 0 || a.ri0 < 0 || a.rj0 < 0 ||
+      a.ri0 + a.rw > a.g.Nx || a.rj0 + a.rh > a.g.Ny)
+    return hipErrorInvalidValue;
+  const int gx = (a.rw + 63) / 64, nsc = gx * ((a.rh + 3) / 4);
+  const int ncp = (int)((np + 1023) / 1024);
+  hipLaunchKernelGGL(project_prologue_kernel, dim3(nsc + 1 + ncp), dim3(256), 0, st, a, gx, nsc, G, Gs,
+                     np, fb, total, h, dt, host_total);
+  return hipGetLastError();
 }
 
 size_t gemv_partial_count(int m) { return (size_t)GEMV_SPLIT * (size_t)(m > 0 ? m : 1); }

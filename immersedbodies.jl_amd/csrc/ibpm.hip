@@ -1444,7 +1444,6 @@ int step_tail(ibpm_solver* s, ibpm_step_stats* stats) {
   const bool spec1 = keeps_spec1(s);
   int rc = IBPM_OK;
   if (s->n > 0) {
-    HIPCHK(launch_forces(s->n, s->Ft, s->fb, s->total, s->h, s->dt, s->force_slot, s->st));
     ScatterArgs sa;
     memset(&sa, 0, sizeof sa);
     sa.g = s->g;
@@ -1464,11 +1463,16 @@ int step_tail(ibpm_solver* s, ibpm_step_stats* stats) {
     // (the Gauss-Seidel support argument): rot and the solve run on the box only, and
     // Gamma_1 = Gamma_s1 elsewhere
     const bool pbox = s->box_on && !spec1;
-    if (pbox) {
-      sa.ri0 = s->bi0[0], sa.rj0 = s->bj0[0], sa.rw = s->bw[0], sa.rh = s->bh[0];
-      HIPCHK(launch_copy(s->G, s->Gs, P, s->st));
+    if (pbox) sa.ri0 = s->bi0[0], sa.rj0 = s->bj0[0], sa.rw = s->bw[0], sa.rh = s->bh[0];
+    if (pbox && sa.rowptr && sa.rw > 0 && sa.rh > 0) {
+      // the forces, rot(E^T Ft) on the box and Gamma_1 = Gamma_s1 in one launch
+      HIPCHK(launch_project_prologue(sa, s->G, s->Gs, P, s->fb, s->total, s->h, s->dt,
+                                     s->force_slot, s->st));
+    } else {
+      HIPCHK(launch_forces(s->n, s->Ft, s->fb, s->total, s->h, s->dt, s->force_slot, s->st));
+      if (pbox) HIPCHK(launch_copy(s->G, s->Gs, P, s->st));
+      HIPCHK(launch_scatter_rot(sa, s->st));
     }
-    HIPCHK(launch_scatter_rot(sa, s->st));
     SolveJob j;
     j.b = s->rhs;
     j.da = 1.0;

@@ -309,6 +309,12 @@ struct ScatterArgs {
 void build_scatter_lists(int n, int Nx, int Ny, const int* coords, const int* perm, const double* w,
                          std::vector<int>& rowptr, std::vector<int>& ep, std::vector<double>& ew);
 hipError_t launch_scatter_rot(const ScatterArgs& a, hipStream_t st);
+// project_circ's first launch for static bodies, three independent jobs in one launch:
+// rot(E^T f[0]) from the vertex lists on the box (nrhs 1), G = Gs over np values, and
+// the forces (launch_forces with Ft = f[0]); the same results as the three launches.
+hipError_t launch_project_prologue(const ScatterArgs& a, double* G, const double* Gs, size_t np,
+                                   double* fb, double* total, double h, double dt,
+                                   double* host_total, hipStream_t st);
 // (min ix, max ix, min iy, max iy) of the panels' flux cells into out[0..3]
 hipError_t launch_body_bbox(const int* coords, int n, double* out, hipStream_t st);
 
