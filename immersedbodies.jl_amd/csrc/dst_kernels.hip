@@ -837,13 +837,19 @@ __global__ __launch_bounds__(256) void dst_pass_mixed_kernel(const PassBatch bat
 // block (grp 0 odd rows, grp 1 even rows), TB values of b x TM quarter rows, K chunk 32 in
 // fold_chunk<32> order, the loads of the next PF chunks in flight; wave w computes the
 // (b-half w&1, m-half w>>1) quarter of both of the block's products, so the odd block's
-// butterfly A +- B stays in registers. The 32-row tiles with one stage of fp32 inputs are
-// held to five work-groups per CU: the 1280 tiles of a 5-plane 1023 x 1023 pass then run
-// in one round (four per CU leave a second round a quarter full).
+// butterfly A +- B stays in registers. PF = 0: one stage of inputs, and the table tiles go
+// from global memory straight to LDS (global_load_lds_dwordx4, no registers), 64-byte rows
+// whose 16-byte pieces are XOR-swizzled by row (piece c of row r at c ^ ((r >> 2) & 3)) so
+// that the 16 rows an MFMA operand read touches fall in distinct banks. The 32-row tiles
+// with one stage of fp32 inputs are held to five work-groups per CU: the 1280 tiles of a
+// 5-plane 1023 x 1023 pass then run in one round (four per CU leave a second round a
+// quarter full).
 // ---------------------------------------------------------------------------------
 template <typename InT, typename OutT, int EPI, int TB, int TM, int PF>
-__global__ __launch_bounds__(256, (TB == 32 && PF == 1 && sizeof(InT) == 4) ? 5 : 1) void dst_pass_mixedq_kernel(
+__global__ __launch_bounds__(256, (TB == 32 && PF <= 1 && sizeof(InT) == 4) ? 5 : 1) void dst_pass_mixedq_kernel(
     const PassBatch batch) {
+  constexpr bool DT = PF == 0;       // tables straight into LDS
+  constexpr int NS = DT ? 1 : PF;    // register stages of prefetch
   constexpr int BS = TB / 32;  // 16-wide b subtiles per wave
   constexpr int MS = TM / 32;  // 16-wide m subtiles per wave
   constexpr int RA = TB / 16;  // (b, i-pair) input slots per thread and chunk
@@ -861,8 +867,9 @@ __global__ __launch_bounds__(256, (TB == 32 && PF == 1 && sizeof(InT) == 4) ? 5 
 
   __shared__ __attribute__((aligned(16))) __bf16 Xh[2][TB][MX_LDK];  // [X0|X1] hi
   __shared__ __attribute__((aligned(16))) __bf16 Xl[2][TB][MX_LDK];  // [X0|X1] lo
-  __shared__ __attribute__((aligned(16))) __bf16 Th[2][TM][MX_LDK];  // [SA|SB] hi
-  __shared__ __attribute__((aligned(16))) __bf16 Tl[2][TM][MX_LDK];  // [SA|SB] lo
+  __shared__ __attribute__((aligned(16))) __bf16 Th[2][DT ? 1 : TM][MX_LDK];  // [SA|SB] hi
+  __shared__ __attribute__((aligned(16))) __bf16 Tl[2][DT ? 1 : TM][MX_LDK];  // [SA|SB] lo
+  __shared__ __attribute__((aligned(16))) __bf16 Tq[DT ? 4 : 1][DT ? TM : 1][MX_KC];  // DT: SA hi, lo, SB hi, lo
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -909,13 +916,31 @@ __global__ __launch_bounds__(256, (TB == 32 && PF == 1 && sizeof(InT) == 4) ? 5 
 #pragma unroll
   for (int r = 0; r < RQ; ++r) line[r] = (unsigned)min(b0 + (tid >> 3) + 32 * r, Nb - 1) * (unsigned)Nk;
   const int q4 = 4 * (tid & 7);
-  InT rx[PF][RQ][16];
+  InT rx[NS][RQ][16];
   // table loader: 16-byte piece (row sm, k8) of SA hi, SA lo, SB hi, SB lo
   const int sm = tid >> 2, sk = (tid & 3) * 8;
   const char* SAc = (const char*)SA;
   const char* SBc = (const char*)SB;
   const unsigned toff = (unsigned)((m0 + sm) * Qm + sk) * 2u, tqb = (unsigned)tq * 2u;
-  bf8 rs[PF][4];
+  bf8 rs[NS][4];
+  // DT: wave instruction j copies 16 rows of part j / (TM / 16) (SA hi, SA lo, SB hi, SB lo);
+  // lane L fills row 16 (j % (TM / 16)) + L / 4, piece L % 4, with logical piece
+  // (L % 4) ^ ((row >> 2) & 3) of that row (rows past the table read its last row: their
+  // outputs are never stored)
+  constexpr int NTI = 4 * TM / 16;  // wave instructions per chunk
+  auto copy_tables = [&](int kc) {
+#pragma unroll
+    for (int jj = 0; jj < (NTI + 3) / 4; ++jj) {
+      const int j = wave * ((NTI + 3) / 4) + jj;
+      if (j < NTI) {
+        const int part = j / (TM / 16), row = 16 * (j % (TM / 16)) + (lane >> 2);
+        const int piece = (lane & 3) ^ ((row >> 2) & 3);
+        const char* src = (const char*)(part < 2 ? SA : SB) + (part & 1) * tqb +
+                          (size_t)((unsigned)(min(m0 + row, Qm - 1) * Qm + kc + 8 * piece) * 2u);
+        __builtin_amdgcn_global_load_lds((global_void*)src, (lds_void*)&Tq[part][16 * (j % (TM / 16))][0], 16, 0, 0);
+      }
+    }
+  };
   auto fetch = [&](const int st, int kc) {
     const unsigned i0 = (unsigned)min(kc + q4, Q - 4);
 #pragma unroll
@@ -932,6 +957,7 @@ __global__ __launch_bounds__(256, (TB == 32 && PF == 1 && sizeof(InT) == 4) ? 5 
         run(l + (unsigned)H + i0, o + 12, 4);                // x_{H+i0+1..H+i0+4}
       }
     }
+    if (DT) return;
     const unsigned off = toff + (unsigned)kc * 2u;
     if (sm < TM && m0 + sm < Qm) {
       rs[st][0] = *(const bf8*)(SAc + off);
@@ -979,7 +1005,7 @@ __global__ __launch_bounds__(256, (TB == 32 && PF == 1 && sizeof(InT) == 4) ? 5 
       *(bf4*)&Xh[1][bl][q4] = ch;
       *(bf4*)&Xl[1][bl][q4] = cl;
     }
-    if (sm < TM) {
+    if (!DT && sm < TM) {
       *(bf8*)&Th[0][sm][sk] = rs[st][0];
       *(bf8*)&Tl[0][sm][sk] = rs[st][1];
       *(bf8*)&Th[1][sm][sk] = rs[st][2];
@@ -999,10 +1025,19 @@ __global__ __launch_bounds__(256, (TB == 32 && PF == 1 && sizeof(InT) == 4) ? 5 
 #pragma unroll
     for (int ms = 0; ms < MS; ++ms) {
       const int row = mh * (TM / 2) + ms * 16 + col;
-      const bf8 sah = *(const bf8*)&Th[0][row][kq * 8];
-      const bf8 sal = *(const bf8*)&Tl[0][row][kq * 8];
-      const bf8 sbh = *(const bf8*)&Th[1][row][kq * 8];
-      const bf8 sbl = *(const bf8*)&Tl[1][row][kq * 8];
+      bf8 sah, sal, sbh, sbl;
+      if (DT) {
+        const int pc = 8 * (kq ^ ((row >> 2) & 3));
+        sah = *(const bf8*)&Tq[0][row][pc];
+        sal = *(const bf8*)&Tq[1][row][pc];
+        sbh = *(const bf8*)&Tq[2][row][pc];
+        sbl = *(const bf8*)&Tq[3][row][pc];
+      } else {
+        sah = *(const bf8*)&Th[0][row][kq * 8];
+        sal = *(const bf8*)&Tl[0][row][kq * 8];
+        sbh = *(const bf8*)&Th[1][row][kq * 8];
+        sbl = *(const bf8*)&Tl[1][row][kq * 8];
+      }
 #pragma unroll
       for (int bs = 0; bs < BS; ++bs) {
         acc0[bs][ms] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x0l[bs], sah, acc0[bs][ms], 0, 0, 0);
@@ -1015,19 +1050,38 @@ __global__ __launch_bounds__(256, (TB == 32 && PF == 1 && sizeof(InT) == 4) ? 5 
     }
   };
   const int nb = (Q + MX_KC - 1) / MX_KC;  // chunks, taken in fold_chunk order
+  if (DT) {
+    // per chunk: stage the inputs fetched one chunk ahead, wait for the table copy,
+    // barrier, fetch the next inputs, multiply, barrier, start the next table copy
+    int kc = fold_chunk<MX_KC>(grp, 0, MX_KC, nb);
+    fetch(0, kc);
+    copy_tables(kc);
+    for (int t = 0; t < nb; ++t) {  // the same for every thread: barriers stay uniform
+      const int kn = t + 1 < nb ? fold_chunk<MX_KC>(grp, t + 1, MX_KC, nb) : 0;
+      stage(0, kc);
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this chunk's table copy has landed
+      __syncthreads();
+      if (t + 1 < nb) fetch(0, kn);
+      multiply();
+      __syncthreads();
+      if (t + 1 < nb) copy_tables(kn);
+      kc = kn;
+    }
+  } else {
 #pragma unroll
-  for (int st = 0; st < PF; ++st)
-    if (st < nb) fetch(st, fold_chunk<MX_KC>(grp, st, MX_KC, nb));
-  for (int t0 = 0; t0 < nb; t0 += PF) {
+    for (int st = 0; st < NS; ++st)
+      if (st < nb) fetch(st, fold_chunk<MX_KC>(grp, st, MX_KC, nb));
+    for (int t0 = 0; t0 < nb; t0 += NS) {
 #pragma unroll
-    for (int st = 0; st < PF; ++st) {
-      const int t = t0 + st;  // the same for every thread: barriers stay uniform
-      if (t < nb) {
-        stage(st, fold_chunk<MX_KC>(grp, t, MX_KC, nb));
-        __syncthreads();
-        if (t + PF < nb) fetch(st, fold_chunk<MX_KC>(grp, t + PF, MX_KC, nb));
-        multiply();
-        __syncthreads();
+      for (int st = 0; st < NS; ++st) {
+        const int t = t0 + st;  // the same for every thread: barriers stay uniform
+        if (t < nb) {
+          stage(st, fold_chunk<MX_KC>(grp, t, MX_KC, nb));
+          __syncthreads();
+          if (t + NS < nb) fetch(st, fold_chunk<MX_KC>(grp, t + NS, MX_KC, nb));
+          multiply();
+          __syncthreads();
+        }
       }
     }
   }
@@ -1064,8 +1118,8 @@ __global__ __launch_bounds__(256, (TB == 32 && PF == 1 && sizeof(InT) == 4) ? 5 
 // quarter rows there), 16..19 double fold with the tables copied straight into LDS
 // (dst_pass_f64d_kernel; where Q is not a multiple of 16 they run as variant 10 or 11).
 // The mixed pass has its own 20: 0..7 one fold (4..7 two prefetch stages), 8..19 the
-// double fold (dst_pass_mixedq_kernel) with 1, 2 or 3 prefetch stages; v % 4 picks the
-// tiles as for 0..15.
+// double fold (dst_pass_mixedq_kernel) with 1 or 2 prefetch stages (8..15) or with the
+// tables copied straight into LDS (16..19); v % 4 picks the tiles as for 0..15.
 constexpr int kVariants = kDstVariants;
 constexpr int kMixedVariants = kDstVariants;
 // the single-fold mixed variant of the same tiles (two prefetch stages for 12..19)
@@ -1211,7 +1265,7 @@ static void launch_mixed_types(int v, const PassBatch& b, int nplanes, int maxNb
 
 static void launch_mixed_variant(int v, const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
                                  bool in_f32, bool out_f32, hipStream_t st) {
-  if (v >= 16) launch_mixed_types<MixQ<3>::K>(v - 16, b, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
+  if (v >= 16) launch_mixed_types<MixQ<0>::K>(v - 16, b, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
   else if (v >= 12) launch_mixed_types<MixQ<2>::K>(v - 12, b, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
   else if (v >= 8) launch_mixed_types<MixQ<1>::K>(v - 8, b, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
   else if (v >= 4) launch_mixed_types<MixKC<2>::K>(v - 4, b, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
@@ -1366,10 +1420,10 @@ int dst_tuned_variant(int mixed, int nplanes, int maxNb, int maxH) {
 
 int dst_variant_tiles(int v, bool mixed, int* TB, int* TM, int* PF, int* fold, int* KC) {
   if (v < 0 || v >= kVariants) return 0;
-  if (mixed) {  // (TB, TM) by v % 4 as for 0..15; PF 1, 2 (one fold), 1, 2, 3 (double fold)
+  if (mixed) {  // (TB, TM) by v % 4 as for 0..15; PF 1, 2 (one fold), 1, 2, 0 (double fold)
     *TB = kVarTB[v % 4];
     *TM = kVarTM[v % 4];
-    *PF = v < 8 ? 1 + v / 4 : 1 + (v - 8) / 4;
+    *PF = v < 8 ? 1 + v / 4 : (v < 16 ? 1 + (v - 8) / 4 : 0);
     if (fold) *fold = v < 8 ? 2 : 4;
     if (KC) *KC = MX_KC;
     return 1;
