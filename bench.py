@@ -93,7 +93,6 @@ def step_roofline(solver, ms_per_step, pass_flops):
     L = g.nlevel
     Nx, Ny = nx - 1, ny - 1
     P = Nx * Ny                       # plane values
-    nq = (nx + 1) * ny + nx * (ny + 1)
     n = solver.state.prob.bodies.npanel
     B8 = 8.0
     comp = []
@@ -105,8 +104,9 @@ def step_roofline(solver, ms_per_step, pass_flops):
     boxes = [solver.box_update(l) for l in range(1, L + 1)]
     pbox = solver.box_update(0)
     on = boxes[0] is not None
-    add("nonlin_rhs (avg_flux, direct_product, rot, CNAB rhs; all levels)", 1, 0,
-        L * (nq + 4 * P) * B8)
+    # the fluxes are formed from psi inside the kernel (curl!), so psi is read instead of q
+    add("nonlin_rhs (curl, avg_flux, direct_product, rot, CNAB rhs; all levels)", 1, 0,
+        L * (P + 4 * P) * B8)
     add("Helmholtz solves, Gauss-Seidel (all levels)", 1, 0, 2 * L * P * B8)
     # two vort2flux per step; coarsify reads the fine plane, writes the coarse centre
     # (trial) or the whole coarse plane with the outer copy fused (final)
@@ -137,7 +137,6 @@ def step_roofline(solver, ms_per_step, pass_flops):
     add("couple_surface (gather, dense B^-1 product, forces)", 3, 2.0 * (2 * n) ** 2,
         ((2 * n) ** 2 + 98 * 2 * n) * B8)
     add("project_circ (copy, rot E^T on the box, Gauss-Seidel on the box)", 3, 0, 2 * P * B8)
-    add("curl (all levels)", 1, 0, L * (P + nq) * B8)
     floor = 0.0
     for c in comp:
         tf = c["flops"] / (PEAK["fp64"] * 1e12)

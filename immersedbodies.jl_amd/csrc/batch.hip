@@ -225,6 +225,16 @@ int capture(ibpm_batch* b, int phase, hipGraphExec_t* out) {
 // state is left as it was.
 int capture_all(ibpm_batch* b) {
   const int np = b->nprob;
+  // the captured steps read the fluxes from q (eager curl): write q first where a step
+  // outside the batch left it to the next nonlinear kernel
+  for (int p = 0; p < np; ++p) {
+    ibpm_solver* s = b->s[p];
+    if (!s->q_stale) continue;
+    int rc0 = materialize_q(s);
+    if (rc0 == IBPM_OK && hipStreamSynchronize(s->st) != hipSuccess)
+      rc0 = set_error(IBPM_E_HIP, "batch capture: stream synchronize failed");
+    if (rc0) return rc0;
+  }
   for (hipGraphExec_t& e : b->exec)
     if (e) (void)hipGraphExecDestroy(e), e = nullptr;
   if (b->execB) (void)hipGraphExecDestroy(b->execB), b->execB = nullptr;
@@ -373,6 +383,12 @@ int ibpm_batch_advance(ibpm_batch* b, int nsteps, const double* freestream, cons
   if (g < 0)
     return set_error(IBPM_E_ARG, "ibpm_batch_advance: the solvers were advanced out of step "
                                  "with each other; create a new batch");
+  // the graphs read the fluxes from q: write q where a step outside the batch left it to
+  // the next nonlinear kernel
+  for (int p = 0; p < b->nprob; ++p) {
+    const int rc0 = materialize_q(b->s[p]);
+    if (rc0) return rc0;
+  }
   for (int p = 0; p < b->nprob; ++p) BCHK(hipStreamSynchronize(b->s[p]->st));
   const int np = b->nprob;
   std::vector<double> dyn((size_t)std::min(nsteps, kHistSteps) * np * kDyn);

@@ -1386,6 +1386,11 @@ int step_head(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step
   na.g = s->g;
   na.nlevel = L;
   na.q = s->q;
+  if (s->q_stale) {
+    // a graph replays the step with fluxes from q: q must be written before a capture
+    if (s->graph_mode) return fail(IBPM_E_ARG, "internal: q not written before a graph capture");
+    na.psi = s->psi;
+  }
   na.G = s->G;
   na.nl_new = s->nl1;
   na.nl_old = s->nl2;
@@ -1494,17 +1499,20 @@ int step_tail(ibpm_solver* s, ibpm_step_stats* stats) {
     double* oc[1] = {s->Gs};
     double* qc[1] = {s->q};
     // with the box update, only the change of Gamma since the trial vort2flux is
-    // transformed (outer = Gamma_s: G and Gs differ inside the boxes only)
-    rc = vort2flux(s, 1, Gc, pc, oc, qc, 0, spec1 ? s->F1 : nullptr,
+    // transformed (outer = Gamma_s: G and Gs differ inside the boxes only); outside a
+    // graph the final curl! is left to the next step's nonlinear kernel (q_stale)
+    rc = vort2flux(s, 1, Gc, pc, oc, s->graph_mode ? qc : nullptr, 0, spec1 ? s->F1 : nullptr,
                    s->box_on && !spec1 ? s->Ftr : nullptr, s->box_on && !spec1);
     if (rc) return rc;
+    s->q_stale = !s->graph_mode;
   } else {
     HIPCHK(launch_copy(s->G, s->Gs, (size_t)L * P, s->st));
     double* Gc[1] = {s->G};
     double* pc[1] = {s->psi};
     double* qc[1] = {s->q};
-    rc = vort2flux(s, 1, Gc, pc, nullptr, qc, 0, spec1 ? s->F1 : nullptr);
+    rc = vort2flux(s, 1, Gc, pc, nullptr, s->graph_mode ? qc : nullptr, 0, spec1 ? s->F1 : nullptr);
     if (rc) return rc;
+    s->q_stale = !s->graph_mode;
     if (stats) {
       stats->cg_iters = 0;
       stats->cg_converged = 1;
@@ -1572,6 +1580,15 @@ void base_flux_dyn(const ibpm_solver* s, const double fs8[8], double out[14]) {
                         om * -sn, om * -c, om * c, om * -sn};
   (void)s;
   for (int k = 0; k < 14; ++k) out[k] = v[k];
+}
+
+// curl! (fluid.jl:230-276) of the current psi into q where the last step left it to the
+// next nonlinear kernel; the same kernel and bits the step would have written.
+int materialize_q(ibpm_solver* s) {
+  if (!s->q_stale) return IBPM_OK;
+  HIPCHK(launch_curl_all(s->g, s->L, s->q, s->psi, s->st));
+  s->q_stale = false;
+  return IBPM_OK;
 }
 }  // namespace ibpm
 
@@ -2141,6 +2158,10 @@ int ibpm_read(ibpm_solver* s, int field, double* dst, size_t n) {
   if (n != need) return fail(IBPM_E_ARG, "size mismatch: expected " + std::to_string(need));
   if (need == 0) return IBPM_OK;
   HIPCHK(hipSetDevice(s->device));
+  if (field == IBPM_Q) {
+    int rc = materialize_q(s);
+    if (rc) return rc;
+  }
   HIPCHK(hipStreamSynchronize(s->st));
   if (field == IBPM_Q0) {
     if (s->q0_shadow_valid) {
@@ -2183,6 +2204,10 @@ int ibpm_write(ibpm_solver* s, int field, const double* src, size_t n) {
   if (n != need) return fail(IBPM_E_ARG, "size mismatch: expected " + std::to_string(need));
   if (need == 0) return IBPM_OK;
   HIPCHK(hipSetDevice(s->device));
+  if (field == IBPM_PSI) {  // q stays the curl of the psi it came from
+    int rc = materialize_q(s);
+    if (rc) return rc;
+  }
   HIPCHK(hipStreamSynchronize(s->st));
   switch (field) {
     case IBPM_Q0:
@@ -2201,6 +2226,7 @@ int ibpm_write(ibpm_solver* s, int field, const double* src, size_t n) {
   }
   HIPCHK(hipMemcpy(field_ptr(s, field), src, need * 8, hipMemcpyHostToDevice));
   if (field == IBPM_FTILDE) s->nhist = 0;  // the extrapolation history no longer applies
+  if (field == IBPM_Q) s->q_stale = false;
   return IBPM_OK;
 }
 
@@ -2442,8 +2468,9 @@ int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, doubl
     *bytes = L * Nk * Nb * (8.0 + (mixed ? 4.0 : 8.0));  // fp64 plane in, plane out
   } else {
     *flops = 0.0;
-    // q (nq) + Gamma + nl_old in, nl_new + rhs out, per level
-    *bytes = (double)s->L * ((double)s->g.nq + 4.0 * (double)P) * 8.0;
+    // q (nq), or psi (nG) where the kernel forms the fluxes, + Gamma + nl_old in,
+    // nl_new + rhs out, per level
+    *bytes = (double)s->L * ((s->q_stale ? (double)P : (double)s->g.nq) + 4.0 * (double)P) * 8.0;
   }
   return IBPM_OK;
 }

@@ -204,6 +204,8 @@ struct NonlinArgs {
   GridDims g;
   int nlevel;
   const double* q;       // nq x L
+  const double* psi;     // nG x L; when set, the fluxes are curl(psi) formed in the kernel
+                         // (q is not read)
   const double* G;       // nG x L (state circulation)
   double* nl_new;        // nG x L, written
   const double* nl_old;  // nG x L

@@ -102,6 +102,10 @@ struct ibpm_solver {
   // dyn_d (14 values, BaseFlux::dyn) and its body motion from `motion`, both filled on the
   // device before the step
   bool graph_mode = false;
+  // q is curl(psi) of the current psi but not written: steps outside a graph leave the
+  // final curl to the next step's nonlinear kernel, which forms the fluxes from psi;
+  // reads and writes of the state write q first (materialize_q)
+  bool q_stale = false;
   double* dyn_d = nullptr;
   int graph_cg_budget = 0;  // CG iterations captured per step (0: cg_maxiter)
   // instrumentation (ibpm_time_kernel): HIP events around each launch of kernel class
@@ -139,6 +143,9 @@ int solver_step(ibpm_solver* s, const double* fsv, const double* motion, ibpm_st
 // Host-driven CG iterations from where a captured step's budget left them, to the stopping
 // test, on the solver's stream.
 int solver_cg_continue(ibpm_solver* s);
+// Writes q = curl(psi) where the last step left it to the next nonlinear kernel (q_stale),
+// on the solver's stream.
+int materialize_q(ibpm_solver* s);
 // BaseFlux::dyn values of a step from its freestream and GridVelocity fs8 = (ux, uy,
 // centre x, y, velocity x, y, angle, angular_vel): the numbers base_flux() puts in the
 // kernel arguments, bit for bit.

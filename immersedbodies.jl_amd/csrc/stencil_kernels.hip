@@ -50,7 +50,11 @@ __device__ __forceinline__ double lane_next(double v) {
                           __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false));
 }
 
-template <bool DYN>
+// PSI: the fluxes come from the streamfunction (a.psi, with the coarser level's ring) by
+// the curl! of curl_all_kernel, evaluated where they are needed: the same expressions,
+// so the same bits as reading the q that curl_all_kernel writes; a lane reads its psi
+// column once per row and takes the next column from the lane beside it.
+template <bool DYN, bool PSI>
 __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
   const GridDims g = a.g;
   const int lev = blockIdx.z;
@@ -86,13 +90,40 @@ __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
   // Qy(I+1, j), j = J0 .. J0+NL_RJ+1 (v-faces up to ny+1); Qy(I, j) is the previous lane's
   const int Iy = min(I + 1, nx);
   double qa[NL_RJ + 2];
-#pragma unroll
-  for (int r = 0; r < NL_RJ + 2; ++r) qa[r] = qy[(Iy - 1) + (size_t)(min(J0 + r, ny + 1) - 1) * nx];
   // Qx(I+1, j), j = J0 .. J0+NL_RJ (u-faces up to ny)
   const int Ix = min(I + 1, nx + 1);
   double qxr[NL_RJ + 1];
+  if (PSI) {
+    // psi(I, j), j = J0-1 .. J0+NL_RJ (index clamped into the plane), and psi(I+1, j) from
+    // the next lane; faces with a boundary vertex, or whose column is clamped, go through
+    // the boundary rules (curl_u / curl_v). Lane 63's v-faces feed no other lane.
+    const double* p = a.psi + (size_t)lev * g.nG;
+    const double* pc = lev + 1 < a.nlevel ? a.psi + (size_t)(lev + 1) * g.nG : nullptr;
+    double pv[NL_RJ + 2];
 #pragma unroll
-  for (int r = 0; r < NL_RJ + 1; ++r) qxr[r] = qx[(Ix - 1) + (size_t)(min(J0 + r, ny) - 1) * (nx + 1)];
+    for (int r = 0; r < NL_RJ + 2; ++r) {
+      const int jc = min(max(J0 - 1 + r, 1), g.Ny);
+      pv[r] = p[(size_t)(Ic - 1) + (size_t)(jc - 1) * g.Nx];
+    }
+#pragma unroll
+    for (int r = 0; r < NL_RJ + 2; ++r) {
+      const double pn = lane_next(pv[r]);  // psi(I + 1, J0 - 1 + r)
+      const int jv = min(J0 + r, ny + 1);
+      const bool in = Iy == I + 1 && I + 1 >= 2 && I + 1 <= nx - 1 && jv >= 2 && jv <= ny;
+      qa[r] = in ? pv[r] - pn : (lane == 63 ? 0.0 : curl_v(p, pc, nx, ny, Iy, jv));
+    }
+#pragma unroll
+    for (int r = 0; r < NL_RJ + 1; ++r) {
+      const int ju = min(J0 + r, ny);
+      const bool in = Ix == I + 1 && I + 1 >= 2 && I + 1 <= nx && ju >= 2 && ju <= ny - 1;
+      qxr[r] = in ? pv[r + 1] - pv[r] : curl_u(p, pc, nx, ny, Ix, ju);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < NL_RJ + 2; ++r) qa[r] = qy[(Iy - 1) + (size_t)(min(J0 + r, ny + 1) - 1) * nx];
+#pragma unroll
+    for (int r = 0; r < NL_RJ + 1; ++r) qxr[r] = qx[(Ix - 1) + (size_t)(min(J0 + r, ny) - 1) * (nx + 1)];
+  }
   double qyv[NL_RJ + 2];
 #pragma unroll
   for (int r = 0; r < NL_RJ + 2; ++r) {
@@ -153,10 +184,17 @@ __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
 hipError_t launch_nonlin_rhs(const NonlinArgs& a, hipStream_t st) {
   if (g_skip) return no_skip();
   dim3 grid((a.g.Nx + NL_W - 1) / NL_W, (a.g.Ny + 4 * NL_RJ - 1) / (4 * NL_RJ), a.nlevel);
-  if (a.bf.dyn)
-    hipLaunchKernelGGL(nonlin_rhs_kernel<true>, grid, dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL(nonlin_rhs_kernel<false>, grid, dim3(256), 0, st, a);
+  if (a.psi) {
+    if (a.bf.dyn)
+      hipLaunchKernelGGL((nonlin_rhs_kernel<true, true>), grid, dim3(256), 0, st, a);
+    else
+      hipLaunchKernelGGL((nonlin_rhs_kernel<false, true>), grid, dim3(256), 0, st, a);
+  } else {
+    if (a.bf.dyn)
+      hipLaunchKernelGGL((nonlin_rhs_kernel<true, false>), grid, dim3(256), 0, st, a);
+    else
+      hipLaunchKernelGGL((nonlin_rhs_kernel<false, false>), grid, dim3(256), 0, st, a);
+  }
   return hipGetLastError();
 }
 
