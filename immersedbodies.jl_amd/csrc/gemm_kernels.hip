@@ -1,15 +1,22 @@
 // FP64 matrix-core products C (+)= A B of the box-restricted spectral updates: the
 // forward sine transform of a change of Gamma that vanishes outside a box of the plane,
 // S_x[:, I] D[I, J] S_y[J, :], as two dense products of the box with the sine-matrix
-// columns it touches (ibpm.hip, box_update). Column-major operands; A or B may be given
-// as the difference of two matrices of one leading dimension (X - X2, the change of
-// Gamma read from the two planes it lies between).
+// columns it touches (ibpm.hip, box_update), and the trial psi on the panels' box.
+// Column-major operands; A or B may be given as the difference of two matrices of one
+// leading dimension (X - X2, the change of Gamma read from the two planes it lies
+// between). The matrix core takes B^T as its first operand so that a lane's accumulator
+// entries run along the column of C. Each C entry sums its K terms in one fixed order, so
+// results do not depend on scheduling.
 //
-// Tile: 64 x 64 of C per work-group, 4 waves of 32 x 32 (2 x 2 v_mfma_f64_16x16x4), K in
-// chunks of 32 staged through LDS with the next chunk's loads in flight. The matrix core
-// takes B^T as its first operand so that a lane's accumulator entries run along the
-// column of C (16 consecutive rows per store). Each C entry sums its K terms in one fixed
-// order, so results do not depend on scheduling.
+// Two kernels, picked per launch by its size:
+//  - gemm_f64_kernel, for launches of many tiles (a full plane of C per job): 64 x 64 of
+//    C per work-group, 4 waves of 32 x 32 (2 x 2 v_mfma_f64_16x16x4), K in chunks of 32
+//    staged through LDS with the next chunk's loads in flight;
+//  - gemm_f64_reg_kernel, for the small products (a few hundred tiles, K of tens to a few
+//    hundred), which are latency-bound: 32 x 32 of C per work-group, one memory round trip
+//    per K chunk of 16 KR values: the chunk is dealt to the 16 (wave, lane quarter) pairs,
+//    each lane loads all its values of A and B straight into registers (no LDS staging),
+//    and the four waves' partial tiles are summed in wave order through LDS.
 #include <string.h>
 
 #include <algorithm>
@@ -21,7 +28,30 @@ namespace ibpm {
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
-constexpr int GT = 64;  // C tile edge
+
+// Tile order of a launch: each job's T tiles, m-tile major with the n-tiles fastest
+// (consecutive tiles share rows of A), are cut into 8 contiguous ranges of near-equal
+// size, [x T / 8, (x + 1) T / 8) for XCD x (each XCD has its own L2); work-groups are dealt
+// round-robin to the XCDs, so work-group w walks the ranges of XCD w % 8 over the jobs in
+// order, ceil(T / 8) slots per job (gb.lbase: the first slot of each job, filled by the
+// launcher). Each XCD brings its rows of A into its L2 once, and every job, short or long
+// in K, is spread over all XCDs.
+template <int TM, int TN>
+__device__ __forceinline__ bool gemm_tile(const GemmBatch& gb, int& job, int& m0, int& n0) {
+  const int x = blockIdx.x & 7, l = blockIdx.x >> 3;
+  job = 0;
+#pragma unroll
+  for (int i = 1; i < kMaxPlanes; ++i) job = (i < gb.njobs && l >= gb.lbase[i]) ? i : job;
+  const GemmJob& jb = gb.j[job];
+  const int ntn = (jb.N + TN - 1) / TN, T = ((jb.M + TM - 1) / TM) * ntn;
+  const int u = x * T / 8 + (l - gb.lbase[job]);
+  if (u >= (x + 1) * T / 8) return false;
+  m0 = (u / ntn) * TM;
+  n0 = (u % ntn) * TN;
+  return true;
+}
+
+constexpr int GT = 64;  // C tile edge of gemm_f64_kernel
 constexpr int GK = 32;  // K chunk (the next chunk's loads fly during this one's products)
 constexpr int GR = GT * GK / 256;  // elements of A (and of B) each thread stages per chunk
 
@@ -30,9 +60,11 @@ constexpr int GR = GT * GK / 256;  // elements of A (and of B) each thread stage
 template <bool DA, bool DB>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
   if (skipped(gb.skip)) return;
+  int job, m0, n0;
+  if (!gemm_tile<GT, GT>(gb, job, m0, n0)) return;  // whole work-group: barriers stay uniform
   // the job's fields in registers (uniform), read once; a missing difference term reads
   // the matrix itself with weight 0 (a - 0 * a = a exactly), so no load is conditional
-  const GemmJob& jb = gb.j[blockIdx.y];
+  const GemmJob& jb = gb.j[job];
   const double *A = jb.A, *B = jb.B;
   const double* A2 = jb.A2 ? jb.A2 : jb.A;
   const double* B2 = jb.B2 ? jb.B2 : jb.B;
@@ -40,9 +72,6 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
   double* C = jb.C;
   const int lda = jb.lda, ldb = jb.ldb, ldc = jb.ldc, M = jb.M, N = jb.N, K = jb.K;
   const bool accumulate = jb.accumulate != 0;
-  const int ntm = (M + GT - 1) / GT, ntn = (N + GT - 1) / GT;
-  if ((int)blockIdx.x >= ntm * ntn) return;  // whole work-group: barriers stay uniform
-  const int m0 = (blockIdx.x % ntm) * GT, n0 = (blockIdx.x / ntm) * GT;
   __shared__ double As[GK][GT + 1];  // As[k][m]
   __shared__ double Bs[GK][GT + 1];  // Bs[k][n]
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -137,6 +166,103 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
     }
 }
 
+constexpr int GM = 2, GN = 2;                // 16-wide subtiles of the C tile along m, n
+constexpr int GTM = 16 * GM, GTN = 16 * GN;  // C tile of gemm_f64_reg_kernel
+constexpr int GCE = GTM * GTN / 256;         // C entries each thread sums and stores
+
+// DA, DB: some job of the launch has a difference term A2 (B2); without, the operand is
+// loaded once. KR: the longest run a lane loads per chunk (chunks of 16 KR values of K).
+template <bool DA, bool DB, int KR>
+__global__ __launch_bounds__(256) void gemm_f64_reg_kernel(const GemmBatch gb) {
+  if (skipped(gb.skip)) return;
+  int job, m0, n0;
+  if (!gemm_tile<GTM, GTN>(gb, job, m0, n0)) return;  // whole work-group: barriers stay uniform
+  // the job's fields in registers (uniform), read once; a missing difference term reads
+  // the matrix itself with weight 0 (a - 0 * a = a exactly), so no load is conditional
+  const GemmJob& jb = gb.j[job];
+  const double *A = jb.A, *B = jb.B;
+  const double* A2 = jb.A2 ? jb.A2 : jb.A;
+  const double* B2 = jb.B2 ? jb.B2 : jb.B;
+  const double wa = jb.A2 ? 1.0 : 0.0, wb = jb.B2 ? 1.0 : 0.0;
+  double* C = jb.C;
+  const int lda = jb.lda, ldb = jb.ldb, ldc = jb.ldc, M = jb.M, N = jb.N, K = jb.K;
+  const bool accumulate = jb.accumulate != 0;
+  __shared__ double red[4][GTN][GTM + 1];  // the waves' partial tiles, [n][m]
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int col = lane & 15, kq = lane >> 4, run = 4 * w + kq;
+
+  // the C entries this thread finishes, loaded first so that they arrive with the operands
+  double cin[GCE];
+#pragma unroll
+  for (int e = 0; e < GCE; ++e) {
+    const int idx = tid + 256 * e, m = idx % GTM, n = idx / GTM;
+    const size_t o = (size_t)min(m0 + m, M - 1) + (size_t)min(n0 + n, N - 1) * ldc;
+    cin[e] = accumulate ? C[o] : 0.0;
+  }
+  size_t ao[GM], bo[GN];
+#pragma unroll
+  for (int i = 0; i < GM; ++i) ao[i] = (size_t)min(m0 + 16 * i + col, M - 1);
+#pragma unroll
+  for (int j = 0; j < GN; ++j) bo[j] = (size_t)min(n0 + 16 * j + col, N - 1) * ldb;
+
+  d4 acc[GN][GM];
+#pragma unroll
+  for (int j = 0; j < GN; ++j)
+#pragma unroll
+    for (int i = 0; i < GM; ++i) acc[j][i] = (d4){0.0, 0.0, 0.0, 0.0};
+
+  // one chunk: lane quarter kq of wave w takes k = kc0 + run + 16 r, r < KR; every load is
+  // unconditional with its index clamped into the matrix (zeroed below), so all of them
+  // issue back to back; past K they repeat the last column's address
+  auto chunk = [&](const int kc0) __attribute__((always_inline)) {
+    double a[GM][KR], b[GN][KR];
+#pragma unroll
+    for (int r = 0; r < KR; ++r) {
+      const size_t k = (size_t)min(kc0 + run + 16 * r, K - 1);
+#pragma unroll
+      for (int i = 0; i < GM; ++i) {
+        const size_t o = ao[i] + k * lda;
+        a[i][r] = DA ? fma(-wa, A2[o], A[o]) : A[o];
+      }
+#pragma unroll
+      for (int j = 0; j < GN; ++j) {
+        const size_t o = k + bo[j];
+        b[j][r] = DB ? fma(-wb, B2[o], B[o]) : B[o];
+      }
+    }
+    const int nr = (min(16 * KR, K - kc0) + 15) / 16;  // steps holding some k < K (uniform)
+#pragma unroll
+    for (int r = 0; r < KR; ++r) {
+      if (r < nr) {
+        const bool ok = kc0 + run + 16 * r < K;
+#pragma unroll
+        for (int j = 0; j < GN; ++j)
+#pragma unroll
+          for (int i = 0; i < GM; ++i)
+            acc[j][i] = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? b[j][r] : 0.0, ok ? a[i][r] : 0.0,
+                                                             acc[j][i], 0, 0, 0);
+      }
+    }
+  };
+  // the first chunk outside the loop: its loads do not wait for those of the C entries
+  chunk(0);
+  for (int kc0 = 16 * KR; kc0 < K; kc0 += 16 * KR) chunk(kc0);
+  // acc[j][i][r] at lane: the wave's partial C(m = m0 + 16 i + col, n = n0 + 16 j + kq + 4 r)
+#pragma unroll
+  for (int j = 0; j < GN; ++j)
+#pragma unroll
+    for (int i = 0; i < GM; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[w][16 * j + kq + 4 * r][16 * i + col] = acc[j][i][r];
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < GCE; ++e) {
+    const int idx = tid + 256 * e, m = idx % GTM, n = idx / GTM;
+    const double v = ((red[0][n][m] + red[1][n][m]) + red[2][n][m]) + red[3][n][m];
+    if (m0 + m < M && n0 + n < N) C[(size_t)(m0 + m) + (size_t)(n0 + n) * ldc] = cin[e] + v;
+  }
+}
+
 // out[m + n ldo] = sum over s of P[s stride + m + n ldp], the slices in order; one job
 // per blockIdx.z
 __global__ __launch_bounds__(256) void sum_slices_kernel(const SliceSumBatch sb) {
@@ -155,6 +281,21 @@ __global__ __launch_bounds__(256) void sum_slices_kernel(const SliceSumBatch sb)
   for (int s = 1; s < 16; ++s)
     if (s < nslice) sum += v[s];
   jb.out[(size_t)m + (size_t)n * jb.ldo] = sum;
+}
+
+// slots of the jobs in the order of gemm_tile (gb.lbase) and the number of TM x TN tiles;
+// returns the number of work-groups
+template <int TM, int TN>
+int gemm_groups(GemmBatch& gb, int njobs, int* tiles) {
+  int l = 0, t = 0;
+  for (int i = 0; i < njobs; ++i) {
+    const int T = ((gb.j[i].M + TM - 1) / TM) * ((gb.j[i].N + TN - 1) / TN);
+    gb.lbase[i] = l;
+    l += (T + 7) / 8;
+    t += T;
+  }
+  *tiles = t;
+  return 8 * l;
 }
 
 }  // namespace
@@ -184,28 +325,60 @@ hipError_t launch_sum_slices(const double* P, size_t stride, int nslice, int ldp
 }
 
 hipError_t launch_gemm_f64(const GemmBatch& gb0, int njobs, hipStream_t st) {
+  if (njobs <= 0) return hipSuccess;
+  if (njobs > kMaxPlanes) return hipErrorInvalidValue;
   GemmBatch gb = gb0;
   gb.skip = g_skip;
-  int tiles = 0;
+  gb.njobs = njobs;
   bool da = false, db = false;
+  int kmax = 0;
   for (int i = 0; i < njobs; ++i) {
     const GemmJob& j = gb.j[i];
     if (j.M <= 0 || j.N <= 0 || j.K <= 0) return hipErrorInvalidValue;
-    const int t = ((j.M + GT - 1) / GT) * ((j.N + GT - 1) / GT);
-    tiles = t > tiles ? t : tiles;
     da = da || j.A2;
     db = db || j.B2;
+    kmax = std::max(kmax, j.K);
   }
-  if (njobs <= 0) return hipSuccess;
-  const dim3 grid(tiles, njobs);
-  if (da && db)
-    hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, dim3(256), 0, st, gb);
-  else if (da)
-    hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, dim3(256), 0, st, gb);
-  else if (db)
-    hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, dim3(256), 0, st, gb);
-  else
-    hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, dim3(256), 0, st, gb);
+  // the 64 x 64 kernel once the launch holds two of its tiles per CU; smaller launches
+  // take the register-staged 32 x 32 one
+  int tiles = 0;
+  const int wide_groups = gemm_groups<GT, GT>(gb, njobs, &tiles);
+  if (tiles >= 512) {
+    const dim3 grid(wide_groups);
+    if (da && db)
+      hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, dim3(256), 0, st, gb);
+    else if (da)
+      hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, dim3(256), 0, st, gb);
+    else if (db)
+      hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, dim3(256), 0, st, gb);
+    else
+      hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, dim3(256), 0, st, gb);
+    return hipGetLastError();
+  }
+  const dim3 grid(gemm_groups<GTM, GTN>(gb, njobs, &tiles));
+  // loads per lane and chunk: enough for the longest K of the launch in one chunk (up to
+  // 256, 128 with a difference term, whose loads are twice as many)
+#define GEMM_LAUNCH(DA_, DB_, KR_) \
+  hipLaunchKernelGGL((gemm_f64_reg_kernel<DA_, DB_, KR_>), grid, dim3(256), 0, st, gb)
+  if (da || db) {
+    const bool k4 = kmax <= 64;
+    if (da && db) {
+      if (k4) GEMM_LAUNCH(true, true, 4); else GEMM_LAUNCH(true, true, 8);
+    } else if (da) {
+      if (k4) GEMM_LAUNCH(true, false, 4); else GEMM_LAUNCH(true, false, 8);
+    } else {
+      if (k4) GEMM_LAUNCH(false, true, 4); else GEMM_LAUNCH(false, true, 8);
+    }
+  } else if (kmax <= 64) {
+    GEMM_LAUNCH(false, false, 4);
+  } else if (kmax <= 128) {
+    GEMM_LAUNCH(false, false, 8);
+  } else if (kmax <= 192) {
+    GEMM_LAUNCH(false, false, 12);
+  } else {
+    GEMM_LAUNCH(false, false, 16);
+  }
+#undef GEMM_LAUNCH
   return hipGetLastError();
 }
 

@@ -427,6 +427,8 @@ struct GemmJob {
 struct GemmBatch {
   GemmJob j[kMaxPlanes];
   const double* skip;  // launch context flag (filled by the launcher)
+  int njobs;                // filled by the launcher
+  int lbase[kMaxPlanes];    // first tile slot of each job per XCD (filled by the launcher)
 };
 hipError_t launch_gemm_f64(const GemmBatch& gb, int njobs, hipStream_t st);
 // out[m + n ldo] = sum over s < nslice (<= 16) of P[s stride + m + n ldp], in order of s
