@@ -163,6 +163,9 @@ int ibpm_advance(ibpm_solver* s, int nsteps, const double* freestream, const dou
 int ibpm_set_body_motion(ibpm_solver* s, int body, double t, const double pos[2],
                          const double vel[2], double angle, double angular_vel);
 
+/* Field copies to and from host memory (state.q, .psi, ... as the table above). The step
+ * forms the fluxes q only where it needs them; ibpm_read(IBPM_Q) first writes q = curl psi
+ * (one launch) when the last step left it unformed, so it always returns the state's q. */
 int ibpm_read(ibpm_solver* s, int field, double* dst, size_t n);
 int ibpm_write(ibpm_solver* s, int field, const double* src, size_t n);
 size_t ibpm_field_size(const ibpm_solver* s, int field);
