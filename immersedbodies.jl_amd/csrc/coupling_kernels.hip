@@ -542,12 +542,13 @@ __global__ __launch_bounds__(1024) void forces_kernel(int n, const double* __res
 }
 
 // project_circ's first launch for static bodies (three independent jobs by block range):
-// rot(E^T Ft) on the box from the vertex lists, Gamma_1 = Gamma_s1 (the solve then
-// subtracts the correction on the box), and the forces of update_vars!.
+// rot(E^T Ft) on the box from the vertex lists, the trial Gamma_1 on that box saved (the
+// solve then writes Gamma_1 = saved - correction there, and the final box update reads
+// the change against it), and the forces of update_vars!.
 __global__ __launch_bounds__(256) void project_prologue_kernel(const ScatterArgs a, int gx, int nsc,
-                                                               double* __restrict__ G,
-                                                               const double* __restrict__ Gs,
-                                                               size_t np, double* __restrict__ fb,
+                                                               const double* __restrict__ G,
+                                                               double* __restrict__ saved,
+                                                               double* __restrict__ fb,
                                                                double* __restrict__ total, double h,
                                                                double dt, double* __restrict__ host_total) {
   const int b = blockIdx.x;
@@ -555,18 +556,24 @@ __global__ __launch_bounds__(256) void project_prologue_kernel(const ScatterArgs
     scatter_rot_lists(a, b % gx, b / gx, 0);
   } else if (b == nsc) {
     forces_sum<256>(a.npanel, a.f[0], fb, total, h, dt, host_total);
-  } else {  // 1024 values per block, all four loads issued before the stores
-    const size_t i0 = (size_t)(b - nsc - 1) * 1024 + threadIdx.x;
+  } else {  // 1024 values of the box per block, all four loads issued before the stores
+    const int n = a.rw * a.rh;
+    const int e0 = (b - nsc - 1) * 1024 + threadIdx.x;
     double v[4];
+    size_t o[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = Gs[i0 + 256 * k < np ? i0 + 256 * k : np - 1];
+    for (int k = 0; k < 4; ++k) {
+      const int e = min(e0 + 256 * k, n - 1);
+      o[k] = (size_t)(a.ri0 + e % a.rw) + (size_t)(a.rj0 + e / a.rw) * a.g.Nx;
+      v[k] = G[o[k]];
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      if (i0 + 256 * k < np) G[i0 + 256 * k] = v[k];
+      if (e0 + 256 * k < n) saved[o[k]] = v[k];
   }
 }
 
-hipError_t launch_project_prologue(const ScatterArgs& a0, double* G, const double* Gs, size_t np,
+hipError_t launch_project_prologue(const ScatterArgs& a0, const double* G, double* saved,
                                    double* fb, double* total, double h, double dt,
                                    double* host_total, hipStream_t st) {
   if (g_skip) return no_skip();
@@ -576,9 +583,9 @@ hipError_t launch_project_prologue(const ScatterArgs& a0, double* G, const doubl
       a.ri0 + a.rw > a.g.Nx || a.rj0 + a.rh > a.g.Ny)
     return hipErrorInvalidValue;
   const int gx = (a.rw + 63) / 64, nsc = gx * ((a.rh + 3) / 4);
-  const int ncp = (int)((np + 1023) / 1024);
-  hipLaunchKernelGGL(project_prologue_kernel, dim3(nsc + 1 + ncp), dim3(256), 0, st, a, gx, nsc, G, Gs,
-                     np, fb, total, h, dt, host_total);
+  const int ncp = (a.rw * a.rh + 1023) / 1024;
+  hipLaunchKernelGGL(project_prologue_kernel, dim3(nsc + 1 + ncp), dim3(256), 0, st, a, gx, nsc, G,
+                     saved, fb, total, h, dt, host_total);
   return hipGetLastError();
 }
 

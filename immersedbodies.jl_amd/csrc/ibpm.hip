@@ -465,9 +465,11 @@ int v2f_group(ibpm_solver* s, int ng, const void* const* in, bool in_f32, double
   return run_pass(s, pb, no, false, epi, mixed, false);
 }
 
-// vort2flux (fluid.jl:175-206) for ncol independent columns of L planes each.
-// With `outer`, the coarse levels outside the centre come from outer (final pass of the
-// step); otherwise coarsify! writes the centre region in place. q: curl all levels.
+// vort2flux (fluid.jl:175-206) for ncol independent columns of L planes each. coarsify!
+// writes the centre regions in place; with fwd_given (the final call of a step with box
+// updates) column 0 changed since the trial call only on level 1's box, so coarsify! is
+// re-evaluated on each level's box only, the trial values it overwrites saved into
+// saved[0] (which holds level 1's already). q: curl all levels.
 // nout: psi is written on the nout finest levels only (the others are only needed for
 // their boundary values, which the spectral ring chain carries); the mixed path's
 // refinement reads every level and always writes all of them. spec0 (FP64): the forward
@@ -478,9 +480,9 @@ int box_update(ibpm_solver* s, const double* G, const double* Gs, double* F, boo
 
 // fwd (FP64, column 0): the L spectral planes of column 0; with fwd_given they hold those
 // of the trial vort2flux of the step, which box_update turns into this call's (Gc[0] minus
-// outer[0] vanishes outside the boxes), else the forward passes write them there.
+// saved[0] on the boxes, zero elsewhere), else the forward passes write them there.
 int vort2flux(ibpm_solver* s, int ncol, double* const* Gc, double* const* psic,
-              double* const* outer, double* const* qc, int nout = 0,
+              double* const* saved, double* const* qc, int nout = 0,
               const double* spec0 = nullptr, double* fwd = nullptr, bool fwd_given = false,
               bool psi_box = false) {
   const size_t P = s->plane();
@@ -489,13 +491,22 @@ int vort2flux(ibpm_solver* s, int ncol, double* const* Gc, double* const* psic,
   if (nout <= 0 || nout > L || mixed || qc) nout = L;
   if (mixed) spec0 = nullptr, fwd = nullptr;
   if (!fwd) fwd_given = false;
+  if (fwd_given && !(saved && saved[0])) return fail(IBPM_E_ARG, "box update without saved planes");
   if (mixed || ncol != 1 || qc || !s->pbox_on) psi_box = false;
-  for (int c = 0; c < ncol; ++c)
+  for (int c = 0; c < ncol; ++c) {
+    if (fwd_given && c == 0) {
+      BoxSet bs;
+      memset(&bs, 0, sizeof bs);
+      for (int l = 0; l < L; ++l)
+        bs.i0[l] = s->bi0[l], bs.j0[l] = s->bj0[l], bs.w[l] = s->bw[l], bs.h[l] = s->bh[l];
+      HIPCHK(launch_coarsify_boxes(s->g, L, Gc[0], saved[0], bs, s->st));
+      continue;
+    }
     for (int l = 1; l < L; ++l)
-      HIPCHK(launch_coarsify(s->g, Gc[c] + l * P, Gc[c] + (l - 1) * P,
-                             outer && outer[c] ? outer[c] + l * P : nullptr, s->st));
+      HIPCHK(launch_coarsify(s->g, Gc[c] + l * P, Gc[c] + (l - 1) * P, nullptr, s->st));
+  }
   if (fwd_given) {
-    int rc = box_update(s, Gc[0], outer[0], fwd, true);
+    int rc = box_update(s, Gc[0], saved[0], fwd, true);
     if (rc) return rc;
   }
   const double* sin0[kMaxPlanes] = {};
@@ -1415,7 +1426,7 @@ int step_head(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step
     j.da = 1.0;
     j.dnum = s->dt;
     j.dden = 2.0 * s->Re * (s->hc[l] * s->hc[l]);
-    j.x = s->Gs + l * P;
+    j.x = s->G + l * P;  // the trial state Gamma_s in place: Gamma^n is not read after rhs
     j.epi = EPI_STORE;
     j.base = nullptr;
     j.a = 1.0;
@@ -1430,7 +1441,7 @@ int step_head(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step
     // trial psi: only couple_surface's E reads it (levels 1 and 2); the final vort2flux
     // below writes every level. Without bodies that final call is the same computation
     // (coarsify! is idempotent), so the trial one is skipped.
-    double* Gc[1] = {s->Gs};
+    double* Gc[1] = {s->G};
     double* pc[1] = {s->psi};
     rc = vort2flux(s, 1, Gc, pc, nullptr, nullptr, std::min(2, L), jobs[0].spec,
                    s->box_on && !spec1 ? s->Ftr : nullptr, false, s->box_on && !spec1);
@@ -1442,10 +1453,12 @@ int step_head(ibpm_solver* s, const double* fsv, const double* motion, ibpm_step
 }
 
 // advance! after couple_surface: the forces, project_circ! (coupling.jl:217-231):
-// Gamma = Gamma_s, Gamma[:,1] -= Ainv1 rot(E^T Ft), and the final vort2flux.
+// Gamma = Gamma_s, Gamma[:,1] -= Ainv1 rot(E^T Ft), and the final vort2flux. Gamma_s is
+// the state G itself (the Helmholtz solves wrote it there): level 1 is saved into Gs where
+// the solve changes it (its box, or the whole plane), the solve writes Gamma_1 = saved -
+// correction, and the final vort2flux re-evaluates coarsify! where that reaches.
 int step_tail(ibpm_solver* s, ibpm_step_stats* stats) {
   const size_t P = s->plane();
-  const int L = s->L;
   const bool spec1 = keeps_spec1(s);
   int rc = IBPM_OK;
   if (s->n > 0) {
@@ -1466,16 +1479,16 @@ int step_tail(ibpm_solver* s, ibpm_step_stats* stats) {
     sa.out[0] = s->rhs;
     // with the box update, Ainv1 rot(E^T Ft) from zero vanishes outside the level-1 box
     // (the Gauss-Seidel support argument): rot and the solve run on the box only, and
-    // Gamma_1 = Gamma_s1 elsewhere
+    // Gamma_1 = Gamma_s1 elsewhere (left in place)
     const bool pbox = s->box_on && !spec1;
     if (pbox) sa.ri0 = s->bi0[0], sa.rj0 = s->bj0[0], sa.rw = s->bw[0], sa.rh = s->bh[0];
     if (pbox && sa.rowptr && sa.rw > 0 && sa.rh > 0) {
-      // the forces, rot(E^T Ft) on the box and Gamma_1 = Gamma_s1 in one launch
-      HIPCHK(launch_project_prologue(sa, s->G, s->Gs, P, s->fb, s->total, s->h, s->dt,
+      // the forces, rot(E^T Ft) on the box and Gamma_s1 saved on the box in one launch
+      HIPCHK(launch_project_prologue(sa, s->G, s->Gs, s->fb, s->total, s->h, s->dt,
                                      s->force_slot, s->st));
     } else {
       HIPCHK(launch_forces(s->n, s->Ft, s->fb, s->total, s->h, s->dt, s->force_slot, s->st));
-      if (pbox) HIPCHK(launch_copy(s->G, s->Gs, P, s->st));
+      HIPCHK(launch_copy(s->Gs, s->G, P, s->st));  // Gamma_s1
       HIPCHK(launch_scatter_rot(sa, s->st));
     }
     SolveJob j;
@@ -1496,18 +1509,17 @@ int step_tail(ibpm_solver* s, ibpm_step_stats* stats) {
     if (spec1) HIPCHK(launch_sub(s->F1, s->F1, s->G1, P, s->st));
     double* Gc[1] = {s->G};
     double* pc[1] = {s->psi};
-    double* oc[1] = {s->Gs};
+    double* sc[1] = {s->Gs};
     double* qc[1] = {s->q};
     // with the box update, only the change of Gamma since the trial vort2flux is
-    // transformed (outer = Gamma_s: G and Gs differ inside the boxes only); outside a
-    // graph the final curl! is left to the next step's nonlinear kernel (q_stale)
-    rc = vort2flux(s, 1, Gc, pc, oc, s->graph_mode ? qc : nullptr, 0, spec1 ? s->F1 : nullptr,
-                   s->box_on && !spec1 ? s->Ftr : nullptr, s->box_on && !spec1);
+    // transformed (it vanishes outside the boxes; Gs holds Gamma_s there); outside a graph
+    // the final curl! is left to the next step's nonlinear kernel (q_stale)
+    rc = vort2flux(s, 1, Gc, pc, pbox ? sc : nullptr, s->graph_mode ? qc : nullptr, 0,
+                   spec1 ? s->F1 : nullptr, pbox ? s->Ftr : nullptr, pbox);
     if (rc) return rc;
     s->q_stale = !s->graph_mode;
   } else {
-    HIPCHK(launch_copy(s->G, s->Gs, (size_t)L * P, s->st));
-    double* Gc[1] = {s->G};
+    double* Gc[1] = {s->G};  // Gamma = Gamma_s, already in place
     double* pc[1] = {s->psi};
     double* qc[1] = {s->q};
     rc = vort2flux(s, 1, Gc, pc, nullptr, s->graph_mode ? qc : nullptr, 0, spec1 ? s->F1 : nullptr);

@@ -220,6 +220,14 @@ hipError_t launch_nonlin_rhs(const NonlinArgs& a, hipStream_t st);
 
 // coarsify! (fluid.jl:208-228) from level `fine` into `coarse`; when `outer` is not null
 // the points outside the centre region are copied from it (fused state copy).
+// boxes of the box updates, per level (0-based vertex offsets and sizes)
+struct BoxSet {
+  int i0[kMaxPlanes], j0[kMaxPlanes], w[kMaxPlanes], h[kMaxPlanes];
+};
+// coarsify! on box l of every level l >= 1 only, in place in G (L planes), the values it
+// overwrites saved at the same offsets of `saved`
+hipError_t launch_coarsify_boxes(GridDims g, int L, double* G, double* saved, const BoxSet& b,
+                                 hipStream_t st);
 hipError_t launch_coarsify(GridDims g, double* coarse, const double* fine, const double* outer,
                            hipStream_t st);
 
@@ -312,9 +320,9 @@ void build_scatter_lists(int n, int Nx, int Ny, const int* coords, const int* pe
                          std::vector<int>& rowptr, std::vector<int>& ep, std::vector<double>& ew);
 hipError_t launch_scatter_rot(const ScatterArgs& a, hipStream_t st);
 // project_circ's first launch for static bodies, three independent jobs in one launch:
-// rot(E^T f[0]) from the vertex lists on the box (nrhs 1), G = Gs over np values, and
-// the forces (launch_forces with Ft = f[0]); the same results as the three launches.
-hipError_t launch_project_prologue(const ScatterArgs& a, double* G, const double* Gs, size_t np,
+// rot(E^T f[0]) from the vertex lists on the box (nrhs 1), saved = G on that box (plane
+// pitch Nx), and the forces (launch_forces with Ft = f[0]).
+hipError_t launch_project_prologue(const ScatterArgs& a, const double* G, double* saved,
                                    double* fb, double* total, double h, double dt,
                                    double* host_total, hipStream_t st);
 // (min ix, max ix, min iy, max iy) of the panels' flux cells into out[0..3]

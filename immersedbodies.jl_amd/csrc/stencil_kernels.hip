@@ -5,6 +5,8 @@
 // neighbours through L2 (neighbouring threads share every load).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "ibpm_device.h"
 #include "ibpm_internal.h"
 
@@ -208,6 +210,14 @@ hipError_t launch_nonlin_rhs(const NonlinArgs& a, hipStream_t st) {
 // ---------------------------------------------------------------------------------
 // (CoarsifyRange, coarsify_range: ibpm_internal.h)
 
+// the full weighting of fine vertex (i, j) (1-based), in the reference's order
+__device__ __forceinline__ double coarsify_value(const double* __restrict__ fine, int Nx, int i, int j) {
+#define GF(ii, jj) fine[((ii)-1) + (size_t)((jj)-1) * Nx]
+  return GF(i, j) + 0.5 * (GF(i + 1, j) + GF(i, j + 1) + GF(i - 1, j) + GF(i, j - 1)) +
+         0.25 * (GF(i + 1, j + 1) + GF(i + 1, j - 1) + GF(i - 1, j - 1) + GF(i - 1, j + 1));
+#undef GF
+}
+
 __global__ __launch_bounds__(256) void coarsify_kernel(GridDims g, double* __restrict__ coarse,
                                                        const double* __restrict__ fine,
                                                        const double* __restrict__ outer,
@@ -230,11 +240,7 @@ __global__ __launch_bounds__(256) void coarsify_kernel(GridDims g, double* __res
     if (outer) coarse[idx] = outer[idx];
     return;
   }
-  const int i = 2 + 2 * a, j = 2 + 2 * c;
-#define GF(ii, jj) fine[((ii)-1) + (size_t)((jj)-1) * g.Nx]
-  coarse[idx] = GF(i, j) + 0.5 * (GF(i + 1, j) + GF(i, j + 1) + GF(i - 1, j) + GF(i, j - 1)) +
-                0.25 * (GF(i + 1, j + 1) + GF(i + 1, j - 1) + GF(i - 1, j - 1) + GF(i - 1, j + 1));
-#undef GF
+  coarse[idx] = coarsify_value(fine, g.Nx, 2 + 2 * a, 2 + 2 * c);
 }
 
 hipError_t launch_coarsify(GridDims g, double* coarse, const double* fine, const double* outer,
@@ -249,6 +255,47 @@ hipError_t launch_coarsify(GridDims g, double* coarse, const double* fine, const
   }
   hipLaunchKernelGGL(coarsify_kernel, grid, dim3(256), 0, st, g, coarse, fine, outer, g_skip);
   return hipGetLastError();
+}
+
+// coarsify! re-evaluated on box l of level l only (intersected with the centre region),
+// where a change of the finer level can reach it (the boxes of the box updates: box l
+// holds every coarse vertex whose weighting touches box l-1); elsewhere the coarse values
+// are already those of the finer level's unchanged values. The values it overwrites are
+// first saved at the same offsets of `saved`. One vertex per thread, one launch per level
+// (level l reads what level l-1's launch wrote).
+__global__ __launch_bounds__(256) void coarsify_box_kernel(GridDims g, double* __restrict__ coarse,
+                                                           const double* __restrict__ fine,
+                                                           double* __restrict__ saved, int i0,
+                                                           int w, int j0, int n, const double* skip) {
+  if (skipped(skip)) return;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= n) return;
+  const CoarsifyRange cr = coarsify_range(g.nx, g.ny);
+  const int Ic = i0 + e % w, Jc = j0 + e / w;
+  const size_t idx = (size_t)(Ic - 1) + (size_t)(Jc - 1) * g.Nx;
+  const double v = coarsify_value(fine, g.Nx, 2 + 2 * (Ic - cr.ic0), 2 + 2 * (Jc - cr.jc0));
+  saved[idx] = coarse[idx];
+  coarse[idx] = v;
+}
+
+hipError_t launch_coarsify_boxes(GridDims g, int L, double* G, double* saved, const BoxSet& b,
+                                 hipStream_t st) {
+  const CoarsifyRange cr = coarsify_range(g.nx, g.ny);
+  if (cr.ni == 0 || cr.nj == 0) return hipSuccess;
+  for (int l = 1; l < L; ++l) {
+    if (b.w[l] <= 0 || b.h[l] <= 0) continue;
+    // box l (1-based vertices) within the centre
+    const int i0 = std::max(b.i0[l] + 1, cr.ic0), i1 = std::min(b.i0[l] + b.w[l], cr.ic0 + cr.ni - 1);
+    const int j0 = std::max(b.j0[l] + 1, cr.jc0), j1 = std::min(b.j0[l] + b.h[l], cr.jc0 + cr.nj - 1);
+    if (i1 < i0 || j1 < j0) continue;
+    const int w = i1 - i0 + 1, n = w * (j1 - j0 + 1);
+    hipLaunchKernelGGL(coarsify_box_kernel, dim3((n + 255) / 256), dim3(256), 0, st, g,
+                       G + (size_t)l * g.nG, G + (size_t)(l - 1) * g.nG, saved + (size_t)l * g.nG,
+                       i0, w, j0, n, g_skip);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------------
