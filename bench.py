@@ -108,14 +108,20 @@ def step_roofline(solver, ms_per_step, pass_flops):
     add("nonlin_rhs (curl, avg_flux, direct_product, rot, CNAB rhs; all levels)", 1, 0,
         L * (P + 4 * P) * B8)
     add("Helmholtz solves, Gauss-Seidel (all levels)", 1, 0, 2 * L * P * B8)
-    # two vort2flux per step; coarsify reads the fine plane, writes the coarse centre
-    # (trial) or the whole coarse plane with the outer copy fused (final)
+    # two vort2flux per step; coarsify reads the fine plane and writes the coarse centre
+    # (trial); the final one re-evaluates each coarser level's box only (the fine values
+    # under it read, the trial values saved, the new ones written), or the whole centre
+    # without box updates
     add("coarsify (trial)", L - 1, 0, (L - 1) * (P + P / 4) * B8)
-    add("coarsify + state copy (final)", L - 1, 0, (L - 1) * 3 * P * B8)
+    if on:
+        cells = sum(b[1] * b[3] for b in boxes[1:] if b is not None)
+        add("coarsify on the boxes (final)", L - 1, 0, 7 * cells * B8)
+    else:
+        add("coarsify (final)", L - 1, 0, (L - 1) * (P + P / 4) * B8)
     # sine passes: forward pair of the trial, inverse pair of the final (box updates),
     # 5 planes in and out per launch
     npass = 4 if on else 8
-    add("sine passes (double fold, FP64 MFMA)", npass, npass * pass_flops, npass * 2 * L * P * B8)
+    add("sine passes (folded, FP64 MFMA)", npass, npass * pass_flops, npass * 2 * L * P * B8)
     # ring chains: every level's spectral plane read, E written where psi is formed, the
     # ring operators M_x, M_y read per level step
     m_bytes = (Nx * Nx + Ny * Ny) * B8
@@ -136,7 +142,13 @@ def step_roofline(solver, ms_per_step, pass_flops):
             2.0 * Nx * Ny * bh + 2.0 * bw * Nx * bh, (P + Ny * bh + bw * Nx + bw * bh) * B8)
     add("couple_surface (gather, dense B^-1 product, forces)", 3, 2.0 * (2 * n) ** 2,
         ((2 * n) ** 2 + 98 * 2 * n) * B8)
-    add("project_circ (copy, rot E^T on the box, Gauss-Seidel on the box)", 3, 0, 2 * P * B8)
+    if on:
+        # the trial Gamma_1 saved on the box, rot E^T there, the box solve (b and the
+        # saved values read, Gamma_1 written)
+        add("project_circ (box save, rot E^T, Gauss-Seidel on the box)", 2, 0,
+            6 * boxes[0][1] * boxes[0][3] * B8)
+    else:
+        add("project_circ (level-1 save, rot E^T, solve)", 3, 0, 5 * P * B8)
     floor = 0.0
     for c in comp:
         tf = c["flops"] / (PEAK["fp64"] * 1e12)

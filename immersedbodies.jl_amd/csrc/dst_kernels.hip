@@ -649,6 +649,244 @@ __global__ __launch_bounds__(256, MINW) void dst_pass_f64d_kernel(const PassBatc
 }
 
 // ---------------------------------------------------------------------------------
+// FP64 pass with a third fold (of the T2 blocks). T2[i][m] = 2 sin(pi (2m+1) k / (2Q)),
+// k = i+1, satisfies T2[i][Q-1-m] = (-1)^(k+1) T2[i][m], so for m < Q/2 the rows m and
+// m~ = Q-1-m of a T2 product come from two half-length sums:
+//   O_m = sum_{i even} T2[i][m] X_{i},  E_m = sum_{i odd} T2[i][m] X_{i}
+//   row m = O_m + E_m,  row m~ = O_m - E_m
+// Both blocks of the double fold hold one T2 product: the odd block's B (X = P_{2i+2})
+// and the even block's rows 4m+2 (X = P'_{i+1}). A work-group takes m in [m0, m0 + TM)
+// of [0, Q/2) and the mirror rows m~, so it also forms the block's other product on both
+// (T1 on P_{2i+1} for the odd block, T4 on Q'_{i+1} for the even block, at columns m and
+// m~), and writes four output rows per m:
+//   odd block:  Y_{2m+1}, Y_{M-1-2m} = A_m +- (O_m + E_m),
+//               Y_{2m~+1}, Y_{M-1-2m~} = A_m~ +- (O_m - E_m)
+//   even block: Y_{4m+2} = O_m + E_m, Y_{4m~+2} = O_m - E_m, Y_{4m+4} = C_m,
+//               Y_{4m~+4} = C_m~ (m~ < Q-1)
+// Three Q-long contractions per pair of m instead of four: three quarters of the double
+// fold's matrix-core work, with the same inputs, the same tables (T1, T4 read at columns
+// m and m~, T2 at columns m) and the same chunk order (fold_chunk) as
+// dst_pass_f64d_kernel, whose table copies and staging it keeps; half as many
+// work-groups read each input line. Every K chunk size sums in the same order: all
+// triple-fold variants give the same bits.
+// ---------------------------------------------------------------------------------
+template <typename InT, typename OutT, int EPI, int TB, int TM, int KC, int MINW>
+__global__ __launch_bounds__(256, MINW) void dst_pass_f64t_kernel(const PassBatch batch) {
+  constexpr int BS = TB / 32, MS = TM / 32, RA = KC * TB / 256;
+  constexpr int RPI = 128 / TM;      // table rows per wave instruction (1 KB)
+  constexpr int NT = KC * TM / 128;  // wave instructions per table and chunk
+  constexpr int NI = (NT + 3) / 4;   // per wave (waves past NT / NI issue none)
+  static_assert(BS >= 1 && MS >= 1 && RA >= 1 && NT >= 1 && KC <= 16 && KC % 8 == 0, "tile");
+  int z, bt, mt2;
+  if (!xcd_tile(batch, z, bt, mt2)) return;
+  const int grp = mt2 & 1;  // 0: odd rows, 1: even rows
+  const int mt = mt2 >> 1;
+  const PassPlane& d = batch.p[z];
+  const int Nk = d.Nk, Nb = d.Nb;
+  const int M = Nk + 1, H = M / 2, Q = M / 4, Qp = quarter_pitch(Q), Q2 = Q / 2;
+  const int b0 = bt * TB;
+  const int m0 = mt * TM;
+  if (b0 >= Nb || m0 >= Q2) return;
+
+  __shared__ double Xs[2][KC][TB];  // [0]: P_{2i+1} or P'_{i+1}, [1]: P_{2i+2} or Q'_{i+1}
+  __shared__ __attribute__((aligned(16))) double Sf_s[KC][TM];  // full product, columns m
+  __shared__ __attribute__((aligned(16))) double Sg_s[KC][TM];  // full product, columns m~
+  __shared__ __attribute__((aligned(16))) double Sh_s[KC][TM];  // T2, columns m
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int bh = wave & 1;
+  const int mh = wave >> 1;
+  const size_t tbytes = (size_t)Qp * Qp * 8;
+  const char* T1 = (const char*)d.Sq;
+  const char* SF = grp == 0 ? T1 : T1 + 2 * tbytes;  // T1 (odd block), T4 (even block)
+  const char* SH = T1 + tbytes;                      // T2
+  const char* xb = (const char*)d.in;
+
+  // inputs as in dst_pass_f64d_kernel
+  unsigned o0[RA], o1[RA], o2[RA], o3[RA];
+  bool rowok[RA];
+#pragma unroll
+  for (int r = 0; r < RA; ++r) {
+    const int e = tid + 256 * r;
+    const int ii = e % KC;
+    const int b = b0 + e / KC;
+    rowok[r] = b < Nb;
+    const unsigned line = (unsigned)min(b, Nb - 1) * (unsigned)Nk;
+    const int k0 = grp == 0 ? 2 * ii : ii;
+    const int k1 = grp == 0 ? M - 2 - 2 * ii : M - 2 - ii;
+    const int k2 = grp == 0 ? 2 * ii + 1 : H - 2 - ii;
+    const int k3 = grp == 0 ? M - 3 - 2 * ii : H + ii;
+    o0[r] = (line + k0) * 8u;
+    o1[r] = (line + k1) * 8u;
+    o2[r] = (line + k2) * 8u;
+    o3[r] = (line + k3) * 8u;
+  }
+  const unsigned sa = grp == 0 ? 16u : 8u;
+  // table copy: wave instruction j fills rows (wave NI + j) RPI + lane / (TM/2), columns
+  // c0 + 2 (lane % (TM/2)) and the next, c0 = m0 (columns m) or Q - m0 - TM (columns m~,
+  // in increasing order: local column TM-1-c holds m~ of local column c)
+  const int rr = lane / (TM / 2), cp = lane % (TM / 2);
+  const unsigned orow = (unsigned)((wave * NI * RPI + rr) * Qp) * 8u;
+  const unsigned ot = orow + (unsigned)(m0 + 2 * cp) * 8u;
+  const unsigned og = orow + (unsigned)(Q - m0 - TM + 2 * cp) * 8u;
+  const unsigned ostep = (unsigned)(RPI * Qp * 8), tstep = (unsigned)(Qp * 8);
+  auto copy_tables = [&](int kc) {
+    const unsigned k = (unsigned)kc * tstep;
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+      const int w = wave * NI + j;
+      if (NT % 4 == 0 || w < NT) {
+        const int r0 = w * RPI;
+        __builtin_amdgcn_global_load_lds((global_void*)(SF + ot + k + j * ostep), (lds_void*)&Sf_s[r0][0], 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((global_void*)(SF + og + k + j * ostep), (lds_void*)&Sg_s[r0][0], 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((global_void*)(SH + ot + k + j * ostep), (lds_void*)&Sh_s[r0][0], 16, 0, 0);
+      }
+    }
+  };
+
+  d4 accF[BS][MS], accG[BS][MS], accO[BS][MS], accE[BS][MS];
+#pragma unroll
+  for (int i = 0; i < BS; ++i)
+#pragma unroll
+    for (int j = 0; j < MS; ++j) {
+      accF[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+      accG[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+      accO[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+      accE[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+    }
+  double r0[RA], r1[RA], r2[RA], r3[RA];
+  auto at = [](const char* base, unsigned off) { return (double)*(const InT*)(base + off); };
+  auto fetch = [&](int kc) {
+    const unsigned a = (unsigned)kc * sa, c = grp == 0 ? a : 0u - (unsigned)kc * 8u;
+#pragma unroll
+    for (int r = 0; r < RA; ++r) {
+      r0[r] = at(xb, o0[r] + a);
+      r1[r] = at(xb, o1[r] - a);
+      r2[r] = at(xb, o2[r] + c);
+      r3[r] = at(xb, o3[r] - c);
+    }
+  };
+  const int col = lane & 15;
+  const int kq = lane >> 4;
+  auto stage = [&](int kc) {
+#pragma unroll
+    for (int r = 0; r < RA; ++r) {
+      const int e = tid + 256 * r;
+      const int ii = e % KC;
+      const int i = kc + ii;
+      const int bl = e / KC;
+      double a, b;
+      if (grp == 0) {
+        const double v3 = 2 * i + 2 < H ? r3[r] : 0.0;  // k = H has no partner
+        a = r0[r] + r1[r];                               // P_{2i+1}
+        b = r2[r] + v3;                                  // P_{2i+2}
+      } else {
+        const double qk = r0[r] - r1[r];  // Q_{i+1}
+        const double qh = r2[r] - r3[r];  // Q_{H-i-1}
+        const bool last = i == Q - 1;     // k = Q is its own partner
+        a = last ? qk : qk + qh;
+        b = last ? 0.0 : qk - qh;
+      }
+      Xs[0][ii][bl ^ ii] = rowok[r] ? a : 0.0;
+      Xs[1][ii][bl ^ ii] = rowok[r] ? b : 0.0;
+    }
+  };
+  // the full product's input: Xs[0] (odd block) or Xs[1] (even block); T2's the other
+  const int xf = grp, xh = 1 - grp;
+  auto multiply = [&]() {
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ++ks) {
+      const int krow = ks * 4 + kq;
+      double a[BS];
+#pragma unroll
+      for (int bs = 0; bs < BS; ++bs) a[bs] = Xs[xf][krow][(bh * (TB / 2) + bs * 16 + col) ^ krow];
+#pragma unroll
+      for (int ms = 0; ms < MS; ++ms) {
+        const int mc = mh * (TM / 2) + ms * 16 + col;
+        const double bf = Sf_s[krow][mc];
+        const double bg = Sg_s[krow][TM - 1 - mc];
+#pragma unroll
+        for (int bs = 0; bs < BS; ++bs) {
+          accF[bs][ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bs], bf, accF[bs][ms], 0, 0, 0);
+          accG[bs][ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bs], bg, accG[bs][ms], 0, 0, 0);
+        }
+      }
+    }
+    // T2: even local rows (i even: k odd) into O, odd ones into E
+#pragma unroll
+    for (int ks = 0; ks < KC / 8; ++ks) {
+      const int ro = 2 * (ks * 4 + kq), re = ro + 1;
+      double ao[BS], ae[BS];
+#pragma unroll
+      for (int bs = 0; bs < BS; ++bs) {
+        const int c = bh * (TB / 2) + bs * 16 + col;
+        ao[bs] = Xs[xh][ro][c ^ ro];
+        ae[bs] = Xs[xh][re][c ^ re];
+      }
+#pragma unroll
+      for (int ms = 0; ms < MS; ++ms) {
+        const int mc = mh * (TM / 2) + ms * 16 + col;
+        const double bo = Sh_s[ro][mc];
+        const double be = Sh_s[re][mc];
+#pragma unroll
+        for (int bs = 0; bs < BS; ++bs) {
+          accO[bs][ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(ao[bs], bo, accO[bs][ms], 0, 0, 0);
+          accE[bs][ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(ae[bs], be, accE[bs][ms], 0, 0, 0);
+        }
+      }
+    }
+  };
+  // chunks in fold_chunk order (Q a multiple of 16: every chunk is whole)
+  const int nb = Q / 16, nsteps = Q / KC;
+  int kc = fold_chunk(grp, 0, KC, nb);
+  fetch(kc);
+  copy_tables(kc);
+  for (int t = 0; t < nsteps; ++t) {  // the same for every thread: barriers stay uniform
+    const int kn = t + 1 < nsteps ? fold_chunk(grp, t + 1, KC, nb) : 0;
+    stage(kc);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this chunk's table copy has landed
+    __syncthreads();
+    if (t + 1 < nsteps) fetch(kn);
+    multiply();
+    __syncthreads();
+    if (t + 1 < nsteps) copy_tables(kn);
+    kc = kn;
+  }
+
+  // epilogue: the f64 16x16 accumulator holds D[row = lane/16 + 4r][col = lane%16]
+  const int rq = lane >> 4;
+#pragma unroll
+  for (int bs = 0; bs < BS; ++bs)
+#pragma unroll
+    for (int ms = 0; ms < MS; ++ms) {
+      const int m = m0 + mh * (TM / 2) + ms * 16 + col;
+      if (m >= Q2) continue;
+      const int mb = Q - 1 - m;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int b = b0 + bh * (TB / 2) + bs * 16 + rq + 4 * r;
+        if (b >= Nb) continue;
+        const double f = accF[bs][ms][r], g = accG[bs][ms][r];
+        const double tp = accO[bs][ms][r] + accE[bs][ms][r];  // T2 product, row m
+        const double tm = accO[bs][ms][r] - accE[bs][ms][r];  // row m~
+        if (grp == 0) {
+          store_out<OutT, EPI>(d, b, 2 * m, f + tp);            // row 2m+1
+          store_out<OutT, EPI>(d, b, M - 2 - 2 * m, f - tp);    // row M-1-2m
+          store_out<OutT, EPI>(d, b, 2 * mb, g + tm);           // row 2m~+1
+          store_out<OutT, EPI>(d, b, M - 2 - 2 * mb, g - tm);   // row M-1-2m~
+        } else {
+          store_out<OutT, EPI>(d, b, 4 * m + 1, tp);                   // row 4m+2
+          store_out<OutT, EPI>(d, b, 4 * mb + 1, tm);                  // row 4m~+2
+          store_out<OutT, EPI>(d, b, 4 * m + 3, f);                    // row 4m+4
+          if (mb < Q - 1) store_out<OutT, EPI>(d, b, 4 * mb + 3, g);  // row 4m~+4
+        }
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------------
 // Mixed pass: fp32 operands split into bf16 hi + lo, three MFMA products, fp32 sums.
 // Work-group tile: TB b x (TM odd + TM even) rows, K chunk 32 (one MFMA k-step), TB, TM
 // in {32, 64} picked by the autotuner. The sine tables are stored per output row with k
@@ -1116,23 +1354,28 @@ __global__ __launch_bounds__(256, (TB == 32 && PF <= 1 && sizeof(InT) == 4) ? 5 
 // Tile variants (fold, TB, TM, prefetch depth, K chunk); the autotuner times them on a
 // launch shape and keeps the best. FP64: 0..7 single fold, 8..15 double fold (TM counts
 // quarter rows there), 16..19 double fold with the tables copied straight into LDS
-// (dst_pass_f64d_kernel; where Q is not a multiple of 16 they run as variant 10 or 11).
+// (dst_pass_f64d_kernel; where Q is not a multiple of 16 they run as variant 10 or 11),
+// 20..23 the third fold (dst_pass_f64t_kernel, TM of the Q/2 rows m; where it does not
+// apply they run as the direct double-fold variant of the same TM and K chunk).
 // The mixed pass has its own 20: 0..7 one fold (4..7 two prefetch stages), 8..19 the
 // double fold (dst_pass_mixedq_kernel) with 1 or 2 prefetch stages (8..15) or with the
 // tables copied straight into LDS (16..19); v % 4 picks the tiles as for 0..15.
 constexpr int kVariants = kDstVariants;
-constexpr int kMixedVariants = kDstVariants;
+constexpr int kMixedVariants = 20;
 // the single-fold mixed variant of the same tiles (two prefetch stages for 12..19)
 static int mixed_single(int v) { return v < 8 ? v : (v < 12 ? v % 4 : 4 + v % 4); }
-constexpr int kVarTB[kVariants] = {64, 64, 32, 32, 64, 64, 32, 32, 64, 64,
-                                   32, 32, 64, 64, 32, 32, 32, 32, 32, 32};
-constexpr int kVarTM[kVariants] = {64, 32, 64, 32, 64, 32, 64, 32, 64, 32,
-                                   64, 32, 64, 32, 64, 32, 64, 64, 32, 32};
-constexpr int kVarPF[kVariants] = {1, 1, 1, 1, 2, 2, 2, 2, 1, 1, 1, 1, 2, 2, 2, 2, 0, 0, 0, 0};
-constexpr int kVarFold[kVariants] = {2, 2, 2, 2, 2, 2, 2, 2, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4};
-constexpr int kVarKC[kVariants] = {16, 16, 16, 16, 16, 16, 16, 16, 16, 16,
-                                   16, 16, 16, 16, 16, 16, 16, 8, 16, 8};
-constexpr int kFirstDirect = 16;
+constexpr int kVarTB[kVariants] = {64, 64, 32, 32, 64, 64, 32, 32, 64, 64, 32, 32,
+                                   64, 64, 32, 32, 32, 32, 32, 32, 32, 32, 32, 32};
+constexpr int kVarTM[kVariants] = {64, 32, 64, 32, 64, 32, 64, 32, 64, 32, 64, 32,
+                                   64, 32, 64, 32, 64, 64, 32, 32, 64, 64, 32, 32};
+constexpr int kVarPF[kVariants] = {1, 1, 1, 1, 2, 2, 2, 2, 1, 1, 1, 1,
+                                   2, 2, 2, 2, 0, 0, 0, 0, 0, 0, 0, 0};
+constexpr int kVarFold[kVariants] = {2, 2, 2, 2, 2, 2, 2, 2, 4, 4, 4, 4,
+                                     4, 4, 4, 4, 4, 4, 4, 4, 8, 8, 8, 8};
+constexpr int kVarKC[kVariants] = {16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16,
+                                   16, 16, 16, 16, 16, 8, 16, 8, 16, 8, 16, 8};
+constexpr int kFirstDirect = 16, kFirstTriple = 20;
+static_assert(kVariants == 24, "variant tables");
 
 static dim3 tile_grid(PassBatch& b, int nplanes, int maxNb, int maxH, int TB, int TM) {
   b.nbt = (maxNb + TB - 1) / TB;
@@ -1183,6 +1426,19 @@ struct F64QKC {
   };
 };
 
+// The third fold: tiles of the Q/2 rows m (the launch grid counts those).
+template <int KC>
+struct F64TKC {
+  template <typename InT, typename OutT, int EPI, int TB, int TM>
+  struct K {
+    static constexpr bool quarter = true, eighth = true;
+    static void go(dim3 g, hipStream_t st, const PassBatch& b) {
+      hipLaunchKernelGGL((dst_pass_f64t_kernel<InT, OutT, EPI, TB, TM, KC, TM == 64 ? 3 : 5>), g,
+                         dim3(256), 0, st, b);
+    }
+  };
+};
+
 // Direct table copies: five work-groups per CU for 64-wide tiles, six for 32-wide ones.
 template <int KC>
 struct F64DKC {
@@ -1204,6 +1460,14 @@ template <class T>
 struct is_quarter<T, decltype((void)T::quarter)> {
   static constexpr bool value = T::quarter;
 };
+template <class T, class = void>
+struct is_eighth {
+  static constexpr bool value = false;
+};
+template <class T>
+struct is_eighth<T, decltype((void)T::eighth)> {
+  static constexpr bool value = T::eighth;
+};
 
 template <template <typename, typename, int, int, int> class K, typename InT, typename OutT,
           int TB, int TM>
@@ -1213,7 +1477,8 @@ static void launch_epi(const PassBatch& b_in, int nplanes, int maxNb, int maxH, 
   b.skip = g_skip;
   // double fold: TM quarter rows of each of the two blocks (odd, even) per tile
   constexpr bool quarter = is_quarter<K<InT, OutT, EPI_STORE, TB, TM>>::value;
-  dim3 g = tile_grid(b, nplanes, maxNb, quarter ? maxH / 2 : maxH, TB, TM);
+  constexpr bool eighth = is_eighth<K<InT, OutT, EPI_STORE, TB, TM>>::value;
+  dim3 g = tile_grid(b, nplanes, maxNb, eighth ? maxH / 4 : (quarter ? maxH / 2 : maxH), TB, TM);
   if (quarter) {
     b.nmt *= 2;
     g = dim3(8 * ((nplanes * b.nbt * b.nmt + 7) / 8));
@@ -1239,7 +1504,16 @@ static void launch_variant(int v, const PassBatch& b, int nplanes, int maxNb, in
 
 static void launch_f64_variant(int v, const PassBatch& b, int nplanes, int maxNb, int maxH, int epi,
                                hipStream_t st) {
-  if (v >= kFirstDirect) {
+  if (v >= kFirstTriple) {
+    const bool tm64 = kVarTM[v] == 64;
+    if (kVarKC[v] == 8) {
+      if (tm64) launch_epi<F64TKC<8>::K, double, double, 32, 64>(b, nplanes, maxNb, maxH, epi, st);
+      else launch_epi<F64TKC<8>::K, double, double, 32, 32>(b, nplanes, maxNb, maxH, epi, st);
+    } else {
+      if (tm64) launch_epi<F64TKC<16>::K, double, double, 32, 64>(b, nplanes, maxNb, maxH, epi, st);
+      else launch_epi<F64TKC<16>::K, double, double, 32, 32>(b, nplanes, maxNb, maxH, epi, st);
+    }
+  } else if (v >= kFirstDirect) {
     const bool tm64 = kVarTM[v] == 64;  // TB is 32 for all of them
     if (kVarKC[v] == 8) {
       if (tm64) launch_epi<F64DKC<8>::K, double, double, 32, 64>(b, nplanes, maxNb, maxH, epi, st);
@@ -1274,10 +1548,12 @@ static void launch_mixed_variant(int v, const PassBatch& b, int nplanes, int max
 
 // ---- autotuner ----
 // Process-wide table: launch shape (precision, planes, Nb and H of the largest plane) ->
-// the fastest variant measured on it. Both paths are tuned over the double-fold variants
-// where the length allows them (else the single fold), so every launch of a shape sums in
-// the same order whichever tiles it gets; shapes never tuned use double fold (32, 64).
-int dst_default_variant(bool mixed) { (void)mixed; return 10; }
+// the fastest variant measured on it. Each path is tuned over the variants of the
+// deepest fold the lengths allow (FP64: the third fold, else the double, else the single
+// one; mixed: the double, else the single), so every launch of a shape sums in the same
+// order whichever tiles it gets; shapes never tuned take the first variant of that fold
+// (FP64 20, mixed 10), which falls back like any variant where its fold does not apply.
+int dst_default_variant(bool mixed) { return mixed ? 10 : kFirstTriple; }
 namespace {
 struct TuneKey {
   int mixed, nplanes, maxNb, maxH;
@@ -1323,15 +1599,36 @@ static bool direct_ok(const PassBatch& b, int nplanes, int KC) {
 }
 static int direct_fallback(int v) { return v < kFirstDirect + 2 ? 10 : 11; }
 
+// The third fold needs, besides what the direct variants need, TM <= Q/2 on every plane
+// (a tile's mirror columns m~ stay inside the tables); elsewhere it runs as the direct
+// double-fold variant of the same TM and K chunk.
+static bool triple_ok(const PassBatch& b, int nplanes, int TM) {
+  for (int i = 0; i < nplanes; ++i)
+    if ((b.p[i].Nk + 1) / 4 / 2 < TM) return false;
+  return true;
+}
+static int triple_fallback(int v) { return kFirstDirect + (v - kFirstTriple); }
+
 int dst_effective_variant(int v, int N, bool mixed) {
   if (mixed) return v >= 8 && !mixed_quarter_ok(N) ? mixed_single(v) : v;
-  if (v >= kFirstDirect && ((N + 1) % 4 || ((N + 1) / 4) % 16)) v = direct_fallback(v);
+  const int Q = (N + 1) / 4;
+  // a 64-row third-fold variant takes 32-row tiles where only those fit (the same bits)
+  if (v >= kFirstTriple && kVarTM[v] == 64 && Q / 2 < 64 && Q / 2 >= 32) v += 2;
+  if (v >= kFirstTriple && ((N + 1) % 4 || Q % 16 || Q / 2 < kVarTM[v])) v = triple_fallback(v);
+  if (v >= kFirstDirect && ((N + 1) % 4 || Q % 16)) v = direct_fallback(v);
   return v >= 8 && (N + 1) % 4 ? v - 8 : v;
 }
 
 hipError_t launch_dst_pass_f64(const PassBatch& batch, int nplanes, int maxNb, int maxH, int epi,
                                hipStream_t st, int variant) {
   int v = variant >= 0 && variant < kVariants ? variant : tuned_variant(0, nplanes, maxNb, maxH);
+  // a 64-row third-fold variant takes 32-row tiles where only those fit (the same bits)
+  if (v >= kFirstTriple && kVarTM[v] == 64 && !triple_ok(batch, nplanes, 64) &&
+      triple_ok(batch, nplanes, 32))
+    v += 2;
+  if (v >= kFirstTriple && !(quarter_ok(batch, nplanes) && direct_ok(batch, nplanes, kVarKC[v]) &&
+                             triple_ok(batch, nplanes, kVarTM[v])))
+    v = triple_fallback(v);
   if (v >= kFirstDirect && !(quarter_ok(batch, nplanes) && direct_ok(batch, nplanes, kVarKC[v])))
     v = direct_fallback(v);
   if (v >= 8 && !quarter_ok(batch, nplanes)) v -= 8;
@@ -1341,7 +1638,7 @@ hipError_t launch_dst_pass_f64(const PassBatch& batch, int nplanes, int maxNb, i
 
 hipError_t launch_dst_pass_mixed(const PassBatch& batch, int nplanes, int maxNb, int maxH, int epi,
                                  bool in_f32, bool out_f32, hipStream_t st, int variant) {
-  int v = variant >= 0 && variant < kVariants ? variant : tuned_variant(1, nplanes, maxNb, maxH);
+  int v = variant >= 0 && variant < kMixedVariants ? variant : tuned_variant(1, nplanes, maxNb, maxH);
   if (v >= 8 && !quarter_ok(batch, nplanes, true)) v = mixed_single(v);
   launch_mixed_variant(v, batch, nplanes, maxNb, maxH, epi, in_f32, out_f32, st);
   return hipGetLastError();
@@ -1370,14 +1667,19 @@ hipError_t tune_dst_pass(const PassBatch& batch, int nplanes, int maxNb, int max
   // three rounds over the variants, best mean of 4 launches per round (interleaving the
   // rounds spreads clock and power-state drift over all variants)
   constexpr int reps = 4, rounds = 3;
-  // the double-fold variants where the length allows them, else the single fold
+  // the variants of the deepest fold the lengths allow: FP64 the third fold (20..23), else
+  // the double fold (8..19), else the single fold (0..7); mixed the double fold, else the
+  // single one
   const bool q = quarter_ok(batch, nplanes, mixed);
-  const int v0 = q ? 8 : 0, v1 = q ? (mixed ? kMixedVariants : kVariants) : 8;
+  const bool tq = !mixed && q && direct_ok(batch, nplanes, 8) && triple_ok(batch, nplanes, 32);
+  const int v0 = tq ? kFirstTriple : (q ? 8 : 0);
+  const int v1 = tq ? kVariants : (q ? (mixed ? kMixedVariants : kFirstTriple) : 8);
   float t[kVariants];
   for (int v = 0; v < kVariants; ++v) t[v] = 1e30f;
   for (int round = 0; round < rounds && e == hipSuccess; ++round) {
     for (int v = v0; v < v1 && e == hipSuccess; ++v) {
       if (v >= kFirstDirect && !direct_ok(batch, nplanes, kVarKC[v])) continue;
+      if (v >= kFirstTriple && !triple_ok(batch, nplanes, kVarTM[v])) continue;
       auto run = [&]() {
         if (mixed) launch_mixed_variant(v, batch, nplanes, maxNb, maxH, EPI_STORE, in_f32, out_f32, st);
         else launch_f64_variant(v, batch, nplanes, maxNb, maxH, EPI_STORE, st);
@@ -1419,7 +1721,7 @@ int dst_tuned_variant(int mixed, int nplanes, int maxNb, int maxH) {
 }
 
 int dst_variant_tiles(int v, bool mixed, int* TB, int* TM, int* PF, int* fold, int* KC) {
-  if (v < 0 || v >= kVariants) return 0;
+  if (v < 0 || v >= (mixed ? kMixedVariants : kVariants)) return 0;
   if (mixed) {  // (TB, TM) by v % 4 as for 0..15; PF 1, 2 (one fold), 1, 2, 0 (double fold)
     *TB = kVarTB[v % 4];
     *TM = kVarTM[v % 4];

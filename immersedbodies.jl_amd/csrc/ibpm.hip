@@ -2465,7 +2465,8 @@ int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, doubl
   if (kernel == 0) {
     // matrix-core flops of the contraction one launch (all levels) runs: the single fold
     // multiplies H x H and (H-1) x (H-1) tables, the double fold three Q x Q and one
-    // (Q-1) x (Q-1)
+    // (Q-1) x (Q-1), the third fold per block and pair of rows (m, Q-1-m) two Q-long and
+    // two Q/2-long contractions (3 Q^2 per block)
     const int L = s->L;
     const double Nk = s->g.Nx, Nb = s->g.Ny;
     const int Hi = (s->g.Nx + 1) / 2;
@@ -2474,8 +2475,9 @@ int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, doubl
     if (v < 0) v = dst_default_variant(mixed);
     int TB = 0, TM = 0, PF = 0, fold = 2;
     dst_variant_tiles(dst_effective_variant(v, s->g.Nx, mixed), mixed, &TB, &TM, &PF, &fold);
-    const double per_product = fold == 4 ? 2.0 * Nb * (3.0 * Q * Q + (Q - 1) * (Q - 1))
-                                         : 2.0 * Nb * (H * H + E * E);
+    const double per_product = fold == 8   ? 2.0 * Nb * 6.0 * Q * Q / 2.0
+                               : fold == 4 ? 2.0 * Nb * (3.0 * Q * Q + (Q - 1) * (Q - 1))
+                                           : 2.0 * Nb * (H * H + E * E);
     *flops = L * per_product * (mixed ? 3.0 : 1.0);
     *bytes = L * Nk * Nb * (8.0 + (mixed ? 4.0 : 8.0));  // fp64 plane in, plane out
   } else {

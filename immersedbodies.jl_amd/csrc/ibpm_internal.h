@@ -108,7 +108,7 @@ hipError_t tune_dst_pass(const PassBatch& b, int nplanes, int maxNb, int maxH, b
                          bool in_f32, bool out_f32, hipStream_t st, int* chosen, float* best_ms);
 // (TB, TM, PF, fold, KC) of variant v; PF 0: sine tables copied straight into LDS
 int dst_variant_tiles(int v, bool mixed, int* TB, int* TM, int* PF, int* fold, int* KC = nullptr);
-constexpr int kDstVariants = 20;
+constexpr int kDstVariants = 24;  // FP64 pass variants (the mixed pass has 20)
 // Register-only MFMA loop (f64 16x16x4 or bf16 16x16x32) on `blocks` x 256 threads; out
 // needs blocks x 256 doubles. flops: matrix flops of the launch.
 hipError_t launch_mfma_probe(bool bf16, int iters, int blocks, double* out, hipStream_t st,
