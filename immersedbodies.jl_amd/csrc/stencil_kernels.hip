@@ -83,6 +83,25 @@ __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
     const int jc = min(max(J0 - 1 + r, 1), g.Ny);
     gv[r] = G[(size_t)(Ic - 1) + (size_t)(jc - 1) * g.Nx];
   }
+  // psi(I, j), j = J0-1 .. J0+NL_RJ (PSI; index clamped into the plane) and the previous
+  // nonlinear term at the window's output rows (only the output lanes' values are used),
+  // loaded with Gamma before any boundary rule: one memory round trip for all of them
+  const double* p = PSI ? a.psi + (size_t)lev * g.nG : nullptr;
+  const double* pc = PSI && lev + 1 < a.nlevel ? a.psi + (size_t)(lev + 1) * g.nG : nullptr;
+  double pv[NL_RJ + 2];
+  if (PSI) {
+#pragma unroll
+    for (int r = 0; r < NL_RJ + 2; ++r) {
+      const int jc = min(max(J0 - 1 + r, 1), g.Ny);
+      pv[r] = p[(size_t)(Ic - 1) + (size_t)(jc - 1) * g.Nx];
+    }
+  }
+  double nlo[NL_RJ];
+#pragma unroll
+  for (int r = 0; r < NL_RJ; ++r) {
+    const int jc = min(J0 + r, g.Ny);
+    nlo[r] = a.nl_old[(size_t)(Ic - 1) + (size_t)(jc - 1) * g.Nx + (size_t)lev * g.nG];
+  }
 #pragma unroll
   for (int r = 0; r < NL_RJ + 2; ++r) {
     const int j = J0 - 1 + r;
@@ -96,17 +115,9 @@ __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
   const int Ix = min(I + 1, nx + 1);
   double qxr[NL_RJ + 1];
   if (PSI) {
-    // psi(I, j), j = J0-1 .. J0+NL_RJ (index clamped into the plane), and psi(I+1, j) from
-    // the next lane; faces with a boundary vertex, or whose column is clamped, go through
-    // the boundary rules (curl_u / curl_v). Lane 63's v-faces feed no other lane.
-    const double* p = a.psi + (size_t)lev * g.nG;
-    const double* pc = lev + 1 < a.nlevel ? a.psi + (size_t)(lev + 1) * g.nG : nullptr;
-    double pv[NL_RJ + 2];
-#pragma unroll
-    for (int r = 0; r < NL_RJ + 2; ++r) {
-      const int jc = min(max(J0 - 1 + r, 1), g.Ny);
-      pv[r] = p[(size_t)(Ic - 1) + (size_t)(jc - 1) * g.Nx];
-    }
+    // psi(I+1, j) from the next lane; faces with a boundary vertex, or whose column is
+    // clamped, go through the boundary rules (curl_u / curl_v). Lane 63's v-faces feed no
+    // other lane.
 #pragma unroll
     for (int r = 0; r < NL_RJ + 2; ++r) {
       const double pn = lane_next(pv[r]);  // psi(I + 1, J0 - 1 + r)
@@ -170,7 +181,7 @@ __global__ __launch_bounds__(256) void nonlin_rhs_kernel(const NonlinArgs a) {
     const double Kg = 4.0 * g00 - gl - gr - gb - gt;
     double v = g00 - a.cA[lev] * Kg;
     v = v + (a.dt * a.beta1) * nl;
-    v = v + (a.dt * a.beta2) * a.nl_old[idx];
+    v = v + (a.dt * a.beta2) * nlo[r];
     if (Gc) {
       double rb = 0.0;
       if (J == 1) rb += fac * ring_value(Gc, nx, ny, 2, I + 1);
