@@ -44,6 +44,8 @@ constexpr int RTY = 16;  // tile height (ky): 4 rows per wave
 //   part_c[bx][side][ky] = sum over the tile's kx of S_x[nx/4 | 3nx/4, kx] E
 // (completed by ring_reduce_kernel; every sum runs in a fixed order). Short tiles give
 // four blocks per CU on a 1023^2 plane, enough loads in flight to cover the latency.
+// F32: the spectral planes F (and E) in FP32 (the mixed path).
+template <bool F32>
 __global__ __launch_bounds__(256) void ring_sweep_kernel(const RingChainArgs a, int do_proj) {
   __shared__ double yv[7][RTY];          // cosy, sy1, syN, vL, vR, wb, wt of the tile's rows
   __shared__ double rs[2][RTY][RT + 1];  // S_x-weighted E of the tile, per row, for the row sums
@@ -60,52 +62,37 @@ __global__ __launch_bounds__(256) void ring_sweep_kernel(const RingChainArgs a, 
   const double* __restrict__ fac = a.fac_in[c];
   // Loads in the order their values are needed, all issued before the first wait: the
   // row tables of row tid % RTY (stored by the first RTY threads), then the plane and
-  // column values. Every index is clamped into range (out-of-range values unused).
+  // column values. Every index is clamped into range (out-of-range values unused), and
+  // every load is unconditional: without ring factors (the coarsest level) they are read
+  // from the edge vectors, of the same layout, and replaced by zeros (a branch around
+  // the loads would make the compiler wait for the earlier ones at its end).
+  const bool hf = fac != nullptr;
+  const double* __restrict__ fsrc = hf ? fac : a.edge;
   const int yr = tid & (RTY - 1);
   const int kyr = min(ky0 + yr, Ny) - 1;
   const double y0 = a.cosy[kyr], y5 = a.pw[kyr], y6 = a.pw[Ny + kyr];
-  double y1 = 0.0, y2 = 0.0, y3 = 0.0, y4 = 0.0;
-  if (fac) {
-    y1 = a.edge[2 * Nx + kyr];
-    y2 = a.edge[2 * Nx + Ny + kyr];
-    y3 = fac[2 * Nx + kyr];
-    y4 = fac[2 * Nx + Ny + kyr];
-  }
+  const double y1 = a.edge[2 * Nx + kyr], y2 = a.edge[2 * Nx + Ny + kyr];
+  const double y3 = fsrc[2 * Nx + kyr], y4 = fsrc[2 * Nx + Ny + kyr];
   const int kc = (okx ? kx : Nx) - 1;
-  const void* __restrict__ F = a.F[c];
   void* __restrict__ E = a.E[c];
   double fv[RTY / 4];
-  float fs[RTY / 4];
-  if (a.F_f32) {  // uniform: each precision's loads issue back to back
 #pragma unroll
-    for (int m = 0; m < RTY / 4; ++m)
-      fs[m] = ((const float*)F)[(size_t)kc + (size_t)(min(ky0 + w * (RTY / 4) + m, Ny) - 1) * Nx];
-  } else {
-#pragma unroll
-    for (int m = 0; m < RTY / 4; ++m)
-      fv[m] = ((const double*)F)[(size_t)kc + (size_t)(min(ky0 + w * (RTY / 4) + m, Ny) - 1) * Nx];
+  for (int m = 0; m < RTY / 4; ++m) {
+    const size_t o = (size_t)kc + (size_t)(min(ky0 + w * (RTY / 4) + m, Ny) - 1) * Nx;
+    fv[m] = F32 ? (double)((const float*)a.F[c])[o] : ((const double*)a.F[c])[o];
   }
   const double cx_k = a.cosx[kc], sxl_k = a.pw[2 * Ny + kc], sxr_k = a.pw[2 * Ny + Nx + kc];
-  double fuB = 0.0, fuT = 0.0, e1 = 0.0, eN = 0.0;
-  if (fac) {
-    fuB = fac[kc];
-    fuT = fac[Nx + kc];
-    e1 = a.edge[kc];
-    eN = a.edge[Nx + kc];
-  }
+  const double fuB = hf ? fsrc[kc] : 0.0, fuT = hf ? fsrc[Nx + kc] : 0.0;
+  const double e1 = hf ? a.edge[kc] : 0.0, eN = hf ? a.edge[Nx + kc] : 0.0;
   if (tid < RTY) {
     const bool ok = ky0 + tid <= Ny;
     yv[0][tid] = ok ? y0 : 0.0;
-    yv[1][tid] = ok ? y1 : 0.0;
-    yv[2][tid] = ok ? y2 : 0.0;
-    yv[3][tid] = ok ? y3 : 0.0;
-    yv[4][tid] = ok ? y4 : 0.0;
+    yv[1][tid] = ok && hf ? y1 : 0.0;
+    yv[2][tid] = ok && hf ? y2 : 0.0;
+    yv[3][tid] = ok && hf ? y3 : 0.0;
+    yv[4][tid] = ok && hf ? y4 : 0.0;
     yv[5][tid] = ok ? y5 : 0.0;
     yv[6][tid] = ok ? y6 : 0.0;
-  }
-  if (a.F_f32) {
-#pragma unroll
-    for (int m = 0; m < RTY / 4; ++m) fv[m] = (double)fs[m];
   }
   __syncthreads();
   const double cxk = okx ? cx_k : 0.0, sxl = okx ? sxl_k : 0.0, sxr = okx ? sxr_k : 0.0;
@@ -122,7 +109,7 @@ __global__ __launch_bounds__(256) void ring_sweep_kernel(const RingChainArgs a, 
       const double lam = -2.0 * ((cxk + yv[0][yy]) - 2.0);
       e = a.scale / lam * f;
       if (a.write_e) {
-        if (a.F_f32) ((float*)E)[idx] = (float)e;
+        if (F32) ((float*)E)[idx] = (float)e;
         else ((double*)E)[idx] = e;
       }
       ab += e * yv[5][yy];
@@ -244,7 +231,10 @@ hipError_t launch_ring_sweep(const RingChainArgs& a0, int nchain, bool do_proj, 
   RingChainArgs a = a0;
   a.skip = g_skip;
   const int ntx = (a.Nx + RT - 1) / RT, nty = (a.Ny + RTY - 1) / RTY;
-  hipLaunchKernelGGL(ring_sweep_kernel, dim3(ntx, nty, nchain), dim3(256), 0, st, a, do_proj ? 1 : 0);
+  if (a.F_f32)
+    hipLaunchKernelGGL(ring_sweep_kernel<true>, dim3(ntx, nty, nchain), dim3(256), 0, st, a, do_proj ? 1 : 0);
+  else
+    hipLaunchKernelGGL(ring_sweep_kernel<false>, dim3(ntx, nty, nchain), dim3(256), 0, st, a, do_proj ? 1 : 0);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !do_proj) return e;
   const int nv = 2 * a.Nx + 2 * a.Ny;
