@@ -46,11 +46,13 @@ class HipCnabSolver(AbstractSolver):
                   still stops on the true residual. cg_precond_refresh > 0 reassembles
                   after a step that needed more iterations than that.
       device: HIP device ordinal
-      dst_tiles: sine-pass tile variant; -1 autotunes at creation, 0..19 fix it:
-                 for v < 16, v % 4 picks (TB, TM) = (64,64), (64,32), (32,64), (32,32),
-                 FP64 loads run 1 (v % 8 < 4) or 2 K chunks ahead, v >= 8 folds the FP64
-                 sine matrix twice; 16..19 fold twice and copy the sine tables straight
-                 into LDS, (TB, TM, KC) = (32,64,16), (32,64,8), (32,32,16), (32,32,8)
+      dst_tiles: sine-pass tile variant; -1 autotunes at creation (over the deepest fold
+                 the lengths allow), 0..23 fix it (mixed 0..19): for v < 16, v % 4 picks
+                 (TB, TM) = (64,64), (64,32), (32,64), (32,32), FP64 loads run 1
+                 (v % 8 < 4) or 2 K chunks ahead, v >= 8 folds the sine matrix twice;
+                 16..19 fold twice and copy the sine tables straight into LDS, (TB, TM,
+                 KC) = (32,64,16), (32,64,8), (32,32,16), (32,32,8); 20..23 (FP64) fold
+                 three times with the same tiles
       helmholtz: "auto" solves the implicit Helmholtz systems by red-black Gauss-Seidel
                  in FP64 where 64 half-sweeps bring the error below 2^-56 max|x| (Jacobi
                  contraction 4k/(1+4k) <= 0.54, k = dt / (2 Re h^2)), by the sine

@@ -91,19 +91,25 @@ typedef struct ibpm_options {
   int cg_tol_mode;    /* 0: reltol*||r0|| (IterativeSolvers), 1: reltol*||b||    */
   int device;         /* HIP device ordinal                                      */
   int dst_tiles;      /* sine-pass tiles: -1 (default) autotuned at creation: the
-                         variants (FP64: the double-fold ones, 8..19) are timed on
-                         each launch shape the step uses and the fastest kept
-                         (process-wide, per shape); 0..19 fixed: for v < 16,
-                         v % 4 picks (TB, TM) = (64,64), (64,32), (32,64), (32,32),
-                         FP64 loads run 1 (v % 8 < 4) or 2 chunks ahead, v >= 8
-                         folds the FP64 sine matrix twice instead of once; 16..19
-                         fold twice and copy the sine tables straight into LDS
-                         with (TB, TM, K chunk) = (32,64,16), (32,64,8), (32,32,16),
-                         (32,32,8) where (N+1)/4 is a multiple of the K chunk (else
-                         they run as 10 or 11); mixed uses v % 8 (one fold; TB, TM
-                         and the prefetch depth as above). Variants of one fold give
-                         identical results (same summation order); the two folds
-                         agree to rounding.                                         */
+                         variants of the deepest fold a launch shape allows (FP64:
+                         the third fold, 20..23, else the double fold, 8..19; mixed:
+                         the double fold) are timed on each launch shape the step
+                         uses and the fastest kept (process-wide, per shape);
+                         0..23 fixed (mixed 0..19): for v < 16, v % 4 picks
+                         (TB, TM) = (64,64), (64,32), (32,64), (32,32), FP64 loads
+                         run 1 (v % 8 < 4) or 2 chunks ahead, v >= 8 folds the sine
+                         matrix twice instead of once; 16..19 fold twice and copy
+                         the sine tables straight into LDS with (TB, TM, K chunk) =
+                         (32,64,16), (32,64,8), (32,32,16), (32,32,8) where (N+1)/4
+                         is a multiple of 16 (else they run as 10 or 11); 20..23
+                         (FP64) fold three times with the same (TB, TM, K chunk),
+                         TM counting the (N+1)/8 row pairs, where (N+1)/4 is a
+                         multiple of 16 and (N+1)/8 >= TM (a 64-row variant takes
+                         32 rows where only those fit; else they run as 16..19);
+                         mixed uses v % 8 for one fold (TB, TM and the prefetch
+                         depth as above). Variants of one fold give identical
+                         results (same summation order); the folds agree to
+                         rounding.                                                  */
   int cg_precond;     /* moving bodies: 0 (default) the reference's unpreconditioned
                          cg!; 1 preconditions it with the inverse of B assembled at
                          an earlier body configuration, applied in each body's frame
@@ -219,8 +225,8 @@ int ibpm_time_kernel(ibpm_solver* s, int kernel, int reps, double* avg_ms, doubl
 
 /* Instrumentation: tile shape (TB values of the batch index x TM output rows per block,
  * loads issued PF K chunks ahead, PF = 0: sine tables copied straight into LDS), fold (2:
- * one even/odd fold, 4: two) and K chunk KC that the solver's sine passes of `nplanes`
- * planes along x (xdir != 0) or y use. fold and KC may be NULL. */
+ * one even/odd fold, 4: two, 8: three) and K chunk KC that the solver's sine passes of
+ * `nplanes` planes along x (xdir != 0) or y use. fold and KC may be NULL. */
 int ibpm_dst_tiles(const ibpm_solver* s, int nplanes, int xdir, int* TB, int* TM, int* PF,
                    int* fold, int* KC);
 
