@@ -58,6 +58,24 @@ def test_poisson_and_helmholtz_any_length(precision, refine, tol, nx, ny):
     s.close()
 
 
+@pytest.mark.parametrize("tiles", [20, 21, 22, 23, -1])
+def test_poisson_third_fold_partial_tiles(tiles):
+    """The third-fold sine passes where Q/2 is not a multiple of the tile's 32 or 64 rows:
+    nx = 320 gives Q = 80 along x (40 row pairs: a whole and a partial 32-row tile, the
+    64-row variants taking 32 rows), ny = 256 gives Q = 64 along y (32 row pairs: one
+    whole 32-row tile, the 64-row variants taking 32 rows); against the oracle at 1e-12."""
+    prob, og, s = _grid_and_solver("fp64", 0, 320, 256, 2)
+    print("poisson third fold, tiles", tiles, flush=True)
+    s.close()
+    s = ib.create_solver(prob, precision="fp64", dst_tiles=tiles)
+    assert s.dst_tiles(1)[3] == 8 and s.dst_tiles(1, False)[3] == 8
+    b = np.random.default_rng(320).standard_normal(og.nG)
+    e = cases.relerr(s.apply("poisson", b), O.lap_inv(og, O.diag_poisson(og), b))
+    print("rel err", e)
+    assert e < 1e-12
+    s.close()
+
+
 @pytest.mark.parametrize("dt", [0.004, 0.1, 0.5])
 def test_helmholtz_gauss_seidel_and_sine_transforms(dt):
     """The two Helmholtz methods against the oracle's sine-transform solve at 1e-12: with
