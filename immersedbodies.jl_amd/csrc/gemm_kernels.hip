@@ -57,7 +57,9 @@ constexpr int GR = GT * GK / 256;  // elements of A (and of B) each thread stage
 
 // DA, DB: some job of the launch has a difference term A2 (B2); without, the operand is
 // loaded once.
-template <bool DA, bool DB>
+// MIR: a launch of mirror jobs (GemmJob::mirror): the two half-length sums over even and
+// odd sine rows, written to the mirrored pairs of output columns.
+template <bool DA, bool DB, bool MIR>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
   if (skipped(gb.skip)) return;
   int job, m0, n0;
@@ -72,17 +74,23 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
   double* C = jb.C;
   const int lda = jb.lda, ldb = jb.ldb, ldc = jb.ldc, M = jb.M, N = jb.N, K = jb.K;
   const bool accumulate = jb.accumulate != 0;
+  constexpr bool mir = MIR;
+  const int pu = jb.kpar & 1;  // mirror: local row parity of the even sine rows (U)
   __shared__ double As[GK][GT + 1];  // As[k][m]
   __shared__ double Bs[GK][GT + 1];  // Bs[k][n]
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   const int col = lane & 15, kq = lane >> 4;
 
-  d4 acc[2][2];
+  d4 acc[2][2], accv[MIR ? 2 : 1][MIR ? 2 : 1];  // accv: mirror jobs' odd-row sums V
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < (MIR ? 2 : 1); ++i)
+#pragma unroll
+    for (int j = 0; j < (MIR ? 2 : 1); ++j) accv[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
 
   // A chunk: 64 m x GK k, thread -> m = tid % 64, k = tid / 64 + 4r (m contiguous)
   // B chunk: GK k x 64 n, thread -> k = tid % GK, n = tid / GK + (256 / GK) r (k contiguous)
@@ -122,6 +130,35 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
     }
     __syncthreads();
     if (k0 + GK < K) GEMM_FETCH(k0 + GK)
+    if (MIR && mir) {
+      // per step, rows ku = 2 (4 s + kq) + pu (even sine rows) into U (acc) and ku ^ 1 (odd
+      // ones) into V; k0 is a multiple of GK, so local and global row parities agree
+      const int ks2 = (min(K - k0, GK) + 7) / 8;
+      for (int ks = 0; ks < ks2; ++ks) {
+        const int ku = 2 * (ks * 4 + kq) + pu, kv = ku ^ 1;
+        double au[2], bu[2], av[2], bv[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          au[i] = Bs[ku][wn * 32 + i * 16 + col];
+          av[i] = Bs[kv][wn * 32 + i * 16 + col];
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          bu[j] = As[ku][wm * 32 + j * 16 + col];
+          bv[j] = As[kv][wm * 32 + j * 16 + col];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(au[i], bu[j], acc[i][j], 0, 0, 0);
+            accv[i & (MIR ? 1 : 0)][j & (MIR ? 1 : 0)] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                av[i], bv[j], accv[i & (MIR ? 1 : 0)][j & (MIR ? 1 : 0)], 0, 0, 0);
+          }
+      }
+      __syncthreads();
+      continue;
+    }
     const int ksteps = (min(K - k0, GK) + 3) / 4;  // zero-padded past K: skip those steps
     for (int ks = 0; ks < ksteps; ++ks) {
       const int kr = ks * 4 + kq;
@@ -139,6 +176,37 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmBatch gb) {
     __syncthreads();
   }
 #undef GEMM_FETCH
+  if (MIR && mir) {
+    // columns n and mref - n: C (+)= U + V and U - V, one 16-column subtile at a time (its
+    // column pair's loads first)
+    const int mref = jb.mref;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      double c0[2][4], c1[2][4];
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = min(m0 + wm * 32 + j * 16 + col, M - 1);
+          const int n = min(n0 + wn * 32 + i * 16 + kq + 4 * r, N - 1);
+          c0[j][r] = accumulate ? C[(size_t)m + (size_t)n * ldc] : 0.0;
+          c1[j][r] = accumulate ? C[(size_t)m + (size_t)(mref - n) * ldc] : 0.0;
+        }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int m = m0 + wm * 32 + j * 16 + col;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = n0 + wn * 32 + i * 16 + kq + 4 * r;
+          if (m >= M || n >= N) continue;
+          const double u = acc[i][j][r], v = accv[i & (MIR ? 1 : 0)][j & (MIR ? 1 : 0)][r];
+          C[(size_t)m + (size_t)n * ldc] = c0[j][r] + (u + v);
+          if (mref - n != n) C[(size_t)m + (size_t)(mref - n) * ldc] = c1[j][r] + (u - v);
+        }
+      }
+    }
+    return;
+  }
   // C entries of this thread (accumulate), loaded after the products: not held in
   // registers through them (other work-groups on the CU cover the wait)
   double cin[2][2][4];
@@ -330,29 +398,37 @@ hipError_t launch_gemm_f64(const GemmBatch& gb0, int njobs, hipStream_t st) {
   GemmBatch gb = gb0;
   gb.skip = g_skip;
   gb.njobs = njobs;
-  bool da = false, db = false;
+  bool da = false, db = false, mir = false;
   int kmax = 0;
   for (int i = 0; i < njobs; ++i) {
     const GemmJob& j = gb.j[i];
     if (j.M <= 0 || j.N <= 0 || j.K <= 0) return hipErrorInvalidValue;
+    if (j.mirror && (j.mref < j.N - 1 || j.mref - (j.N - 1) > j.N)) return hipErrorInvalidValue;
+    if ((j.mirror != 0) != (gb.j[0].mirror != 0)) return hipErrorInvalidValue;  // all or none
     da = da || j.A2;
     db = db || j.B2;
+    mir = mir || j.mirror;
     kmax = std::max(kmax, j.K);
   }
   // the 64 x 64 kernel once the launch holds two of its tiles per CU; smaller launches
   // take the register-staged 32 x 32 one
   int tiles = 0;
   const int wide_groups = gemm_groups<GT, GT>(gb, njobs, &tiles);
+  if (mir) {  // mirror jobs only (never with a difference term): the 64 x 64 kernel
+    if (da || db) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((gemm_f64_kernel<false, false, true>), dim3(wide_groups), dim3(256), 0, st, gb);
+    return hipGetLastError();
+  }
   if (tiles >= 512) {
     const dim3 grid(wide_groups);
     if (da && db)
-      hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, dim3(256), 0, st, gb);
+      hipLaunchKernelGGL((gemm_f64_kernel<true, true, false>), grid, dim3(256), 0, st, gb);
     else if (da)
-      hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, dim3(256), 0, st, gb);
+      hipLaunchKernelGGL((gemm_f64_kernel<true, false, false>), grid, dim3(256), 0, st, gb);
     else if (db)
-      hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, dim3(256), 0, st, gb);
+      hipLaunchKernelGGL((gemm_f64_kernel<false, true, false>), grid, dim3(256), 0, st, gb);
     else
-      hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, dim3(256), 0, st, gb);
+      hipLaunchKernelGGL((gemm_f64_kernel<false, false, false>), grid, dim3(256), 0, st, gb);
     return hipGetLastError();
   }
   const dim3 grid(gemm_groups<GTM, GTN>(gb, njobs, &tiles));

@@ -573,12 +573,13 @@ double cA(const ibpm_solver* s, int l) { return s->dt / (2.0 * s->Re * (s->hc[l]
 int box_update(ibpm_solver* s, const double* G, const double* Gs, double* F, bool accumulate) {
   const size_t P = s->plane();
   const int Nx = s->g.Nx, Ny = s->g.Ny;
-  GemmBatch g1, g2;
+  GemmBatch g1, g2, g2m;  // g2m: second products of the x-first levels (mirror jobs)
   SliceSumBatch sb;
   memset(&g1, 0, sizeof g1);
   memset(&g2, 0, sizeof g2);
+  memset(&g2m, 0, sizeof g2m);
   memset(&sb, 0, sizeof sb);
-  int n1 = 0, n2 = 0, nsum = 0;
+  int n1 = 0, n2 = 0, n2m = 0, nsum = 0;
   for (int l = 0; l < s->L; ++l) {
     if (s->bw[l] <= 0 || s->bh[l] <= 0) continue;
     const size_t o = (size_t)s->bi0[l] + (size_t)s->bj0[l] * Nx;
@@ -606,23 +607,28 @@ int box_update(ibpm_solver* s, const double* G, const double* Gs, double* F, boo
       a.M = M1, a.N = N1, a.K = std::min(kk, K1 - k0);
     }
     if (ns > 1) sb.j[nsum++] = SliceSumJob{s->Tpart + s->bpoff[l], tsize, made, ldt, M1, N1, T, ldt};
-    GemmJob& b = g2.j[n2++];
+    GemmJob& b = xf ? g2m.j[n2m++] : g2.j[n2++];
+    b.M = Nx, b.N = Ny;
     if (xf) {
       b.A = T, b.lda = Nx;
       b.B = s->Sfy + s->bj0[l], b.ldb = Ny;
       b.K = s->bh[l];
+      // S_y[j, Ny-1-n] = (-1)^j S_y[j, n]: columns n and Ny-1-n from the sums over even and
+      // odd rows j = bj0 + k, half the products
+      b.mirror = 1, b.mref = Ny - 1, b.kpar = s->bj0[l] & 1;
+      b.N = (Ny + 1) / 2;
     } else {
       b.A = s->Sfx + (size_t)s->bi0[l] * Nx, b.lda = Nx;
       b.B = T, b.ldb = s->bw[l];
       b.K = s->bw[l];
     }
     b.C = F + l * P, b.ldc = Nx;
-    b.M = Nx, b.N = Ny;
     b.accumulate = accumulate ? 1 : 0;
   }
   HIPCHK(launch_gemm_f64(g1, n1, s->st));
   if (nsum) HIPCHK(launch_sum_slices_batch(sb, nsum, s->st));
-  HIPCHK(launch_gemm_f64(g2, n2, s->st));
+  if (n2m) HIPCHK(launch_gemm_f64(g2m, n2m, s->st));
+  if (n2) HIPCHK(launch_gemm_f64(g2, n2, s->st));
   return IBPM_OK;
 }
 

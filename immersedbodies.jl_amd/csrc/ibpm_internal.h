@@ -431,6 +431,11 @@ struct GemmJob {
   int lda, ldb, ldc;
   int M, N, K;
   int accumulate;  // 1: C += A B, 0: C = A B
+  // mirror (B rows k of sine-matrix rows j = k + kpar, modulo 2, and columns n whose
+  // mirror column mref - n has B(k, mref - n) = (-1)^j B(k, n)): only columns n < N are
+  // formed, from the sums U over even j and V over odd j, C(:, n) (+)= U + V and
+  // C(:, mref - n) (+)= U - V (once where the two columns are one)
+  int mirror, mref, kpar;
 };
 struct GemmBatch {
   GemmJob j[kMaxPlanes];
