@@ -240,7 +240,8 @@ constexpr int GCE = GTM * GTN / 256;         // C entries each thread sums and s
 
 // DA, DB: some job of the launch has a difference term A2 (B2); without, the operand is
 // loaded once. KR: the longest run a lane loads per chunk (chunks of 16 KR values of K).
-template <bool DA, bool DB, int KR>
+// MK: the launch's jobs have a mirrored A (GemmJob::amirror; DA false).
+template <bool DA, bool DB, int KR, bool MK = false>
 __global__ __launch_bounds__(256) void gemm_f64_reg_kernel(const GemmBatch gb) {
   if (skipped(gb.skip)) return;
   int job, m0, n0;
@@ -254,6 +255,8 @@ __global__ __launch_bounds__(256) void gemm_f64_reg_kernel(const GemmBatch gb) {
   const double wa = jb.A2 ? 1.0 : 0.0, wb = jb.B2 ? 1.0 : 0.0;
   double* C = jb.C;
   const int lda = jb.lda, ldb = jb.ldb, ldc = jb.ldc, M = jb.M, N = jb.N, K = jb.K;
+  const int amid = jb.amid;
+  const double asign = jb.asign;
   const bool accumulate = jb.accumulate != 0;
   __shared__ double red[4][GTN][GTM + 1];  // the waves' partial tiles, [n][m]
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -290,7 +293,10 @@ __global__ __launch_bounds__(256) void gemm_f64_reg_kernel(const GemmBatch gb) {
 #pragma unroll
       for (int i = 0; i < GM; ++i) {
         const size_t o = ao[i] + k * lda;
-        a[i][r] = DA ? fma(-wa, A2[o], A[o]) : A[o];
+        if (MK)  // A(m, k) + asign A(m, mirror of k): fma with weight +-1 or 0 rounds once
+          a[i][r] = fma((int)k == amid ? 0.0 : asign, A2[(long long)ao[i] - (long long)k * lda], A[o]);
+        else
+          a[i][r] = DA ? fma(-wa, A2[o], A[o]) : A[o];
       }
 #pragma unroll
       for (int j = 0; j < GN; ++j) {
@@ -398,16 +404,18 @@ hipError_t launch_gemm_f64(const GemmBatch& gb0, int njobs, hipStream_t st) {
   GemmBatch gb = gb0;
   gb.skip = g_skip;
   gb.njobs = njobs;
-  bool da = false, db = false, mir = false;
+  bool da = false, db = false, mir = false, mk = false;
   int kmax = 0;
   for (int i = 0; i < njobs; ++i) {
     const GemmJob& j = gb.j[i];
     if (j.M <= 0 || j.N <= 0 || j.K <= 0) return hipErrorInvalidValue;
     if (j.mirror && (j.mref < j.N - 1 || j.mref - (j.N - 1) > j.N)) return hipErrorInvalidValue;
     if ((j.mirror != 0) != (gb.j[0].mirror != 0)) return hipErrorInvalidValue;  // all or none
-    da = da || j.A2;
+    da = da || (j.A2 && !j.amirror);  // a mirrored A's A2 is its mirror, not a difference
     db = db || j.B2;
     mir = mir || j.mirror;
+    mk = mk || j.amirror;
+    if ((j.amirror != 0) != (gb.j[0].amirror != 0)) return hipErrorInvalidValue;  // all or none
     kmax = std::max(kmax, j.K);
   }
   // the 64 x 64 kernel once the launch holds two of its tiles per CU; smaller launches
@@ -417,6 +425,17 @@ hipError_t launch_gemm_f64(const GemmBatch& gb0, int njobs, hipStream_t st) {
   if (mir) {  // mirror jobs only (never with a difference term): the 64 x 64 kernel
     if (da || db) return hipErrorInvalidValue;
     hipLaunchKernelGGL((gemm_f64_kernel<false, false, true>), dim3(wide_groups), dim3(256), 0, st, gb);
+    return hipGetLastError();
+  }
+  if (mk) {  // mirrored A (no difference terms): the register-staged kernel
+    if (da || db || mir) return hipErrorInvalidValue;
+    const dim3 grid(gemm_groups<GTM, GTN>(gb, njobs, &tiles));
+    if (kmax <= 64)
+      hipLaunchKernelGGL((gemm_f64_reg_kernel<false, false, 4, true>), grid, dim3(256), 0, st, gb);
+    else if (kmax <= 128)
+      hipLaunchKernelGGL((gemm_f64_reg_kernel<false, false, 8, true>), grid, dim3(256), 0, st, gb);
+    else
+      return hipErrorInvalidValue;
     return hipGetLastError();
   }
   if (tiles >= 512) {

@@ -273,6 +273,7 @@ int solve_chunk(ibpm_solver* s, const SolveJob* jobs, int nj) {
 // ---- Helmholtz solves by red-black Gauss-Seidel (gs_kernels.hip) ----
 constexpr int kGsMaxSweeps = 64;
 constexpr int kPsiSlicesA = 8, kPsiSlicesB = 16;  // K slices of the trial psi box products
+constexpr int kPsiFoldSlices = 4;  // K slices of the folded first product (<= kPsiSlicesA)
 constexpr int kBoxSliceK = 128;  // K per slice of a long first box-update product
 
 // Half-sweeps m with rho^(m-1) <= 2^-56 for (da I + k (-L)) x = b, k = dnum / dden, where
@@ -832,16 +833,29 @@ int psi_box_product(ibpm_solver* s, const double* E, double* psi) {
   const int Nx = s->g.Nx, Ny = s->g.Ny, bw = s->pbw, bh = s->pbh;
   GemmBatch g;
   memset(&g, 0, sizeof g);
-  const int ka = ((Ny + kPsiSlicesA - 1) / kPsiSlicesA + 31) / 32 * 32;
-  int na = 0;
-  for (int k0 = 0; k0 < Ny && na < kPsiSlicesA; k0 += ka, ++na) {
-    GemmJob& j = g.j[na];
-    j.A = E + (size_t)k0 * Nx, j.lda = Nx;
-    j.B = s->Sfy + k0 + (size_t)s->pj0 * Ny, j.ldb = Ny;
-    j.C = s->Pa + (size_t)na * Nx * bh, j.ldc = Nx;
-    j.M = Nx, j.N = bh, j.K = std::min(ka, Ny - k0);
+  // the first product folded over its contraction: S_y[Ny-1-k, j] = (-1)^j S_y[k, j], so
+  // column j of E S_y[:, J] is sum over k < (Ny+1)/2 of (E[:, k] + (-1)^j E[:, Ny-1-k])
+  // S_y[k, j] (the middle column of an odd Ny once): half the contraction, one job per
+  // K slice and parity of j (every other column of B and of the slice's output)
+  const int Kh = (Ny + 1) / 2, mid = (Ny & 1) ? (Ny - 1) / 2 : -1;
+  const int ka = ((Kh + kPsiFoldSlices - 1) / kPsiFoldSlices + 31) / 32 * 32;
+  int na = 0, nj = 0;
+  for (int k0 = 0; k0 < Kh && na < kPsiFoldSlices; k0 += ka, ++na) {
+    const int K = std::min(ka, Kh - k0);
+    for (int p = 0; p < 2; ++p) {
+      const int np0 = ((p - s->pj0) % 2 + 2) % 2;  // first box column of parity p
+      if (np0 >= bh) continue;
+      GemmJob& j = g.j[nj++];
+      j.A = E + (size_t)k0 * Nx, j.lda = Nx;
+      j.A2 = E + (size_t)(Ny - 1 - k0) * Nx;
+      j.amirror = 1, j.asign = p == 0 ? 1.0 : -1.0;
+      j.amid = mid >= k0 && mid < k0 + K ? mid - k0 : -1;
+      j.B = s->Sfy + k0 + (size_t)(s->pj0 + np0) * Ny, j.ldb = 2 * Ny;
+      j.C = s->Pa + (size_t)na * Nx * bh + (size_t)np0 * Nx, j.ldc = 2 * Nx;
+      j.M = Nx, j.N = (bh - np0 + 1) / 2, j.K = K;
+    }
   }
-  HIPCHK(launch_gemm_f64(g, na, s->st));
+  HIPCHK(launch_gemm_f64(g, nj, s->st));
   HIPCHK(launch_sum_slices(s->Pa, (size_t)Nx * bh, na, Nx, Nx, bh, s->T2, Nx, s->st));
   memset(&g, 0, sizeof g);
   const int kb = ((Nx + kPsiSlicesB - 1) / kPsiSlicesB + 31) / 32 * 32;

@@ -436,6 +436,11 @@ struct GemmJob {
   // formed, from the sums U over even j and V over odd j, C(:, n) (+)= U + V and
   // C(:, mref - n) (+)= U - V (once where the two columns are one)
   int mirror, mref, kpar;
+  // mirrored A (amirror): A(m, k) = A[m + k lda] + asign A2[m - k lda], the second term left
+  // out at k = amid (the middle column, its own mirror; -1: none), A2 pointing at the mirror
+  // of column 0
+  int amirror, amid;
+  double asign;
 };
 struct GemmBatch {
   GemmJob j[kMaxPlanes];
