@@ -133,13 +133,18 @@ def step_roofline(solver, ms_per_step, pass_flops):
             if b is None or b[1] == 0:
                 continue
             bw, bh = b[1], b[3]
-            fl += min(2.0 * Nx * bw * bh + 2.0 * Nx * bh * Ny, 2.0 * bw * bh * Ny + 2.0 * Nx * bw * Ny)
+            # x first: the second product folded over the sine rows' parity (columns n and
+            # Ny-1-n from half the products)
+            fl += min(2.0 * Nx * bw * bh + 2.0 * Nx * bh * ((Ny + 1) // 2),
+                      2.0 * bw * bh * Ny + 2.0 * Nx * bw * Ny)
             by += 2 * P * B8 + (bw * bh + Nx * bw + Ny * bh) * B8
         add("box update products (FP64 MFMA)", 2, fl, by)
     if pbox is not None:
         bw, bh = pbox[1], pbox[3]
+        # the first product folded over its contraction (half the sine rows)
         add("trial psi on the panels' box (FP64 MFMA, K-split)", 4,
-            2.0 * Nx * Ny * bh + 2.0 * bw * Nx * bh, (P + Ny * bh + bw * Nx + bw * bh) * B8)
+            2.0 * Nx * ((Ny + 1) // 2) * bh + 2.0 * bw * Nx * bh,
+            (P + Ny * bh + bw * Nx + bw * bh) * B8)
     add("couple_surface (gather, dense B^-1 product, forces)", 3, 2.0 * (2 * n) ** 2,
         ((2 * n) ** 2 + 98 * 2 * n) * B8)
     if on:
