@@ -416,6 +416,7 @@ hipError_t launch_gemm_f64(const GemmBatch& gb0, int njobs, hipStream_t st) {
     mir = mir || j.mirror;
     mk = mk || j.amirror;
     if ((j.amirror != 0) != (gb.j[0].amirror != 0)) return hipErrorInvalidValue;  // all or none
+    if (j.amirror && (!j.A2 || j.amid < -1 || j.amid >= j.K)) return hipErrorInvalidValue;
     kmax = std::max(kmax, j.K);
   }
   // the 64 x 64 kernel once the launch holds two of its tiles per CU; smaller launches
@@ -427,15 +428,10 @@ hipError_t launch_gemm_f64(const GemmBatch& gb0, int njobs, hipStream_t st) {
     hipLaunchKernelGGL((gemm_f64_kernel<false, false, true>), dim3(wide_groups), dim3(256), 0, st, gb);
     return hipGetLastError();
   }
-  if (mk) {  // mirrored A (no difference terms): the register-staged kernel
+  if (mk) {  // mirrored A (no difference terms): the register-staged kernel, K chunks of 64
     if (da || db || mir) return hipErrorInvalidValue;
     const dim3 grid(gemm_groups<GTM, GTN>(gb, njobs, &tiles));
-    if (kmax <= 64)
-      hipLaunchKernelGGL((gemm_f64_reg_kernel<false, false, 4, true>), grid, dim3(256), 0, st, gb);
-    else if (kmax <= 128)
-      hipLaunchKernelGGL((gemm_f64_reg_kernel<false, false, 8, true>), grid, dim3(256), 0, st, gb);
-    else
-      return hipErrorInvalidValue;
+    hipLaunchKernelGGL((gemm_f64_reg_kernel<false, false, 4, true>), grid, dim3(256), 0, st, gb);
     return hipGetLastError();
   }
   if (tiles >= 512) {
@@ -452,18 +448,17 @@ hipError_t launch_gemm_f64(const GemmBatch& gb0, int njobs, hipStream_t st) {
   }
   const dim3 grid(gemm_groups<GTM, GTN>(gb, njobs, &tiles));
   // loads per lane and chunk: enough for the longest K of the launch in one chunk (up to
-  // 256, 128 with a difference term, whose loads are twice as many)
+  // 256); with a difference term or a mirrored A (twice the loads) chunks of 64, whose
+  // registers leave room for 4 waves per SIMD: two round trips at K = 128 measured faster
+  // than one at half the occupancy
 #define GEMM_LAUNCH(DA_, DB_, KR_) \
   hipLaunchKernelGGL((gemm_f64_reg_kernel<DA_, DB_, KR_>), grid, dim3(256), 0, st, gb)
-  if (da || db) {
-    const bool k4 = kmax <= 64;
-    if (da && db) {
-      if (k4) GEMM_LAUNCH(true, true, 4); else GEMM_LAUNCH(true, true, 8);
-    } else if (da) {
-      if (k4) GEMM_LAUNCH(true, false, 4); else GEMM_LAUNCH(true, false, 8);
-    } else {
-      if (k4) GEMM_LAUNCH(false, true, 4); else GEMM_LAUNCH(false, true, 8);
-    }
+  if (da && db) {
+    GEMM_LAUNCH(true, true, 4);
+  } else if (da) {
+    GEMM_LAUNCH(true, false, 4);
+  } else if (db) {
+    GEMM_LAUNCH(false, true, 4);
   } else if (kmax <= 64) {
     GEMM_LAUNCH(false, false, 4);
   } else if (kmax <= 128) {
