@@ -405,7 +405,6 @@ hipError_t launch_gemm_f64(const GemmBatch& gb0, int njobs, hipStream_t st) {
   gb.skip = g_skip;
   gb.njobs = njobs;
   bool da = false, db = false, mir = false, mk = false;
-  int kmax = 0;
   for (int i = 0; i < njobs; ++i) {
     const GemmJob& j = gb.j[i];
     if (j.M <= 0 || j.N <= 0 || j.K <= 0) return hipErrorInvalidValue;
@@ -417,7 +416,6 @@ hipError_t launch_gemm_f64(const GemmBatch& gb0, int njobs, hipStream_t st) {
     mk = mk || j.amirror;
     if ((j.amirror != 0) != (gb.j[0].amirror != 0)) return hipErrorInvalidValue;  // all or none
     if (j.amirror && (!j.A2 || j.amid < -1 || j.amid >= j.K)) return hipErrorInvalidValue;
-    kmax = std::max(kmax, j.K);
   }
   // the 64 x 64 kernel once the launch holds two of its tiles per CU; smaller launches
   // take the register-staged 32 x 32 one
@@ -447,10 +445,10 @@ hipError_t launch_gemm_f64(const GemmBatch& gb0, int njobs, hipStream_t st) {
     return hipGetLastError();
   }
   const dim3 grid(gemm_groups<GTM, GTN>(gb, njobs, &tiles));
-  // loads per lane and chunk: enough for the longest K of the launch in one chunk (up to
-  // 256); with a difference term or a mirrored A (twice the loads) chunks of 64, whose
-  // registers leave room for 4 waves per SIMD: two round trips at K = 128 measured faster
-  // than one at half the occupancy
+  // K chunks of 64 (KR 4): their registers leave room for 4 waves per SIMD (the most the
+  // reduction buffer allows), and more round trips at that occupancy measured faster than
+  // one chunk over the whole K at 2 or 3 waves (the box update's first product 15.0 ->
+  // 13.0 us at K = 128; a B application of the moving bodies 0.336 -> 0.334 ms)
 #define GEMM_LAUNCH(DA_, DB_, KR_) \
   hipLaunchKernelGGL((gemm_f64_reg_kernel<DA_, DB_, KR_>), grid, dim3(256), 0, st, gb)
   if (da && db) {
@@ -459,14 +457,8 @@ hipError_t launch_gemm_f64(const GemmBatch& gb0, int njobs, hipStream_t st) {
     GEMM_LAUNCH(true, false, 4);
   } else if (db) {
     GEMM_LAUNCH(false, true, 4);
-  } else if (kmax <= 64) {
-    GEMM_LAUNCH(false, false, 4);
-  } else if (kmax <= 128) {
-    GEMM_LAUNCH(false, false, 8);
-  } else if (kmax <= 192) {
-    GEMM_LAUNCH(false, false, 12);
   } else {
-    GEMM_LAUNCH(false, false, 16);
+    GEMM_LAUNCH(false, false, 4);
   }
 #undef GEMM_LAUNCH
   return hipGetLastError();
