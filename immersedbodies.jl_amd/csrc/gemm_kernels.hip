@@ -448,7 +448,7 @@ hipError_t launch_gemm_f64(const GemmBatch& gb0, int njobs, hipStream_t st) {
   // K chunks of 64 (KR 4): their registers leave room for 4 waves per SIMD (the most the
   // reduction buffer allows), and more round trips at that occupancy measured faster than
   // one chunk over the whole K at 2 or 3 waves (the box update's first product 15.0 ->
-  // 13.0 us at K = 128; a B application of the moving bodies 0.336 -> 0.334 ms)
+  // 13.0 us at K = 128)
 #define GEMM_LAUNCH(DA_, DB_, KR_) \
   hipLaunchKernelGGL((gemm_f64_reg_kernel<DA_, DB_, KR_>), grid, dim3(256), 0, st, gb)
   if (da && db) {
