@@ -49,6 +49,24 @@ def test_static_fp64_matches_oracle(builder, box):
     assert abs(solver.state.t - 5 * prob.scheme.dt) < 1e-15
 
 
+@pytest.mark.parametrize("y0", [-1.5, -1.55])
+def test_static_fp64_even_ny_matches_oracle(y0):
+    """ny = 61: the interior length Ny = 60 is even, so the folded first product of the
+    trial psi on the panels' box pairs every sine row with its mirror (no middle row); the
+    two origins, one cell apart, start the box on rows of both parities."""
+    prob = cases.cylinder(nx=60, ny=61, x0=(-1.0, y0))
+    print("case even Ny", y0, flush=True)
+    solver = ib.create_solver(prob, box_update="force")
+    assert solver.box_update(0) is not None
+    orc = cases.oracle_for(prob)
+    nsteps = 6
+    stats = solver.advance(nsteps)
+    f_ref = cases.oracle_advance(orc, prob, nsteps)
+    f_gpu = solver.forces(stats)
+    assert cases.relerr(f_gpu, f_ref) <= FORCE_TOL, (f_gpu, f_ref)
+    _compare_state(solver, orc)
+
+
 def test_no_body_uniform_flow_stays_irrotational():
     prob = cases.no_body()
     solver = ib.create_solver(prob)
